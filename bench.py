@@ -1,0 +1,142 @@
+"""Benchmark: vector pairs graded per second on the 50k-gate netlist (one GPU).
+
+A step is one full grading pass (sim -> mark -> merge -> cover over every
+chunk, the reference's chunk loop main.cu:171-231) of 65,536 pairs on the
+seeded 50,000-gate / 60-level netlist, with the netlist and vectors already
+resident in HBM.  Prints one JSON line.
+
+The roofline object is for the dominant kernel (the fused sim + forward-mark
+level kernel), timed live with HIP events around each of its launches on the
+grader's stream.  The CPU baseline is the serial C oracle on a bounded sample
+of the same workload, on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "gpu-path-delay-coverage_amd")]
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="single-50k-gates")
+    ap.add_argument("--cpu-sample", type=int, default=4096, help="pairs graded by the CPU baseline (0 = skip)")
+    ap.add_argument("--robust", type=int, default=1)
+    return ap.parse_args()
+
+
+def cpu_baseline(nl, bits, pairs, sample, robust):
+    """Serial C oracle (one core) on the first `sample` pairs of the workload."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    net = orc.OracleNet(nl.arrays())
+    f0 = np.full(net.n, 0xFFFFFFFF, np.uint32)
+    f1 = f0.copy()
+    gc = np.zeros(net.n, np.uint64)
+    t0 = time.perf_counter()
+    codes = orc.sim_chunk(net, bits, pairs, 0, sample)
+    marks, _ = orc.mark_chunk(net, codes, robust)
+    orc.merge_chunk(net, codes, marks, 0, f0, f1)
+    orc.cover_chunk(net, marks, 0, f0, f1, gc)
+    dt = time.perf_counter() - t0
+    return sample / dt
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build grades on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design only, see DESIGN.md)"}))
+        sys.exit(2)
+    import pdfgrade as pg
+    from pdfgrade.configs import WORKLOADS
+
+    wl = WORKLOADS[args.workload]
+    nl = pg.Netlist.generate(wl.netlist)
+    info = nl.info()
+    n, I = info["n"], info["num_pi"]
+    bits = pg.gen_vectors(wl.vector_seed, I, wl.pairs)
+    robust = bool(args.robust)
+    g = pg.Grader(nl, device=0)
+    g.set_vectors(bits, wl.pairs)
+    g.set_chunk(wl.chunk)
+    total = None
+    for _ in range(args.warmup):
+        total = g.grade(robust)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t = g.grade(robust)
+        assert total is None or t == total
+        total = t
+    dt = time.perf_counter() - t0
+    pairs_per_s = wl.pairs * args.steps / dt
+    ms_per_step = dt * 1e3 / args.steps
+
+    # live per-launch timing of the dominant kernel family
+    t_total, kt = g.grade_timed(robust)
+    assert t_total == total
+    fam = max(("fwd", "reverse", "cover", "events"), key=lambda k: kt[f"{k}_ms"])
+    # algorithmic bytes of the fused forward kernel per pair: PI vector bits
+    # read once plus the A, B and M planes written once (fanin re-reads are
+    # intra-pass intermediates)
+    fwd_bytes = (3 * n + I) / 8.0 * wl.pairs
+    # reverse (removal + merge): M read and written, B read
+    rev_bytes = 3 * n / 8.0 * wl.pairs
+    fam_bytes = {"fwd": fwd_bytes, "reverse": rev_bytes}.get(fam, 0.0)
+    launches = max(1, kt[f"{fam}_launches"])
+    avg_ms = kt[f"{fam}_ms"] / launches
+    achieved = (fam_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # end-to-end algorithmic rate (10n + I bits per pair: sim, mark, remove,
+    # merge, cover each reading/writing its dense planes once)
+    e2e_bytes = (10 * n + I) / 8.0 * wl.pairs
+    e2e_gbs = e2e_bytes / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if args.cpu_sample > 0:
+        sample = min(args.cpu_sample, wl.pairs)
+        cpu = {"value": cpu_baseline(nl, bits, wl.pairs, sample, robust), "unit": "pairs/s", "cores": 1,
+               "kind": "port",
+               "sample": f"first {sample} of {wl.pairs} pairs of the {wl.name} netlist, one chunk, serial C oracle"}
+
+    out = {
+        "metric": "vector-pairs graded/s at 50k gates",
+        "value": pairs_per_s,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded netlist and vectors)",
+        "config": {"workload": wl.name, "gates": info["n"] - I, "levels": info["levels"], "pis": I,
+                   "pos": info["num_po"], "pairs": wl.pairs, "chunk": wl.chunk,
+                   "mode": "robust" if robust else "non-robust", "total_paths": total},
+        "gate_evals_per_s": pairs_per_s * (n - I),
+        "roofline": {"bound": "hbm", "kernel": f"k_{fam}", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": avg_ms * 1e3, "launches_per_step": launches},
+        "end_to_end": {"algorithmic_GBps": e2e_gbs, "frac": e2e_gbs / HBM_PEAK_GBS},
+        "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "wall_ms")},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,924 @@
+// MI355X grading path: sim -> mark -> merge -> cover on gate-major bit-planes.
+//
+// Layout in HBM (one chunk of W pairs, words = W/64, rows padded to `pitch`
+// words so every row starts on a 128-byte line):
+//   A[n][pitch]  first-frame value of each node, bit j of word k = pair 64k+j
+//   B[n][pitch]  second-frame value
+//   M[n][pitch]  path mark (forward marks, then final marks after removal)
+// A node's 4-valued code is (b | (a^b) << 1): S0, S1, T0 (1->0), T1 (0->1),
+// the encoding the reference's tables use (simkernel.cuh:158-165).
+//
+// Kernels (one launch per level where the reference launches per level):
+//   k_fwd      sim (simkernel.cuh:96-156) and forward marking
+//              (markkernel.cuh:215-270), fused or alone
+//   k_reverse  invalid-mark removal (markkernel.cuh:271-293) and the
+//              first-detection merge (mergekernel.cuh:116-198), fused or alone;
+//              also lists each word's marked nodes in reverse level order
+//   k_events   distinct first-detection patterns of the chunk
+//   k_cover    exact uint64 reverse path-count DP (coverkernel.cuh:171-230)
+//              over the marked nodes of each such pattern only; a pattern
+//              with no first detection contributes nothing (c = 0 for every
+//              node, coverkernel.cuh:214-220), so the others are skipped.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pdf_internal.hpp"
+
+using pdf::Netlist;
+using pdf::set_error;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxRegFanin = 8;  // fanins kept in registers by the mark pass
+
+struct DevNet {
+  const uint8_t *type;
+  const uint8_t *po;
+  const uint32_t *fin_off;
+  const uint32_t *fin;
+  const uint32_t *fout_off;
+  const uint32_t *fout;
+  const uint32_t *level;
+  uint32_t n;
+};
+
+struct Planes {
+  uint64_t *A, *B, *M;
+  uint64_t pitch;  // words per row
+  uint32_t words;  // words in this chunk
+};
+
+// Thread -> (gate, word) for a level launch.  A block of kBlock threads covers
+// (kBlock >> twlog) gates x (1 << twlog) consecutive words; block b takes word
+// tile b % T and gate group b / T, so with T a multiple of 8 a word tile stays
+// on one XCD (blocks are dealt round-robin to the 8 XCDs) at every level and
+// its fanin rows are re-read from that XCD's L2.
+struct LevelMap {
+  uint32_t T;      // word tiles
+  uint32_t twlog;  // log2 tile width in words (<= 6)
+  uint32_t words;  // words in the chunk
+  uint32_t lo, count;
+};
+
+__device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t &g, uint32_t &w) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t grp = b / mp.T, tile = b - grp * mp.T;
+  w = (tile << mp.twlog) + (threadIdx.x & ((1u << mp.twlog) - 1));
+  const uint32_t gi = grp * (kBlock >> mp.twlog) + (threadIdx.x >> mp.twlog);
+  g = mp.lo + gi;
+  return w < mp.words && gi < mp.count;
+}
+
+// Off-path sensitization predicate S(on, off) of markkernel.cuh:36-41 in
+// bit-plane form (one bit per pair).  LUT_R: AND/NAND on in {T0,T1}, off = S1;
+// OR/NOR off = S0.  LUT_NR: AND/NAND (on=T0, off in {S1,T1}) or (on=T1,
+// off=S1); OR/NOR (on=T1, off in {S0,T0}) or (on=T0, off=S0).
+template <bool ROBUST, bool ANDLIKE>
+__device__ __forceinline__ uint64_t sens(uint64_t aon, uint64_t bon, uint64_t aoff, uint64_t boff) {
+  if (ROBUST) return (aon ^ bon) & (ANDLIKE ? (aoff & boff) : ~(aoff | boff));
+  if (ANDLIKE) return (aon & ~bon & boff) | (~aon & bon & aoff & boff);
+  return (~aon & bon & ~boff) | (aon & ~bon & ~aoff & ~boff);
+}
+
+// OR_i AND_{j != i} S(fanin_i, fanin_j) over fanin positions.
+template <bool ROBUST, bool ANDLIKE>
+__device__ __forceinline__ uint64_t side_ok(const uint64_t *fa, const uint64_t *fb, uint32_t nfi) {
+  uint64_t r = 0;
+  for (uint32_t i = 0; i < nfi; ++i) {
+    uint64_t acc = ~0ULL;
+    for (uint32_t j = 0; j < nfi; ++j)
+      if (j != i) acc &= sens<ROBUST, ANDLIKE>(fa[i], fb[i], fa[j], fb[j]);
+    r |= acc;
+  }
+  return r;
+}
+
+// Same predicate for gates with more fanins than fit in registers: operands
+// are re-read from the planes.
+template <bool ROBUST, bool ANDLIKE>
+__device__ uint64_t side_ok_mem(const Planes &pl, const uint32_t *fins, uint32_t nfi, uint32_t w) {
+  uint64_t r = 0;
+  for (uint32_t i = 0; i < nfi; ++i) {
+    const uint64_t ai = pl.A[fins[i] * pl.pitch + w], bi = pl.B[fins[i] * pl.pitch + w];
+    uint64_t acc = ~0ULL;
+    for (uint32_t j = 0; j < nfi && acc; ++j)
+      if (j != i) acc &= sens<ROBUST, ANDLIKE>(ai, bi, pl.A[fins[j] * pl.pitch + w], pl.B[fins[j] * pl.pitch + w]);
+    r |= acc;
+  }
+  return r;
+}
+
+template <bool ROBUST>
+__device__ __forceinline__ uint64_t side_dispatch(bool andlike, const uint64_t *fa, const uint64_t *fb,
+                                                  uint32_t nfi) {
+  return andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+}
+
+// Forward pass over one level: simulation (DO_SIM) and/or forward marking
+// (DO_MARK).  Thread = (gate, word).
+template <bool DO_SIM, bool DO_MARK, bool ROBUST>
+__global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uint64_t *__restrict__ va,
+                                                const uint64_t *__restrict__ vb, uint64_t vwords,
+                                                uint64_t wbase, LevelMap mp) {
+  uint32_t g, w;
+  if (!map_item(mp, g, w)) return;
+  const uint64_t row = (uint64_t)g * pl.pitch + w;
+  const uint8_t ty = net.type[g];
+  uint64_t a, b;
+  if (ty == PDF_INPT) {
+    if (DO_SIM) {
+      // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P])
+      const uint64_t q = wbase + w;
+      a = va[(uint64_t)g * vwords + q];
+      if (vb) {
+        b = vb[(uint64_t)g * vwords + q];
+      } else {
+        const uint64_t q1 = (q + 1 == vwords) ? 0 : q + 1;
+        b = (a >> 1) | (va[(uint64_t)g * vwords + q1] << 63);
+      }
+      pl.A[row] = a;
+      pl.B[row] = b;
+    } else {
+      a = pl.A[row];
+      b = pl.B[row];
+    }
+    if (DO_MARK) pl.M[row] = a ^ b;
+    return;
+  }
+  const uint32_t f0 = net.fin_off[g];
+  const uint32_t nfi = net.fin_off[g + 1] - f0;
+  const uint32_t *fins = net.fin + f0;
+  const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+  const bool andlike = ty == PDF_AND || ty == PDF_NAND;
+  uint64_t fa[kMaxRegFanin], fb[kMaxRegFanin];
+  const uint32_t nreg = nfi < kMaxRegFanin ? nfi : kMaxRegFanin;
+  if (DO_SIM) {
+    fa[0] = pl.A[(uint64_t)fins[0] * pl.pitch + w];
+    fb[0] = pl.B[(uint64_t)fins[0] * pl.pitch + w];
+    a = fa[0];
+    b = fb[0];
+    if (ty == PDF_NOT) { a = ~a; b = ~b; }
+    if (nfi > 1) {
+      if (multi) {
+        for (uint32_t j = 1; j < nfi; ++j) {
+          const uint64_t xa = pl.A[(uint64_t)fins[j] * pl.pitch + w];
+          const uint64_t xb = pl.B[(uint64_t)fins[j] * pl.pitch + w];
+          if (j < kMaxRegFanin) { fa[j] = xa; fb[j] = xb; }
+          // the chained 2-input fold of simkernel.cuh:142-148
+          switch (ty) {
+            case PDF_AND: a &= xa; b &= xb; break;
+            case PDF_NAND: a = ~(a & xa); b = ~(b & xb); break;
+            case PDF_OR: a |= xa; b |= xb; break;
+            default: a = ~(a | xa); b = ~(b | xb); break;
+          }
+        }
+      } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
+        a = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
+        b = 0;
+      }
+    }
+    pl.A[row] = a;
+    pl.B[row] = b;
+  } else {
+    a = pl.A[row];
+    b = pl.B[row];
+  }
+  if (!DO_MARK) return;
+  const uint64_t tr = a ^ b;
+  uint64_t m = 0;
+  if (tr) {
+    const uint64_t m0 = pl.M[(uint64_t)fins[0] * pl.pitch + w];
+    if (ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT) {
+      m = m0 & tr;
+    } else if (ty >= PDF_AND && ty <= PDF_XNOR) {
+      if (nfi == 1) {
+        m = m0 & tr;  // empty AND over off-path inputs
+      } else if (multi && (m0 & tr)) {
+        uint64_t r;
+        if (nfi <= kMaxRegFanin) {
+          if (!DO_SIM) {
+            for (uint32_t j = 0; j < nreg; ++j) {
+              fa[j] = pl.A[(uint64_t)fins[j] * pl.pitch + w];
+              fb[j] = pl.B[(uint64_t)fins[j] * pl.pitch + w];
+            }
+          }
+          r = ROBUST ? side_dispatch<true>(andlike, fa, fb, nfi) : side_dispatch<false>(andlike, fa, fb, nfi);
+        } else {
+          r = andlike ? side_ok_mem<ROBUST, true>(pl, fins, nfi, w) : side_ok_mem<ROBUST, false>(pl, fins, nfi, w);
+        }
+        // markkernel.cuh:253-261: the on-path mark is always fanin 0's
+        m = r & m0 & tr;
+      }
+      // XOR/XNOR with >= 2 fanins never sensitize (LUT entries 0)
+    }
+  }
+  pl.M[row] = m;
+}
+
+// Reverse pass over one level.  DO_REMOVE: m &= po | OR_fanouts m
+// (markkernel.cuh:279-291).  DO_MERGE: earliest marked T0/T1 pair per node
+// (mergekernel.cuh:141-197) and the per-word list of marked nodes.
+template <bool DO_REMOVE, bool DO_MERGE>
+__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, Planes pl, LevelMap mp, uint32_t start, uint32_t *__restrict__ first_t0,
+                                                    uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                                    uint32_t *__restrict__ list_cnt, uint64_t list_stride) {
+  uint32_t g, w;
+  if (!map_item(mp, g, w)) return;
+  const uint64_t row = (uint64_t)g * pl.pitch + w;
+  uint64_t m = pl.M[row];
+  if (!m) return;
+  if (DO_REMOVE && !net.po[g]) {
+    uint64_t r = 0;
+    const uint32_t e = net.fout_off[g + 1];
+    for (uint32_t k = net.fout_off[g]; k < e && (m & ~r); ++k) r |= pl.M[(uint64_t)net.fout[k] * pl.pitch + w];
+    if (m & ~r) {
+      m &= r;
+      pl.M[row] = m;
+    }
+  }
+  if (DO_MERGE && m) {
+    const uint64_t b = pl.B[row];
+    const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
+    const uint32_t base = start + 64u * w;
+    if (t0) atomicMin(&first_t0[g], base + (uint32_t)__builtin_ctzll(t0));
+    if (t1) atomicMin(&first_t1[g], base + (uint32_t)__builtin_ctzll(t1));
+    const uint32_t pos = atomicAdd(&list_cnt[w], 1u);
+    list[(uint64_t)w * list_stride + pos] = g;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Distinct first-detection patterns of this chunk (relative index).
+__global__ void k_events(uint32_t n, const uint32_t *__restrict__ first_t0, const uint32_t *__restrict__ first_t1,
+                         uint32_t start, uint32_t width, uint64_t *__restrict__ actmap,
+                         uint32_t *__restrict__ act_list, uint32_t *__restrict__ act_count) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t x[2] = {first_t0[g], first_t1[g]};
+  for (int k = 0; k < 2; ++k) {
+    if (x[k] == PDF_NONE || x[k] < start || x[k] - start >= width) continue;
+    const uint32_t rel = x[k] - start;
+    const uint64_t bit = 1ULL << (rel & 63);
+    const uint64_t old = atomicOr((unsigned long long *)&actmap[rel >> 6], (unsigned long long)bit);
+    if (!(old & bit)) act_list[atomicAdd(act_count, 1u)] = rel;
+  }
+}
+
+__device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
+  const uint64_t s = acc + v;
+  const bool o = s < acc;
+  acc = s;
+  return o;
+}
+
+// Reverse path-count DP (coverkernel.cuh:194-228) for each distinct
+// first-detection pattern p of the chunk, over the nodes marked at p.  One
+// wave per pattern; the word's node list is in descending level order, so a
+// run of equal levels is processed lane-parallel.  c/h per node live in this
+// wave's slice of cs/hs and are read only for fanouts marked at p, all of
+// which were written earlier in the same run over the list.
+__global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
+                                              const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
+                                              const uint32_t *__restrict__ act_list,
+                                              const uint32_t *__restrict__ act_count,
+                                              const uint32_t *__restrict__ first_t0,
+                                              const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
+                                              uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ totals,
+                                              uint32_t *__restrict__ err) {
+  const uint32_t lane = threadIdx.x;
+  uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
+  uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
+  const uint32_t nact = *act_count;
+  for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
+    const uint32_t rel = act_list[k];
+    const uint32_t w = rel >> 6, j = rel & 63;
+    const uint32_t p = start + rel;
+    const uint32_t len = list_cnt[w];
+    const uint32_t *L = list + (uint64_t)w * list_stride;
+    uint64_t tot = 0;
+    bool ovf = false;
+    for (uint32_t i = 0; i < len;) {
+      const uint32_t idx = i + lane;
+      const bool valid = idx < len;
+      const uint32_t g = valid ? L[idx] : 0;
+      const uint32_t lv = valid ? net.level[g] : 0xFFFFFFFFu;
+      const uint32_t lv0 = __shfl(lv, 0);
+      const uint64_t same = __ballot(valid && lv == lv0);
+      // the run is a contiguous prefix of the lanes
+      const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
+      if (lane < run && ((pl.M[(uint64_t)g * pl.pitch + w] >> j) & 1)) {
+        uint64_t C = 0, H = net.po[g] == 1 ? 1 : 0;
+        for (uint32_t e = net.fout_off[g]; e < net.fout_off[g + 1]; ++e) {
+          const uint32_t fo = net.fout[e];
+          if ((pl.M[(uint64_t)fo * pl.pitch + w] >> j) & 1) {
+            ovf |= add_ovf(C, c_s[fo]);
+            ovf |= add_ovf(H, h_s[fo]);
+          }
+        }
+        const bool is_new = first_t0[g] == p || first_t1[g] == p;
+        uint64_t c = C, h = H;
+        if (is_new) { ovf |= add_ovf(c, H); h = 0; }
+        if (net.type[g] == PDF_INPT) h = 0;
+        c_s[g] = c;
+        h_s[g] = h;
+        if (c) {
+          const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
+          ovf |= old + c < old;
+          if (net.type[g] == PDF_INPT) ovf |= add_ovf(tot, c);
+        }
+      }
+      __threadfence_block();
+      i += run;
+    }
+    // wave sum of the PI counts of this pattern (kernSumSingle)
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint64_t o = __shfl_down(tot, off);
+      ovf |= add_ovf(tot, o);
+    }
+    if (lane == 0 && tot) {
+      const uint64_t o0 = atomicAdd((unsigned long long *)&totals[0], (unsigned long long)tot);
+      const uint64_t o1 = atomicAdd((unsigned long long *)&totals[1], (unsigned long long)tot);
+      ovf |= (o0 + tot < o0) || (o1 + tot < o1);
+    }
+    if (__ballot(ovf)) { if (lane == 0) atomicOr(err, 1u); }
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- host side
+
+struct pdf_grader {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Netlist nl;
+  // netlist in HBM
+  uint8_t *d_type = nullptr, *d_po = nullptr;
+  uint32_t *d_fin_off = nullptr, *d_fin = nullptr, *d_fout_off = nullptr, *d_fout = nullptr, *d_level = nullptr;
+  // vectors
+  uint64_t num_pairs = 0, vwords = 0;
+  uint64_t *d_va = nullptr, *d_vb = nullptr;
+  // chunk planes
+  uint64_t chunk_pairs = 0, pitch = 0;
+  uint64_t *d_A = nullptr, *d_B = nullptr, *d_M = nullptr;
+  // merge / cover state
+  uint32_t *d_first0 = nullptr, *d_first1 = nullptr;
+  uint64_t *d_gate = nullptr;
+  uint64_t *d_totals = nullptr;  // [0] = current chunk, [1] = all chunks
+  uint32_t *d_err = nullptr;
+  uint32_t *d_list = nullptr, *d_list_cnt = nullptr;
+  uint64_t *d_actmap = nullptr;
+  uint32_t *d_act_list = nullptr, *d_act_count = nullptr;
+  uint64_t *d_cs = nullptr, *d_hs = nullptr;
+  uint32_t cover_waves = 0;
+  // per-launch event timing (pdf_grade_timed)
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_family;
+  bool timing = false;
+  // call-order tracking for the stage API
+  int64_t stage_chunk = -1;
+  int stage = 0;  // 1 = simulated, 2 = marked, 3 = merged, 4 = covered
+  int stage_robust = 0;
+
+  DevNet net() const {
+    return DevNet{d_type, d_po, d_fin_off, d_fin, d_fout_off, d_fout, d_level, nl.n};
+  }
+};
+
+namespace {
+
+bool hip_ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+#define HIPCHK(expr)                                   \
+  do {                                                 \
+    if (!hip_ok((expr), #expr)) return PDF_ERR_HIP;    \
+  } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc((void **)p, sizeof(T) * count) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("hipMalloc of " + std::to_string(sizeof(T) * count) + " bytes failed");
+    return PDF_ERR_NOMEM;
+  }
+  return PDF_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+uint32_t grid_for(uint64_t items, uint32_t block) { return (uint32_t)((items + block - 1) / block); }
+
+enum Family { kFwd = 0, kRev = 1, kEvents = 2, kCover = 3, kFill = 4 };
+
+// With g->timing set, every launch is bracketed by a pair of pooled events.
+void tm_begin(pdf_grader *g, int family) {
+  if (!g->timing) return;
+  const size_t i = g->ev_family.size() * 2;
+  while (g->ev_pool.size() < i + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { g->timing = false; return; }
+    g->ev_pool.push_back(e);
+  }
+  g->ev_family.push_back(family);
+  (void)hipEventRecord(g->ev_pool[i], g->stream);
+}
+void tm_end(pdf_grader *g) {
+  if (!g->timing || g->ev_family.empty()) return;
+  (void)hipEventRecord(g->ev_pool[g->ev_family.size() * 2 - 1], g->stream);
+}
+
+int fill32(pdf_grader *g, uint32_t *p, uint32_t v, uint64_t count) {
+  if (!count) return PDF_OK;
+  const uint32_t grid = std::min<uint64_t>(grid_for(count, 256), 4096);
+  tm_begin(g, kFill);
+  k_fill_u32<<<grid, 256, 0, g->stream>>>(p, v, count);
+  tm_end(g);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+int fill64(pdf_grader *g, uint64_t *p, uint64_t v, uint64_t count) {
+  if (!count) return PDF_OK;
+  const uint32_t grid = std::min<uint64_t>(grid_for(count, 256), 4096);
+  tm_begin(g, kFill);
+  k_fill_u64<<<grid, 256, 0, g->stream>>>(p, v, count);
+  tm_end(g);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+int check_chunk(pdf_grader *g, uint64_t chunk, uint64_t start, uint64_t *width) {
+  if (!g->num_pairs) { set_error("no vectors loaded"); return PDF_ERR_STATE; }
+  if (!g->chunk_pairs) { set_error("chunk size not set"); return PDF_ERR_STATE; }
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  if (chunk >= nchunks) { set_error("chunk index out of range"); return PDF_ERR_ARG; }
+  if (start != chunk * g->chunk_pairs) { set_error("start_pattern must equal chunk * chunk_pairs"); return PDF_ERR_ARG; }
+  *width = std::min(g->chunk_pairs, g->num_pairs - start);
+  return PDF_OK;
+}
+
+Planes planes(pdf_grader *g, uint64_t width) { return Planes{g->d_A, g->d_B, g->d_M, g->pitch, (uint32_t)(width / 64)}; }
+
+LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, uint32_t *grid) {
+  uint32_t twlog = 0;
+  while (twlog < 6 && (1u << twlog) < words) ++twlog;
+  LevelMap mp{(words + (1u << twlog) - 1) >> twlog, twlog, words, lo, count};
+  const uint32_t per_block = kBlock >> twlog;
+  *grid = mp.T * ((count + per_block - 1) / per_block);
+  return mp;
+}
+
+template <bool S, bool M>
+int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
+  const Planes pl = planes(g, width);
+  const DevNet net = g->net();
+  for (uint32_t k = 0; k <= g->nl.L; ++k) {
+    uint32_t grid;
+    const LevelMap mp = level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], &grid);
+    tm_begin(g, kFwd);
+    if (robust)
+      k_fwd<S, M, true><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, start / 64, mp);
+    else
+      k_fwd<S, M, false><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, start / 64, mp);
+    tm_end(g);
+  }
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+template <bool R, bool M>
+int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width) {
+  const Planes pl = planes(g, width);
+  const DevNet net = g->net();
+  if (M) {
+    int rc = fill32(g, g->d_list_cnt, 0, pl.words);
+    if (rc) return rc;
+  }
+  for (uint32_t k = g->nl.L + 1; k-- > 0;) {
+    uint32_t grid;
+    const LevelMap mp = level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], &grid);
+    tm_begin(g, kRev);
+    k_reverse<R, M><<<grid, kBlock, 0, g->stream>>>(net, pl, mp, (uint32_t)start, g->d_first0, g->d_first1,
+                                                    g->d_list, g->d_list_cnt, g->nl.n);
+    tm_end(g);
+  }
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
+  const Planes pl = planes(g, width);
+  int rc;
+  if ((rc = fill64(g, g->d_actmap, 0, pl.words))) return rc;
+  if ((rc = fill32(g, g->d_act_count, 0, 1))) return rc;
+  if ((rc = fill64(g, g->d_totals, 0, 1))) return rc;
+  tm_begin(g, kEvents);
+  k_events<<<grid_for(g->nl.n, 256), 256, 0, g->stream>>>(g->nl.n, g->d_first0, g->d_first1, (uint32_t)start,
+                                                         (uint32_t)width, g->d_actmap, g->d_act_list, g->d_act_count);
+  tm_end(g);
+  tm_begin(g, kCover);
+  k_cover<<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt, g->nl.n,
+                                                g->d_act_list, g->d_act_count, g->d_first0, g->d_first1, g->d_cs,
+                                                g->d_hs, g->d_gate, g->d_totals, g->d_err);
+  tm_end(g);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+void free_chunk(pdf_grader *g) {
+  dfree(g->d_A);
+  dfree(g->d_B);
+  dfree(g->d_M);
+  dfree(g->d_list);
+  dfree(g->d_list_cnt);
+  dfree(g->d_actmap);
+  dfree(g->d_act_list);
+  dfree(g->d_cs);
+  dfree(g->d_hs);
+  g->chunk_pairs = 0;
+}
+
+int reset_state(pdf_grader *g) {
+  int rc;
+  if ((rc = fill32(g, g->d_first0, PDF_NONE, g->nl.n))) return rc;
+  if ((rc = fill32(g, g->d_first1, PDF_NONE, g->nl.n))) return rc;
+  if ((rc = fill64(g, g->d_gate, 0, g->nl.n))) return rc;
+  if ((rc = fill64(g, g->d_totals, 0, 2))) return rc;
+  if ((rc = fill32(g, g->d_err, 0, 1))) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  g->stage = 0;
+  g->stage_chunk = -1;
+  return PDF_OK;
+}
+
+struct EventTimer {
+  pdf_grader *g;
+  float *ms;
+  explicit EventTimer(pdf_grader *g_, float *ms_) : g(g_), ms(ms_) {
+    if (ms) (void)hipEventRecord(g->ev0, g->stream);
+  }
+  int finish() {
+    if (ms) {
+      HIPCHK(hipEventRecord(g->ev1, g->stream));
+      HIPCHK(hipEventSynchronize(g->ev1));
+      HIPCHK(hipEventElapsedTime(ms, g->ev0, g->ev1));
+    } else {
+      HIPCHK(hipStreamSynchronize(g->stream));
+    }
+    return PDF_OK;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
+  if (!nlh || !out) { set_error("pdf_grader_create: null argument"); return PDF_ERR_ARG; }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    set_error("pdf_grader_create: no HIP device available");
+    return PDF_ERR_HIP;
+  }
+  if (device < 0 || device >= ndev) { set_error("pdf_grader_create: bad device index"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(device));
+  auto *g = new (std::nothrow) pdf_grader;
+  if (!g) { set_error("out of memory"); return PDF_ERR_NOMEM; }
+  g->device = device;
+  g->nl = nlh->nl;
+  const Netlist &nl = g->nl;
+  int rc = PDF_OK;
+  auto up = [&](auto *&dptr, const auto &vec) {
+    if (rc) return;
+    using T = std::remove_reference_t<decltype(vec[0])>;
+    rc = dalloc(&dptr, vec.size());
+    if (!rc && !vec.empty() && !hip_ok(hipMemcpy(dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice), "hipMemcpy"))
+      rc = PDF_ERR_HIP;
+  };
+  if (!hip_ok(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+      !hip_ok(hipEventCreate(&g->ev0), "hipEventCreate") || !hip_ok(hipEventCreate(&g->ev1), "hipEventCreate")) {
+    pdf_grader_destroy(g);
+    return PDF_ERR_HIP;
+  }
+  up(g->d_type, nl.type);
+  up(g->d_po, nl.po);
+  up(g->d_fin_off, nl.fin_off);
+  up(g->d_fin, nl.fin);
+  up(g->d_fout_off, nl.fout_off);
+  up(g->d_fout, nl.fout);
+  up(g->d_level, nl.level);
+  if (!rc) rc = dalloc(&g->d_first0, nl.n);
+  if (!rc) rc = dalloc(&g->d_first1, nl.n);
+  if (!rc) rc = dalloc(&g->d_gate, nl.n);
+  if (!rc) rc = dalloc(&g->d_totals, 2);
+  if (!rc) rc = dalloc(&g->d_err, 1);
+  if (!rc) rc = dalloc(&g->d_act_count, 1);
+  if (!rc) rc = reset_state(g);
+  if (rc) { pdf_grader_destroy(g); return rc; }
+  *out = g;
+  return PDF_OK;
+}
+
+void pdf_grader_destroy(pdf_grader *g) {
+  if (!g) return;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  free_chunk(g);
+  dfree(g->d_type);
+  dfree(g->d_po);
+  dfree(g->d_fin_off);
+  dfree(g->d_fin);
+  dfree(g->d_fout_off);
+  dfree(g->d_fout);
+  dfree(g->d_level);
+  dfree(g->d_va);
+  dfree(g->d_vb);
+  dfree(g->d_first0);
+  dfree(g->d_first1);
+  dfree(g->d_gate);
+  dfree(g->d_totals);
+  dfree(g->d_err);
+  dfree(g->d_act_count);
+  for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+static int load_vectors(pdf_grader *g, const uint64_t *a, const uint64_t *b, uint64_t num) {
+  if (!g || !a) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (num == 0 || num % 64) { set_error("number of pairs must be a positive multiple of 64"); return PDF_ERR_ARG; }
+  if (num > 0xFFFFFFFFull - 64) { set_error("at most 2^32-65 pairs (pattern ids are uint32)"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  dfree(g->d_va);
+  dfree(g->d_vb);
+  free_chunk(g);
+  g->num_pairs = 0;
+  const uint64_t words = num / 64;
+  const size_t bytes = sizeof(uint64_t) * words * g->nl.num_pi;
+  int rc = dalloc(&g->d_va, words * g->nl.num_pi);
+  if (!rc && b) rc = dalloc(&g->d_vb, words * g->nl.num_pi);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(g->d_va, a, bytes, hipMemcpyHostToDevice));
+  if (b) HIPCHK(hipMemcpy(g->d_vb, b, bytes, hipMemcpyHostToDevice));
+  g->num_pairs = num;
+  g->vwords = words;
+  rc = pdf_grader_set_chunk(g, 0);
+  return rc;
+}
+
+int pdf_grader_set_vectors(pdf_grader *g, const uint64_t *bits, uint64_t num_vectors) {
+  return load_vectors(g, bits, nullptr, num_vectors);
+}
+
+int pdf_grader_set_pairs(pdf_grader *g, const uint64_t *a_bits, const uint64_t *b_bits, uint64_t num_pairs) {
+  if (!b_bits) { set_error("pdf_grader_set_pairs: null b_bits"); return PDF_ERR_ARG; }
+  return load_vectors(g, a_bits, b_bits, num_pairs);
+}
+
+int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  if (!g->num_pairs) { set_error("pdf_grader_set_chunk: load vectors first"); return PDF_ERR_STATE; }
+  if (chunk_pairs % 64) { set_error("pdf_grader_set_chunk: chunk must be a multiple of 64 pairs"); return PDF_ERR_ARG; }
+  if (chunk_pairs == 0 || chunk_pairs > g->num_pairs) chunk_pairs = g->num_pairs;
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  free_chunk(g);
+  const uint64_t words = chunk_pairs / 64;
+  g->pitch = (words + 15) / 16 * 16;
+  const uint64_t n = g->nl.n;
+  // cover scratch: one n-long c/h slice per wave, about 1 GiB at most
+  const uint64_t per_wave = 2 * n * sizeof(uint64_t);
+  g->cover_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(512, (1ull << 30) / per_wave));
+  int rc = dalloc(&g->d_A, n * g->pitch);
+  if (!rc) rc = dalloc(&g->d_B, n * g->pitch);
+  if (!rc) rc = dalloc(&g->d_M, n * g->pitch);
+  if (!rc) rc = dalloc(&g->d_list, n * words);
+  if (!rc) rc = dalloc(&g->d_list_cnt, words);
+  if (!rc) rc = dalloc(&g->d_actmap, words);
+  if (!rc) rc = dalloc(&g->d_act_list, chunk_pairs);
+  if (!rc) rc = dalloc(&g->d_cs, n * g->cover_waves);
+  if (!rc) rc = dalloc(&g->d_hs, n * g->cover_waves);
+  if (rc) { free_chunk(g); return rc; }
+  g->chunk_pairs = chunk_pairs;
+  return reset_state(g);
+}
+
+int pdf_grader_num_chunks(const pdf_grader *g, uint64_t *num_chunks) {
+  if (!g || !num_chunks) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("no vectors / chunk configured"); return PDF_ERR_STATE; }
+  *num_chunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  return PDF_OK;
+}
+
+int pdf_grader_chunk_width(const pdf_grader *g, uint64_t chunk, uint64_t *pairs) {
+  if (!g || !pairs) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("no vectors / chunk configured"); return PDF_ERR_STATE; }
+  const uint64_t start = chunk * g->chunk_pairs;
+  if (start >= g->num_pairs) { set_error("chunk index out of range"); return PDF_ERR_ARG; }
+  *pairs = std::min(g->chunk_pairs, g->num_pairs - start);
+  return PDF_OK;
+}
+
+int pdf_grader_reset(pdf_grader *g) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  return reset_state(g);
+}
+
+int pdf_sim(pdf_grader *g, uint64_t chunk, uint64_t start, float *ms) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  uint64_t width;
+  int rc = check_chunk(g, chunk, start, &width);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(g->device));
+  EventTimer tm(g, ms);
+  if ((rc = launch_fwd<true, false>(g, start, width, 0))) return rc;
+  if ((rc = tm.finish())) return rc;
+  g->stage_chunk = (int64_t)chunk;
+  g->stage = 1;
+  return PDF_OK;
+}
+
+int pdf_mark(pdf_grader *g, uint64_t chunk, uint64_t start, int robust, float *ms) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  uint64_t width;
+  int rc = check_chunk(g, chunk, start, &width);
+  if (rc) return rc;
+  if (g->stage_chunk != (int64_t)chunk || g->stage < 1) { set_error("pdf_mark: simulate this chunk first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  EventTimer tm(g, ms);
+  if ((rc = launch_fwd<false, true>(g, start, width, robust))) return rc;
+  if ((rc = launch_reverse<true, false>(g, start, width))) return rc;
+  if ((rc = tm.finish())) return rc;
+  g->stage = 2;
+  g->stage_robust = robust;
+  return PDF_OK;
+}
+
+int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, float *ms) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  if (segs != 1) { set_error("pdf_merge: only segment length 1 is supported"); return PDF_ERR_UNSUPPORTED; }
+  uint64_t width;
+  int rc = check_chunk(g, chunk, start, &width);
+  if (rc) return rc;
+  if (g->stage_chunk != (int64_t)chunk || g->stage < 2) { set_error("pdf_merge: mark this chunk first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  EventTimer tm(g, ms);
+  if ((rc = launch_reverse<false, true>(g, start, width))) return rc;
+  if ((rc = tm.finish())) return rc;
+  g->stage = 3;
+  return PDF_OK;
+}
+
+int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t *coverage, float *ms) {
+  if (!g || !coverage) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (segs != 1) { set_error("pdf_cover: only segment length 1 is supported"); return PDF_ERR_UNSUPPORTED; }
+  uint64_t width;
+  int rc = check_chunk(g, chunk, start, &width);
+  if (rc) return rc;
+  if (g->stage_chunk != (int64_t)chunk || g->stage < 3) { set_error("pdf_cover: merge this chunk first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  EventTimer tm(g, ms);
+  if ((rc = launch_cover(g, start, width))) return rc;
+  if ((rc = tm.finish())) return rc;
+  uint64_t tot[2];
+  uint32_t err = 0;
+  HIPCHK(hipMemcpy(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost));
+  g->stage = 4;
+  if (err) { set_error("pdf_cover: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  *coverage = tot[0];
+  return PDF_OK;
+}
+
+static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
+  if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_grade: load vectors first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  int rc = reset_state(g);
+  if (rc) return rc;
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  for (uint64_t c = 0; c < nchunks; ++c) {
+    const uint64_t start = c * g->chunk_pairs;
+    const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
+    if ((rc = launch_fwd<true, true>(g, start, width, robust))) return rc;
+    if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
+    if ((rc = launch_cover(g, start, width))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
+  uint64_t tot[2];
+  uint32_t err = 0;
+  HIPCHK(hipMemcpy(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost));
+  g->stage = 4;
+  g->stage_chunk = (int64_t)nchunks - 1;
+  g->stage_robust = robust;
+  if (err) { set_error("pdf_grade: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  *total = tot[1];
+  return PDF_OK;
+}
+
+int pdf_grade(pdf_grader *g, int robust, uint64_t *total) { return grade_impl(g, robust, total); }
+
+int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times) {
+  if (!g || !total || !times) { set_error("null argument"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  g->ev_family.clear();
+  g->timing = true;
+  const int rc = grade_impl(g, robust, total);
+  g->timing = false;
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
+  std::memset(times, 0, sizeof(*times));
+  double *ms[5] = {&times->fwd_ms, &times->reverse_ms, &times->events_ms, &times->cover_ms, &times->fill_ms};
+  uint32_t *cnt[5] = {&times->fwd_launches, &times->reverse_launches, &times->events_launches,
+                      &times->cover_launches, &times->fill_launches};
+  for (size_t i = 0; i < g->ev_family.size(); ++i) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, g->ev_pool[2 * i], g->ev_pool[2 * i + 1]));
+    *ms[g->ev_family[i]] += t;
+    *cnt[g->ev_family[i]] += 1;
+  }
+  if (!g->ev_family.empty()) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, g->ev_pool[0], g->ev_pool[2 * g->ev_family.size() - 1]));
+    times->wall_ms = t;
+  }
+  return PDF_OK;
+}
+
+int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane, uint64_t *b_plane) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  if (g->stage < 1) { set_error("nothing simulated yet"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
+  const uint64_t words = width / 64;
+  if (a_plane) HIPCHK(hipMemcpy2D(a_plane, words * 8, g->d_A, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
+  if (b_plane) HIPCHK(hipMemcpy2D(b_plane, words * 8, g->d_B, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
+int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane) {
+  if (!g || !mark_plane) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (g->stage < 2) { set_error("nothing marked yet"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
+  const uint64_t words = width / 64;
+  HIPCHK(hipMemcpy2D(mark_plane, words * 8, g->d_M, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
+int pdf_get_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  if (first_t0) HIPCHK(hipMemcpy(first_t0, g->d_first0, sizeof(uint32_t) * g->nl.n, hipMemcpyDeviceToHost));
+  if (first_t1) HIPCHK(hipMemcpy(first_t1, g->d_first1, sizeof(uint32_t) * g->nl.n, hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
+int pdf_get_gate_counts(const pdf_grader *g, uint64_t *counts) {
+  if (!g || !counts) { set_error("null argument"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipMemcpy(counts, g->d_gate, sizeof(uint64_t) * g->nl.n, hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
+int pdf_get_total(const pdf_grader *g, uint64_t *total) {
+  if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  uint64_t tot[2];
+  HIPCHK(hipMemcpy(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
+  *total = tot[1];
+  return PDF_OK;
+}
+
+}  // extern "C"

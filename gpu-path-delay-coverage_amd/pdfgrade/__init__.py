@@ -1,0 +1,382 @@
+"""Python host side of the MI355X path-delay fault grader (ctypes over the C ABI
+declared in include/pdfgrade.h).
+
+The grader mirrors the reference's four host wrappers and its chunk loop:
+
+    Grader.sim(chunk, start)            gpuRunSimulation   simkernel.cuh:200-249
+    Grader.mark(chunk, start, robust)   gpuMarkPaths       markkernel.cuh:295-355
+    Grader.merge(chunk, start, segs)    gpuMergeSegments   mergekernel.cuh:200-267
+    Grader.cover(chunk, start, segs)    gpuCountPaths      coverkernel.cuh:233-333
+    Grader.grade(robust)                chunk loop         main.cu:171-231
+
+Every call runs the HIP kernels in libpdfgrade.so; there is no CPU fallback.
+Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdfgrade.so")
+
+# node types / codes (include/pdfgrade.h)
+UNKN, INPT, AND, NAND, OR, NOR, XOR, XNOR, BUFF, NOT, FROM, DFF = range(12)
+TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF", "NOT", "FROM", "DFF"]
+S0, S1, T0, T1 = 0, 1, 2, 3
+NONE = 0xFFFFFFFF
+
+OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8)
+
+
+class PdfError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class _GenParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_pi", C.c_uint32), ("num_po", C.c_uint32),
+                ("num_gates", C.c_uint32), ("levels", C.c_uint32), ("fanin_p", C.c_double * 4),
+                ("prev_level_frac", C.c_double), ("fanout_alpha", C.c_double),
+                ("fanout_cap", C.c_uint32), ("max_attempts", C.c_uint32)]
+
+
+class _Desc(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("num_pi", C.c_uint32), ("type", C.c_void_p), ("po", C.c_void_p),
+                ("fin_off", C.c_void_p), ("fin", C.c_void_p)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("num_pi", C.c_uint32), ("num_po", C.c_uint32), ("levels", C.c_uint32),
+                ("edges", C.c_uint64), ("structural_paths", C.c_uint64)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("fwd_ms", C.c_double), ("reverse_ms", C.c_double), ("events_ms", C.c_double),
+                ("cover_ms", C.c_double), ("fill_ms", C.c_double), ("fwd_launches", C.c_uint32),
+                ("reverse_launches", C.c_uint32), ("events_launches", C.c_uint32),
+                ("cover_launches", C.c_uint32), ("fill_launches", C.c_uint32), ("wall_ms", C.c_double)]
+
+
+# exported symbols of include/pdfgrade.h with their argument types
+_P = C.c_void_p
+_U64 = C.c_uint64
+_U64P = C.POINTER(C.c_uint64)
+_FP = C.POINTER(C.c_float)
+SIGNATURES = {
+    "pdf_last_error": ([], C.c_char_p),
+    "pdf_abi_version": ([], C.c_int),
+    "pdf_netlist_create": ([C.POINTER(_Desc), C.POINTER(_P)], C.c_int),
+    "pdf_netlist_read_bench": ([C.c_char_p, C.POINTER(_P)], C.c_int),
+    "pdf_netlist_read_level": ([C.c_char_p, C.POINTER(_P)], C.c_int),
+    "pdf_netlist_write_level": ([_P, C.c_char_p], C.c_int),
+    "pdf_netlist_free": ([_P], None),
+    "pdf_netlist_get_info": ([_P, C.POINTER(_Info)], C.c_int),
+    "pdf_netlist_export": ([_P] + [C.c_void_p] * 8, C.c_int),
+    "pdf_netlist_generate": ([C.POINTER(_GenParams), C.POINTER(_P)], C.c_int),
+    "pdf_gen_vectors": ([_U64, C.c_uint32, _U64, C.c_void_p], C.c_int),
+    "pdf_grader_create": ([C.c_int, _P, C.POINTER(_P)], C.c_int),
+    "pdf_grader_destroy": ([_P], None),
+    "pdf_grader_set_vectors": ([_P, C.c_void_p, _U64], C.c_int),
+    "pdf_grader_set_pairs": ([_P, C.c_void_p, C.c_void_p, _U64], C.c_int),
+    "pdf_grader_set_chunk": ([_P, _U64], C.c_int),
+    "pdf_grader_num_chunks": ([_P, _U64P], C.c_int),
+    "pdf_grader_chunk_width": ([_P, _U64, _U64P], C.c_int),
+    "pdf_grader_reset": ([_P], C.c_int),
+    "pdf_sim": ([_P, _U64, _U64, _FP], C.c_int),
+    "pdf_mark": ([_P, _U64, _U64, C.c_int, _FP], C.c_int),
+    "pdf_merge": ([_P, _U64, _U64, C.c_int, _FP], C.c_int),
+    "pdf_cover": ([_P, _U64, _U64, C.c_int, _U64P, _FP], C.c_int),
+    "pdf_grade": ([_P, C.c_int, _U64P], C.c_int),
+    "pdf_grade_timed": ([_P, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
+    "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
+    "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_get_gate_counts": ([_P, C.c_void_p], C.c_int),
+    "pdf_get_total": ([_P, _U64P], C.c_int),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libpdfgrade.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        h = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(h, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = h
+    return _lib
+
+
+def _check(rc: int):
+    if rc != OK:
+        raise PdfError(rc, lib().pdf_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@dataclass(frozen=True)
+class GenConfig:
+    """Synthetic netlist + vector recipe (see configs.py)."""
+    seed: int
+    num_pi: int
+    num_po: int
+    num_gates: int
+    levels: int
+    fanin_p: tuple
+    prev_level_frac: float = 0.7
+    fanout_alpha: float = 0.0
+    fanout_cap: int = 0
+    max_attempts: int = 16
+
+
+class Netlist:
+    """Levelized netlist held by the C library (host memory)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().pdf_netlist_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    @classmethod
+    def generate(cls, cfg: GenConfig) -> "Netlist":
+        p = _GenParams()
+        p.seed, p.num_pi, p.num_po, p.num_gates, p.levels = cfg.seed, cfg.num_pi, cfg.num_po, cfg.num_gates, cfg.levels
+        fp = list(cfg.fanin_p) + [0.0] * (4 - len(cfg.fanin_p))
+        for i in range(4):
+            p.fanin_p[i] = fp[i]
+        p.prev_level_frac, p.fanout_alpha = cfg.prev_level_frac, cfg.fanout_alpha
+        p.fanout_cap, p.max_attempts = cfg.fanout_cap, cfg.max_attempts
+        h = C.c_void_p()
+        _check(lib().pdf_netlist_generate(C.byref(p), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, num_pi: int, types, po, fanins) -> "Netlist":
+        """types/po: per node; fanins: list of fanin-id lists (ordered)."""
+        n = len(types)
+        t = np.asarray(types, dtype=np.uint8)
+        o = np.asarray(po, dtype=np.uint8)
+        off = np.zeros(n + 1, dtype=np.uint32)
+        off[1:] = np.cumsum([len(f) for f in fanins])
+        fin = np.asarray([x for f in fanins for x in f] or [0], dtype=np.uint32)
+        d = _Desc(n, num_pi, _ptr(t).value, _ptr(o).value, _ptr(off).value, _ptr(fin).value)
+        h = C.c_void_p()
+        _check(lib().pdf_netlist_create(C.byref(d), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read_bench(cls, path: str) -> "Netlist":
+        h = C.c_void_p()
+        _check(lib().pdf_netlist_read_bench(path.encode(), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def read_level(cls, path: str) -> "Netlist":
+        h = C.c_void_p()
+        _check(lib().pdf_netlist_read_level(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def write_level(self, path: str):
+        _check(lib().pdf_netlist_write_level(self._h, path.encode()))
+
+    def info(self) -> dict:
+        i = _Info()
+        _check(lib().pdf_netlist_get_info(self._h, C.byref(i)))
+        return dict(n=i.n, num_pi=i.num_pi, num_po=i.num_po, levels=i.levels, edges=i.edges,
+                    structural_paths=i.structural_paths)
+
+    def arrays(self) -> dict:
+        """CSR arrays as numpy: type, po, level, fin_off, fin, fout_off, fout, level_off."""
+        inf = self.info()
+        n, e, L = inf["n"], inf["edges"], inf["levels"]
+        a = dict(type=np.zeros(n, np.uint8), po=np.zeros(n, np.uint8), level=np.zeros(n, np.uint32),
+                 fin_off=np.zeros(n + 1, np.uint32), fin=np.zeros(max(e, 1), np.uint32),
+                 fout_off=np.zeros(n + 1, np.uint32), fout=np.zeros(max(e, 1), np.uint32),
+                 level_off=np.zeros(L + 2, np.uint32))
+        _check(lib().pdf_netlist_export(self._h, *[_ptr(a[k]) for k in
+                                                  ("type", "po", "level", "fin_off", "fin", "fout_off", "fout",
+                                                   "level_off")]))
+        a["fin"] = a["fin"][:e]
+        a["fout"] = a["fout"][:e]
+        a["num_pi"] = inf["num_pi"]
+        a["n"] = n
+        a["L"] = L
+        return a
+
+
+def gen_vectors(seed: int, num_pi: int, num_vectors: int) -> np.ndarray:
+    """iid Bernoulli(1/2) vectors, PI-major packed: shape (num_pi, ceil(V/64)) uint64."""
+    words = (num_vectors + 63) // 64
+    out = np.zeros((num_pi, words), dtype=np.uint64)
+    _check(lib().pdf_gen_vectors(seed, num_pi, num_vectors, _ptr(out)))
+    return out
+
+
+class Grader:
+    """One netlist resident in HBM plus the chunk bit-planes of one vector set."""
+
+    def __init__(self, netlist: Netlist, device: int = 0):
+        self._h = None
+        h = C.c_void_p()
+        _check(lib().pdf_grader_create(device, netlist.handle, C.byref(h)))
+        self._h = h
+        self.n = netlist.info()["n"]
+
+    def close(self):
+        if self._h:
+            lib().pdf_grader_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- inputs
+    def set_vectors(self, bits: np.ndarray, num_vectors: int):
+        bits = np.ascontiguousarray(bits, dtype=np.uint64)
+        _check(lib().pdf_grader_set_vectors(self._h, _ptr(bits), num_vectors))
+        self._num_pairs = num_vectors
+
+    def set_pairs(self, a_bits: np.ndarray, b_bits: np.ndarray, num_pairs: int):
+        a_bits = np.ascontiguousarray(a_bits, dtype=np.uint64)
+        b_bits = np.ascontiguousarray(b_bits, dtype=np.uint64)
+        _check(lib().pdf_grader_set_pairs(self._h, _ptr(a_bits), _ptr(b_bits), num_pairs))
+        self._num_pairs = num_pairs
+
+    def set_chunk(self, chunk_pairs: int):
+        _check(lib().pdf_grader_set_chunk(self._h, chunk_pairs))
+
+    def num_chunks(self) -> int:
+        v = C.c_uint64()
+        _check(lib().pdf_grader_num_chunks(self._h, C.byref(v)))
+        return v.value
+
+    def chunk_width(self, chunk: int) -> int:
+        v = C.c_uint64()
+        _check(lib().pdf_grader_chunk_width(self._h, chunk, C.byref(v)))
+        return v.value
+
+    def reset(self):
+        _check(lib().pdf_grader_reset(self._h))
+
+    # -- the four stages (reference host wrappers)
+    def sim(self, chunk: int, start: int) -> float:
+        ms = C.c_float()
+        _check(lib().pdf_sim(self._h, chunk, start, C.byref(ms)))
+        return ms.value
+
+    def mark(self, chunk: int, start: int, robust: bool) -> float:
+        ms = C.c_float()
+        _check(lib().pdf_mark(self._h, chunk, start, int(robust), C.byref(ms)))
+        return ms.value
+
+    def merge(self, chunk: int, start: int, segs: int = 1) -> float:
+        ms = C.c_float()
+        _check(lib().pdf_merge(self._h, chunk, start, segs, C.byref(ms)))
+        return ms.value
+
+    def cover(self, chunk: int, start: int, segs: int = 1):
+        ms = C.c_float()
+        cov = C.c_uint64()
+        _check(lib().pdf_cover(self._h, chunk, start, segs, C.byref(cov), C.byref(ms)))
+        return cov.value, ms.value
+
+    def grade(self, robust: bool) -> int:
+        tot = C.c_uint64()
+        _check(lib().pdf_grade(self._h, int(robust), C.byref(tot)))
+        return tot.value
+
+    def grade_timed(self, robust: bool):
+        """pdf_grade with every launch timed by HIP events: (total, times dict)."""
+        tot = C.c_uint64()
+        t = KernelTimes()
+        _check(lib().pdf_grade_timed(self._h, int(robust), C.byref(tot), C.byref(t)))
+        return tot.value, {k: getattr(t, k) for k, _ in KernelTimes._fields_}
+
+    def grade_stages(self, robust: bool, segs: int = 1):
+        """The reference's chunk loop (main.cu:171-231) through the four stage
+        calls; returns (total, per-chunk coverage list)."""
+        self.reset()
+        totals, per_chunk = 0, []
+        start = 0
+        for chunk in range(self.num_chunks()):
+            self.sim(chunk, start)
+            self.mark(chunk, start, robust)
+            self.merge(chunk, start, segs)
+            cov, _ = self.cover(chunk, start, segs)
+            per_chunk.append(cov)
+            totals += cov
+            start += self.chunk_width(chunk)
+        return totals, per_chunk
+
+    # -- parity getters
+    def sim_planes(self, width: int):
+        words = width // 64
+        a = np.zeros((self.n, words), np.uint64)
+        b = np.zeros((self.n, words), np.uint64)
+        _check(lib().pdf_get_sim_planes(self._h, _ptr(a), _ptr(b)))
+        return a, b
+
+    def mark_plane(self, width: int):
+        m = np.zeros((self.n, width // 64), np.uint64)
+        _check(lib().pdf_get_mark_plane(self._h, _ptr(m)))
+        return m
+
+    def first(self):
+        f0 = np.zeros(self.n, np.uint32)
+        f1 = np.zeros(self.n, np.uint32)
+        _check(lib().pdf_get_first(self._h, _ptr(f0), _ptr(f1)))
+        return f0, f1
+
+    def gate_counts(self) -> np.ndarray:
+        c = np.zeros(self.n, np.uint64)
+        _check(lib().pdf_get_gate_counts(self._h, _ptr(c)))
+        return c
+
+    def total(self) -> int:
+        t = C.c_uint64()
+        _check(lib().pdf_get_total(self._h, C.byref(t)))
+        return t.value
+
+
+def planes_to_codes(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Unpack bit-planes (n, words) into byte codes (n, words*64)."""
+    n, words = a.shape
+    au = np.unpackbits(a.view(np.uint8).reshape(n, words * 8), axis=1, bitorder="little")
+    bu = np.unpackbits(b.view(np.uint8).reshape(n, words * 8), axis=1, bitorder="little")
+    return (bu | ((au ^ bu) << 1)).astype(np.uint8)
+
+
+def plane_to_bytes(m: np.ndarray) -> np.ndarray:
+    n, words = m.shape
+    return np.unpackbits(m.view(np.uint8).reshape(n, words * 8), axis=1, bitorder="little").astype(np.uint8)

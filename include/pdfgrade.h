@@ -1,0 +1,222 @@
+/*
+ * pdfgrade.h -- C ABI of the MI355X path-delay fault grader.
+ *
+ * The grading path is sim -> mark -> merge -> cover, called once per chunk of
+ * vector pairs, exactly as the reference's chunk loop does (main.cu:171-231).
+ * Every entry point returns PDF_OK (0) or a negative status; the text of the
+ * last failure on the calling thread is available from pdf_last_error().
+ * No entry point calls exit() or abort() (the reference does, through its
+ * HANDLE_ERROR macros, e.g. simkernel.cuh:32-39).
+ *
+ * Data model
+ *   - Netlist: levelized, node ids sorted by level; INPT nodes are exactly ids
+ *     0..num_pi-1 and form level 0.  Fanins are ordered (fanin 0 is the on-path
+ *     input for marking).  This replaces the util/ GPU_Circuit + GPUNODE
+ *     contract used at simkernel.cuh:108,129; markkernel.cuh:220-223,277;
+ *     coverkernel.cuh:184,204-205.
+ *   - Vectors: bit-packed, PI-major.  Row i holds ceil(V/64) uint64 words; bit
+ *     j of word k is the value of PI i in vector 64k+j.  Sequence mode grades
+ *     pair p = (V[p], V[(p+1) mod V]) as simkernel.cuh:115-124 does; explicit
+ *     mode grades caller-given (A_p, B_p).
+ *   - Node codes: S0=0, S1=1, T0=2 (1->0), T1=3 (0->1).
+ *
+ * Threading: a grader handle belongs to one host thread at a time.  All work
+ * is issued on the handle's own HIP stream.
+ */
+#ifndef PDFGRADE_H
+#define PDFGRADE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDF_ABI_VERSION 1
+
+/* status codes */
+#define PDF_OK 0
+#define PDF_ERR_ARG (-1)         /* invalid argument / shape */
+#define PDF_ERR_NETLIST (-2)     /* malformed netlist */
+#define PDF_ERR_IO (-3)          /* file could not be read / parsed */
+#define PDF_ERR_HIP (-4)         /* HIP runtime error */
+#define PDF_ERR_NOMEM (-5)       /* allocation failed */
+#define PDF_ERR_UNSUPPORTED (-6) /* feature not available (e.g. segs > 1) */
+#define PDF_ERR_STATE (-7)       /* call out of order */
+#define PDF_ERR_OVERFLOW (-8)    /* exact uint64 accumulation overflowed */
+
+/* node types (numbering is this library's; the reference's util/defines.h
+ * values are not available -- only AND..XNOR = 2..7 is implied by
+ * markkernel.cuh:49 indexing LUT_R[type-2]) */
+#define PDF_UNKN 0
+#define PDF_INPT 1
+#define PDF_AND 2
+#define PDF_NAND 3
+#define PDF_OR 4
+#define PDF_NOR 5
+#define PDF_XOR 6
+#define PDF_XNOR 7
+#define PDF_BUFF 8
+#define PDF_NOT 9
+#define PDF_FROM 10
+#define PDF_DFF 11
+
+/* node codes */
+#define PDF_S0 0
+#define PDF_S1 1
+#define PDF_T0 2
+#define PDF_T1 3
+
+/* no first-detection pattern yet (the reference's int2 (-1,-1),
+ * mergekernel.cuh:215) */
+#define PDF_NONE 0xFFFFFFFFu
+
+const char *pdf_last_error(void);
+int pdf_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Netlists (host objects; no GPU needed)                                    */
+/* ------------------------------------------------------------------------ */
+typedef struct pdf_netlist pdf_netlist;
+
+/* Caller-supplied CSR view of a levelized netlist.  Validation rejects: ids
+ * not sorted by level, level(g) != 1 + max level(fanin) for gates, INPT not
+ * exactly 0..num_pi-1, gates without fanins, duplicate fanins, fanin or
+ * fanout lists that disagree, more than 65535 fanins or fanouts on one node. */
+typedef struct pdf_netlist_desc {
+  uint32_t n;               /* nodes */
+  uint32_t num_pi;          /* INPT nodes = ids 0..num_pi-1 */
+  const uint8_t *type;      /* [n] PDF_* type */
+  const uint8_t *po;        /* [n] primary-output flag (0/1) */
+  const uint32_t *fin_off;  /* [n+1] */
+  const uint32_t *fin;      /* [fin_off[n]] ordered fanins */
+} pdf_netlist_desc;
+
+typedef struct pdf_netlist_info {
+  uint32_t n, num_pi, num_po, levels; /* levels = max level index L */
+  uint64_t edges;
+  uint64_t structural_paths;          /* PI->PO paths, saturated at 2^64-1 */
+} pdf_netlist_info;
+
+/* Build from arrays: fanouts and levels are derived (ordered by fanout id). */
+int pdf_netlist_create(const pdf_netlist_desc *desc, pdf_netlist **out);
+/* ISCAS .bench text (INPUT/OUTPUT/assignments), levelized and renumbered so
+ * inputs keep declaration order and gates are sorted by (level, file order).
+ * Replaces GPU_Circuit::read_bench (main.cu:129-130; util, absent). */
+int pdf_netlist_read_bench(const char *path, pdf_netlist **out);
+/* Presorted ".level" text written by pdf_netlist_write_level (format defined
+ * by this library; the reference's GPU_Circuit::load format is not
+ * available, main.cu:131-133). */
+int pdf_netlist_read_level(const char *path, pdf_netlist **out);
+int pdf_netlist_write_level(const pdf_netlist *nl, const char *path);
+void pdf_netlist_free(pdf_netlist *nl);
+int pdf_netlist_get_info(const pdf_netlist *nl, pdf_netlist_info *info);
+/* Copy out CSR arrays; any pointer may be NULL.  Sizes: type/po/level [n],
+ * fin_off/fout_off [n+1], fin/fout [edges], level_off [levels+2]. */
+int pdf_netlist_export(const pdf_netlist *nl, uint8_t *type, uint8_t *po,
+                       uint32_t *level, uint32_t *fin_off, uint32_t *fin,
+                       uint32_t *fout_off, uint32_t *fout,
+                       uint32_t *level_off);
+
+/* Seeded synthetic netlists (SplitMix64-seeded xoshiro256**).  The
+ * reference ships no generator; this one is defined by this library. */
+typedef struct pdf_gen_params {
+  uint64_t seed;
+  uint32_t num_pi, num_po, num_gates, levels;
+  double fanin_p[4];        /* P(fan-in = 1..4); fan-in 1 -> NOT/BUFF */
+  double prev_level_frac;   /* share of fanins drawn from level-1 */
+  double fanout_alpha;      /* power-law exponent of fanout weights; 0 = uniform */
+  uint32_t fanout_cap;      /* max fanout of any node (0 = none) */
+  uint32_t max_attempts;    /* regenerations allowed when paths >= 2^63 */
+} pdf_gen_params;
+
+int pdf_netlist_generate(const pdf_gen_params *params, pdf_netlist **out);
+
+/* iid Bernoulli(1/2) vectors, PI-major packed as described above: writes
+ * num_pi * ceil(num_vectors/64) words; bits past num_vectors are zero. */
+int pdf_gen_vectors(uint64_t seed, uint32_t num_pi, uint64_t num_vectors,
+                    uint64_t *out_bits);
+
+/* ------------------------------------------------------------------------ */
+/* Grader (GPU)                                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct pdf_grader pdf_grader;
+
+/* Create a grader on HIP device `device` and copy the netlist to HBM
+ * (GPU_Circuit::copy, main.cu:137).  The grader keeps its own copy. */
+int pdf_grader_create(int device, const pdf_netlist *nl, pdf_grader **out);
+void pdf_grader_destroy(pdf_grader *g);
+
+/* Sequence-mode vectors from host memory (read_vectors, main.cu:146-156).
+ * num_vectors must be a multiple of 64. */
+int pdf_grader_set_vectors(pdf_grader *g, const uint64_t *bits,
+                           uint64_t num_vectors);
+/* Explicit pairs (A_p, B_p), both PI-major packed; num_pairs % 64 == 0. */
+int pdf_grader_set_pairs(pdf_grader *g, const uint64_t *a_bits,
+                         const uint64_t *b_bits, uint64_t num_pairs);
+/* Pairs per chunk (the reference's simul_patterns, main.cu:150-153); a
+ * multiple of 64.  0 = one chunk for all pairs.  Allocates the chunk's
+ * bit-planes.  Also resets the cross-chunk state. */
+int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs);
+/* Number of chunks for the loaded pairs and the width (pairs) of one. */
+int pdf_grader_num_chunks(const pdf_grader *g, uint64_t *num_chunks);
+int pdf_grader_chunk_width(const pdf_grader *g, uint64_t chunk,
+                           uint64_t *pairs);
+/* Forget first-detection state and per-gate counts (new vector set). */
+int pdf_grader_reset(pdf_grader *g);
+
+/* The four stages.  `chunk` selects the pairs [start, start+width);
+ * start_pattern must equal chunk * chunk_pairs (the reference passes both,
+ * main.cu:175,182,187,208).  `ms` (may be NULL) receives wall milliseconds
+ * measured with HIP events on the grader's stream.  Each call returns after
+ * its work has completed. */
+/* gpuRunSimulation (simkernel.cuh:16,200-249) */
+int pdf_sim(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, float *ms);
+/* gpuMarkPaths (markkernel.cuh:31,295-355); robust replaces the global
+ * robust_flag (main.cu:7; markkernel.cuh:313) */
+int pdf_mark(pdf_grader *g, uint64_t chunk, uint64_t start_pattern,
+             int robust, float *ms);
+/* gpuMergeSegments<K> (mergekernel.cuh:200-267); only segs == 1 */
+int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
+              float *ms);
+/* gpuCountPaths<K> (coverkernel.cuh:233-333): *coverage receives (overwrites)
+ * this chunk's exact total; per-gate counts accumulate inside the grader. */
+int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
+              uint64_t *coverage, float *ms);
+
+/* Whole chunk loop of main.cu:171-231 (sim, mark, merge, cover per chunk,
+ * totals accumulated) using the fused kernels; *total receives the exact
+ * total.  Equivalent to calling the four stages per chunk. */
+int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
+
+/* Diagnostic variant of pdf_grade (no reference counterpart): the same
+ * launches, each bracketed by HIP events on the grader's stream, so the
+ * summed device time of every kernel family is reported.  Slower than
+ * pdf_grade because of the event records; meant for roofline reporting. */
+typedef struct pdf_kernel_times {
+  double fwd_ms, reverse_ms, events_ms, cover_ms, fill_ms; /* summed durations */
+  uint32_t fwd_launches, reverse_launches, events_launches, cover_launches, fill_launches;
+  double wall_ms; /* first to last event */
+} pdf_kernel_times;
+int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times);
+
+/* Parity getters (the reference's --verbose dumps, simkernel.cuh:250-295;
+ * markkernel.cuh:357-395; mergekernel.cuh:25-47; coverkernel.cuh:44-66).
+ * Planes are node-major: [n][words] uint64, words = chunk width / 64; bit j
+ * of word k is pair start+64k+j of the chunk last simulated / marked. */
+int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane,
+                       uint64_t *b_plane);
+int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane);
+/* first_T0 / first_T1: [n], PDF_NONE when never detected. */
+int pdf_get_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1);
+/* Per-gate exact counts sum_p c(g,p) over all chunks graded so far: [n]. */
+int pdf_get_gate_counts(const pdf_grader *g, uint64_t *counts);
+/* Exact total over all chunks graded so far. */
+int pdf_get_total(const pdf_grader *g, uint64_t *total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDFGRADE_H */

@@ -1,0 +1,228 @@
+"""Bit-exact parity of the HIP grading path (through the C ABI) against the CPU
+oracle, on the same seeded inputs.  Integer work: every output must be equal."""
+import sys
+
+import numpy as np
+import pytest
+
+import pdfgrade as pg
+from pdfgrade.configs import GATES_50K, TINY
+from oracle import oracle as orc
+
+from helpers import biased_vectors, edge_netlist, oracle_grade, pack, small_config, unpack
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_grade(nl, bits, P, chunk, robust, b_bits=None, stages=False):
+    with pg.Grader(nl) as g:
+        if b_bits is None:
+            g.set_vectors(bits, P)
+        else:
+            g.set_pairs(bits, b_bits, P)
+        g.set_chunk(chunk)
+        if stages:
+            total, per_chunk = g.grade_stages(robust)
+        else:
+            total, per_chunk = g.grade(robust), None
+        f0, f1 = g.first()
+        return dict(total=total, per_chunk=per_chunk, first_t0=f0, first_t1=f1, gate_counts=g.gate_counts())
+
+
+def assert_same(got, ref, ctx=""):
+    assert got["total"] == ref["total"], ctx
+    assert np.array_equal(got["first_t0"], ref["first_t0"]), ctx
+    assert np.array_equal(got["first_t1"], ref["first_t1"]), ctx
+    assert np.array_equal(got["gate_counts"], ref["gate_counts"]), ctx
+    if got.get("per_chunk") is not None:
+        assert np.array_equal(np.array(got["per_chunk"], np.uint64), ref["chunk_totals"]), ctx
+
+
+@pytest.mark.parametrize("robust", [True, False])
+@pytest.mark.parametrize("chunk", [64, 256, 1024])
+def test_tiny_workload(robust, chunk):
+    say("tiny", robust, chunk)
+    nl = pg.Netlist.generate(TINY.netlist)
+    bits = pg.gen_vectors(TINY.vector_seed, nl.info()["num_pi"], TINY.pairs)
+    ref = oracle_grade(nl, bits, TINY.pairs, chunk, robust)
+    assert_same(gpu_grade(nl, bits, TINY.pairs, chunk, robust), ref, "fused")
+    assert_same(gpu_grade(nl, bits, TINY.pairs, chunk, robust, stages=True), ref, "stages")
+
+
+def check_intermediates(nl, bits, P, chunk, robust, b_bits=None):
+    """Sim codes and final marks of every chunk against the oracle."""
+    net = orc.OracleNet(nl.arrays())
+    with pg.Grader(nl) as g:
+        if b_bits is None:
+            g.set_vectors(bits, P)
+        else:
+            g.set_pairs(bits, b_bits, P)
+        g.set_chunk(chunk)
+        f0 = np.full(net.n, 0xFFFFFFFF, np.uint32)
+        f1 = f0.copy()
+        gc = np.zeros(net.n, np.uint64)
+        start = 0
+        for c in range(g.num_chunks()):
+            w = g.chunk_width(c)
+            g.sim(c, start)
+            a, b = g.sim_planes(w)
+            codes = orc.sim_chunk(net, bits, P, start, w, b_bits=b_bits)
+            assert np.array_equal(pg.planes_to_codes(a, b), codes), f"sim chunk {c}"
+            g.mark(c, start, robust)
+            marks, _ = orc.mark_chunk(net, codes, robust)
+            assert np.array_equal(pg.plane_to_bytes(g.mark_plane(w)), marks), f"mark chunk {c}"
+            g.merge(c, start)
+            orc.merge_chunk(net, codes, marks, start, f0, f1)
+            gf0, gf1 = g.first()
+            assert np.array_equal(gf0, f0) and np.array_equal(gf1, f1), f"merge chunk {c}"
+            cov, _ = g.cover(c, start)
+            ref_cov, _ = orc.cover_chunk(net, marks, start, f0, f1, gc)
+            assert cov == ref_cov, f"cover chunk {c}"
+            assert np.array_equal(g.gate_counts(), gc), f"gate counts chunk {c}"
+            start += w
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_tiny_intermediates(robust):
+    say("tiny intermediates", robust)
+    nl = pg.Netlist.generate(TINY.netlist)
+    bits = pg.gen_vectors(TINY.vector_seed, nl.info()["num_pi"], TINY.pairs)
+    check_intermediates(nl, bits, TINY.pairs, 256, robust)
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_small_circuits(seed):
+    say("random", seed)
+    cfg = small_config(seed, num_pi=6 + seed % 5, num_gates=30 + 7 * seed, levels=4 + seed % 6,
+                       wide=seed % 2 == 0, alpha=2.2 if seed % 3 == 0 else 0.0)
+    nl = pg.Netlist.generate(cfg)
+    P = 64 * (2 + seed % 4)
+    bits = biased_vectors(seed, nl.info()["num_pi"], P, p_hold=0.3 + 0.1 * (seed % 5))
+    for robust in (True, False):
+        chunk = 64 * (1 + seed % 3)
+        ref = oracle_grade(nl, bits, P, chunk, robust)
+        assert_same(gpu_grade(nl, bits, P, chunk, robust), ref, f"seed {seed} robust {robust}")
+    check_intermediates(nl, bits, P, 64, True)
+
+
+def test_explicit_pairs():
+    say("explicit pairs")
+    nl = pg.Netlist.generate(small_config(21, num_gates=60, levels=7))
+    P = 256
+    I = nl.info()["num_pi"]
+    a = biased_vectors(1, I, P, p_hold=0.0)
+    b = pack(np.where(np.random.default_rng(2).random((P, I)) < 0.3, 1 - unpack(a, P), unpack(a, P)))
+    for robust in (True, False):
+        ref = oracle_grade(nl, a, P, 128, robust, b_bits=b)
+        assert_same(gpu_grade(nl, a, P, 128, robust, b_bits=b), ref, "explicit")
+    check_intermediates(nl, a, P, 128, False, b_bits=b)
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_edge_netlist(robust):
+    say("edge netlist", robust)
+    nl = edge_netlist()
+    P = 512
+    bits = biased_vectors(9, 8, P, p_hold=0.5)
+    ref = oracle_grade(nl, bits, P, 128, robust)
+    assert_same(gpu_grade(nl, bits, P, 128, robust), ref, "edge")
+    check_intermediates(nl, bits, P, 192, robust)
+
+
+def test_constant_vectors_give_zero():
+    say("constant")
+    nl = pg.Netlist.generate(TINY.netlist)
+    bits = np.zeros((nl.info()["num_pi"], 16), np.uint64)
+    r = gpu_grade(nl, bits, 1024, 512, True)
+    assert r["total"] == 0 and not r["gate_counts"].any()
+    assert np.all(r["first_t0"] == pg.NONE) and np.all(r["first_t1"] == pg.NONE)
+
+
+def test_50k_gates_sample_against_oracle():
+    """The 50k-gate netlist on its first 4,096 pairs (as a 4,096-vector set)."""
+    say("50k sample")
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = 4096
+    bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
+    ref = oracle_grade(nl, bits, P, 1024, True)
+    assert_same(gpu_grade(nl, bits, P, 1024, True), ref, "50k fused")
+    assert_same(gpu_grade(nl, bits, P, 4096, True, stages=True), ref, "50k stages")
+
+
+def test_50k_gates_full_size_properties():
+    """Full 65,536 pairs: chunk invariance, stage path == fused path, and the
+    planes of an interior chunk equal the oracle's for that chunk."""
+    say("50k full")
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = GATES_50K.pairs
+    bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
+    full = gpu_grade(nl, bits, P, P, True)
+    for chunk in (8192, 4096 * 3):
+        r = gpu_grade(nl, bits, P, chunk, True)
+        r["per_chunk"] = None
+        assert_same(r, full, f"chunk {chunk}")
+    st = gpu_grade(nl, bits, P, 16384, True, stages=True)
+    assert sum(st["per_chunk"]) == full["total"]
+    assert_same({**st, "per_chunk": None}, full, "stages")
+    # every first-detection pattern is a valid pair index, and counts exist only
+    # on nodes that were detected or sit above a detected node
+    det = (full["first_t0"] != pg.NONE) | (full["first_t1"] != pg.NONE)
+    assert np.all(full["first_t0"][det & (full["first_t0"] != pg.NONE)] < P)
+    assert full["total"] <= int(full["gate_counts"].sum())
+    # interior chunk planes vs the oracle (planes carry no cross-chunk state)
+    net = orc.OracleNet(nl.arrays())
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(2048)
+        c = 17
+        start = c * 2048
+        g.sim(c, start)
+        a, b = g.sim_planes(2048)
+        codes = orc.sim_chunk(net, bits, P, start, 2048)
+        assert np.array_equal(pg.planes_to_codes(a, b), codes)
+        g.mark(c, start, True)
+        marks, _ = orc.mark_chunk(net, codes, True)
+        assert np.array_equal(pg.plane_to_bytes(g.mark_plane(2048)), marks)
+    # the last chunk wraps to vector 0 (simkernel.cuh:116-120)
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(1024)
+        c = 63
+        g.sim(c, c * 1024)
+        a, b = g.sim_planes(1024)
+        codes = orc.sim_chunk(net, bits, P, c * 1024, 1024)
+        assert np.array_equal(pg.planes_to_codes(a, b), codes)
+
+
+def test_api_errors():
+    say("api errors")
+    nl = pg.Netlist.generate(TINY.netlist)
+    bits = pg.gen_vectors(1, nl.info()["num_pi"], 1024)
+    with pg.Grader(nl) as g:
+        with pytest.raises(pg.PdfError) as e:
+            g.set_vectors(bits, 1000)
+        assert e.value.code == pg.ERR_ARG
+        g.set_vectors(bits, 1024)
+        with pytest.raises(pg.PdfError) as e:
+            g.set_chunk(100)
+        assert e.value.code == pg.ERR_ARG
+        g.set_chunk(256)
+        with pytest.raises(pg.PdfError) as e:
+            g.mark(0, 0, True)
+        assert e.value.code == pg.ERR_STATE
+        g.sim(0, 0)
+        with pytest.raises(pg.PdfError) as e:
+            g.sim(1, 0)  # start must be chunk * chunk_pairs
+        assert e.value.code == pg.ERR_ARG
+        g.mark(0, 0, True)
+        with pytest.raises(pg.PdfError) as e:
+            g.merge(0, 0, segs=2)
+        assert e.value.code == pg.ERR_UNSUPPORTED
+        with pytest.raises(pg.PdfError) as e:
+            g.sim(4, 1024)
+        assert e.value.code == pg.ERR_ARG

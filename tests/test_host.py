@@ -1,0 +1,165 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+symbol of include/pdfgrade.h, and the netlist/vector host code behaves."""
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import pdfgrade as pg
+from pdfgrade.configs import GATES_50K, TINY
+
+from helpers import C17_BENCH
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "pdfgrade.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pdf_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) >= 25
+    lib = pg.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the ctypes signatures cover exactly the declared functions
+    assert sorted(pg.SIGNATURES) == names
+    assert lib.pdf_abi_version() == 1
+
+
+def _arr(nl):
+    return nl.arrays()
+
+
+def test_generator_is_deterministic_and_seeded():
+    a = _arr(pg.Netlist.generate(TINY.netlist))
+    b = _arr(pg.Netlist.generate(TINY.netlist))
+    for k in ("type", "po", "fin_off", "fin"):
+        assert np.array_equal(a[k], b[k])
+    c = _arr(pg.Netlist.generate(pg.GenConfig(**{**TINY.netlist.__dict__, "seed": 99})))
+    assert not np.array_equal(a["fin"][:50], c["fin"][:50])
+
+
+def check_structure(arrs, cfg):
+    n, I = arrs["n"], arrs["num_pi"]
+    assert n == cfg.num_pi + cfg.num_gates and I == cfg.num_pi
+    assert arrs["L"] == cfg.levels
+    assert int(arrs["po"].sum()) == cfg.num_po
+    lv = arrs["level"]
+    assert np.all(np.diff(lv.astype(np.int64)) >= 0)
+    assert np.all(arrs["type"][:I] == pg.INPT) and np.all(arrs["type"][I:] != pg.INPT)
+    fo, fi = arrs["fin_off"], arrs["fin"]
+    nfi = np.diff(fo)
+    assert nfi.max() <= 4 and nfi[I:].min() >= 1
+    for g in range(I, n):
+        ins = fi[fo[g]:fo[g + 1]]
+        assert len(set(ins.tolist())) == len(ins)
+        assert lv[ins].max() == lv[g] - 1  # at least one fanin on the previous level
+        if nfi[g] == 1:
+            assert arrs["type"][g] in (pg.NOT, pg.BUFF)
+        else:
+            assert arrs["type"][g] in (pg.AND, pg.NAND, pg.OR, pg.NOR, pg.XOR, pg.XNOR)
+    fanout = np.diff(arrs["fout_off"])
+    if cfg.fanout_cap:
+        assert fanout.max() <= cfg.fanout_cap
+    return nfi[I:], fanout
+
+
+def test_generator_tiny_structure():
+    arrs = _arr(pg.Netlist.generate(TINY.netlist))
+    nfi, _ = check_structure(arrs, TINY.netlist)
+    assert 0.05 < np.mean(nfi == 1) < 0.2
+    lv = arrs["level"]
+    fo, fi = arrs["fin_off"], arrs["fin"]
+    prev = sum(int(lv[f] == lv[g] - 1) for g in range(arrs["num_pi"], arrs["n"]) for f in fi[fo[g]:fo[g + 1]])
+    assert 0.6 < prev / len(fi) < 0.9
+
+
+def test_generator_50k_structure_and_path_bound():
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    inf = nl.info()
+    assert inf["structural_paths"] < 2 ** 63
+    arrs = nl.arrays()
+    nfi, fanout = check_structure(arrs, GATES_50K.netlist)
+    hist = np.bincount(nfi, minlength=5)[1:5] / len(nfi)
+    assert np.allclose(hist, (0.1, 0.6, 0.2, 0.1), atol=0.02)
+    # heavy-tailed fanout: some nodes far above the mean
+    assert fanout.max() >= 20 * max(1.0, fanout.mean()) / 4
+
+
+def test_vectors_packed_and_deterministic():
+    v = pg.gen_vectors(5, 7, 200)
+    assert v.shape == (7, 4)
+    assert np.array_equal(v, pg.gen_vectors(5, 7, 200))
+    assert np.all(v[:, 3] >> np.uint64(8) == 0)  # bits past vector 199 are zero
+    ones = sum(bin(int(x)).count("1") for x in v[:, :3].ravel())
+    assert 0.4 < ones / (7 * 192) < 0.6
+
+
+def write_tmp(text, suffix):
+    f = tempfile.NamedTemporaryFile("w", suffix=suffix, delete=False)
+    f.write(text)
+    f.close()
+    return f.name
+
+
+def test_bench_reader_c17():
+    path = write_tmp(C17_BENCH, ".bench")
+    try:
+        nl = pg.Netlist.read_bench(path)
+    finally:
+        os.unlink(path)
+    a = nl.arrays()
+    assert a["n"] == 11 and a["num_pi"] == 5 and a["L"] == 3
+    assert list(a["type"][5:]) == [pg.NAND] * 6
+    assert list(a["po"]) == [0] * 9 + [1, 1]
+    fo, fi = a["fin_off"], a["fin"]
+    assert [list(fi[fo[g]:fo[g + 1]]) for g in range(5, 11)] == [[0, 2], [2, 3], [1, 6], [6, 4], [5, 7], [7, 8]]
+    assert nl.info()["structural_paths"] == 11
+
+
+@pytest.mark.parametrize("text,code", [
+    ("INPUT(a)\nOUTPUT(c)\nb = AND(a, c)\nc = NOT(b)\n", pg.ERR_NETLIST),   # cycle
+    ("INPUT(a)\nOUTPUT(b)\nb = AND(a, z)\n", pg.ERR_NETLIST),              # undriven
+    ("INPUT(a)\nOUTPUT(b)\nb = FOO(a)\n", pg.ERR_IO),                      # unknown gate
+    ("INPUT(a)\nOUTPUT(q)\nb = NOT(a)\n", pg.ERR_NETLIST),                 # undefined output
+])
+def test_bench_reader_errors(text, code):
+    path = write_tmp(text, ".bench")
+    try:
+        with pytest.raises(pg.PdfError) as e:
+            pg.Netlist.read_bench(path)
+        assert e.value.code == code
+    finally:
+        os.unlink(path)
+
+
+def test_level_round_trip(tmp_path):
+    nl = pg.Netlist.generate(TINY.netlist)
+    p = str(tmp_path / "tiny.level")
+    nl.write_level(p)
+    back = pg.Netlist.read_level(p).arrays()
+    a = nl.arrays()
+    for k in ("type", "po", "fin_off", "fin", "fout_off", "fout", "level_off"):
+        assert np.array_equal(a[k], back[k])
+
+
+@pytest.mark.parametrize("types,po,fanins,why", [
+    ([pg.AND, pg.INPT], [0, 1], [[1], []], "INPT"),
+    ([pg.INPT, pg.AND, pg.AND], [0, 0, 1], [[], [2], [0]], "earlier"),
+    ([pg.INPT, pg.AND], [0, 1], [[], []], "no fanins"),
+    ([pg.INPT, pg.AND, pg.OR], [0, 1, 1], [[], [0], [0]], ""),  # valid
+])
+def test_netlist_validation(types, po, fanins, why):
+    if not why:
+        nl = pg.Netlist.from_arrays(1, types, po, fanins)
+        assert nl.info()["levels"] == 1
+        return
+    with pytest.raises(pg.PdfError) as e:
+        pg.Netlist.from_arrays(1, types, po, fanins)
+    assert e.value.code == pg.ERR_NETLIST and why in str(e.value)

@@ -1,0 +1,237 @@
+// fcount: command-line grader with the reference's flags and result line
+// (main.cu:32-237), driving the C ABI of libpdfgrade.
+//
+//   fcount [--robust] [--segs 1] [--num_patterns N] [--verbose [--sim]
+//          [--mark] [--merge] [--cover]] [--fused] --bench CIRCUIT VECTORS...
+//
+// CIRCUIT: a .bench file when the name contains "bench", otherwise a
+// presorted .level file (main.cu:129-134).  VECTORS: text files with one
+// vector per line as '0'/'1' characters (one per PI, in INPUT order);
+// whitespace is ignored and '#' starts a comment.  The reference's vector
+// reader lives in its absent util/ submodule, so this format is this tool's.
+//
+// For every vector file one line goes to stdout, as main.cu:233-234 prints it:
+//   file:V,n,K;gpu;sim;mark;merge;cover;total
+// with gpu = summed stage milliseconds and sim..cover = the last chunk's
+// (stage times are HIP-event wall time here, not process CPU time).
+
+#include <getopt.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+#include "pdfgrade.h"
+
+namespace {
+
+int verbose_flag = 0, sim_flag = 0, mark_flag = 0, merge_flag = 0, cover_flag = 0, robust_flag = 0,
+    fused_flag = 0;
+
+[[noreturn]] void die(const std::string &what) {
+  std::cerr << "fcount: " << what << "\n";
+  std::exit(EXIT_FAILURE);
+}
+
+void check(int rc, const char *what) {
+  if (rc != PDF_OK) die(std::string(what) + ": " + pdf_last_error());
+}
+
+// Reads one vector per line; returns PI-major packed bits.
+std::vector<uint64_t> read_vectors(const std::string &path, uint32_t num_pi, uint64_t *count) {
+  std::ifstream in(path);
+  if (!in) die("cannot open vector file " + path);
+  std::vector<std::string> rows;
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::string bits;
+    for (char c : line) {
+      if (c == '0' || c == '1') bits.push_back(c);
+      else if (!std::isspace((unsigned char)c)) die("bad character in " + path);
+    }
+    if (bits.empty()) continue;
+    if (bits.size() != num_pi)
+      die(path + ": vector of width " + std::to_string(bits.size()) + ", circuit has " + std::to_string(num_pi) +
+          " inputs");
+    rows.push_back(bits);
+  }
+  if (rows.empty()) die("no vectors in " + path);
+  const uint64_t V = rows.size(), words = (V + 63) / 64;
+  std::vector<uint64_t> packed(num_pi * words, 0);
+  for (uint64_t v = 0; v < V; ++v)
+    for (uint32_t i = 0; i < num_pi; ++i)
+      if (rows[v][i] == '1') packed[i * words + v / 64] |= 1ULL << (v % 64);
+  *count = V;
+  return packed;
+}
+
+const char *code_name(int c) {
+  static const char *n[] = {"S0", "S1", "T0", "T1"};
+  return n[c & 3];
+}
+
+// --verbose dumps in the layout of the reference's debug files
+// (simkernel.cuh:250-295 gpusim-p2.log, markkernel.cuh:357-395 gpumark.log,
+// mergekernel.cuh:25-47 gpumerge.log); cover writes per-gate totals.
+void dump_planes(pdf_grader *g, uint32_t n, uint64_t width, uint64_t start, bool first_chunk) {
+  const uint64_t words = (width + 63) / 64;
+  std::vector<uint64_t> a(n * words), b(n * words), m(n * words);
+  if (sim_flag) {
+    check(pdf_get_sim_planes(g, a.data(), b.data()), "pdf_get_sim_planes");
+    std::ofstream f("gpusim-p2.log", first_chunk ? std::ios::out : std::ios::app);
+    for (uint64_t p = 0; p < width; ++p) {
+      f << "Vector " << start + p << ":\t";
+      for (uint32_t i = 0; i < n; ++i) {
+        const int av = (a[i * words + p / 64] >> (p % 64)) & 1, bv = (b[i * words + p / 64] >> (p % 64)) & 1;
+        f << code_name(bv | ((av ^ bv) << 1)) << " ";
+      }
+      f << "\n";
+    }
+  }
+  if (mark_flag) {
+    check(pdf_get_mark_plane(g, m.data()), "pdf_get_mark_plane");
+    std::ofstream f("gpumark.log", first_chunk ? std::ios::out : std::ios::app);
+    for (uint64_t p = 0; p < width; ++p) {
+      f << "Vector " << start + p << ":\t";
+      for (uint32_t i = 0; i < n; ++i) f << (((m[i * words + p / 64] >> (p % 64)) & 1) ? "Y " : "N ");
+      f << "\n";
+    }
+  }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string infile;
+  std::vector<std::string> benchmarks;
+  long override_patterns = 0;
+  int segs = 1;
+  static struct option long_options[] = {{"verbose", no_argument, &verbose_flag, 1},
+                                         {"cover", no_argument, &cover_flag, 1},
+                                         {"mark", no_argument, &mark_flag, 1},
+                                         {"merge", no_argument, &merge_flag, 1},
+                                         {"sim", no_argument, &sim_flag, 1},
+                                         {"robust", no_argument, &robust_flag, 1},
+                                         {"brief", no_argument, &verbose_flag, 0},
+                                         {"fused", no_argument, &fused_flag, 1},
+                                         {"help", no_argument, 0, 'h'},
+                                         {"bench", required_argument, 0, 'b'},
+                                         {"num_patterns", required_argument, 0, 'o'},
+                                         {"segs", required_argument, 0, 's'},
+                                         {0, 0, 0, 0}};
+  while (true) {
+    int idx = 0;
+    const int c = getopt_long(argc, argv, "b:s:", long_options, &idx);
+    if (c == -1) break;
+    switch (c) {
+      case 0: break;
+      case 'b': infile = optarg; break;
+      case 'o': override_patterns = std::atol(optarg); break;
+      case 's':
+        segs = std::atoi(optarg);
+        if (segs > 8) die("Segment lengths over 8 are not supported!");
+        break;
+      case 'h':
+        std::printf("Usage: %s (options) /path/to/vectors1 /path/to/vectors2\n", argv[0]);
+        std::printf("\t--segs N : segment length (only 1 is implemented)\n");
+        std::printf("\t--bench /path/to/ckt : the circuit to grade (.bench or .level)\n");
+        std::printf("\t--robust : robust sensitization (default non-robust)\n");
+        std::printf("\t--num_patterns N : pairs per chunk = N-1, rounded down to a multiple of 64\n");
+        std::printf("\t--fused : whole chunk loop in one call (pdf_grade)\n");
+        return 0;
+      default: return EXIT_FAILURE;
+    }
+  }
+  for (int i = optind; i < argc; ++i) benchmarks.push_back(argv[i]);
+  if (infile.empty()) die("--bench argument is required.");
+  if (segs != 1) die("only segment length 1 is implemented");
+
+  pdf_netlist *nl = nullptr;
+  if (infile.find("bench") != std::string::npos) check(pdf_netlist_read_bench(infile.c_str(), &nl), "read_bench");
+  else check(pdf_netlist_read_level(infile.c_str(), &nl), "read_level");
+  pdf_netlist_info info;
+  check(pdf_netlist_get_info(nl, &info), "get_info");
+  std::clog << "Circuit size is: " << info.n << " Levels: " << info.levels << std::endl;
+  pdf_grader *g = nullptr;
+  check(pdf_grader_create(0, nl, &g), "pdf_grader_create");
+
+  for (const std::string &vf : benchmarks) {
+    uint64_t V = 0;
+    std::vector<uint64_t> bits = read_vectors(vf, info.num_pi, &V);
+    check(pdf_grader_set_vectors(g, bits.data(), V), "set_vectors");
+    // pairs per chunk: --num_patterns N gives N-1 (main.cu:151-153), rounded
+    // to whole 64-pair words; otherwise as many as fit in half of free HBM
+    uint64_t chunk = 0;
+    if (override_patterns > 0) {
+      chunk = (uint64_t)(override_patterns - 1) / 64 * 64;
+      if (chunk == 0) chunk = 64;
+    } else {
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const double per_pair = info.n * (3.0 / 8.0 + 4.0 / 64.0) + 16.0;
+        chunk = (uint64_t)(free_b * 0.5 / per_pair) / 64 * 64;
+      }
+      if (chunk == 0 || chunk >= V) chunk = 0;
+    }
+    check(pdf_grader_set_chunk(g, chunk), "set_chunk");
+    uint64_t nchunks = 0;
+    check(pdf_grader_num_chunks(g, &nchunks), "num_chunks");
+    std::clog << "Maximum patterns per pass: " << (chunk ? chunk : V) << " / " << V << std::endl;
+    float sim = 0, mark = 0, merge = 0, cover = 0, gpu = 0;
+    uint64_t totals = 0;
+    if (fused_flag) {
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      (void)hipEventRecord(e0, 0);
+      check(pdf_grade(g, robust_flag, &totals), "pdf_grade");
+      (void)hipEventRecord(e1, 0);
+      (void)hipEventSynchronize(e1);
+      (void)hipEventElapsedTime(&gpu, e0, e1);
+    } else {
+      uint64_t start = 0;
+      for (uint64_t c = 0; c < nchunks; ++c) {
+        uint64_t width = 0, coverage = 0;
+        check(pdf_grader_chunk_width(g, c, &width), "chunk_width");
+        check(pdf_sim(g, c, start, &sim), "pdf_sim");
+        check(pdf_mark(g, c, start, robust_flag, &mark), "pdf_mark");
+        if (verbose_flag) dump_planes(g, info.n, width, start, c == 0);
+        check(pdf_merge(g, c, start, segs, &merge), "pdf_merge");
+        check(pdf_cover(g, c, start, segs, &coverage, &cover), "pdf_cover");
+        gpu += sim + mark + merge + cover;
+        totals += coverage;
+        std::cerr << "GPU Coverage: " << coverage << ", total: " << totals << std::endl;
+        start += width;
+      }
+    }
+    if (verbose_flag && (merge_flag || cover_flag)) {
+      std::vector<uint32_t> f0(info.n), f1(info.n);
+      std::vector<uint64_t> counts(info.n);
+      check(pdf_get_first(g, f0.data(), f1.data()), "get_first");
+      check(pdf_get_gate_counts(g, counts.data()), "get_gate_counts");
+      if (merge_flag) {
+        std::ofstream f("gpumerge.log");
+        for (uint32_t i = 0; i < info.n; ++i)
+          f << "Segment " << i << "(" << i << "): " << (int32_t)f0[i] << "," << (int32_t)f1[i] << " \n";
+      }
+      if (cover_flag) {
+        std::ofstream f("gpucover.log");
+        for (uint32_t i = 0; i < info.n; ++i) f << "Gate " << i << ": " << counts[i] << "\n";
+      }
+    }
+    std::cerr << "   GPU: " << gpu << " ms" << std::endl;
+    std::cout << vf << ":" << V << "," << info.n << "," << segs << ";" << gpu << ";" << sim << ";" << mark << ";"
+              << merge << ";" << cover << ";" << totals << std::endl;
+  }
+  pdf_grader_destroy(g);
+  pdf_netlist_free(nl);
+  return 0;
+}

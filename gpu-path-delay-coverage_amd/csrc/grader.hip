@@ -36,7 +36,11 @@ using pdf::set_error;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxRegFanin = 8;  // fanins kept in registers by the mark pass
+// Fanins kept in registers by the mark pass.  The copies are indexed only by
+// compile-time constants (fully unrolled loops): a runtime index into a
+// private array becomes an indexed register write, which for an index past
+// the array overwrites unrelated live registers.
+constexpr int kRegFanin = 4;
 
 struct DevNet {
   const uint8_t *type;
@@ -87,15 +91,19 @@ __device__ __forceinline__ uint64_t sens(uint64_t aon, uint64_t bon, uint64_t ao
   return (~aon & bon & ~boff) | (aon & ~bon & ~aoff & ~boff);
 }
 
-// OR_i AND_{j != i} S(fanin_i, fanin_j) over fanin positions.
+// OR_i AND_{j != i} S(fanin_i, fanin_j) over fanin positions, for
+// nfi <= kRegFanin fanins held in registers.
 template <bool ROBUST, bool ANDLIKE>
-__device__ __forceinline__ uint64_t side_ok(const uint64_t *fa, const uint64_t *fb, uint32_t nfi) {
+__device__ __forceinline__ uint64_t side_ok(const uint64_t (&fa)[kRegFanin], const uint64_t (&fb)[kRegFanin],
+                                            uint32_t nfi) {
   uint64_t r = 0;
-  for (uint32_t i = 0; i < nfi; ++i) {
+#pragma unroll
+  for (int i = 0; i < kRegFanin; ++i) {
     uint64_t acc = ~0ULL;
-    for (uint32_t j = 0; j < nfi; ++j)
-      if (j != i) acc &= sens<ROBUST, ANDLIKE>(fa[i], fb[i], fa[j], fb[j]);
-    r |= acc;
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j)
+      if (j != i && (uint32_t)j < nfi) acc &= sens<ROBUST, ANDLIKE>(fa[i], fb[i], fa[j], fb[j]);
+    if ((uint32_t)i < nfi) r |= acc;
   }
   return r;
 }
@@ -116,8 +124,8 @@ __device__ uint64_t side_ok_mem(const Planes &pl, const uint32_t *fins, uint32_t
 }
 
 template <bool ROBUST>
-__device__ __forceinline__ uint64_t side_dispatch(bool andlike, const uint64_t *fa, const uint64_t *fb,
-                                                  uint32_t nfi) {
+__device__ __forceinline__ uint64_t side_dispatch(bool andlike, const uint64_t (&fa)[kRegFanin],
+                                                  const uint64_t (&fb)[kRegFanin], uint32_t nfi) {
   return andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
 }
 
@@ -126,7 +134,7 @@ __device__ __forceinline__ uint64_t side_dispatch(bool andlike, const uint64_t *
 template <bool DO_SIM, bool DO_MARK, bool ROBUST>
 __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uint64_t *__restrict__ va,
                                                 const uint64_t *__restrict__ vb, uint64_t vwords,
-                                                uint64_t wbase, LevelMap mp) {
+                                                uint32_t tail, uint64_t wbase, LevelMap mp) {
   uint32_t g, w;
   if (!map_item(mp, g, w)) return;
   const uint64_t row = (uint64_t)g * pl.pitch + w;
@@ -134,11 +142,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
   uint64_t a, b;
   if (ty == PDF_INPT) {
     if (DO_SIM) {
-      // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P])
+      // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P]).  Bits past P in
+      // the last word are zero in both frames, so those lanes never mark.
       const uint64_t q = wbase + w;
       a = va[(uint64_t)g * vwords + q];
       if (vb) {
         b = vb[(uint64_t)g * vwords + q];
+      } else if (q + 1 == vwords && tail) {
+        b = (a >> 1) | ((va[(uint64_t)g * vwords] & 1ULL) << (tail - 1));
       } else {
         const uint64_t q1 = (q + 1 == vwords) ? 0 : q + 1;
         b = (a >> 1) | (va[(uint64_t)g * vwords + q1] << 63);
@@ -157,8 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
   const uint32_t *fins = net.fin + f0;
   const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
   const bool andlike = ty == PDF_AND || ty == PDF_NAND;
-  uint64_t fa[kMaxRegFanin], fb[kMaxRegFanin];
-  const uint32_t nreg = nfi < kMaxRegFanin ? nfi : kMaxRegFanin;
+  uint64_t fa[kRegFanin], fb[kRegFanin];  // constant indices only
   if (DO_SIM) {
     fa[0] = pl.A[(uint64_t)fins[0] * pl.pitch + w];
     fb[0] = pl.B[(uint64_t)fins[0] * pl.pitch + w];
@@ -167,18 +177,25 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
     if (ty == PDF_NOT) { a = ~a; b = ~b; }
     if (nfi > 1) {
       if (multi) {
-        for (uint32_t j = 1; j < nfi; ++j) {
-          const uint64_t xa = pl.A[(uint64_t)fins[j] * pl.pitch + w];
-          const uint64_t xb = pl.B[(uint64_t)fins[j] * pl.pitch + w];
-          if (j < kMaxRegFanin) { fa[j] = xa; fb[j] = xb; }
-          // the chained 2-input fold of simkernel.cuh:142-148
+        // the chained 2-input fold of simkernel.cuh:142-148
+        auto fold = [&](uint64_t xa, uint64_t xb) {
           switch (ty) {
             case PDF_AND: a &= xa; b &= xb; break;
             case PDF_NAND: a = ~(a & xa); b = ~(b & xb); break;
             case PDF_OR: a |= xa; b |= xb; break;
             default: a = ~(a | xa); b = ~(b | xb); break;
           }
+        };
+#pragma unroll
+        for (int j = 1; j < kRegFanin; ++j) {
+          if ((uint32_t)j < nfi) {
+            fa[j] = pl.A[(uint64_t)fins[j] * pl.pitch + w];
+            fb[j] = pl.B[(uint64_t)fins[j] * pl.pitch + w];
+            fold(fa[j], fb[j]);
+          }
         }
+        for (uint32_t j = kRegFanin; j < nfi; ++j)
+          fold(pl.A[(uint64_t)fins[j] * pl.pitch + w], pl.B[(uint64_t)fins[j] * pl.pitch + w]);
       } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
         a = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
         b = 0;
@@ -202,11 +219,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
         m = m0 & tr;  // empty AND over off-path inputs
       } else if (multi && (m0 & tr)) {
         uint64_t r;
-        if (nfi <= kMaxRegFanin) {
+        if (nfi <= kRegFanin) {
           if (!DO_SIM) {
-            for (uint32_t j = 0; j < nreg; ++j) {
-              fa[j] = pl.A[(uint64_t)fins[j] * pl.pitch + w];
-              fb[j] = pl.B[(uint64_t)fins[j] * pl.pitch + w];
+#pragma unroll
+            for (int j = 0; j < kRegFanin; ++j) {
+              if ((uint32_t)j < nfi) {
+                fa[j] = pl.A[(uint64_t)fins[j] * pl.pitch + w];
+                fb[j] = pl.B[(uint64_t)fins[j] * pl.pitch + w];
+              }
             }
           }
           r = ROBUST ? side_dispatch<true>(andlike, fa, fb, nfi) : side_dispatch<false>(andlike, fa, fb, nfi);
@@ -373,6 +393,7 @@ struct pdf_grader {
   uint32_t *d_fin_off = nullptr, *d_fin = nullptr, *d_fout_off = nullptr, *d_fout = nullptr, *d_level = nullptr;
   // vectors
   uint64_t num_pairs = 0, vwords = 0;
+  uint32_t tail = 0;  // num_pairs % 64
   uint64_t *d_va = nullptr, *d_vb = nullptr;
   // chunk planes
   uint64_t chunk_pairs = 0, pitch = 0;
@@ -482,7 +503,9 @@ int check_chunk(pdf_grader *g, uint64_t chunk, uint64_t start, uint64_t *width) 
   return PDF_OK;
 }
 
-Planes planes(pdf_grader *g, uint64_t width) { return Planes{g->d_A, g->d_B, g->d_M, g->pitch, (uint32_t)(width / 64)}; }
+Planes planes(pdf_grader *g, uint64_t width) {
+  return Planes{g->d_A, g->d_B, g->d_M, g->pitch, (uint32_t)((width + 63) / 64)};
+}
 
 LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, uint32_t *grid) {
   uint32_t twlog = 0;
@@ -502,9 +525,9 @@ int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
     const LevelMap mp = level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], &grid);
     tm_begin(g, kFwd);
     if (robust)
-      k_fwd<S, M, true><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, start / 64, mp);
+      k_fwd<S, M, true><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, g->tail, start / 64, mp);
     else
-      k_fwd<S, M, false><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, start / 64, mp);
+      k_fwd<S, M, false><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, g->tail, start / 64, mp);
     tm_end(g);
   }
   HIPCHK(hipGetLastError());
@@ -674,7 +697,7 @@ void pdf_grader_destroy(pdf_grader *g) {
 
 static int load_vectors(pdf_grader *g, const uint64_t *a, const uint64_t *b, uint64_t num) {
   if (!g || !a) { set_error("null argument"); return PDF_ERR_ARG; }
-  if (num == 0 || num % 64) { set_error("number of pairs must be a positive multiple of 64"); return PDF_ERR_ARG; }
+  if (num == 0) { set_error("no pairs"); return PDF_ERR_ARG; }
   if (num > 0xFFFFFFFFull - 64) { set_error("at most 2^32-65 pairs (pattern ids are uint32)"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
   HIPCHK(hipStreamSynchronize(g->stream));
@@ -682,15 +705,28 @@ static int load_vectors(pdf_grader *g, const uint64_t *a, const uint64_t *b, uin
   dfree(g->d_vb);
   free_chunk(g);
   g->num_pairs = 0;
-  const uint64_t words = num / 64;
+  const uint64_t words = (num + 63) / 64;
+  const uint32_t tail = (uint32_t)(num % 64);
   const size_t bytes = sizeof(uint64_t) * words * g->nl.num_pi;
   int rc = dalloc(&g->d_va, words * g->nl.num_pi);
   if (!rc && b) rc = dalloc(&g->d_vb, words * g->nl.num_pi);
   if (rc) return rc;
-  HIPCHK(hipMemcpy(g->d_va, a, bytes, hipMemcpyHostToDevice));
-  if (b) HIPCHK(hipMemcpy(g->d_vb, b, bytes, hipMemcpyHostToDevice));
+  auto upload = [&](uint64_t *dst, const uint64_t *src) -> int {
+    if (!tail) {
+      HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+      return PDF_OK;
+    }
+    // clear the bits past the last pair so padded lanes hold S0 everywhere
+    std::vector<uint64_t> tmp(src, src + words * g->nl.num_pi);
+    for (uint64_t i = 0; i < g->nl.num_pi; ++i) tmp[i * words + words - 1] &= (1ULL << tail) - 1;
+    HIPCHK(hipMemcpy(dst, tmp.data(), bytes, hipMemcpyHostToDevice));
+    return PDF_OK;
+  };
+  if ((rc = upload(g->d_va, a))) return rc;
+  if (b && (rc = upload(g->d_vb, b))) return rc;
   g->num_pairs = num;
   g->vwords = words;
+  g->tail = tail;
   rc = pdf_grader_set_chunk(g, 0);
   return rc;
 }
@@ -707,12 +743,12 @@ int pdf_grader_set_pairs(pdf_grader *g, const uint64_t *a_bits, const uint64_t *
 int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
   if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
   if (!g->num_pairs) { set_error("pdf_grader_set_chunk: load vectors first"); return PDF_ERR_STATE; }
-  if (chunk_pairs % 64) { set_error("pdf_grader_set_chunk: chunk must be a multiple of 64 pairs"); return PDF_ERR_ARG; }
-  if (chunk_pairs == 0 || chunk_pairs > g->num_pairs) chunk_pairs = g->num_pairs;
+  if (chunk_pairs == 0 || chunk_pairs >= g->num_pairs) chunk_pairs = g->num_pairs;
+  else if (chunk_pairs % 64) { set_error("pdf_grader_set_chunk: chunk must be a multiple of 64 pairs"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
   HIPCHK(hipStreamSynchronize(g->stream));
   free_chunk(g);
-  const uint64_t words = chunk_pairs / 64;
+  const uint64_t words = (chunk_pairs + 63) / 64;
   g->pitch = (words + 15) / 16 * 16;
   const uint64_t n = g->nl.n;
   // cover scratch: one n-long c/h slice per wave, about 1 GiB at most
@@ -881,7 +917,7 @@ int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane, uint64_t *b_plane
   if (g->stage < 1) { set_error("nothing simulated yet"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
-  const uint64_t words = width / 64;
+  const uint64_t words = (width + 63) / 64;
   if (a_plane) HIPCHK(hipMemcpy2D(a_plane, words * 8, g->d_A, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
   if (b_plane) HIPCHK(hipMemcpy2D(b_plane, words * 8, g->d_B, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
   return PDF_OK;
@@ -892,7 +928,7 @@ int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane) {
   if (g->stage < 2) { set_error("nothing marked yet"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
-  const uint64_t words = width / 64;
+  const uint64_t words = (width + 63) / 64;
   HIPCHK(hipMemcpy2D(mark_plane, words * 8, g->d_M, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
   return PDF_OK;
 }

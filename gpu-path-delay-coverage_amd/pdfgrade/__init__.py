@@ -341,14 +341,14 @@ class Grader:
 
     # -- parity getters
     def sim_planes(self, width: int):
-        words = width // 64
+        words = (width + 63) // 64
         a = np.zeros((self.n, words), np.uint64)
         b = np.zeros((self.n, words), np.uint64)
         _check(lib().pdf_get_sim_planes(self._h, _ptr(a), _ptr(b)))
         return a, b
 
     def mark_plane(self, width: int):
-        m = np.zeros((self.n, width // 64), np.uint64)
+        m = np.zeros((self.n, (width + 63) // 64), np.uint64)
         _check(lib().pdf_get_mark_plane(self._h, _ptr(m)))
         return m
 
@@ -369,14 +369,15 @@ class Grader:
         return t.value
 
 
-def planes_to_codes(a: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Unpack bit-planes (n, words) into byte codes (n, words*64)."""
+def planes_to_codes(a: np.ndarray, b: np.ndarray, width: Optional[int] = None) -> np.ndarray:
+    """Unpack bit-planes (n, words) into byte codes (n, width)."""
     n, words = a.shape
     au = np.unpackbits(a.view(np.uint8).reshape(n, words * 8), axis=1, bitorder="little")
     bu = np.unpackbits(b.view(np.uint8).reshape(n, words * 8), axis=1, bitorder="little")
-    return (bu | ((au ^ bu) << 1)).astype(np.uint8)
+    return (bu | ((au ^ bu) << 1)).astype(np.uint8)[:, :width]
 
 
-def plane_to_bytes(m: np.ndarray) -> np.ndarray:
+def plane_to_bytes(m: np.ndarray, width: Optional[int] = None) -> np.ndarray:
     n, words = m.shape
-    return np.unpackbits(m.view(np.uint8).reshape(n, words * 8), axis=1, bitorder="little").astype(np.uint8)
+    return np.unpackbits(m.view(np.uint8).reshape(n, words * 8), axis=1,
+                         bitorder="little").astype(np.uint8)[:, :width]

@@ -149,14 +149,15 @@ int pdf_grader_create(int device, const pdf_netlist *nl, pdf_grader **out);
 void pdf_grader_destroy(pdf_grader *g);
 
 /* Sequence-mode vectors from host memory (read_vectors, main.cu:146-156).
- * num_vectors must be a multiple of 64. */
+ * Any count >= 1; when it is not a multiple of 64 the bits past the last
+ * vector are ignored (the last pair wraps to vector 0, simkernel.cuh:116-120). */
 int pdf_grader_set_vectors(pdf_grader *g, const uint64_t *bits,
                            uint64_t num_vectors);
-/* Explicit pairs (A_p, B_p), both PI-major packed; num_pairs % 64 == 0. */
+/* Explicit pairs (A_p, B_p), both PI-major packed, num_pairs >= 1. */
 int pdf_grader_set_pairs(pdf_grader *g, const uint64_t *a_bits,
                          const uint64_t *b_bits, uint64_t num_pairs);
 /* Pairs per chunk (the reference's simul_patterns, main.cu:150-153); a
- * multiple of 64.  0 = one chunk for all pairs.  Allocates the chunk's
+ * multiple of 64, or 0 (or >= the number of pairs) for one chunk.  Allocates the chunk's
  * bit-planes.  Also resets the cross-chunk state. */
 int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs);
 /* Number of chunks for the loaded pairs and the width (pairs) of one. */

@@ -95,7 +95,7 @@ def edge_netlist():
     l1 = [add(t, ins) for t, ins in [
         (pg.AND, [0, 1]), (pg.NAND, [1, 2, 3]), (pg.OR, [2, 3]), (pg.NOR, [3, 4, 5]), (pg.XOR, [4, 5]),
         (pg.XNOR, [5]), (pg.NOT, [6]), (pg.BUFF, [0, 1]), (pg.FROM, [2]), (pg.DFF, [3]), (pg.NAND, [4]),
-        (pg.UNKN, [5, 6]), (pg.NOT, [1, 2]), (pg.XOR, [6]), (pg.UNKN, [0])]]
+        (pg.UNKN, [5, 6]), (pg.NOT, [1, 2]), (pg.XOR, [6]), (pg.UNKN, [0]), (pg.OR, [0, 1, 2, 3, 4])]]
     big_and = add(pg.AND, [0, 1, 2, 3, 4, 5, 6, l1[0], l1[2], l1[6], l1[8], l1[9]], out=1)
     big_or = add(pg.NOR, [l1[1], l1[3], 0, 1, 2, 3, 4, 5, 6, l1[13], l1[14], l1[7]], out=1)
     l2 = [add(t, ins, out) for t, ins, out in [

@@ -72,10 +72,12 @@ def check_intermediates(nl, bits, P, chunk, robust, b_bits=None):
             g.sim(c, start)
             a, b = g.sim_planes(w)
             codes = orc.sim_chunk(net, bits, P, start, w, b_bits=b_bits)
-            assert np.array_equal(pg.planes_to_codes(a, b), codes), f"sim chunk {c}"
+            assert np.array_equal(pg.planes_to_codes(a, b, w), codes), f"sim chunk {c}"
+            # lanes past the last pair never transition
+            assert (pg.planes_to_codes(a, b)[:, w:] < 2).all()
             g.mark(c, start, robust)
             marks, _ = orc.mark_chunk(net, codes, robust)
-            assert np.array_equal(pg.plane_to_bytes(g.mark_plane(w)), marks), f"mark chunk {c}"
+            assert np.array_equal(pg.plane_to_bytes(g.mark_plane(w), w), marks), f"mark chunk {c}"
             g.merge(c, start)
             orc.merge_chunk(net, codes, marks, start, f0, f1)
             gf0, gf1 = g.first()
@@ -132,6 +134,20 @@ def test_edge_netlist(robust):
     ref = oracle_grade(nl, bits, P, 128, robust)
     assert_same(gpu_grade(nl, bits, P, 128, robust), ref, "edge")
     check_intermediates(nl, bits, P, 192, robust)
+
+
+@pytest.mark.parametrize("P,chunk", [(1000, 256), (130, 64), (77, 0)])
+def test_pair_count_not_multiple_of_64(P, chunk):
+    """The last pair wraps to vector 0 (simkernel.cuh:116-120); lanes past the
+    last pair never transition."""
+    say("ragged", P, chunk)
+    nl = pg.Netlist.generate(small_config(33, num_gates=70, levels=6))
+    bits = biased_vectors(33, nl.info()["num_pi"], P, p_hold=0.5)
+    for robust in (True, False):
+        ref = oracle_grade(nl, bits, P, chunk, robust)
+        assert_same(gpu_grade(nl, bits, P, chunk, robust), ref, f"P={P}")
+        assert_same(gpu_grade(nl, bits, P, chunk, robust, stages=True), ref, f"P={P} stages")
+    check_intermediates(nl, bits, P, chunk or P, True)
 
 
 def test_constant_vectors_give_zero():
@@ -205,7 +221,7 @@ def test_api_errors():
     bits = pg.gen_vectors(1, nl.info()["num_pi"], 1024)
     with pg.Grader(nl) as g:
         with pytest.raises(pg.PdfError) as e:
-            g.set_vectors(bits, 1000)
+            g.set_vectors(bits, 0)
         assert e.value.code == pg.ERR_ARG
         g.set_vectors(bits, 1024)
         with pytest.raises(pg.PdfError) as e:

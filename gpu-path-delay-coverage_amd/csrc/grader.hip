@@ -1,24 +1,30 @@
-// MI355X grading path: sim -> mark -> merge -> cover on gate-major bit-planes.
+// MI355X grading path: sim -> mark -> merge -> cover on node-major bit-planes.
 //
-// Layout in HBM (one chunk of W pairs, words = W/64, rows padded to `pitch`
-// words so every row starts on a 128-byte line):
-//   A[n][pitch]  first-frame value of each node, bit j of word k = pair 64k+j
-//   B[n][pitch]  second-frame value
-//   M[n][pitch]  path mark (forward marks, then final marks after removal)
+// Layout in HBM for one chunk of W pairs (words = ceil(W/64); rows padded to
+// `pitch` words, a multiple of 16, so every row starts on a 128-byte line):
+//   V[n][pitch]   value of each node under vector start+64k+j (bit j, word k).
+//                 In sequence mode pair p is (V[p], V[p+1]) (simkernel.cuh:
+//                 115-124), so one plane holds both frames: the frame-2 word
+//                 is the frame-1 word shifted by one bit, and word `words`
+//                 holds the 64 vectors after the chunk (for the last pair).
+//   VB[n][pitch]  frame-2 values, explicit-pair mode only.
+//   M[n][pitch]   path marks, written only where non-zero;
+//   NZ[n][tiles]  bit (w & 63) of word (w >> 6): M[g][w] was written in this
+//                 chunk (a mark word with its bit clear reads as zero).
 // A node's 4-valued code is (b | (a^b) << 1): S0, S1, T0 (1->0), T1 (0->1),
-// the encoding the reference's tables use (simkernel.cuh:158-165).
+// the encoding of the reference's tables (simkernel.cuh:158-165).
 //
 // Kernels (one launch per level where the reference launches per level):
-//   k_fwd      sim (simkernel.cuh:96-156) and forward marking
+//   k_fwd      simulation (simkernel.cuh:96-156) and forward marking
 //              (markkernel.cuh:215-270), fused or alone
 //   k_reverse  invalid-mark removal (markkernel.cuh:271-293) and the
 //              first-detection merge (mergekernel.cuh:116-198), fused or alone;
-//              also lists each word's marked nodes in reverse level order
-//   k_events   distinct first-detection patterns of the chunk
+//              also lists each word's marked nodes in descending level order
+//   k_events   distinct first-detection pairs of the chunk
 //   k_cover    exact uint64 reverse path-count DP (coverkernel.cuh:171-230)
-//              over the marked nodes of each such pattern only; a pattern
-//              with no first detection contributes nothing (c = 0 for every
-//              node, coverkernel.cuh:214-220), so the others are skipped.
+//              over the nodes marked at each such pair only: a pair with no
+//              first detection has c = 0 at every node (coverkernel.cuh:
+//              214-220), so it contributes nothing and is skipped.
 
 #include <hip/hip_runtime.h>
 
@@ -54,21 +60,26 @@ struct DevNet {
 };
 
 struct Planes {
-  uint64_t *A, *B, *M;
-  uint64_t pitch;  // words per row
-  uint32_t words;  // words in this chunk
+  uint64_t *V, *VB, *M, *NZ;
+  uint64_t pitch;      // words per plane row
+  uint32_t words;      // pair words in this chunk
+  uint32_t ntiles;     // NZ words per node
+  uint64_t last_mask;  // valid pairs of the chunk's last word
 };
 
 // Thread -> (gate, word) for a level launch.  A block of kBlock threads covers
 // (kBlock >> twlog) gates x (1 << twlog) consecutive words; block b takes word
 // tile b % T and gate group b / T, so with T a multiple of 8 a word tile stays
 // on one XCD (blocks are dealt round-robin to the 8 XCDs) at every level and
-// its fanin rows are re-read from that XCD's L2.
+// its fanin rows are re-read from that XCD's L2.  Blocks from `main_blocks`
+// on (sequence-mode simulation only) compute the extra column `words`, one
+// gate per thread.
 struct LevelMap {
   uint32_t T;      // word tiles
   uint32_t twlog;  // log2 tile width in words (<= 6)
   uint32_t words;  // words in the chunk
   uint32_t lo, count;
+  uint32_t main_blocks;
 };
 
 __device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t &g, uint32_t &w) {
@@ -78,6 +89,22 @@ __device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t &g, uint32
   const uint32_t gi = grp * (kBlock >> mp.twlog) + (threadIdx.x >> mp.twlog);
   g = mp.lo + gi;
   return w < mp.words && gi < mp.count;
+}
+
+__device__ __forceinline__ uint64_t shift_next(uint64_t cur, uint64_t nxt) { return (cur >> 1) | (nxt << 63); }
+
+// Frames (a, b) of node f at word w.
+template <bool SEQ>
+__device__ __forceinline__ void frames(const Planes &pl, uint32_t f, uint32_t w, uint64_t &a, uint64_t &b) {
+  const uint64_t at = (uint64_t)f * pl.pitch + w;
+  a = pl.V[at];
+  b = SEQ ? shift_next(a, pl.V[at + 1]) : pl.VB[at];
+}
+
+// Mark word of node f (zero unless written in this chunk).
+__device__ __forceinline__ uint64_t mark_of(const Planes &pl, uint32_t f, uint32_t w) {
+  const uint64_t nz = pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)];
+  return ((nz >> (w & 63)) & 1) ? pl.M[(uint64_t)f * pl.pitch + w] : 0;
 }
 
 // Off-path sensitization predicate S(on, off) of markkernel.cuh:36-41 in
@@ -108,163 +135,211 @@ __device__ __forceinline__ uint64_t side_ok(const uint64_t (&fa)[kRegFanin], con
   return r;
 }
 
-// Same predicate for gates with more fanins than fit in registers: operands
-// are re-read from the planes.
-template <bool ROBUST, bool ANDLIKE>
+// Same predicate for gates with more fanins: operands re-read from the planes.
+template <bool SEQ, bool ROBUST, bool ANDLIKE>
 __device__ uint64_t side_ok_mem(const Planes &pl, const uint32_t *fins, uint32_t nfi, uint32_t w) {
   uint64_t r = 0;
   for (uint32_t i = 0; i < nfi; ++i) {
-    const uint64_t ai = pl.A[fins[i] * pl.pitch + w], bi = pl.B[fins[i] * pl.pitch + w];
+    uint64_t ai, bi;
+    frames<SEQ>(pl, fins[i], w, ai, bi);
     uint64_t acc = ~0ULL;
-    for (uint32_t j = 0; j < nfi && acc; ++j)
-      if (j != i) acc &= sens<ROBUST, ANDLIKE>(ai, bi, pl.A[fins[j] * pl.pitch + w], pl.B[fins[j] * pl.pitch + w]);
+    for (uint32_t j = 0; j < nfi && acc; ++j) {
+      if (j == i) continue;
+      uint64_t aj, bj;
+      frames<SEQ>(pl, fins[j], w, aj, bj);
+      acc &= sens<ROBUST, ANDLIKE>(ai, bi, aj, bj);
+    }
     r |= acc;
   }
   return r;
 }
 
-template <bool ROBUST>
-__device__ __forceinline__ uint64_t side_dispatch(bool andlike, const uint64_t (&fa)[kRegFanin],
-                                                  const uint64_t (&fb)[kRegFanin], uint32_t nfi) {
-  return andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+// The chained 2-input fold of simkernel.cuh:142-148 for AND/NAND/OR/NOR.
+__device__ __forceinline__ void fold(uint8_t ty, uint64_t &v, uint64_t x) {
+  switch (ty) {
+    case PDF_AND: v &= x; break;
+    case PDF_NAND: v = ~(v & x); break;
+    case PDF_OR: v |= x; break;
+    default: v = ~(v | x); break;
+  }
+}
+
+// One frame of one node at word w (used for the extra sequence column).
+__device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Planes &pl, uint32_t g, uint32_t w,
+                                                  uint8_t ty) {
+  const uint32_t f0 = net.fin_off[g];
+  const uint32_t nfi = net.fin_off[g + 1] - f0;
+  const uint32_t *fins = net.fin + f0;
+  uint64_t v = pl.V[(uint64_t)fins[0] * pl.pitch + w];
+  if (ty == PDF_NOT) v = ~v;
+  if (nfi > 1) {
+    if (ty >= PDF_AND && ty <= PDF_NOR) {
+      for (uint32_t j = 1; j < nfi; ++j) fold(ty, v, pl.V[(uint64_t)fins[j] * pl.pitch + w]);
+    } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
+      v = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
+    }
+  }
+  return v;
 }
 
 // Forward pass over one level: simulation (DO_SIM) and/or forward marking
-// (DO_MARK).  Thread = (gate, word).
-template <bool DO_SIM, bool DO_MARK, bool ROBUST>
+// (DO_MARK).  SEQ: consecutive-vector pairs from one value plane; otherwise
+// explicit pairs with a second plane.
+template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST>
 __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uint64_t *__restrict__ va,
-                                                const uint64_t *__restrict__ vb, uint64_t vwords,
-                                                uint32_t tail, uint64_t wbase, LevelMap mp) {
+                                                const uint64_t *__restrict__ vb, uint64_t vstride, uint64_t wbase,
+                                                LevelMap mp) {
+  if (SEQ && DO_SIM && blockIdx.x >= mp.main_blocks) {
+    // extra column: the node values of the 64 vectors after the chunk
+    const uint32_t gi = (blockIdx.x - mp.main_blocks) * kBlock + threadIdx.x;
+    if (gi >= mp.count) return;
+    const uint32_t g = mp.lo + gi;
+    const uint8_t ty = net.type[g];
+    const uint64_t v = ty == PDF_INPT ? va[(uint64_t)g * vstride + wbase + mp.words]
+                                      : sim_one_frame(net, pl, g, mp.words, ty);
+    pl.V[(uint64_t)g * pl.pitch + mp.words] = v;
+    return;
+  }
   uint32_t g, w;
   if (!map_item(mp, g, w)) return;
   const uint64_t row = (uint64_t)g * pl.pitch + w;
   const uint8_t ty = net.type[g];
-  uint64_t a, b;
+  uint64_t a, b = 0, m = 0;
   if (ty == PDF_INPT) {
+    // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P])
+    const uint64_t q = (uint64_t)g * vstride + wbase + w;
     if (DO_SIM) {
-      // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P]).  Bits past P in
-      // the last word are zero in both frames, so those lanes never mark.
-      const uint64_t q = wbase + w;
-      a = va[(uint64_t)g * vwords + q];
-      if (vb) {
-        b = vb[(uint64_t)g * vwords + q];
-      } else if (q + 1 == vwords && tail) {
-        b = (a >> 1) | ((va[(uint64_t)g * vwords] & 1ULL) << (tail - 1));
-      } else {
-        const uint64_t q1 = (q + 1 == vwords) ? 0 : q + 1;
-        b = (a >> 1) | (va[(uint64_t)g * vwords + q1] << 63);
+      a = va[q];
+      pl.V[row] = a;
+      if (!SEQ) {
+        b = vb[q];
+        pl.VB[row] = b;
+      } else if (DO_MARK) {
+        b = shift_next(a, va[q + 1]);
       }
-      pl.A[row] = a;
-      pl.B[row] = b;
-    } else {
-      a = pl.A[row];
-      b = pl.B[row];
+    } else if (DO_MARK) {
+      frames<SEQ>(pl, g, w, a, b);
     }
-    if (DO_MARK) pl.M[row] = a ^ b;
-    return;
-  }
-  const uint32_t f0 = net.fin_off[g];
-  const uint32_t nfi = net.fin_off[g + 1] - f0;
-  const uint32_t *fins = net.fin + f0;
-  const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
-  const bool andlike = ty == PDF_AND || ty == PDF_NAND;
-  uint64_t fa[kRegFanin], fb[kRegFanin];  // constant indices only
-  if (DO_SIM) {
-    fa[0] = pl.A[(uint64_t)fins[0] * pl.pitch + w];
-    fb[0] = pl.B[(uint64_t)fins[0] * pl.pitch + w];
-    a = fa[0];
-    b = fb[0];
-    if (ty == PDF_NOT) { a = ~a; b = ~b; }
-    if (nfi > 1) {
-      if (multi) {
-        // the chained 2-input fold of simkernel.cuh:142-148
-        auto fold = [&](uint64_t xa, uint64_t xb) {
-          switch (ty) {
-            case PDF_AND: a &= xa; b &= xb; break;
-            case PDF_NAND: a = ~(a & xa); b = ~(b & xb); break;
-            case PDF_OR: a |= xa; b |= xb; break;
-            default: a = ~(a | xa); b = ~(b | xb); break;
-          }
-        };
-#pragma unroll
-        for (int j = 1; j < kRegFanin; ++j) {
-          if ((uint32_t)j < nfi) {
-            fa[j] = pl.A[(uint64_t)fins[j] * pl.pitch + w];
-            fb[j] = pl.B[(uint64_t)fins[j] * pl.pitch + w];
-            fold(fa[j], fb[j]);
-          }
-        }
-        for (uint32_t j = kRegFanin; j < nfi; ++j)
-          fold(pl.A[(uint64_t)fins[j] * pl.pitch + w], pl.B[(uint64_t)fins[j] * pl.pitch + w]);
-      } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
-        a = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
-        b = 0;
-      }
-    }
-    pl.A[row] = a;
-    pl.B[row] = b;
+    // pairs past the last one of the chunk never mark; every other mark needs
+    // its fanin 0 marked, so this keeps them clear everywhere
+    if (DO_MARK) m = (a ^ b) & (w + 1 == mp.words ? pl.last_mask : ~0ULL);
   } else {
-    a = pl.A[row];
-    b = pl.B[row];
-  }
-  if (!DO_MARK) return;
-  const uint64_t tr = a ^ b;
-  uint64_t m = 0;
-  if (tr) {
-    const uint64_t m0 = pl.M[(uint64_t)fins[0] * pl.pitch + w];
-    if (ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT) {
-      m = m0 & tr;
-    } else if (ty >= PDF_AND && ty <= PDF_XNOR) {
-      if (nfi == 1) {
-        m = m0 & tr;  // empty AND over off-path inputs
-      } else if (multi && (m0 & tr)) {
-        uint64_t r;
-        if (nfi <= kRegFanin) {
-          if (!DO_SIM) {
+    const uint32_t f0 = net.fin_off[g];
+    const uint32_t nfi = net.fin_off[g + 1] - f0;
+    const uint32_t *fins = net.fin + f0;
+    const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+    uint64_t fa[kRegFanin], fb[kRegFanin];  // constant indices only
+    if (DO_SIM) {
+      if (DO_MARK || !SEQ) {
+        frames<SEQ>(pl, fins[0], w, fa[0], fb[0]);
+      } else {
+        fa[0] = pl.V[(uint64_t)fins[0] * pl.pitch + w];
+        fb[0] = 0;
+      }
+      a = fa[0];
+      b = fb[0];
+      if (ty == PDF_NOT) { a = ~a; b = ~b; }
+      if (nfi > 1) {
+        if (multi) {
 #pragma unroll
-            for (int j = 0; j < kRegFanin; ++j) {
-              if ((uint32_t)j < nfi) {
-                fa[j] = pl.A[(uint64_t)fins[j] * pl.pitch + w];
-                fb[j] = pl.B[(uint64_t)fins[j] * pl.pitch + w];
+          for (int j = 1; j < kRegFanin; ++j) {
+            if ((uint32_t)j < nfi) {
+              if (DO_MARK || !SEQ) {
+                frames<SEQ>(pl, fins[j], w, fa[j], fb[j]);
+              } else {
+                fa[j] = pl.V[(uint64_t)fins[j] * pl.pitch + w];
+                fb[j] = 0;
               }
+              fold(ty, a, fa[j]);
+              fold(ty, b, fb[j]);
             }
           }
-          r = ROBUST ? side_dispatch<true>(andlike, fa, fb, nfi) : side_dispatch<false>(andlike, fa, fb, nfi);
-        } else {
-          r = andlike ? side_ok_mem<ROBUST, true>(pl, fins, nfi, w) : side_ok_mem<ROBUST, false>(pl, fins, nfi, w);
+          for (uint32_t j = kRegFanin; j < nfi; ++j) {
+            uint64_t xa, xb = 0;
+            if (DO_MARK || !SEQ) frames<SEQ>(pl, fins[j], w, xa, xb);
+            else xa = pl.V[(uint64_t)fins[j] * pl.pitch + w];
+            fold(ty, a, xa);
+            fold(ty, b, xb);
+          }
+        } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
+          a = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
+          b = 0;
         }
-        // markkernel.cuh:253-261: the on-path mark is always fanin 0's
-        m = r & m0 & tr;
       }
-      // XOR/XNOR with >= 2 fanins never sensitize (LUT entries 0)
+      pl.V[row] = a;
+      if (!SEQ) pl.VB[row] = b;
+    } else if (DO_MARK) {
+      frames<SEQ>(pl, g, w, a, b);
+    }
+    if (DO_MARK) {
+      const uint64_t tr = a ^ b;
+      if (tr) {
+        const uint64_t m0 = mark_of(pl, fins[0], w);
+        if (ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT) {
+          m = m0 & tr;
+        } else if (ty >= PDF_AND && ty <= PDF_XNOR) {
+          if (nfi == 1) {
+            m = m0 & tr;  // empty AND over off-path inputs
+          } else if (multi && (m0 & tr)) {
+            uint64_t r;
+            if (nfi <= kRegFanin) {
+              if (!DO_SIM) {
+#pragma unroll
+                for (int j = 0; j < kRegFanin; ++j)
+                  if ((uint32_t)j < nfi) frames<SEQ>(pl, fins[j], w, fa[j], fb[j]);
+              }
+              const bool andlike = ty == PDF_AND || ty == PDF_NAND;
+              r = andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+            } else {
+              r = (ty == PDF_AND || ty == PDF_NAND) ? side_ok_mem<SEQ, ROBUST, true>(pl, fins, nfi, w)
+                                                    : side_ok_mem<SEQ, ROBUST, false>(pl, fins, nfi, w);
+            }
+            // markkernel.cuh:253-261: the on-path mark is always fanin 0's
+            m = r & m0 & tr;
+          }
+          // XOR/XNOR with >= 2 fanins never sensitize (LUT entries 0)
+        }
+      }
     }
   }
-  pl.M[row] = m;
+  if (DO_MARK) {
+    if (m) pl.M[row] = m;
+    // one NZ word per (gate, tile): bits from the lanes of this gate's group
+    const uint64_t bal = __ballot(m != 0);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tw = 1u << mp.twlog;
+    if ((lane & (tw - 1)) == 0) {
+      const uint64_t bits = mp.twlog == 6 ? bal : (bal >> lane) & ((1ULL << tw) - 1);
+      pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)] = bits;
+    }
+  }
 }
 
 // Reverse pass over one level.  DO_REMOVE: m &= po | OR_fanouts m
 // (markkernel.cuh:279-291).  DO_MERGE: earliest marked T0/T1 pair per node
 // (mergekernel.cuh:141-197) and the per-word list of marked nodes.
-template <bool DO_REMOVE, bool DO_MERGE>
-__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, Planes pl, LevelMap mp, uint32_t start, uint32_t *__restrict__ first_t0,
-                                                    uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
-                                                    uint32_t *__restrict__ list_cnt, uint64_t list_stride) {
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, Planes pl, LevelMap mp, uint32_t start,
+                                                    uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
+                                                    uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                    uint64_t list_stride) {
   uint32_t g, w;
   if (!map_item(mp, g, w)) return;
-  const uint64_t row = (uint64_t)g * pl.pitch + w;
-  uint64_t m = pl.M[row];
+  uint64_t m = mark_of(pl, g, w);
   if (!m) return;
+  const uint64_t row = (uint64_t)g * pl.pitch + w;
   if (DO_REMOVE && !net.po[g]) {
     uint64_t r = 0;
     const uint32_t e = net.fout_off[g + 1];
-    for (uint32_t k = net.fout_off[g]; k < e && (m & ~r); ++k) r |= pl.M[(uint64_t)net.fout[k] * pl.pitch + w];
+    for (uint32_t k = net.fout_off[g]; k < e && (m & ~r); ++k) r |= mark_of(pl, net.fout[k], w);
     if (m & ~r) {
       m &= r;
       pl.M[row] = m;
     }
   }
   if (DO_MERGE && m) {
-    const uint64_t b = pl.B[row];
+    uint64_t a, b;
+    frames<SEQ>(pl, g, w, a, b);
     const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
     const uint32_t base = start + 64u * w;
     if (t0) atomicMin(&first_t0[g], base + (uint32_t)__builtin_ctzll(t0));
@@ -274,13 +349,31 @@ __global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, Planes pl, Level
   }
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t count) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = v;
+// Cross-chunk state to its initial value (one launch per grade).
+__global__ void k_reset_grade(uint32_t n, uint32_t *first_t0, uint32_t *first_t1, uint64_t *gate_counts,
+                              uint64_t *totals, uint32_t *err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    first_t0[i] = PDF_NONE;
+    first_t1[i] = PDF_NONE;
+    gate_counts[i] = 0;
+  }
+  if (i < 2) totals[i] = 0;
+  if (i == 0) *err = 0;
 }
-__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t count) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
-    p[i] = v;
+
+// Per-chunk merge/cover scratch to zero (one launch per chunk).
+__global__ void k_reset_chunk(uint32_t words, uint32_t *list_cnt, uint64_t *actmap, uint32_t *act_count,
+                              uint64_t *totals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < words) {
+    list_cnt[i] = 0;
+    actmap[i] = 0;
+  }
+  if (i == 0) {
+    *act_count = 0;
+    totals[0] = 0;
+  }
 }
 
 // Distinct first-detection patterns of this chunk (relative index).
@@ -290,6 +383,7 @@ __global__ void k_events(uint32_t n, const uint32_t *__restrict__ first_t0, cons
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const uint32_t x[2] = {first_t0[g], first_t1[g]};
+#pragma unroll
   for (int k = 0; k < 2; ++k) {
     if (x[k] == PDF_NONE || x[k] < start || x[k] - start >= width) continue;
     const uint32_t rel = x[k] - start;
@@ -311,7 +405,7 @@ __device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
 // wave per pattern; the word's node list is in descending level order, so a
 // run of equal levels is processed lane-parallel.  c/h per node live in this
 // wave's slice of cs/hs and are read only for fanouts marked at p, all of
-// which were written earlier in the same run over the list.
+// which were written earlier in the same walk over the list.
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                               const uint32_t *__restrict__ act_list,
@@ -341,11 +435,11 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       const uint64_t same = __ballot(valid && lv == lv0);
       // the run is a contiguous prefix of the lanes
       const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
-      if (lane < run && ((pl.M[(uint64_t)g * pl.pitch + w] >> j) & 1)) {
+      if (lane < run && ((mark_of(pl, g, w) >> j) & 1)) {
         uint64_t C = 0, H = net.po[g] == 1 ? 1 : 0;
         for (uint32_t e = net.fout_off[g]; e < net.fout_off[g + 1]; ++e) {
           const uint32_t fo = net.fout[e];
-          if ((pl.M[(uint64_t)fo * pl.pitch + w] >> j) & 1) {
+          if ((mark_of(pl, fo, w) >> j) & 1) {
             ovf |= add_ovf(C, c_s[fo]);
             ovf |= add_ovf(H, h_s[fo]);
           }
@@ -391,13 +485,14 @@ struct pdf_grader {
   // netlist in HBM
   uint8_t *d_type = nullptr, *d_po = nullptr;
   uint32_t *d_fin_off = nullptr, *d_fin = nullptr, *d_fout_off = nullptr, *d_fout = nullptr, *d_level = nullptr;
-  // vectors
-  uint64_t num_pairs = 0, vwords = 0;
-  uint32_t tail = 0;  // num_pairs % 64
+  // vectors: sequence mode keeps vwords + 1 words per PI row (positions
+  // wrapped mod num_pairs, so the word after the last is available)
+  uint64_t num_pairs = 0, vwords = 0, vstride = 0;
+  bool seq = true;
   uint64_t *d_va = nullptr, *d_vb = nullptr;
   // chunk planes
-  uint64_t chunk_pairs = 0, pitch = 0;
-  uint64_t *d_A = nullptr, *d_B = nullptr, *d_M = nullptr;
+  uint64_t chunk_pairs = 0, pitch = 0, ntiles = 0;
+  uint64_t *d_V = nullptr, *d_VB = nullptr, *d_M = nullptr, *d_NZ = nullptr;
   // merge / cover state
   uint32_t *d_first0 = nullptr, *d_first1 = nullptr;
   uint64_t *d_gate = nullptr;
@@ -417,9 +512,7 @@ struct pdf_grader {
   int stage = 0;  // 1 = simulated, 2 = marked, 3 = merged, 4 = covered
   int stage_robust = 0;
 
-  DevNet net() const {
-    return DevNet{d_type, d_po, d_fin_off, d_fin, d_fout_off, d_fout, d_level, nl.n};
-  }
+  DevNet net() const { return DevNet{d_type, d_po, d_fin_off, d_fin, d_fout_off, d_fout, d_level, nl.n}; }
 };
 
 namespace {
@@ -430,9 +523,9 @@ bool hip_ok(hipError_t e, const char *what) {
   return false;
 }
 
-#define HIPCHK(expr)                                   \
-  do {                                                 \
-    if (!hip_ok((expr), #expr)) return PDF_ERR_HIP;    \
+#define HIPCHK(expr)                                \
+  do {                                              \
+    if (!hip_ok((expr), #expr)) return PDF_ERR_HIP; \
   } while (0)
 
 template <typename T>
@@ -474,25 +567,6 @@ void tm_end(pdf_grader *g) {
   (void)hipEventRecord(g->ev_pool[g->ev_family.size() * 2 - 1], g->stream);
 }
 
-int fill32(pdf_grader *g, uint32_t *p, uint32_t v, uint64_t count) {
-  if (!count) return PDF_OK;
-  const uint32_t grid = std::min<uint64_t>(grid_for(count, 256), 4096);
-  tm_begin(g, kFill);
-  k_fill_u32<<<grid, 256, 0, g->stream>>>(p, v, count);
-  tm_end(g);
-  HIPCHK(hipGetLastError());
-  return PDF_OK;
-}
-int fill64(pdf_grader *g, uint64_t *p, uint64_t v, uint64_t count) {
-  if (!count) return PDF_OK;
-  const uint32_t grid = std::min<uint64_t>(grid_for(count, 256), 4096);
-  tm_begin(g, kFill);
-  k_fill_u64<<<grid, 256, 0, g->stream>>>(p, v, count);
-  tm_end(g);
-  HIPCHK(hipGetLastError());
-  return PDF_OK;
-}
-
 int check_chunk(pdf_grader *g, uint64_t chunk, uint64_t start, uint64_t *width) {
   if (!g->num_pairs) { set_error("no vectors loaded"); return PDF_ERR_STATE; }
   if (!g->chunk_pairs) { set_error("chunk size not set"); return PDF_ERR_STATE; }
@@ -504,62 +578,81 @@ int check_chunk(pdf_grader *g, uint64_t chunk, uint64_t start, uint64_t *width) 
 }
 
 Planes planes(pdf_grader *g, uint64_t width) {
-  return Planes{g->d_A, g->d_B, g->d_M, g->pitch, (uint32_t)((width + 63) / 64)};
+  const uint32_t tail = (uint32_t)(width % 64);
+  return Planes{g->d_V, g->d_VB, g->d_M, g->d_NZ, g->pitch, (uint32_t)((width + 63) / 64), (uint32_t)g->ntiles,
+                tail ? (1ULL << tail) - 1 : ~0ULL};
 }
 
-LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, uint32_t *grid) {
+LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, bool extra_column, uint32_t *grid) {
   uint32_t twlog = 0;
   while (twlog < 6 && (1u << twlog) < words) ++twlog;
-  LevelMap mp{(words + (1u << twlog) - 1) >> twlog, twlog, words, lo, count};
+  LevelMap mp{(words + (1u << twlog) - 1) >> twlog, twlog, words, lo, count, 0};
   const uint32_t per_block = kBlock >> twlog;
-  *grid = mp.T * ((count + per_block - 1) / per_block);
+  mp.main_blocks = mp.T * ((count + per_block - 1) / per_block);
+  *grid = mp.main_blocks + (extra_column ? grid_for(count, kBlock) : 0);
   return mp;
 }
 
-template <bool S, bool M>
-int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
+template <bool SEQ, bool S, bool M>
+void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
   for (uint32_t k = 0; k <= g->nl.L; ++k) {
     uint32_t grid;
-    const LevelMap mp = level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], &grid);
+    const LevelMap mp =
+        level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], SEQ && S, &grid);
     tm_begin(g, kFwd);
     if (robust)
-      k_fwd<S, M, true><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, g->tail, start / 64, mp);
+      k_fwd<SEQ, S, M, true><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
     else
-      k_fwd<S, M, false><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vwords, g->tail, start / 64, mp);
+      k_fwd<SEQ, S, M, false><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
     tm_end(g);
   }
+}
+
+template <bool S, bool M>
+int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
+  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust);
+  else launch_fwd_t<false, S, M>(g, start, width, robust);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+template <bool SEQ, bool R, bool M>
+void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width) {
+  const Planes pl = planes(g, width);
+  const DevNet net = g->net();
+  for (uint32_t k = g->nl.L + 1; k-- > 0;) {
+    uint32_t grid;
+    const LevelMap mp =
+        level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], false, &grid);
+    tm_begin(g, kRev);
+    k_reverse<SEQ, R, M><<<grid, kBlock, 0, g->stream>>>(net, pl, mp, (uint32_t)start, g->d_first0, g->d_first1,
+                                                         g->d_list, g->d_list_cnt, g->nl.n);
+    tm_end(g);
+  }
+}
+
+int reset_chunk(pdf_grader *g, uint64_t width) {
+  const uint32_t words = (uint32_t)((width + 63) / 64);
+  tm_begin(g, kFill);
+  k_reset_chunk<<<grid_for(std::max(words, 1u), 256), 256, 0, g->stream>>>(words, g->d_list_cnt, g->d_actmap,
+                                                                         g->d_act_count, g->d_totals);
+  tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
 
 template <bool R, bool M>
 int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width) {
-  const Planes pl = planes(g, width);
-  const DevNet net = g->net();
-  if (M) {
-    int rc = fill32(g, g->d_list_cnt, 0, pl.words);
-    if (rc) return rc;
-  }
-  for (uint32_t k = g->nl.L + 1; k-- > 0;) {
-    uint32_t grid;
-    const LevelMap mp = level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], &grid);
-    tm_begin(g, kRev);
-    k_reverse<R, M><<<grid, kBlock, 0, g->stream>>>(net, pl, mp, (uint32_t)start, g->d_first0, g->d_first1,
-                                                    g->d_list, g->d_list_cnt, g->nl.n);
-    tm_end(g);
-  }
+  if (g->seq) launch_reverse_t<true, R, M>(g, start, width);
+  else launch_reverse_t<false, R, M>(g, start, width);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
 
 int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
   const Planes pl = planes(g, width);
-  int rc;
-  if ((rc = fill64(g, g->d_actmap, 0, pl.words))) return rc;
-  if ((rc = fill32(g, g->d_act_count, 0, 1))) return rc;
-  if ((rc = fill64(g, g->d_totals, 0, 1))) return rc;
   tm_begin(g, kEvents);
   k_events<<<grid_for(g->nl.n, 256), 256, 0, g->stream>>>(g->nl.n, g->d_first0, g->d_first1, (uint32_t)start,
                                                          (uint32_t)width, g->d_actmap, g->d_act_list, g->d_act_count);
@@ -574,9 +667,10 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
 }
 
 void free_chunk(pdf_grader *g) {
-  dfree(g->d_A);
-  dfree(g->d_B);
+  dfree(g->d_V);
+  dfree(g->d_VB);
   dfree(g->d_M);
+  dfree(g->d_NZ);
   dfree(g->d_list);
   dfree(g->d_list_cnt);
   dfree(g->d_actmap);
@@ -586,16 +680,22 @@ void free_chunk(pdf_grader *g) {
   g->chunk_pairs = 0;
 }
 
-int reset_state(pdf_grader *g) {
-  int rc;
-  if ((rc = fill32(g, g->d_first0, PDF_NONE, g->nl.n))) return rc;
-  if ((rc = fill32(g, g->d_first1, PDF_NONE, g->nl.n))) return rc;
-  if ((rc = fill64(g, g->d_gate, 0, g->nl.n))) return rc;
-  if ((rc = fill64(g, g->d_totals, 0, 2))) return rc;
-  if ((rc = fill32(g, g->d_err, 0, 1))) return rc;
-  HIPCHK(hipStreamSynchronize(g->stream));
+// Queue the reset of the cross-chunk state (no host synchronization).
+int queue_reset(pdf_grader *g) {
+  tm_begin(g, kFill);
+  k_reset_grade<<<grid_for(std::max(g->nl.n, 2u), 256), 256, 0, g->stream>>>(g->nl.n, g->d_first0, g->d_first1,
+                                                                            g->d_gate, g->d_totals, g->d_err);
+  tm_end(g);
+  HIPCHK(hipGetLastError());
   g->stage = 0;
   g->stage_chunk = -1;
+  return PDF_OK;
+}
+
+int reset_state(pdf_grader *g) {
+  int rc = queue_reset(g);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(g->stream));
   return PDF_OK;
 }
 
@@ -616,6 +716,12 @@ struct EventTimer {
     return PDF_OK;
   }
 };
+
+// Copy rows [n][cols] of a pitched device plane to a dense host array.
+int copy_rows(pdf_grader *g, uint64_t *dst, const uint64_t *src, uint64_t cols) {
+  HIPCHK(hipMemcpy2D(dst, cols * 8, src, g->pitch * 8, cols * 8, g->nl.n, hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
 
 }  // namespace
 
@@ -641,7 +747,8 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
     if (rc) return;
     using T = std::remove_reference_t<decltype(vec[0])>;
     rc = dalloc(&dptr, vec.size());
-    if (!rc && !vec.empty() && !hip_ok(hipMemcpy(dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice), "hipMemcpy"))
+    if (!rc && !vec.empty() &&
+        !hip_ok(hipMemcpy(dptr, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice), "hipMemcpy"))
       rc = PDF_ERR_HIP;
   };
   if (!hip_ok(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate") ||
@@ -705,30 +812,39 @@ static int load_vectors(pdf_grader *g, const uint64_t *a, const uint64_t *b, uin
   dfree(g->d_vb);
   free_chunk(g);
   g->num_pairs = 0;
+  const uint64_t I = g->nl.num_pi;
   const uint64_t words = (num + 63) / 64;
   const uint32_t tail = (uint32_t)(num % 64);
-  const size_t bytes = sizeof(uint64_t) * words * g->nl.num_pi;
-  int rc = dalloc(&g->d_va, words * g->nl.num_pi);
-  if (!rc && b) rc = dalloc(&g->d_vb, words * g->nl.num_pi);
-  if (rc) return rc;
-  auto upload = [&](uint64_t *dst, const uint64_t *src) -> int {
-    if (!tail) {
-      HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-      return PDF_OK;
+  g->seq = b == nullptr;
+  g->vstride = g->seq ? words + 1 : words;
+  std::vector<uint64_t> host(I * g->vstride);
+  auto bit = [&](const uint64_t *src, uint64_t i, uint64_t pos) { return (src[i * words + pos / 64] >> (pos % 64)) & 1; };
+  auto stage = [&](const uint64_t *src) {
+    for (uint64_t i = 0; i < I; ++i) {
+      uint64_t *row = host.data() + i * g->vstride;
+      std::copy(src + i * words, src + (i + 1) * words, row);
+      if (tail) row[words - 1] &= (1ULL << tail) - 1;
+      if (g->seq) {
+        // positions past the last vector wrap to vector 0, 1, ...
+        // (simkernel.cuh:116-120 pairs the last vector with vector 0)
+        for (uint64_t pos = num; pos < 64 * (words + 1); ++pos)
+          if (bit(src, i, (pos - num) % num)) row[pos / 64] |= 1ULL << (pos % 64);
+      }
     }
-    // clear the bits past the last pair so padded lanes hold S0 everywhere
-    std::vector<uint64_t> tmp(src, src + words * g->nl.num_pi);
-    for (uint64_t i = 0; i < g->nl.num_pi; ++i) tmp[i * words + words - 1] &= (1ULL << tail) - 1;
-    HIPCHK(hipMemcpy(dst, tmp.data(), bytes, hipMemcpyHostToDevice));
-    return PDF_OK;
   };
-  if ((rc = upload(g->d_va, a))) return rc;
-  if (b && (rc = upload(g->d_vb, b))) return rc;
+  int rc = dalloc(&g->d_va, host.size());
+  if (!rc && b) rc = dalloc(&g->d_vb, host.size());
+  if (rc) return rc;
+  stage(a);
+  HIPCHK(hipMemcpy(g->d_va, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+  if (b) {
+    std::fill(host.begin(), host.end(), 0);
+    stage(b);
+    HIPCHK(hipMemcpy(g->d_vb, host.data(), host.size() * 8, hipMemcpyHostToDevice));
+  }
   g->num_pairs = num;
   g->vwords = words;
-  g->tail = tail;
-  rc = pdf_grader_set_chunk(g, 0);
-  return rc;
+  return pdf_grader_set_chunk(g, 0);
 }
 
 int pdf_grader_set_vectors(pdf_grader *g, const uint64_t *bits, uint64_t num_vectors) {
@@ -749,14 +865,16 @@ int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
   HIPCHK(hipStreamSynchronize(g->stream));
   free_chunk(g);
   const uint64_t words = (chunk_pairs + 63) / 64;
-  g->pitch = (words + 15) / 16 * 16;
+  g->pitch = (words + 1 + 15) / 16 * 16;  // + the sequence-mode extra column
+  g->ntiles = (words + 63) / 64;
   const uint64_t n = g->nl.n;
   // cover scratch: one n-long c/h slice per wave, about 1 GiB at most
   const uint64_t per_wave = 2 * n * sizeof(uint64_t);
   g->cover_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(512, (1ull << 30) / per_wave));
-  int rc = dalloc(&g->d_A, n * g->pitch);
-  if (!rc) rc = dalloc(&g->d_B, n * g->pitch);
+  int rc = dalloc(&g->d_V, n * g->pitch);
+  if (!rc && !g->seq) rc = dalloc(&g->d_VB, n * g->pitch);
   if (!rc) rc = dalloc(&g->d_M, n * g->pitch);
+  if (!rc) rc = dalloc(&g->d_NZ, n * g->ntiles);
   if (!rc) rc = dalloc(&g->d_list, n * words);
   if (!rc) rc = dalloc(&g->d_list_cnt, words);
   if (!rc) rc = dalloc(&g->d_actmap, words);
@@ -829,6 +947,7 @@ int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, float *ms
   if (g->stage_chunk != (int64_t)chunk || g->stage < 2) { set_error("pdf_merge: mark this chunk first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   EventTimer tm(g, ms);
+  if ((rc = reset_chunk(g, width))) return rc;
   if ((rc = launch_reverse<false, true>(g, start, width))) return rc;
   if ((rc = tm.finish())) return rc;
   g->stage = 3;
@@ -841,7 +960,8 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
   uint64_t width;
   int rc = check_chunk(g, chunk, start, &width);
   if (rc) return rc;
-  if (g->stage_chunk != (int64_t)chunk || g->stage < 3) { set_error("pdf_cover: merge this chunk first"); return PDF_ERR_STATE; }
+  // exactly once per merged chunk: a second call would count its pairs again
+  if (g->stage_chunk != (int64_t)chunk || g->stage != 3) { set_error("pdf_cover: merge this chunk first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   EventTimer tm(g, ms);
   if ((rc = launch_cover(g, start, width))) return rc;
@@ -860,21 +980,22 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
   if (!g->chunk_pairs) { set_error("pdf_grade: load vectors first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
-  int rc = reset_state(g);
+  int rc = queue_reset(g);
   if (rc) return rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   for (uint64_t c = 0; c < nchunks; ++c) {
     const uint64_t start = c * g->chunk_pairs;
     const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
     if ((rc = launch_fwd<true, true>(g, start, width, robust))) return rc;
+    if ((rc = reset_chunk(g, width))) return rc;
     if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
     if ((rc = launch_cover(g, start, width))) return rc;
   }
-  HIPCHK(hipStreamSynchronize(g->stream));
   uint64_t tot[2];
   uint32_t err = 0;
-  HIPCHK(hipMemcpy(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
@@ -912,24 +1033,46 @@ int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times
   return PDF_OK;
 }
 
-int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane, uint64_t *b_plane) {
+int pdf_get_sim_planes(const pdf_grader *gc, uint64_t *a_plane, uint64_t *b_plane) {
+  pdf_grader *g = const_cast<pdf_grader *>(gc);
   if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
   if (g->stage < 1) { set_error("nothing simulated yet"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
-  const uint64_t words = (width + 63) / 64;
-  if (a_plane) HIPCHK(hipMemcpy2D(a_plane, words * 8, g->d_A, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
-  if (b_plane) HIPCHK(hipMemcpy2D(b_plane, words * 8, g->d_B, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
+  const uint64_t words = (width + 63) / 64, n = g->nl.n;
+  std::vector<uint64_t> v(n * (words + 1)), vb;
+  int rc = copy_rows(g, v.data(), g->d_V, words + 1);
+  if (rc) return rc;
+  if (!g->seq) {
+    vb.resize(n * words);
+    if ((rc = copy_rows(g, vb.data(), g->d_VB, words))) return rc;
+  }
+  const uint64_t last = width % 64 ? (1ULL << (width % 64)) - 1 : ~0ULL;
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t k = 0; k < words; ++k) {
+      const uint64_t a = v[i * (words + 1) + k];
+      uint64_t b = g->seq ? ((a >> 1) | (v[i * (words + 1) + k + 1] << 63)) : vb[i * words + k];
+      if (k + 1 == words) b = (b & last) | (a & ~last);  // lanes past the last pair read as stable
+      if (a_plane) a_plane[i * words + k] = a;
+      if (b_plane) b_plane[i * words + k] = b;
+    }
   return PDF_OK;
 }
 
-int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane) {
+int pdf_get_mark_plane(const pdf_grader *gc, uint64_t *mark_plane) {
+  pdf_grader *g = const_cast<pdf_grader *>(gc);
   if (!g || !mark_plane) { set_error("null argument"); return PDF_ERR_ARG; }
   if (g->stage < 2) { set_error("nothing marked yet"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
-  const uint64_t words = (width + 63) / 64;
-  HIPCHK(hipMemcpy2D(mark_plane, words * 8, g->d_M, g->pitch * 8, words * 8, g->nl.n, hipMemcpyDeviceToHost));
+  const uint64_t words = (width + 63) / 64, n = g->nl.n;
+  std::vector<uint64_t> nz(n * g->ntiles);
+  int rc = copy_rows(g, mark_plane, g->d_M, words);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(nz.data(), g->d_NZ, nz.size() * 8, hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t k = 0; k < words; ++k)
+      if (!((nz[i * g->ntiles + k / 64] >> (k % 64)) & 1)) mark_plane[i * words + k] = 0;
   return PDF_OK;
 }
 

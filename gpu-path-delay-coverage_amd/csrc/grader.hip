@@ -48,7 +48,19 @@ constexpr int kBlock = 256;
 // the array overwrites unrelated live registers.
 constexpr int kRegFanin = 4;
 
+// Per-node record read with one 32-byte load (a scalar load when the node is
+// the same for the whole wave): type, PO flag, the first four fanins inline and
+// the CSR offsets of the full fanin and fanout lists.
+struct alignas(32) GateRec {
+  uint32_t fin[4];  // first four fanins (0 past nfi)
+  uint32_t fin_off, fout_off;
+  uint16_t nfi, nfo;
+  uint8_t type, po, pad0, pad1;
+};
+static_assert(sizeof(GateRec) == 32, "GateRec is one 32-byte load");
+
 struct DevNet {
+  const GateRec *rec;
   const uint8_t *type;
   const uint8_t *po;
   const uint32_t *fin_off;
@@ -61,6 +73,7 @@ struct DevNet {
 
 struct Planes {
   uint64_t *V, *VB, *M, *NZ;
+  uint64_t *ACC;       // removal accumulator: OR of fanouts' final marks
   uint64_t pitch;      // words per plane row
   uint32_t words;      // pair words in this chunk
   uint32_t ntiles;     // NZ words per node
@@ -165,16 +178,18 @@ __device__ __forceinline__ void fold(uint8_t ty, uint64_t &v, uint64_t x) {
 }
 
 // One frame of one node at word w (used for the extra sequence column).
-__device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Planes &pl, uint32_t g, uint32_t w,
-                                                  uint8_t ty) {
-  const uint32_t f0 = net.fin_off[g];
-  const uint32_t nfi = net.fin_off[g + 1] - f0;
-  const uint32_t *fins = net.fin + f0;
-  uint64_t v = pl.V[(uint64_t)fins[0] * pl.pitch + w];
+__device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Planes &pl, const GateRec &r,
+                                                  uint32_t w) {
+  const uint8_t ty = r.type;
+  uint64_t v = pl.V[(uint64_t)r.fin[0] * pl.pitch + w];
   if (ty == PDF_NOT) v = ~v;
-  if (nfi > 1) {
+  if (r.nfi > 1) {
     if (ty >= PDF_AND && ty <= PDF_NOR) {
-      for (uint32_t j = 1; j < nfi; ++j) fold(ty, v, pl.V[(uint64_t)fins[j] * pl.pitch + w]);
+#pragma unroll
+      for (int j = 1; j < kRegFanin; ++j)
+        if ((uint32_t)j < r.nfi) fold(ty, v, pl.V[(uint64_t)r.fin[j] * pl.pitch + w]);
+      for (uint32_t j = kRegFanin; j < r.nfi; ++j)
+        fold(ty, v, pl.V[(uint64_t)net.fin[r.fin_off + j] * pl.pitch + w]);
     } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
       v = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
     }
@@ -184,9 +199,11 @@ __device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Plane
 
 // Forward pass over one level: simulation (DO_SIM) and/or forward marking
 // (DO_MARK).  SEQ: consecutive-vector pairs from one value plane; otherwise
-// explicit pairs with a second plane.
-template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST>
-__global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uint64_t *__restrict__ va,
+// explicit pairs with a second plane.  WIDE: 64-word tiles, so each wave
+// works on one node and its record is read with scalar loads.
+template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                const uint64_t *__restrict__ va,
                                                 const uint64_t *__restrict__ vb, uint64_t vstride, uint64_t wbase,
                                                 LevelMap mp) {
   if (SEQ && DO_SIM && blockIdx.x >= mp.main_blocks) {
@@ -194,16 +211,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
     const uint32_t gi = (blockIdx.x - mp.main_blocks) * kBlock + threadIdx.x;
     if (gi >= mp.count) return;
     const uint32_t g = mp.lo + gi;
-    const uint8_t ty = net.type[g];
-    const uint64_t v = ty == PDF_INPT ? va[(uint64_t)g * vstride + wbase + mp.words]
-                                      : sim_one_frame(net, pl, g, mp.words, ty);
+    const GateRec r = recs[g];
+    const uint64_t v =
+        r.type == PDF_INPT ? va[(uint64_t)g * vstride + wbase + mp.words] : sim_one_frame(net, pl, r, mp.words);
     pl.V[(uint64_t)g * pl.pitch + mp.words] = v;
     return;
   }
   uint32_t g, w;
   if (!map_item(mp, g, w)) return;
+  if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
   const uint64_t row = (uint64_t)g * pl.pitch + w;
-  const uint8_t ty = net.type[g];
+  const GateRec r = recs[g];
+  const uint8_t ty = r.type;
   uint64_t a, b = 0, m = 0;
   if (ty == PDF_INPT) {
     // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P])
@@ -224,18 +243,14 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
     // its fanin 0 marked, so this keeps them clear everywhere
     if (DO_MARK) m = (a ^ b) & (w + 1 == mp.words ? pl.last_mask : ~0ULL);
   } else {
-    const uint32_t f0 = net.fin_off[g];
-    const uint32_t nfi = net.fin_off[g + 1] - f0;
-    const uint32_t *fins = net.fin + f0;
+    const uint32_t nfi = r.nfi;
+    const uint32_t *more = net.fin + r.fin_off;  // fanins from kRegFanin on
     const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
-    uint64_t fa[kRegFanin], fb[kRegFanin];  // constant indices only
+    constexpr bool kBoth = DO_MARK || !SEQ;  // frame 2 of the fanins needed
+    uint64_t fa[kRegFanin], fb[kRegFanin];   // constant indices only
     if (DO_SIM) {
-      if (DO_MARK || !SEQ) {
-        frames<SEQ>(pl, fins[0], w, fa[0], fb[0]);
-      } else {
-        fa[0] = pl.V[(uint64_t)fins[0] * pl.pitch + w];
-        fb[0] = 0;
-      }
+      if (kBoth) frames<SEQ>(pl, r.fin[0], w, fa[0], fb[0]);
+      else { fa[0] = pl.V[(uint64_t)r.fin[0] * pl.pitch + w]; fb[0] = 0; }
       a = fa[0];
       b = fb[0];
       if (ty == PDF_NOT) { a = ~a; b = ~b; }
@@ -244,20 +259,16 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
 #pragma unroll
           for (int j = 1; j < kRegFanin; ++j) {
             if ((uint32_t)j < nfi) {
-              if (DO_MARK || !SEQ) {
-                frames<SEQ>(pl, fins[j], w, fa[j], fb[j]);
-              } else {
-                fa[j] = pl.V[(uint64_t)fins[j] * pl.pitch + w];
-                fb[j] = 0;
-              }
+              if (kBoth) frames<SEQ>(pl, r.fin[j], w, fa[j], fb[j]);
+              else { fa[j] = pl.V[(uint64_t)r.fin[j] * pl.pitch + w]; fb[j] = 0; }
               fold(ty, a, fa[j]);
               fold(ty, b, fb[j]);
             }
           }
           for (uint32_t j = kRegFanin; j < nfi; ++j) {
             uint64_t xa, xb = 0;
-            if (DO_MARK || !SEQ) frames<SEQ>(pl, fins[j], w, xa, xb);
-            else xa = pl.V[(uint64_t)fins[j] * pl.pitch + w];
+            if (kBoth) frames<SEQ>(pl, more[j], w, xa, xb);
+            else xa = pl.V[(uint64_t)more[j] * pl.pitch + w];
             fold(ty, a, xa);
             fold(ty, b, xb);
           }
@@ -274,28 +285,28 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
     if (DO_MARK) {
       const uint64_t tr = a ^ b;
       if (tr) {
-        const uint64_t m0 = mark_of(pl, fins[0], w);
+        const uint64_t m0 = mark_of(pl, r.fin[0], w);
         if (ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT) {
           m = m0 & tr;
         } else if (ty >= PDF_AND && ty <= PDF_XNOR) {
           if (nfi == 1) {
             m = m0 & tr;  // empty AND over off-path inputs
           } else if (multi && (m0 & tr)) {
-            uint64_t r;
+            const bool andlike = ty == PDF_AND || ty == PDF_NAND;
+            uint64_t rr;
             if (nfi <= kRegFanin) {
               if (!DO_SIM) {
 #pragma unroll
                 for (int j = 0; j < kRegFanin; ++j)
-                  if ((uint32_t)j < nfi) frames<SEQ>(pl, fins[j], w, fa[j], fb[j]);
+                  if ((uint32_t)j < nfi) frames<SEQ>(pl, r.fin[j], w, fa[j], fb[j]);
               }
-              const bool andlike = ty == PDF_AND || ty == PDF_NAND;
-              r = andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+              rr = andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
             } else {
-              r = (ty == PDF_AND || ty == PDF_NAND) ? side_ok_mem<SEQ, ROBUST, true>(pl, fins, nfi, w)
-                                                    : side_ok_mem<SEQ, ROBUST, false>(pl, fins, nfi, w);
+              rr = andlike ? side_ok_mem<SEQ, ROBUST, true>(pl, more, nfi, w)
+                           : side_ok_mem<SEQ, ROBUST, false>(pl, more, nfi, w);
             }
             // markkernel.cuh:253-261: the on-path mark is always fanin 0's
-            m = r & m0 & tr;
+            m = rr & m0 & tr;
           }
           // XOR/XNOR with >= 2 fanins never sensitize (LUT entries 0)
         }
@@ -316,25 +327,41 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, Planes pl, const uin
 }
 
 // Reverse pass over one level.  DO_REMOVE: m &= po | OR_fanouts m
-// (markkernel.cuh:279-291).  DO_MERGE: earliest marked T0/T1 pair per node
+// (markkernel.cuh:279-291), gathered by the fanouts: every node whose final
+// mark is non-zero ORs it into ACC of each fanin marked at the same word
+// (levels run in descending order, so all fanouts of a node have pushed
+// before its level runs); the reader clears ACC again, so it is all zero
+// between passes.  DO_MERGE: earliest marked T0/T1 pair per node
 // (mergekernel.cuh:141-197) and the per-word list of marked nodes.
-template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
-__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, Planes pl, LevelMap mp, uint32_t start,
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                    LevelMap mp, uint32_t start,
                                                     uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
                                                     uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
                                                     uint64_t list_stride) {
   uint32_t g, w;
   if (!map_item(mp, g, w)) return;
+  if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
   uint64_t m = mark_of(pl, g, w);
   if (!m) return;
   const uint64_t row = (uint64_t)g * pl.pitch + w;
-  if (DO_REMOVE && !net.po[g]) {
-    uint64_t r = 0;
-    const uint32_t e = net.fout_off[g + 1];
-    for (uint32_t k = net.fout_off[g]; k < e && (m & ~r); ++k) r |= mark_of(pl, net.fout[k], w);
-    if (m & ~r) {
-      m &= r;
+  const GateRec r = recs[g];
+  if (DO_REMOVE) {
+    const uint64_t acc = pl.ACC[row];
+    if (acc) pl.ACC[row] = 0;
+    if (!r.po && (m & ~acc)) {
+      m &= acc;
       pl.M[row] = m;
+    }
+    if (m) {
+      auto push = [&](uint32_t f) {
+        const uint64_t mf = mark_of(pl, f, w) & m;
+        if (mf) atomicOr((unsigned long long *)&pl.ACC[(uint64_t)f * pl.pitch + w], (unsigned long long)mf);
+      };
+#pragma unroll
+      for (int j = 0; j < kRegFanin; ++j)
+        if ((uint32_t)j < r.nfi) push(r.fin[j]);
+      for (uint32_t j = kRegFanin; j < r.nfi; ++j) push(net.fin[r.fin_off + j]);
     }
   }
   if (DO_MERGE && m) {
@@ -400,12 +427,21 @@ __device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
   return o;
 }
 
+constexpr uint32_t kCoverCap = 1024;   // nodes marked at one pair kept in LDS
+constexpr uint32_t kHashSlots = 2048;  // open-addressing node -> LDS index
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 2654435761u) >> 21; }  // 11 bits
+
 // Reverse path-count DP (coverkernel.cuh:194-228) for each distinct
-// first-detection pattern p of the chunk, over the nodes marked at p.  One
-// wave per pattern; the word's node list is in descending level order, so a
-// run of equal levels is processed lane-parallel.  c/h per node live in this
-// wave's slice of cs/hs and are read only for fanouts marked at p, all of
-// which were written earlier in the same walk over the list.
+// first-detection pair p of the chunk, over the nodes marked at p.  One wave
+// per pair.  The word's node list (descending level order) is filtered to the
+// nodes marked at p and kept in LDS with their c/h and a hash from node id to
+// LDS index; runs of equal level are then processed lane-parallel, each node
+// summing the c/h of its fanouts found in the hash (a fanout marked at p has a
+// higher level, so it was finished in an earlier run).  If more than
+// kCoverCap nodes are marked at p the DP keeps c/h in this wave's slice of
+// cs/hs instead.
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                               const uint32_t *__restrict__ act_list,
@@ -414,7 +450,12 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ totals,
                                               uint32_t *__restrict__ err) {
+  __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap];
+  __shared__ uint64_t s_c[kCoverCap], s_h[kCoverCap];
+  __shared__ uint32_t h_key[kHashSlots];
+  __shared__ uint16_t h_val[kHashSlots];
   const uint32_t lane = threadIdx.x;
+  const uint64_t below = (1ULL << lane) - 1;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
   uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
   const uint32_t nact = *act_count;
@@ -424,22 +465,59 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
     const uint32_t p = start + rel;
     const uint32_t len = list_cnt[w];
     const uint32_t *L = list + (uint64_t)w * list_stride;
+    // 1. the nodes marked at p, in list order
+    for (uint32_t t = lane; t < kHashSlots; t += 64) h_key[t] = kEmpty;
+    uint32_t K = 0;
+    for (uint32_t base = 0; base < len; base += 64) {
+      const uint32_t idx = base + lane;
+      const uint32_t g = idx < len ? L[idx] : 0;
+      const bool marked = idx < len && ((mark_of(pl, g, w) >> j) & 1);
+      const uint64_t bal = __ballot(marked);
+      const uint32_t pos = K + (uint32_t)__builtin_popcountll(bal & below);
+      if (marked && pos < kCoverCap) {
+        s_node[pos] = g;
+        s_level[pos] = net.level[g];
+      }
+      K += (uint32_t)__builtin_popcountll(bal);
+    }
+    __syncthreads();
+    const bool in_lds = K <= kCoverCap;
+    if (in_lds) {
+      for (uint32_t idx = lane; idx < K; idx += 64) {
+        const uint32_t g = s_node[idx];
+        uint32_t h = hash_slot(g);
+        while (atomicCAS(&h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
+        h_val[h] = (uint16_t)idx;
+      }
+    }
+    __syncthreads();
     uint64_t tot = 0;
     bool ovf = false;
-    for (uint32_t i = 0; i < len;) {
+    // 2. the DP over runs of equal level
+    const uint32_t steps = in_lds ? K : len;
+    for (uint32_t i = 0; i < steps;) {
       const uint32_t idx = i + lane;
-      const bool valid = idx < len;
-      const uint32_t g = valid ? L[idx] : 0;
-      const uint32_t lv = valid ? net.level[g] : 0xFFFFFFFFu;
+      const bool valid = idx < steps;
+      const uint32_t g = valid ? (in_lds ? s_node[idx] : L[idx]) : 0;
+      const uint32_t lv = valid ? (in_lds ? s_level[idx] : net.level[g]) : 0xFFFFFFFFu;
       const uint32_t lv0 = __shfl(lv, 0);
       const uint64_t same = __ballot(valid && lv == lv0);
       // the run is a contiguous prefix of the lanes
       const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
-      if (lane < run && ((mark_of(pl, g, w) >> j) & 1)) {
-        uint64_t C = 0, H = net.po[g] == 1 ? 1 : 0;
-        for (uint32_t e = net.fout_off[g]; e < net.fout_off[g + 1]; ++e) {
-          const uint32_t fo = net.fout[e];
-          if ((mark_of(pl, fo, w) >> j) & 1) {
+      if (lane < run && (in_lds || ((mark_of(pl, g, w) >> j) & 1))) {
+        const GateRec r = net.rec[g];
+        uint64_t C = 0, H = r.po == 1 ? 1 : 0;
+        for (uint32_t e = 0; e < r.nfo; ++e) {
+          const uint32_t fo = net.fout[r.fout_off + e];
+          if (in_lds) {
+            uint32_t h = hash_slot(fo);
+            uint32_t key;
+            while ((key = h_key[h]) != kEmpty && key != fo) h = (h + 1) & (kHashSlots - 1);
+            if (key == fo) {
+              ovf |= add_ovf(C, s_c[h_val[h]]);
+              ovf |= add_ovf(H, s_h[h_val[h]]);
+            }
+          } else if ((mark_of(pl, fo, w) >> j) & 1) {
             ovf |= add_ovf(C, c_s[fo]);
             ovf |= add_ovf(H, h_s[fo]);
           }
@@ -447,16 +525,22 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
         const bool is_new = first_t0[g] == p || first_t1[g] == p;
         uint64_t c = C, h = H;
         if (is_new) { ovf |= add_ovf(c, H); h = 0; }
-        if (net.type[g] == PDF_INPT) h = 0;
-        c_s[g] = c;
-        h_s[g] = h;
+        if (r.type == PDF_INPT) h = 0;
+        if (in_lds) {
+          s_c[idx] = c;
+          s_h[idx] = h;
+        } else {
+          c_s[g] = c;
+          h_s[g] = h;
+        }
         if (c) {
           const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
           ovf |= old + c < old;
-          if (net.type[g] == PDF_INPT) ovf |= add_ovf(tot, c);
+          if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
         }
       }
       __threadfence_block();
+      __syncthreads();
       i += run;
     }
     // wave sum of the PI counts of this pattern (kernSumSingle)
@@ -470,6 +554,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       ovf |= (o0 + tot < o0) || (o1 + tot < o1);
     }
     if (__ballot(ovf)) { if (lane == 0) atomicOr(err, 1u); }
+    __syncthreads();
   }
 }
 
@@ -483,6 +568,7 @@ struct pdf_grader {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Netlist nl;
   // netlist in HBM
+  GateRec *d_rec = nullptr;
   uint8_t *d_type = nullptr, *d_po = nullptr;
   uint32_t *d_fin_off = nullptr, *d_fin = nullptr, *d_fout_off = nullptr, *d_fout = nullptr, *d_level = nullptr;
   // vectors: sequence mode keeps vwords + 1 words per PI row (positions
@@ -492,7 +578,7 @@ struct pdf_grader {
   uint64_t *d_va = nullptr, *d_vb = nullptr;
   // chunk planes
   uint64_t chunk_pairs = 0, pitch = 0, ntiles = 0;
-  uint64_t *d_V = nullptr, *d_VB = nullptr, *d_M = nullptr, *d_NZ = nullptr;
+  uint64_t *d_V = nullptr, *d_VB = nullptr, *d_M = nullptr, *d_NZ = nullptr, *d_ACC = nullptr;
   // merge / cover state
   uint32_t *d_first0 = nullptr, *d_first1 = nullptr;
   uint64_t *d_gate = nullptr;
@@ -512,7 +598,7 @@ struct pdf_grader {
   int stage = 0;  // 1 = simulated, 2 = marked, 3 = merged, 4 = covered
   int stage_robust = 0;
 
-  DevNet net() const { return DevNet{d_type, d_po, d_fin_off, d_fin, d_fout_off, d_fout, d_level, nl.n}; }
+  DevNet net() const { return DevNet{d_rec, d_type, d_po, d_fin_off, d_fin, d_fout_off, d_fout, d_level, nl.n}; }
 };
 
 namespace {
@@ -579,8 +665,8 @@ int check_chunk(pdf_grader *g, uint64_t chunk, uint64_t start, uint64_t *width) 
 
 Planes planes(pdf_grader *g, uint64_t width) {
   const uint32_t tail = (uint32_t)(width % 64);
-  return Planes{g->d_V, g->d_VB, g->d_M, g->d_NZ, g->pitch, (uint32_t)((width + 63) / 64), (uint32_t)g->ntiles,
-                tail ? (1ULL << tail) - 1 : ~0ULL};
+  return Planes{g->d_V,  g->d_VB, g->d_M, g->d_NZ, g->d_ACC, g->pitch, (uint32_t)((width + 63) / 64),
+                (uint32_t)g->ntiles, tail ? (1ULL << tail) - 1 : ~0ULL};
 }
 
 LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, bool extra_column, uint32_t *grid) {
@@ -602,10 +688,15 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
     const LevelMap mp =
         level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], SEQ && S, &grid);
     tm_begin(g, kFwd);
-    if (robust)
-      k_fwd<SEQ, S, M, true><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
+    const bool wide = mp.twlog == 6;
+    if (robust && wide)
+      k_fwd<SEQ, S, M, true, true><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
+    else if (robust)
+      k_fwd<SEQ, S, M, true, false><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
+    else if (wide)
+      k_fwd<SEQ, S, M, false, true><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
     else
-      k_fwd<SEQ, S, M, false><<<grid, kBlock, 0, g->stream>>>(net, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
+      k_fwd<SEQ, S, M, false, false><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
     tm_end(g);
   }
 }
@@ -627,8 +718,12 @@ void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width) {
     const LevelMap mp =
         level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], false, &grid);
     tm_begin(g, kRev);
-    k_reverse<SEQ, R, M><<<grid, kBlock, 0, g->stream>>>(net, pl, mp, (uint32_t)start, g->d_first0, g->d_first1,
-                                                         g->d_list, g->d_list_cnt, g->nl.n);
+    if (mp.twlog == 6)
+      k_reverse<SEQ, R, M, true><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, mp, (uint32_t)start, g->d_first0,
+                                                                 g->d_first1, g->d_list, g->d_list_cnt, g->nl.n);
+    else
+      k_reverse<SEQ, R, M, false><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, mp, (uint32_t)start, g->d_first0,
+                                                                  g->d_first1, g->d_list, g->d_list_cnt, g->nl.n);
     tm_end(g);
   }
 }
@@ -671,6 +766,7 @@ void free_chunk(pdf_grader *g) {
   dfree(g->d_VB);
   dfree(g->d_M);
   dfree(g->d_NZ);
+  dfree(g->d_ACC);
   dfree(g->d_list);
   dfree(g->d_list_cnt);
   dfree(g->d_actmap);
@@ -756,6 +852,19 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
     pdf_grader_destroy(g);
     return PDF_ERR_HIP;
   }
+  std::vector<GateRec> rec(nl.n);
+  for (uint32_t i = 0; i < nl.n; ++i) {
+    GateRec &r = rec[i];
+    std::memset(&r, 0, sizeof(r));
+    r.fin_off = nl.fin_off[i];
+    r.fout_off = nl.fout_off[i];
+    r.nfi = (uint16_t)(nl.fin_off[i + 1] - nl.fin_off[i]);
+    r.nfo = (uint16_t)(nl.fout_off[i + 1] - nl.fout_off[i]);
+    for (uint32_t j = 0; j < r.nfi && j < (uint32_t)kRegFanin; ++j) r.fin[j] = nl.fin[nl.fin_off[i] + j];
+    r.type = nl.type[i];
+    r.po = nl.po[i];
+  }
+  up(g->d_rec, rec);
   up(g->d_type, nl.type);
   up(g->d_po, nl.po);
   up(g->d_fin_off, nl.fin_off);
@@ -780,6 +889,7 @@ void pdf_grader_destroy(pdf_grader *g) {
   (void)hipSetDevice(g->device);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   free_chunk(g);
+  dfree(g->d_rec);
   dfree(g->d_type);
   dfree(g->d_po);
   dfree(g->d_fin_off);
@@ -875,6 +985,9 @@ int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
   if (!rc && !g->seq) rc = dalloc(&g->d_VB, n * g->pitch);
   if (!rc) rc = dalloc(&g->d_M, n * g->pitch);
   if (!rc) rc = dalloc(&g->d_NZ, n * g->ntiles);
+  if (!rc) rc = dalloc(&g->d_ACC, n * g->pitch);
+  // the removal accumulator is zero between passes (each reader clears it)
+  if (!rc && !hip_ok(hipMemset(g->d_ACC, 0, n * g->pitch * 8), "hipMemset")) rc = PDF_ERR_HIP;
   if (!rc) rc = dalloc(&g->d_list, n * words);
   if (!rc) rc = dalloc(&g->d_list_cnt, words);
   if (!rc) rc = dalloc(&g->d_actmap, words);

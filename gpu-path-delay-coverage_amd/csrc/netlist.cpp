@@ -60,6 +60,7 @@ int finalize_netlist(Netlist &nl, std::string &err) {
       continue;
     }
     if (nfi == 0) { err = "gate " + std::to_string(g) + " has no fanins"; return PDF_ERR_NETLIST; }
+    if (nfi > 65535) { err = "gate " + std::to_string(g) + " has more than 65535 fanins"; return PDF_ERR_NETLIST; }
     uint32_t lv = 0;
     for (uint32_t k = nl.fin_off[g]; k < nl.fin_off[g + 1]; ++k) {
       const uint32_t f = nl.fin[k];
@@ -86,6 +87,11 @@ int finalize_netlist(Netlist &nl, std::string &err) {
   // fanouts, sorted by fanout id (CSR transpose in id order)
   nl.fout_off.assign(n + 1, 0);
   for (uint32_t f : nl.fin) nl.fout_off[f + 1]++;
+  for (uint32_t g = 0; g < n; ++g)
+    if (nl.fout_off[g + 1] > 65535) {
+      err = "node " + std::to_string(g) + " has more than 65535 fanouts";
+      return PDF_ERR_NETLIST;
+    }
   for (uint32_t g = 0; g < n; ++g) nl.fout_off[g + 1] += nl.fout_off[g];
   nl.fout.assign(nl.fin.size(), 0);
   std::vector<uint32_t> fill(nl.fout_off.begin(), nl.fout_off.end() - 1);

@@ -82,8 +82,9 @@ typedef struct pdf_netlist pdf_netlist;
 
 /* Caller-supplied CSR view of a levelized netlist.  Validation rejects: ids
  * not sorted by level, level(g) != 1 + max level(fanin) for gates, INPT not
- * exactly 0..num_pi-1, gates without fanins, duplicate fanins, fanin or
- * fanout lists that disagree, more than 65535 fanins or fanouts on one node. */
+ * exactly 0..num_pi-1, gates without fanins, fanins that are not earlier
+ * nodes, empty levels, more than 65535 fanins or fanouts on one node.
+ * Duplicate fanins are allowed (each counts as its own edge). */
 typedef struct pdf_netlist_desc {
   uint32_t n;               /* nodes */
   uint32_t num_pi;          /* INPT nodes = ids 0..num_pi-1 */

@@ -247,6 +247,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
     const uint32_t *more = net.fin + r.fin_off;  // fanins from kRegFanin on
     const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
     constexpr bool kBoth = DO_MARK || !SEQ;  // frame 2 of the fanins needed
+    // issued with the value loads, so a marked fanin 0 costs one more round
+    // trip (its mark word) instead of two
+    const uint64_t nz0 = DO_MARK ? pl.NZ[(uint64_t)r.fin[0] * pl.ntiles + (w >> 6)] : 0;
     uint64_t fa[kRegFanin], fb[kRegFanin];   // constant indices only
     if (DO_SIM) {
       if (kBoth) frames<SEQ>(pl, r.fin[0], w, fa[0], fb[0]);
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
     if (DO_MARK) {
       const uint64_t tr = a ^ b;
       if (tr) {
-        const uint64_t m0 = mark_of(pl, r.fin[0], w);
+        const uint64_t m0 = ((nz0 >> (w & 63)) & 1) ? pl.M[(uint64_t)r.fin[0] * pl.pitch + w] : 0;
         if (ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT) {
           m = m0 & tr;
         } else if (ty >= PDF_AND && ty <= PDF_XNOR) {
@@ -342,31 +345,47 @@ __global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *_
   uint32_t g, w;
   if (!map_item(mp, g, w)) return;
   if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
-  uint64_t m = mark_of(pl, g, w);
-  if (!m) return;
-  const uint64_t row = (uint64_t)g * pl.pitch + w;
+  // round 1: this node's mark-present word and record
+  const uint64_t nzw = pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)];
   const GateRec r = recs[g];
+  if (!((nzw >> (w & 63)) & 1)) return;
+  const uint64_t row = (uint64_t)g * pl.pitch + w;
+  // round 2: mark, accumulator, the fanins' mark-present words and, for the
+  // merge, both frames
+  uint64_t m = pl.M[row];
+  const uint64_t acc = DO_REMOVE ? pl.ACC[row] : 0;
+  uint64_t nzf[kRegFanin];
   if (DO_REMOVE) {
-    const uint64_t acc = pl.ACC[row];
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j)
+      nzf[j] = (uint32_t)j < r.nfi ? pl.NZ[(uint64_t)r.fin[j] * pl.ntiles + (w >> 6)] : 0;
+  }
+  uint64_t a = 0, b = 0;
+  if (DO_MERGE) frames<SEQ>(pl, g, w, a, b);
+  if (!m) return;
+  if (DO_REMOVE) {
     if (acc) pl.ACC[row] = 0;
     if (!r.po && (m & ~acc)) {
       m &= acc;
       pl.M[row] = m;
     }
     if (m) {
-      auto push = [&](uint32_t f) {
-        const uint64_t mf = mark_of(pl, f, w) & m;
+      // round 3: marks of the fanins present in this word, then the pushes
+      auto push = [&](uint32_t f, uint64_t nz) {
+        if (!((nz >> (w & 63)) & 1)) return;
+        const uint64_t mf = pl.M[(uint64_t)f * pl.pitch + w] & m;
         if (mf) atomicOr((unsigned long long *)&pl.ACC[(uint64_t)f * pl.pitch + w], (unsigned long long)mf);
       };
 #pragma unroll
       for (int j = 0; j < kRegFanin; ++j)
-        if ((uint32_t)j < r.nfi) push(r.fin[j]);
-      for (uint32_t j = kRegFanin; j < r.nfi; ++j) push(net.fin[r.fin_off + j]);
+        if ((uint32_t)j < r.nfi) push(r.fin[j], nzf[j]);
+      for (uint32_t j = kRegFanin; j < r.nfi; ++j) {
+        const uint32_t f = net.fin[r.fin_off + j];
+        push(f, pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)]);
+      }
     }
   }
   if (DO_MERGE && m) {
-    uint64_t a, b;
-    frames<SEQ>(pl, g, w, a, b);
     const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
     const uint32_t base = start + 64u * w;
     if (t0) atomicMin(&first_t0[g], base + (uint32_t)__builtin_ctzll(t0));
@@ -427,21 +446,22 @@ __device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
   return o;
 }
 
-constexpr uint32_t kCoverCap = 1024;   // nodes marked at one pair kept in LDS
-constexpr uint32_t kHashSlots = 2048;  // open-addressing node -> LDS index
+constexpr uint32_t kCoverCap = 512;    // nodes marked at one pair kept in LDS
+constexpr uint32_t kHashSlots = 1024;  // open-addressing node -> LDS index
+constexpr uint32_t kPool = 4096;       // fanout ids of those nodes kept in LDS
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 2654435761u) >> 21; }  // 11 bits
+__device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 2654435761u) >> 22; }  // 10 bits
 
 // Reverse path-count DP (coverkernel.cuh:194-228) for each distinct
 // first-detection pair p of the chunk, over the nodes marked at p.  One wave
 // per pair.  The word's node list (descending level order) is filtered to the
-// nodes marked at p and kept in LDS with their c/h and a hash from node id to
-// LDS index; runs of equal level are then processed lane-parallel, each node
-// summing the c/h of its fanouts found in the hash (a fanout marked at p has a
-// higher level, so it was finished in an earlier run).  If more than
-// kCoverCap nodes are marked at p the DP keeps c/h in this wave's slice of
-// cs/hs instead.
+// nodes marked at p; their records, fanout ids and c/h live in LDS with a
+// hash from node id to LDS index.  Runs of equal level are then processed
+// lane-parallel, each node summing the c/h of its fanouts found in the hash
+// (a fanout marked at p has a higher level, so it was finished in an earlier
+// run).  If more than kCoverCap nodes are marked at p the DP keeps c/h in
+// this wave's slice of cs/hs and reads everything from global memory.
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                               const uint32_t *__restrict__ act_list,
@@ -450,10 +470,13 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ totals,
                                               uint32_t *__restrict__ err) {
-  __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap];
+  __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap];
+  __shared__ uint16_t s_nfo[kCoverCap];
+  __shared__ uint8_t s_flags[kCoverCap];  // bit0 po, bit1 INPT, bit2 new
   __shared__ uint64_t s_c[kCoverCap], s_h[kCoverCap];
   __shared__ uint32_t h_key[kHashSlots];
   __shared__ uint16_t h_val[kHashSlots];
+  __shared__ uint32_t s_pool[kPool];
   const uint32_t lane = threadIdx.x;
   const uint64_t below = (1ULL << lane) - 1;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
@@ -474,42 +497,65 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       const bool marked = idx < len && ((mark_of(pl, g, w) >> j) & 1);
       const uint64_t bal = __ballot(marked);
       const uint32_t pos = K + (uint32_t)__builtin_popcountll(bal & below);
-      if (marked && pos < kCoverCap) {
-        s_node[pos] = g;
-        s_level[pos] = net.level[g];
-      }
+      if (marked && pos < kCoverCap) s_node[pos] = g;
       K += (uint32_t)__builtin_popcountll(bal);
     }
     __syncthreads();
     const bool in_lds = K <= kCoverCap;
-    if (in_lds) {
-      for (uint32_t idx = lane; idx < K; idx += 64) {
-        const uint32_t g = s_node[idx];
-        uint32_t h = hash_slot(g);
-        while (atomicCAS(&h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
-        h_val[h] = (uint16_t)idx;
-      }
-    }
-    __syncthreads();
     uint64_t tot = 0;
     bool ovf = false;
-    // 2. the DP over runs of equal level
-    const uint32_t steps = in_lds ? K : len;
-    for (uint32_t i = 0; i < steps;) {
-      const uint32_t idx = i + lane;
-      const bool valid = idx < steps;
-      const uint32_t g = valid ? (in_lds ? s_node[idx] : L[idx]) : 0;
-      const uint32_t lv = valid ? (in_lds ? s_level[idx] : net.level[g]) : 0xFFFFFFFFu;
-      const uint32_t lv0 = __shfl(lv, 0);
-      const uint64_t same = __ballot(valid && lv == lv0);
-      // the run is a contiguous prefix of the lanes
-      const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
-      if (lane < run && (in_lds || ((mark_of(pl, g, w) >> j) & 1))) {
-        const GateRec r = net.rec[g];
-        uint64_t C = 0, H = r.po == 1 ? 1 : 0;
-        for (uint32_t e = 0; e < r.nfo; ++e) {
-          const uint32_t fo = net.fout[r.fout_off + e];
-          if (in_lds) {
+    if (in_lds) {
+      // 2. records, hash, and the fanout-id pool offsets
+      uint32_t carry = 0;
+      for (uint32_t base = 0; base < K; base += 64) {
+        const uint32_t idx = base + lane;
+        uint32_t nfo = 0;
+        if (idx < K) {
+          const uint32_t g = s_node[idx];
+          const GateRec r = net.rec[g];
+          s_level[idx] = net.level[g];
+          nfo = r.nfo;
+          s_nfo[idx] = r.nfo;
+          s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
+                                   ((first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
+          uint32_t h = hash_slot(g);
+          while (atomicCAS(&h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
+          h_val[h] = (uint16_t)idx;
+        }
+        uint32_t incl = nfo;  // inclusive wave scan of the fanout counts
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t o = __shfl_up(incl, off);
+          if (lane >= (uint32_t)off) incl += o;
+        }
+        if (idx < K) s_pool_off[idx] = carry + incl - nfo;
+        carry += __shfl(incl, 63);
+      }
+      __syncthreads();
+      // 3. fanout ids into the pool (nodes past its end read them later)
+      for (uint32_t idx = lane; idx < K; idx += 64) {
+        const uint32_t off = s_pool_off[idx], nfo = s_nfo[idx];
+        if (off + nfo > kPool) continue;
+        const uint32_t fo_off = net.rec[s_node[idx]].fout_off;
+        for (uint32_t e = 0; e < nfo; ++e) s_pool[off + e] = net.fout[fo_off + e];
+      }
+      __syncthreads();
+      // 4. the DP over runs of equal level, in LDS
+      for (uint32_t i = 0; i < K;) {
+        const uint32_t idx = i + lane;
+        const bool valid = idx < K;
+        const uint32_t lv = valid ? s_level[idx] : 0xFFFFFFFFu;
+        const uint32_t lv0 = __shfl(lv, 0);
+        const uint64_t same = __ballot(valid && lv == lv0);
+        const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;  // contiguous prefix
+        if (lane < run) {
+          const uint32_t g = s_node[idx];
+          const uint8_t fl = s_flags[idx];
+          const uint32_t off = s_pool_off[idx], nfo = s_nfo[idx];
+          const bool pooled = off + nfo <= kPool;
+          const uint32_t fo_off = pooled ? 0 : net.rec[g].fout_off;
+          uint64_t C = 0, H = fl & 1;
+          for (uint32_t e = 0; e < nfo; ++e) {
+            const uint32_t fo = pooled ? s_pool[off + e] : net.fout[fo_off + e];
             uint32_t h = hash_slot(fo);
             uint32_t key;
             while ((key = h_key[h]) != kEmpty && key != fo) h = (h + 1) & (kHashSlots - 1);
@@ -517,31 +563,56 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
               ovf |= add_ovf(C, s_c[h_val[h]]);
               ovf |= add_ovf(H, s_h[h_val[h]]);
             }
-          } else if ((mark_of(pl, fo, w) >> j) & 1) {
-            ovf |= add_ovf(C, c_s[fo]);
-            ovf |= add_ovf(H, h_s[fo]);
           }
-        }
-        const bool is_new = first_t0[g] == p || first_t1[g] == p;
-        uint64_t c = C, h = H;
-        if (is_new) { ovf |= add_ovf(c, H); h = 0; }
-        if (r.type == PDF_INPT) h = 0;
-        if (in_lds) {
+          uint64_t c = C, h = H;
+          if (fl & 4) { ovf |= add_ovf(c, H); h = 0; }
+          if (fl & 2) h = 0;
           s_c[idx] = c;
           s_h[idx] = h;
-        } else {
+          if (c) {
+            const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
+            ovf |= old + c < old;
+            if (fl & 2) ovf |= add_ovf(tot, c);
+          }
+        }
+        __syncthreads();
+        i += run;
+      }
+    } else {
+      // too many marked nodes for LDS: c/h in global scratch, the whole list
+      for (uint32_t i = 0; i < len;) {
+        const uint32_t idx = i + lane;
+        const bool valid = idx < len;
+        const uint32_t g = valid ? L[idx] : 0;
+        const uint32_t lv = valid ? net.level[g] : 0xFFFFFFFFu;
+        const uint32_t lv0 = __shfl(lv, 0);
+        const uint64_t same = __ballot(valid && lv == lv0);
+        const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
+        if (lane < run && ((mark_of(pl, g, w) >> j) & 1)) {
+          const GateRec r = net.rec[g];
+          uint64_t C = 0, H = r.po == 1 ? 1 : 0;
+          for (uint32_t e = 0; e < r.nfo; ++e) {
+            const uint32_t fo = net.fout[r.fout_off + e];
+            if ((mark_of(pl, fo, w) >> j) & 1) {
+              ovf |= add_ovf(C, c_s[fo]);
+              ovf |= add_ovf(H, h_s[fo]);
+            }
+          }
+          uint64_t c = C, h = H;
+          if (first_t0[g] == p || first_t1[g] == p) { ovf |= add_ovf(c, H); h = 0; }
+          if (r.type == PDF_INPT) h = 0;
           c_s[g] = c;
           h_s[g] = h;
+          if (c) {
+            const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
+            ovf |= old + c < old;
+            if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
+          }
         }
-        if (c) {
-          const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
-          ovf |= old + c < old;
-          if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
-        }
+        __threadfence_block();
+        __syncthreads();
+        i += run;
       }
-      __threadfence_block();
-      __syncthreads();
-      i += run;
     }
     // wave sum of the PI counts of this pattern (kernSumSingle)
     for (int off = 32; off > 0; off >>= 1) {

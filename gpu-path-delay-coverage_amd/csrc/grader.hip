@@ -470,7 +470,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ totals,
                                               uint32_t *__restrict__ err) {
-  __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap];
+  __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap], s_fo_off[kCoverCap];
   __shared__ uint16_t s_nfo[kCoverCap];
   __shared__ uint8_t s_flags[kCoverCap];  // bit0 po, bit1 INPT, bit2 new
   __shared__ uint64_t s_c[kCoverCap], s_h[kCoverCap];
@@ -516,6 +516,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
           s_level[idx] = net.level[g];
           nfo = r.nfo;
           s_nfo[idx] = r.nfo;
+          s_fo_off[idx] = r.fout_off;
           s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
                                    ((first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
           uint32_t h = hash_slot(g);
@@ -531,12 +532,30 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
         carry += __shfl(incl, 63);
       }
       __syncthreads();
-      // 3. fanout ids into the pool (nodes past its end read them later)
-      for (uint32_t idx = lane; idx < K; idx += 64) {
-        const uint32_t off = s_pool_off[idx], nfo = s_nfo[idx];
-        if (off + nfo > kPool) continue;
-        const uint32_t fo_off = net.rec[s_node[idx]].fout_off;
-        for (uint32_t e = 0; e < nfo; ++e) s_pool[off + e] = net.fout[fo_off + e];
+      // 3. fanout ids into the pool, one pool slot per lane and eight loads in
+      // flight (nodes past the pool's end read theirs in the DP)
+      const uint32_t F = carry < kPool ? carry : kPool;
+      for (uint32_t base = 0; base < F; base += 8 * 64) {
+        uint32_t ids[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const uint32_t slot = base + u * 64 + lane;
+          ids[u] = 0;
+          if (slot < F) {
+            uint32_t lo = 0, hi = K - 1;  // last node whose pool range starts at or before slot
+            while (lo < hi) {
+              const uint32_t mid = (lo + hi + 1) >> 1;
+              if (s_pool_off[mid] <= slot) lo = mid;
+              else hi = mid - 1;
+            }
+            ids[u] = net.fout[s_fo_off[lo] + (slot - s_pool_off[lo])];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const uint32_t slot = base + u * 64 + lane;
+          if (slot < F) s_pool[slot] = ids[u];
+        }
       }
       __syncthreads();
       // 4. the DP over runs of equal level, in LDS
@@ -552,7 +571,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
           const uint8_t fl = s_flags[idx];
           const uint32_t off = s_pool_off[idx], nfo = s_nfo[idx];
           const bool pooled = off + nfo <= kPool;
-          const uint32_t fo_off = pooled ? 0 : net.rec[g].fout_off;
+          const uint32_t fo_off = s_fo_off[idx];
           uint64_t C = 0, H = fl & 1;
           for (uint32_t e = 0; e < nfo; ++e) {
             const uint32_t fo = pooled ? s_pool[off + e] : net.fout[fo_off + e];

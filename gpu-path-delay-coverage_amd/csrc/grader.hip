@@ -200,15 +200,23 @@ __device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Plane
 // Forward pass over one level: simulation (DO_SIM) and/or forward marking
 // (DO_MARK).  SEQ: consecutive-vector pairs from one value plane; otherwise
 // explicit pairs with a second plane.  WIDE: 64-word tiles, so each wave
-// works on one node and its record is read with scalar loads.
-template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST, bool WIDE>
+// works on one node and its record is read with scalar loads.  NW (WIDE
+// only): words per lane, w and w + 64 of a 128-word tile; every load of both
+// words is issued before any of them is used, so a level's waves fit in one
+// resident round with twice the loads in flight.
+template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST, bool WIDE, int NW>
 __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__restrict__ recs, Planes pl,
                                                 const uint64_t *__restrict__ va,
                                                 const uint64_t *__restrict__ vb, uint64_t vstride, uint64_t wbase,
-                                                LevelMap mp) {
+                                                LevelMap mp, uint32_t *__restrict__ clear32, uint32_t clear32_n) {
+  static_assert(NW == 1 || WIDE, "two words per lane needs whole-wave tiles");
   if (SEQ && DO_SIM && blockIdx.x >= mp.main_blocks) {
     // extra column: the node values of the 64 vectors after the chunk
     const uint32_t gi = (blockIdx.x - mp.main_blocks) * kBlock + threadIdx.x;
+    // these threads also clear a caller-given u32 range (the next pass's
+    // counters), striding over it
+    const uint32_t nthreads = (gridDim.x - mp.main_blocks) * kBlock;
+    for (uint32_t i = gi; i < clear32_n; i += nthreads) clear32[i] = 0;
     if (gi >= mp.count) return;
     const uint32_t g = mp.lo + gi;
     const GateRec r = recs[g];
@@ -218,30 +226,46 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
     return;
   }
   uint32_t g, w;
-  if (!map_item(mp, g, w)) return;
+  if (NW == 1) {
+    if (!map_item(mp, g, w)) return;
+  } else {
+    const uint32_t b = blockIdx.x;
+    const uint32_t grp = b / mp.T, tile = b - grp * mp.T;
+    const uint32_t gi = grp * (kBlock >> 6) + (threadIdx.x >> 6);
+    if (gi >= mp.count) return;
+    g = mp.lo + gi;
+    w = tile * 64 * NW + (threadIdx.x & 63);
+  }
   if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
-  const uint64_t row = (uint64_t)g * pl.pitch + w;
   const GateRec r = recs[g];
   const uint8_t ty = r.type;
-  uint64_t a, b = 0, m = 0;
+  bool live[NW];
+#pragma unroll
+  for (int u = 0; u < NW; ++u) live[u] = NW == 1 || w + 64 * u < mp.words;
+  uint64_t a[NW], b[NW], m[NW];
+#pragma unroll
+  for (int u = 0; u < NW; ++u) { b[u] = 0; m[u] = 0; }
   if (ty == PDF_INPT) {
     // simkernel.cuh:115-124: pair (V[q], V[q+1 mod P])
-    const uint64_t q = (uint64_t)g * vstride + wbase + w;
-    if (DO_SIM) {
-      a = va[q];
-      pl.V[row] = a;
-      if (!SEQ) {
-        b = vb[q];
-        pl.VB[row] = b;
+#pragma unroll
+    for (int u = 0; u < NW; ++u) {
+      if (!live[u]) continue;
+      const uint32_t wu = w + 64 * u;
+      const uint64_t row = (uint64_t)g * pl.pitch + wu;
+      const uint64_t q = (uint64_t)g * vstride + wbase + wu;
+      if (DO_SIM) {
+        a[u] = va[q];
+        if (!SEQ) b[u] = vb[q];
+        else if (DO_MARK) b[u] = shift_next(a[u], va[q + 1]);
+        pl.V[row] = a[u];
+        if (!SEQ) pl.VB[row] = b[u];
       } else if (DO_MARK) {
-        b = shift_next(a, va[q + 1]);
+        frames<SEQ>(pl, g, wu, a[u], b[u]);
       }
-    } else if (DO_MARK) {
-      frames<SEQ>(pl, g, w, a, b);
+      // pairs past the last one of the chunk never mark; every other mark
+      // needs its fanin 0 marked, so this keeps them clear everywhere
+      if (DO_MARK) m[u] = (a[u] ^ b[u]) & (wu + 1 == mp.words ? pl.last_mask : ~0ULL);
     }
-    // pairs past the last one of the chunk never mark; every other mark needs
-    // its fanin 0 marked, so this keeps them clear everywhere
-    if (DO_MARK) m = (a ^ b) & (w + 1 == mp.words ? pl.last_mask : ~0ULL);
   } else {
     const uint32_t nfi = r.nfi;
     const uint32_t *more = net.fin + r.fin_off;  // fanins from kRegFanin on
@@ -249,51 +273,73 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
     constexpr bool kBoth = DO_MARK || !SEQ;  // frame 2 of the fanins needed
     // issued with the value loads, so a marked fanin 0 costs one more round
     // trip (its mark word) instead of two
-    const uint64_t nz0 = DO_MARK ? pl.NZ[(uint64_t)r.fin[0] * pl.ntiles + (w >> 6)] : 0;
-    uint64_t fa[kRegFanin], fb[kRegFanin];   // constant indices only
-    if (DO_SIM) {
-      if (kBoth) frames<SEQ>(pl, r.fin[0], w, fa[0], fb[0]);
-      else { fa[0] = pl.V[(uint64_t)r.fin[0] * pl.pitch + w]; fb[0] = 0; }
-      a = fa[0];
-      b = fb[0];
-      if (ty == PDF_NOT) { a = ~a; b = ~b; }
-      if (nfi > 1) {
-        if (multi) {
+    uint64_t nz0[NW];
 #pragma unroll
-          for (int j = 1; j < kRegFanin; ++j) {
-            if ((uint32_t)j < nfi) {
-              if (kBoth) frames<SEQ>(pl, r.fin[j], w, fa[j], fb[j]);
-              else { fa[j] = pl.V[(uint64_t)r.fin[j] * pl.pitch + w]; fb[j] = 0; }
-              fold(ty, a, fa[j]);
-              fold(ty, b, fb[j]);
-            }
+    for (int u = 0; u < NW; ++u)
+      nz0[u] = DO_MARK && live[u] ? pl.NZ[(uint64_t)r.fin[0] * pl.ntiles + ((w + 64 * u) >> 6)] : 0;
+    uint64_t fa[NW][kRegFanin], fb[NW][kRegFanin];  // constant indices only
+    if (DO_SIM) {
+#pragma unroll
+      for (int j = 0; j < kRegFanin; ++j) {
+        if (j == 0 || (multi && (uint32_t)j < nfi)) {
+#pragma unroll
+          for (int u = 0; u < NW; ++u) {
+            fa[u][j] = fb[u][j] = 0;
+            if (!live[u]) continue;
+            if (kBoth) frames<SEQ>(pl, r.fin[j], w + 64 * u, fa[u][j], fb[u][j]);
+            else fa[u][j] = pl.V[(uint64_t)r.fin[j] * pl.pitch + w + 64 * u];
           }
-          for (uint32_t j = kRegFanin; j < nfi; ++j) {
-            uint64_t xa, xb = 0;
-            if (kBoth) frames<SEQ>(pl, more[j], w, xa, xb);
-            else xa = pl.V[(uint64_t)more[j] * pl.pitch + w];
-            fold(ty, a, xa);
-            fold(ty, b, xb);
-          }
-        } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
-          a = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
-          b = 0;
         }
       }
-      pl.V[row] = a;
-      if (!SEQ) pl.VB[row] = b;
+#pragma unroll
+      for (int u = 0; u < NW; ++u) {
+        a[u] = fa[u][0];
+        b[u] = fb[u][0];
+        if (ty == PDF_NOT) { a[u] = ~a[u]; b[u] = ~b[u]; }
+        if (nfi > 1) {
+          if (multi) {
+#pragma unroll
+            for (int j = 1; j < kRegFanin; ++j)
+              if ((uint32_t)j < nfi) { fold(ty, a[u], fa[u][j]); fold(ty, b[u], fb[u][j]); }
+            for (uint32_t j = kRegFanin; j < nfi && live[u]; ++j) {
+              uint64_t xa, xb = 0;
+              if (kBoth) frames<SEQ>(pl, more[j], w + 64 * u, xa, xb);
+              else xa = pl.V[(uint64_t)more[j] * pl.pitch + w + 64 * u];
+              fold(ty, a[u], xa);
+              fold(ty, b[u], xb);
+            }
+          } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
+            a[u] = 0;  // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
+            b[u] = 0;
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NW; ++u) {
+        if (!live[u]) continue;
+        const uint64_t row = (uint64_t)g * pl.pitch + w + 64 * u;
+        pl.V[row] = a[u];
+        if (!SEQ) pl.VB[row] = b[u];
+      }
     } else if (DO_MARK) {
-      frames<SEQ>(pl, g, w, a, b);
+#pragma unroll
+      for (int u = 0; u < NW; ++u) {
+        a[u] = b[u] = 0;
+        if (live[u]) frames<SEQ>(pl, g, w + 64 * u, a[u], b[u]);
+      }
     }
     if (DO_MARK) {
-      const uint64_t tr = a ^ b;
-      if (tr) {
-        const uint64_t m0 = ((nz0 >> (w & 63)) & 1) ? pl.M[(uint64_t)r.fin[0] * pl.pitch + w] : 0;
+#pragma unroll
+      for (int u = 0; u < NW; ++u) {
+        const uint32_t wu = w + 64 * u;
+        const uint64_t tr = live[u] ? a[u] ^ b[u] : 0;
+        if (!tr) continue;
+        const uint64_t m0 = ((nz0[u] >> (wu & 63)) & 1) ? pl.M[(uint64_t)r.fin[0] * pl.pitch + wu] : 0;
         if (ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT) {
-          m = m0 & tr;
+          m[u] = m0 & tr;
         } else if (ty >= PDF_AND && ty <= PDF_XNOR) {
           if (nfi == 1) {
-            m = m0 & tr;  // empty AND over off-path inputs
+            m[u] = m0 & tr;  // empty AND over off-path inputs
           } else if (multi && (m0 & tr)) {
             const bool andlike = ty == PDF_AND || ty == PDF_NAND;
             uint64_t rr;
@@ -301,15 +347,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
               if (!DO_SIM) {
 #pragma unroll
                 for (int j = 0; j < kRegFanin; ++j)
-                  if ((uint32_t)j < nfi) frames<SEQ>(pl, r.fin[j], w, fa[j], fb[j]);
+                  if ((uint32_t)j < nfi) frames<SEQ>(pl, r.fin[j], wu, fa[u][j], fb[u][j]);
               }
-              rr = andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+              rr = andlike ? side_ok<ROBUST, true>(fa[u], fb[u], nfi) : side_ok<ROBUST, false>(fa[u], fb[u], nfi);
             } else {
-              rr = andlike ? side_ok_mem<SEQ, ROBUST, true>(pl, more, nfi, w)
-                           : side_ok_mem<SEQ, ROBUST, false>(pl, more, nfi, w);
+              rr = andlike ? side_ok_mem<SEQ, ROBUST, true>(pl, more, nfi, wu)
+                           : side_ok_mem<SEQ, ROBUST, false>(pl, more, nfi, wu);
             }
             // markkernel.cuh:253-261: the on-path mark is always fanin 0's
-            m = rr & m0 & tr;
+            m[u] = rr & m0 & tr;
           }
           // XOR/XNOR with >= 2 fanins never sensitize (LUT entries 0)
         }
@@ -317,14 +363,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
     }
   }
   if (DO_MARK) {
-    if (m) pl.M[row] = m;
-    // one NZ word per (gate, tile): bits from the lanes of this gate's group
-    const uint64_t bal = __ballot(m != 0);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t tw = 1u << mp.twlog;
-    if ((lane & (tw - 1)) == 0) {
-      const uint64_t bits = mp.twlog == 6 ? bal : (bal >> lane) & ((1ULL << tw) - 1);
-      pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)] = bits;
+#pragma unroll
+    for (int u = 0; u < NW; ++u) {
+      const uint32_t wu = w + 64 * u;
+      if (live[u] && m[u]) pl.M[(uint64_t)g * pl.pitch + wu] = m[u];
+      // one NZ word per (gate, 64-word tile): bits from this gate's lanes
+      const uint64_t bal = __ballot(live[u] && m[u] != 0);
+      if ((lane & (tw - 1)) == 0 && live[u]) {
+        const uint64_t bits = mp.twlog == 6 ? bal : (bal >> lane) & ((1ULL << tw) - 1);
+        pl.NZ[(uint64_t)g * pl.ntiles + (wu >> 6)] = bits;
+      }
     }
   }
 }
@@ -397,29 +447,23 @@ __global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *_
 
 // Cross-chunk state to its initial value (one launch per grade).
 __global__ void k_reset_grade(uint32_t n, uint32_t *first_t0, uint32_t *first_t1, uint64_t *gate_counts,
-                              uint64_t *totals, uint32_t *err) {
+                              uint64_t *grand_total, uint32_t *err) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     first_t0[i] = PDF_NONE;
     first_t1[i] = PDF_NONE;
     gate_counts[i] = 0;
   }
-  if (i < 2) totals[i] = 0;
-  if (i == 0) *err = 0;
+  if (i == 0) {
+    *grand_total = 0;
+    *err = 0;
+  }
 }
 
-// Per-chunk merge/cover scratch to zero (one launch per chunk).
-__global__ void k_reset_chunk(uint32_t words, uint32_t *list_cnt, uint64_t *actmap, uint32_t *act_count,
-                              uint64_t *totals) {
+// Per-chunk merge/cover counters to zero (when no forward launch clears them).
+__global__ void k_reset_chunk(uint32_t *state, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < words) {
-    list_cnt[i] = 0;
-    actmap[i] = 0;
-  }
-  if (i == 0) {
-    *act_count = 0;
-    totals[0] = 0;
-  }
+  if (i < n) state[i] = 0;
 }
 
 // Distinct first-detection patterns of this chunk (relative index).
@@ -468,8 +512,8 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               const uint32_t *__restrict__ act_count,
                                               const uint32_t *__restrict__ first_t0,
                                               const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
-                                              uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ totals,
-                                              uint32_t *__restrict__ err) {
+                                              uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
+                                              uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err) {
   __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap], s_fo_off[kCoverCap];
   __shared__ uint16_t s_nfo[kCoverCap];
   __shared__ uint8_t s_flags[kCoverCap];  // bit0 po, bit1 INPT, bit2 new
@@ -567,7 +611,6 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
         const uint64_t same = __ballot(valid && lv == lv0);
         const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;  // contiguous prefix
         if (lane < run) {
-          const uint32_t g = s_node[idx];
           const uint8_t fl = s_flags[idx];
           const uint32_t off = s_pool_off[idx], nfo = s_nfo[idx];
           const bool pooled = off + nfo <= kPool;
@@ -588,14 +631,17 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
           if (fl & 2) h = 0;
           s_c[idx] = c;
           s_h[idx] = h;
-          if (c) {
-            const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
-            ovf |= old + c < old;
-            if (fl & 2) ovf |= add_ovf(tot, c);
-          }
         }
         __syncthreads();
         i += run;
+      }
+      // 5. per-gate counts and the PI total, all nodes at once
+      for (uint32_t idx = lane; idx < K; idx += 64) {
+        const uint64_t c = s_c[idx];
+        if (!c) continue;
+        const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[s_node[idx]], (unsigned long long)c);
+        ovf |= old + c < old;
+        if (s_flags[idx] & 2) ovf |= add_ovf(tot, c);
       }
     } else {
       // too many marked nodes for LDS: c/h in global scratch, the whole list
@@ -639,8 +685,8 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       ovf |= add_ovf(tot, o);
     }
     if (lane == 0 && tot) {
-      const uint64_t o0 = atomicAdd((unsigned long long *)&totals[0], (unsigned long long)tot);
-      const uint64_t o1 = atomicAdd((unsigned long long *)&totals[1], (unsigned long long)tot);
+      const uint64_t o0 = atomicAdd((unsigned long long *)chunk_total, (unsigned long long)tot);
+      const uint64_t o1 = atomicAdd((unsigned long long *)grand_total, (unsigned long long)tot);
       ovf |= (o0 + tot < o0) || (o1 + tot < o1);
     }
     if (__ballot(ovf)) { if (lane == 0) atomicOr(err, 1u); }
@@ -672,11 +718,14 @@ struct pdf_grader {
   // merge / cover state
   uint32_t *d_first0 = nullptr, *d_first1 = nullptr;
   uint64_t *d_gate = nullptr;
-  uint64_t *d_totals = nullptr;  // [0] = current chunk, [1] = all chunks
+  uint64_t *d_grand = nullptr;  // exact total over all chunks
   uint32_t *d_err = nullptr;
-  uint32_t *d_list = nullptr, *d_list_cnt = nullptr;
-  uint64_t *d_actmap = nullptr;
-  uint32_t *d_act_list = nullptr, *d_act_count = nullptr;
+  uint32_t *d_list = nullptr, *d_act_list = nullptr;
+  // per-chunk counters in one zeroed u32 block: actmap (u64 per word),
+  // list_cnt (per word), act_count, chunk total (u64)
+  uint32_t *d_cstate = nullptr, n_cstate = 0;
+  uint64_t *d_actmap = nullptr, *d_chunk_total = nullptr;
+  uint32_t *d_list_cnt = nullptr, *d_act_count = nullptr;
   uint64_t *d_cs = nullptr, *d_hs = nullptr;
   uint32_t cover_waves = 0;
   // per-launch event timing (pdf_grade_timed)
@@ -769,32 +818,59 @@ LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, bool extra_colum
   return mp;
 }
 
+// Words per lane for the fused sequence-mode pass: two when the chunk splits
+// into 128-word tiles whose count is a multiple of 8 (each tile keeps its XCD).
+uint32_t words_per_lane(bool seq, bool sim, bool mark, uint64_t width) {
+  const uint64_t words = (width + 63) / 64;
+  return (seq && sim && mark && width % 64 == 0 && words % 128 == 0 && (words / 128) % 8 == 0) ? 2 : 1;
+}
+
+template <bool SEQ, bool S, bool M, bool R, bool WIDE, int NW>
+void fwd_one(pdf_grader *g, uint32_t grid, const DevNet &net, const Planes &pl, uint64_t wbase, const LevelMap &mp,
+             uint32_t *clear, uint32_t nclear) {
+  k_fwd<SEQ, S, M, R, WIDE, NW><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride,
+                                                                 wbase, mp, clear, nclear);
+}
+
+// clear_state: the last level's extra-column threads zero the per-chunk
+// merge/cover counters (sequence-mode simulation only).
 template <bool SEQ, bool S, bool M>
-void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
+void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
+  const uint32_t nw = words_per_lane(SEQ, S, M, width);
   for (uint32_t k = 0; k <= g->nl.L; ++k) {
     uint32_t grid;
-    const LevelMap mp =
-        level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], SEQ && S, &grid);
+    const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
+    LevelMap mp = level_map(pl.words, lo, count, SEQ && S, &grid);
+    if (nw == 2) {  // 128-word tiles, one node per wave
+      mp.T = pl.words / 128;
+      mp.main_blocks = mp.T * ((count + (kBlock >> 6) - 1) / (kBlock >> 6));
+      grid = mp.main_blocks + grid_for(count, kBlock);
+    }
+    const bool last = k == g->nl.L && SEQ && S && clear_state;
+    uint32_t *clear = last ? g->d_cstate : nullptr;
+    const uint32_t nclear = last ? g->n_cstate : 0;
+    const uint64_t wbase = start / 64;
     tm_begin(g, kFwd);
-    const bool wide = mp.twlog == 6;
-    if (robust && wide)
-      k_fwd<SEQ, S, M, true, true><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
-    else if (robust)
-      k_fwd<SEQ, S, M, true, false><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
-    else if (wide)
-      k_fwd<SEQ, S, M, false, true><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
-    else
-      k_fwd<SEQ, S, M, false, false><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64, mp);
+    if (nw == 2) {
+      if (robust) fwd_one<SEQ, S, M, true, true, (SEQ && S && M) ? 2 : 1>(g, grid, net, pl, wbase, mp, clear, nclear);
+      else fwd_one<SEQ, S, M, false, true, (SEQ && S && M) ? 2 : 1>(g, grid, net, pl, wbase, mp, clear, nclear);
+    } else if (mp.twlog == 6) {
+      if (robust) fwd_one<SEQ, S, M, true, true, 1>(g, grid, net, pl, wbase, mp, clear, nclear);
+      else fwd_one<SEQ, S, M, false, true, 1>(g, grid, net, pl, wbase, mp, clear, nclear);
+    } else {
+      if (robust) fwd_one<SEQ, S, M, true, false, 1>(g, grid, net, pl, wbase, mp, clear, nclear);
+      else fwd_one<SEQ, S, M, false, false, 1>(g, grid, net, pl, wbase, mp, clear, nclear);
+    }
     tm_end(g);
   }
 }
 
 template <bool S, bool M>
-int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
-  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust);
-  else launch_fwd_t<false, S, M>(g, start, width, robust);
+int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state = false) {
+  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust, clear_state);
+  else launch_fwd_t<false, S, M>(g, start, width, robust, false);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
@@ -818,11 +894,9 @@ void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width) {
   }
 }
 
-int reset_chunk(pdf_grader *g, uint64_t width) {
-  const uint32_t words = (uint32_t)((width + 63) / 64);
+int reset_chunk(pdf_grader *g) {
   tm_begin(g, kFill);
-  k_reset_chunk<<<grid_for(std::max(words, 1u), 256), 256, 0, g->stream>>>(words, g->d_list_cnt, g->d_actmap,
-                                                                         g->d_act_count, g->d_totals);
+  k_reset_chunk<<<grid_for(g->n_cstate, 256), 256, 0, g->stream>>>(g->d_cstate, g->n_cstate);
   tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
@@ -845,7 +919,7 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
   tm_begin(g, kCover);
   k_cover<<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt, g->nl.n,
                                                 g->d_act_list, g->d_act_count, g->d_first0, g->d_first1, g->d_cs,
-                                                g->d_hs, g->d_gate, g->d_totals, g->d_err);
+                                                g->d_hs, g->d_gate, g->d_chunk_total, g->d_grand, g->d_err);
   tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
@@ -858,8 +932,10 @@ void free_chunk(pdf_grader *g) {
   dfree(g->d_NZ);
   dfree(g->d_ACC);
   dfree(g->d_list);
-  dfree(g->d_list_cnt);
-  dfree(g->d_actmap);
+  dfree(g->d_cstate);
+  g->d_actmap = g->d_chunk_total = nullptr;
+  g->d_list_cnt = g->d_act_count = nullptr;
+  g->n_cstate = 0;
   dfree(g->d_act_list);
   dfree(g->d_cs);
   dfree(g->d_hs);
@@ -870,7 +946,7 @@ void free_chunk(pdf_grader *g) {
 int queue_reset(pdf_grader *g) {
   tm_begin(g, kFill);
   k_reset_grade<<<grid_for(std::max(g->nl.n, 2u), 256), 256, 0, g->stream>>>(g->nl.n, g->d_first0, g->d_first1,
-                                                                            g->d_gate, g->d_totals, g->d_err);
+                                                                            g->d_gate, g->d_grand, g->d_err);
   tm_end(g);
   HIPCHK(hipGetLastError());
   g->stage = 0;
@@ -965,9 +1041,8 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   if (!rc) rc = dalloc(&g->d_first0, nl.n);
   if (!rc) rc = dalloc(&g->d_first1, nl.n);
   if (!rc) rc = dalloc(&g->d_gate, nl.n);
-  if (!rc) rc = dalloc(&g->d_totals, 2);
+  if (!rc) rc = dalloc(&g->d_grand, 1);
   if (!rc) rc = dalloc(&g->d_err, 1);
-  if (!rc) rc = dalloc(&g->d_act_count, 1);
   if (!rc) rc = reset_state(g);
   if (rc) { pdf_grader_destroy(g); return rc; }
   *out = g;
@@ -992,9 +1067,8 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_first0);
   dfree(g->d_first1);
   dfree(g->d_gate);
-  dfree(g->d_totals);
+  dfree(g->d_grand);
   dfree(g->d_err);
-  dfree(g->d_act_count);
   for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
@@ -1079,8 +1153,15 @@ int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
   // the removal accumulator is zero between passes (each reader clears it)
   if (!rc && !hip_ok(hipMemset(g->d_ACC, 0, n * g->pitch * 8), "hipMemset")) rc = PDF_ERR_HIP;
   if (!rc) rc = dalloc(&g->d_list, n * words);
-  if (!rc) rc = dalloc(&g->d_list_cnt, words);
-  if (!rc) rc = dalloc(&g->d_actmap, words);
+  const uint32_t ct_off = (uint32_t)((3 * words + 2) & ~1ULL);
+  g->n_cstate = ct_off + 2;
+  if (!rc) rc = dalloc(&g->d_cstate, g->n_cstate);
+  if (!rc) {
+    g->d_actmap = reinterpret_cast<uint64_t *>(g->d_cstate);
+    g->d_list_cnt = g->d_cstate + 2 * words;
+    g->d_act_count = g->d_cstate + 3 * words;
+    g->d_chunk_total = reinterpret_cast<uint64_t *>(g->d_cstate + ct_off);
+  }
   if (!rc) rc = dalloc(&g->d_act_list, chunk_pairs);
   if (!rc) rc = dalloc(&g->d_cs, n * g->cover_waves);
   if (!rc) rc = dalloc(&g->d_hs, n * g->cover_waves);
@@ -1150,7 +1231,7 @@ int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, float *ms
   if (g->stage_chunk != (int64_t)chunk || g->stage < 2) { set_error("pdf_merge: mark this chunk first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   EventTimer tm(g, ms);
-  if ((rc = reset_chunk(g, width))) return rc;
+  if ((rc = reset_chunk(g))) return rc;
   if ((rc = launch_reverse<false, true>(g, start, width))) return rc;
   if ((rc = tm.finish())) return rc;
   g->stage = 3;
@@ -1169,13 +1250,13 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
   EventTimer tm(g, ms);
   if ((rc = launch_cover(g, start, width))) return rc;
   if ((rc = tm.finish())) return rc;
-  uint64_t tot[2];
+  uint64_t tot = 0;
   uint32_t err = 0;
-  HIPCHK(hipMemcpy(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&tot, g->d_chunk_total, sizeof(tot), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost));
   g->stage = 4;
   if (err) { set_error("pdf_cover: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
-  *coverage = tot[0];
+  *coverage = tot;
   return PDF_OK;
 }
 
@@ -1189,21 +1270,21 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   for (uint64_t c = 0; c < nchunks; ++c) {
     const uint64_t start = c * g->chunk_pairs;
     const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
-    if ((rc = launch_fwd<true, true>(g, start, width, robust))) return rc;
-    if ((rc = reset_chunk(g, width))) return rc;
+    if ((rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true))) return rc;
+    if (!g->seq && (rc = reset_chunk(g))) return rc;
     if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
     if ((rc = launch_cover(g, start, width))) return rc;
   }
-  uint64_t tot[2];
+  uint64_t tot = 0;
   uint32_t err = 0;
-  HIPCHK(hipMemcpyAsync(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(&tot, g->d_grand, sizeof(tot), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipMemcpyAsync(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
   if (err) { set_error("pdf_grade: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
-  *total = tot[1];
+  *total = tot;
   return PDF_OK;
 }
 
@@ -1297,9 +1378,7 @@ int pdf_get_gate_counts(const pdf_grader *g, uint64_t *counts) {
 int pdf_get_total(const pdf_grader *g, uint64_t *total) {
   if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
-  uint64_t tot[2];
-  HIPCHK(hipMemcpy(tot, g->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
-  *total = tot[1];
+  HIPCHK(hipMemcpy(total, g->d_grand, sizeof(uint64_t), hipMemcpyDeviceToHost));
   return PDF_OK;
 }
 

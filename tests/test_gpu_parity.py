@@ -244,3 +244,19 @@ def test_api_errors():
         with pytest.raises(pg.PdfError) as e:
             g.sim(4, 1024)
         assert e.value.code == pg.ERR_ARG
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_two_words_per_lane_path(robust):
+    """A 65,536-pair chunk (1,024 words) takes the two-words-per-lane forward
+    kernel; a 1,500-gate netlist keeps the oracle fast."""
+    say("two words per lane", robust)
+    cfg = pg.GenConfig(seed=77, num_pi=64, num_po=64, num_gates=1500, levels=20, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 65536
+    bits = biased_vectors(77, nl.info()["num_pi"], P, p_hold=0.6)
+    ref = oracle_grade(nl, bits, P, P, robust)
+    assert ref["total"] > 0
+    assert_same(gpu_grade(nl, bits, P, P, robust), ref, "nw2")
+    assert_same(gpu_grade(nl, bits, P, 8192, robust), ref, "nw1 chunks")

@@ -732,6 +732,12 @@ struct pdf_grader {
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_family;
   bool timing = false;
+  // tuning knobs (pdf_grader_set_tuning)
+  uint32_t tune_words_per_lane = 1;
+  uint32_t tune_use_graph = 1;
+  // captured fused grade (pdf_grade), rebuilt when its shape changes
+  hipGraphExec_t graph = nullptr;
+  int graph_robust = -1;
   // call-order tracking for the stage API
   int64_t stage_chunk = -1;
   int stage = 0;  // 1 = simulated, 2 = marked, 3 = merged, 4 = covered
@@ -818,11 +824,12 @@ LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, bool extra_colum
   return mp;
 }
 
-// Words per lane for the fused sequence-mode pass: two when the chunk splits
-// into 128-word tiles whose count is a multiple of 8 (each tile keeps its XCD).
-uint32_t words_per_lane(bool seq, bool sim, bool mark, uint64_t width) {
+// Words per lane for the fused sequence-mode pass (tuning knob, default 1):
+// two only when the chunk splits into 128-word tiles whose count is a
+// multiple of 8 (each tile keeps its XCD).
+uint32_t words_per_lane(uint32_t want, bool seq, bool sim, bool mark, uint64_t width) {
   const uint64_t words = (width + 63) / 64;
-  return (seq && sim && mark && width % 64 == 0 && words % 128 == 0 && (words / 128) % 8 == 0) ? 2 : 1;
+  return (want == 2 && seq && sim && mark && width % 64 == 0 && words % 128 == 0 && (words / 128) % 8 == 0) ? 2 : 1;
 }
 
 template <bool SEQ, bool S, bool M, bool R, bool WIDE, int NW>
@@ -838,7 +845,7 @@ template <bool SEQ, bool S, bool M>
 void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
-  const uint32_t nw = words_per_lane(SEQ, S, M, width);
+  const uint32_t nw = words_per_lane(g->tune_words_per_lane, SEQ, S, M, width);
   for (uint32_t k = 0; k <= g->nl.L; ++k) {
     uint32_t grid;
     const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
@@ -925,7 +932,14 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
   return PDF_OK;
 }
 
+void drop_graph(pdf_grader *g) {
+  if (g->graph) (void)hipGraphExecDestroy(g->graph);
+  g->graph = nullptr;
+  g->graph_robust = -1;
+}
+
 void free_chunk(pdf_grader *g) {
+  drop_graph(g);
   dfree(g->d_V);
   dfree(g->d_VB);
   dfree(g->d_M);
@@ -1260,10 +1274,9 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
   return PDF_OK;
 }
 
-static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
-  if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
-  if (!g->chunk_pairs) { set_error("pdf_grade: load vectors first"); return PDF_ERR_STATE; }
-  HIPCHK(hipSetDevice(g->device));
+// Every launch of one fused grade (reset, then per chunk: forward levels,
+// reverse levels, events, cover), in stream order.
+static int queue_grade(pdf_grader *g, int robust) {
   int rc = queue_reset(g);
   if (rc) return rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
@@ -1274,6 +1287,38 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
     if (!g->seq && (rc = reset_chunk(g))) return rc;
     if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
     if ((rc = launch_cover(g, start, width))) return rc;
+  }
+  return PDF_OK;
+}
+
+static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
+  if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_grade: load vectors first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  int rc;
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  if (g->tune_use_graph && !g->timing) {
+    // the launch sequence depends only on the chunk shape and the robust
+    // flag, so it is captured once and replayed
+    if (!g->graph || g->graph_robust != (robust != 0)) {
+      drop_graph(g);
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+      rc = queue_grade(g, robust);
+      const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(e);
+      const hipError_t ie = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) { g->graph = nullptr; HIPCHK(ie); }
+      g->graph_robust = robust != 0;
+    }
+    HIPCHK(hipGraphLaunch(g->graph, g->stream));
+  } else if ((rc = queue_grade(g, robust))) {
+    return rc;
   }
   uint64_t tot = 0;
   uint32_t err = 0;
@@ -1289,6 +1334,26 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
 }
 
 int pdf_grade(pdf_grader *g, int robust, uint64_t *total) { return grade_impl(g, robust, total); }
+
+int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  switch (key) {
+    case PDF_TUNE_WORDS_PER_LANE:
+      if (value != 1 && value != 2) { set_error("words per lane must be 1 or 2"); return PDF_ERR_ARG; }
+      g->tune_words_per_lane = (uint32_t)value;
+      break;
+    case PDF_TUNE_USE_GRAPH:
+      g->tune_use_graph = value != 0;
+      break;
+    default:
+      set_error("unknown tuning key");
+      return PDF_ERR_ARG;
+  }
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  drop_graph(g);
+  return PDF_OK;
+}
 
 int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times) {
   if (!g || !total || !times) { set_error("null argument"); return PDF_ERR_ARG; }

@@ -29,6 +29,7 @@ UNKN, INPT, AND, NAND, OR, NOR, XOR, XNOR, BUFF, NOT, FROM, DFF = range(12)
 TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF", "NOT", "FROM", "DFF"]
 S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
+TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH = 1, 2
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
@@ -97,6 +98,7 @@ SIGNATURES = {
     "pdf_cover": ([_P, _U64, _U64, C.c_int, _U64P, _FP], C.c_int),
     "pdf_grade": ([_P, C.c_int, _U64P], C.c_int),
     "pdf_grade_timed": ([_P, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
+    "pdf_grader_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
     "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
     "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
@@ -288,6 +290,10 @@ class Grader:
 
     def reset(self):
         _check(lib().pdf_grader_reset(self._h))
+
+    def set_tuning(self, key: int, value: int):
+        """TUNE_WORDS_PER_LANE (1/2) or TUNE_USE_GRAPH (0/1); results never change."""
+        _check(lib().pdf_grader_set_tuning(self._h, key, value))
 
     # -- the four stages (reference host wrappers)
     def sim(self, chunk: int, start: int) -> float:

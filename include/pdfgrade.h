@@ -192,6 +192,14 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
  * total.  Equivalent to calling the four stages per chunk. */
 int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
 
+/* Tuning knobs (no reference counterpart; the reference fixes its shapes
+ * with compile-time constants such as SIM_BLOCK, MARK_BLOCK, PARALLEL_SEGS,
+ * simkernel.cuh:29-30, markkernel.cuh:7-8, mergekernel.cuh:49-52).
+ * Results never depend on them. */
+#define PDF_TUNE_WORDS_PER_LANE 1 /* 1 or 2 words per lane in the fused forward pass */
+#define PDF_TUNE_USE_GRAPH 2      /* 1: pdf_grade replays a captured HIP graph */
+int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
+
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same
  * launches, each bracketed by HIP events on the grader's stream, so the
  * summed device time of every kernel family is reported.  Slower than

@@ -260,3 +260,30 @@ def test_two_words_per_lane_path(robust):
     assert ref["total"] > 0
     assert_same(gpu_grade(nl, bits, P, P, robust), ref, "nw2")
     assert_same(gpu_grade(nl, bits, P, 8192, robust), ref, "nw1 chunks")
+
+
+def test_graph_replays_match_eager():
+    """pdf_grade replays a captured HIP graph; every replay, with eager stage
+    calls and synchronizations in between, equals the eager launch sequence."""
+    say("graph replays")
+    nl = pg.Netlist.generate(small_config(41, num_gates=300, levels=12))
+    P = 4096
+    bits = biased_vectors(41, nl.info()["num_pi"], P, p_hold=0.6)
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(1024)
+        g.set_tuning(pg.TUNE_USE_GRAPH, 0)
+        eager = [g.grade(True), g.gate_counts().copy(), *g.first()]
+        g.set_tuning(pg.TUNE_USE_GRAPH, 1)
+        for rep in range(4):
+            assert g.grade(True) == eager[0], rep
+            assert np.array_equal(g.gate_counts(), eager[1])
+            f0, f1 = g.first()
+            assert np.array_equal(f0, eager[2]) and np.array_equal(f1, eager[3])
+            # eager work between replays
+            total, _ = g.grade_stages(True)
+            assert total == eager[0]
+        assert g.grade(False) == oracle_grade(nl, bits, P, 1024, False)["total"]
+        for nw in (1, 2):
+            g.set_tuning(pg.TUNE_WORDS_PER_LANE, nw)
+            assert g.grade(True) == eager[0]

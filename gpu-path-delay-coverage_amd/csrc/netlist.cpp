@@ -271,10 +271,15 @@ int pdf_netlist_generate(const pdf_gen_params *p, pdf_netlist **out) {
     Rng rng(p->seed + 0xD1B54A32D192ED03ULL * a);
     int rc = generate_once(*p, rng, h->nl, err);
     if (rc != PDF_OK) { delete h; set_error("pdf_netlist_generate: " + err); return rc; }
-    if (h->nl.structural_paths < (1ULL << 63)) { *out = h; return PDF_OK; }
+    if (p->path_limit_log2 == 0 || p->path_limit_log2 >= 64 ||
+        h->nl.structural_paths < (1ULL << p->path_limit_log2)) {
+      *out = h;
+      return PDF_OK;
+    }
   }
   delete h;
-  set_error("pdf_netlist_generate: every attempt had >= 2^63 structural paths");
+  set_error("pdf_netlist_generate: every attempt had >= 2^" + std::to_string(p->path_limit_log2) +
+            " structural paths");
   return PDF_ERR_NETLIST;
 }
 

@@ -47,7 +47,7 @@ class _GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("num_pi", C.c_uint32), ("num_po", C.c_uint32),
                 ("num_gates", C.c_uint32), ("levels", C.c_uint32), ("fanin_p", C.c_double * 4),
                 ("prev_level_frac", C.c_double), ("fanout_alpha", C.c_double),
-                ("fanout_cap", C.c_uint32), ("max_attempts", C.c_uint32)]
+                ("fanout_cap", C.c_uint32), ("max_attempts", C.c_uint32), ("path_limit_log2", C.c_uint32)]
 
 
 class _Desc(C.Structure):
@@ -146,6 +146,7 @@ class GenConfig:
     fanout_alpha: float = 0.0
     fanout_cap: int = 0
     max_attempts: int = 16
+    path_limit_log2: int = 63  # reject netlists with >= 2^this PI->PO paths (0 = keep any)
 
 
 class Netlist:
@@ -173,6 +174,7 @@ class Netlist:
             p.fanin_p[i] = fp[i]
         p.prev_level_frac, p.fanout_alpha = cfg.prev_level_frac, cfg.fanout_alpha
         p.fanout_cap, p.max_attempts = cfg.fanout_cap, cfg.max_attempts
+        p.path_limit_log2 = cfg.path_limit_log2
         h = C.c_void_p()
         _check(lib().pdf_netlist_generate(C.byref(p), C.byref(h)))
         return cls(h)

@@ -129,7 +129,8 @@ typedef struct pdf_gen_params {
   double prev_level_frac;   /* share of fanins drawn from level-1 */
   double fanout_alpha;      /* power-law exponent of fanout weights; 0 = uniform */
   uint32_t fanout_cap;      /* max fanout of any node (0 = none) */
-  uint32_t max_attempts;    /* regenerations allowed when paths >= 2^63 */
+  uint32_t max_attempts;    /* regenerations allowed when paths >= 2^limit */
+  uint32_t path_limit_log2; /* reject when PI->PO paths >= 2^this; 0 = keep any */
 } pdf_gen_params;
 
 int pdf_netlist_generate(const pdf_gen_params *params, pdf_netlist **out);

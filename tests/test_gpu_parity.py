@@ -287,3 +287,30 @@ def test_graph_replays_match_eager():
         for nw in (1, 2):
             g.set_tuning(pg.TUNE_WORDS_PER_LANE, nw)
             assert g.grade(True) == eager[0]
+
+
+def test_2m_gates_properties():
+    """The 2,000,000-gate / 200-level netlist on 65,536 of its pairs: chunk
+    invariance across 32,768- and 16,384-pair blocks, and the planes of an
+    interior 64-pair chunk and of the wrapping last chunk equal the oracle's."""
+    from pdfgrade.configs import GATES_2M
+    say("2m gates")
+    nl = pg.Netlist.generate(GATES_2M.netlist)
+    P = 65536
+    bits = pg.gen_vectors(GATES_2M.vector_seed, nl.info()["num_pi"], P)
+    a = gpu_grade(nl, bits, P, 32768, True)
+    b = gpu_grade(nl, bits, P, 16384, True)
+    b["per_chunk"] = None
+    assert_same(b, a, "2m chunking")
+    net = orc.OracleNet(nl.arrays())
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(64)
+        for c in (300, P // 64 - 1):
+            g.sim(c, c * 64)
+            pa, pb = g.sim_planes(64)
+            codes = orc.sim_chunk(net, bits, P, c * 64, 64)
+            assert np.array_equal(pg.planes_to_codes(pa, pb), codes), c
+            g.mark(c, c * 64, True)
+            marks, _ = orc.mark_chunk(net, codes, True)
+            assert np.array_equal(pg.plane_to_bytes(g.mark_plane(64)), marks), c

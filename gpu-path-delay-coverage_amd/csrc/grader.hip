@@ -1090,6 +1090,24 @@ void pdf_grader_destroy(pdf_grader *g) {
   delete g;
 }
 
+// Pairs per chunk when the caller gives none: all pairs if the chunk buffers
+// fit in half of the free HBM, else the largest multiple of 64 that does (the
+// reference sizes its chunk from free GPU memory, gpuCalculateSimulPatterns,
+// main.cu:150).
+static uint64_t default_chunk(pdf_grader *g) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  // V, M, ACC (and VB) planes plus the marked-node lists, per pair
+  const double per_pair = g->nl.n * ((g->seq ? 3.0 : 4.0) / 8.0 + 4.0 / 64.0) + 16.0;
+  const double budget = 0.5 * (double)free_b - (double)(1ull << 30);
+  if (budget <= 0) return 64;
+  const uint64_t fit = (uint64_t)(budget / per_pair) / 64 * 64;
+  return fit >= g->num_pairs ? 0 : std::max<uint64_t>(fit, 64);
+}
+
 static int load_vectors(pdf_grader *g, const uint64_t *a, const uint64_t *b, uint64_t num) {
   if (!g || !a) { set_error("null argument"); return PDF_ERR_ARG; }
   if (num == 0) { set_error("no pairs"); return PDF_ERR_ARG; }
@@ -1132,7 +1150,7 @@ static int load_vectors(pdf_grader *g, const uint64_t *a, const uint64_t *b, uin
   }
   g->num_pairs = num;
   g->vwords = words;
-  return pdf_grader_set_chunk(g, 0);
+  return pdf_grader_set_chunk(g, default_chunk(g));
 }
 
 int pdf_grader_set_vectors(pdf_grader *g, const uint64_t *bits, uint64_t num_vectors) {

@@ -159,7 +159,9 @@ int pdf_grader_set_vectors(pdf_grader *g, const uint64_t *bits,
 int pdf_grader_set_pairs(pdf_grader *g, const uint64_t *a_bits,
                          const uint64_t *b_bits, uint64_t num_pairs);
 /* Pairs per chunk (the reference's simul_patterns, main.cu:150-153); a
- * multiple of 64, or 0 (or >= the number of pairs) for one chunk.  Allocates the chunk's
+ * multiple of 64, or 0 (or >= the number of pairs) for one chunk.  Loading
+ * vectors picks a default: one chunk if its buffers fit in half of the free
+ * HBM, else the largest multiple of 64 that fits.  Allocates the chunk's
  * bit-planes.  Also resets the cross-chunk state. */
 int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs);
 /* Number of chunks for the loaded pairs and the width (pairs) of one. */

@@ -35,6 +35,17 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(workload: str, family: str):
+    """HBM bytes per launch of the kernel family, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic_<workload>.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["bytes_per_launch"].get(f"k_{family}")
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(nl, bits, pairs, sample, robust):
     """Serial C oracle (one core) on the first `sample` pairs of the workload."""
     import numpy as np
@@ -88,12 +99,11 @@ def main():
     t_total, kt = g.grade_timed(robust)
     assert t_total == total
     fam = max(("fwd", "reverse", "cover", "events"), key=lambda k: kt[f"{k}_ms"])
-    # algorithmic bytes of the fused forward kernel per pair: PI vector bits
-    # read once plus the A, B and M planes written once (fanin re-reads are
-    # intra-pass intermediates)
-    fwd_bytes = (3 * n + I) / 8.0 * wl.pairs
-    # reverse (removal + merge): M read and written, B read
-    rev_bytes = 3 * n / 8.0 * wl.pairs
+    # algorithmic bytes per pair, from the per-stage figures of the
+    # measurement plan: sim (2n + I)/8 and forward mark 3n/8 for the fused
+    # forward kernel; removal 2n/8 and merge 2n/8 for the fused reverse kernel
+    fwd_bytes = (5 * n + I) / 8.0 * wl.pairs
+    rev_bytes = 4 * n / 8.0 * wl.pairs
     fam_bytes = {"fwd": fwd_bytes, "reverse": rev_bytes}.get(fam, 0.0)
     launches = max(1, kt[f"{fam}_launches"])
     avg_ms = kt[f"{fam}_ms"] / launches
@@ -128,7 +138,7 @@ def main():
                    "mode": "robust" if robust else "non-robust", "total_paths": total},
         "gate_evals_per_s": pairs_per_s * (n - I),
         "roofline": {"bound": "hbm", "kernel": f"k_{fam}", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name, fam),
                      "avg_launch_us": avg_ms * 1e3, "launches_per_step": launches},
         "end_to_end": {"algorithmic_GBps": e2e_gbs, "frac": e2e_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "wall_ms")},

@@ -27,6 +27,7 @@
 //              214-220), so it contributes nothing and is skipped.
 
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -506,6 +507,10 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 26544357
 // (a fanout marked at p has a higher level, so it was finished in an earlier
 // run).  If more than kCoverCap nodes are marked at p the DP keeps c/h in
 // this wave's slice of cs/hs and reads everything from global memory.
+// ALL_NEW: every pair of the chunk, every marked node counted as new (an
+// empty detection history), so a pair's PI total is its number of marked
+// PI->PO paths; written to pair_gain instead of the running totals.
+template <bool ALL_NEW>
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                               const uint32_t *__restrict__ act_list,
@@ -513,7 +518,8 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               const uint32_t *__restrict__ first_t0,
                                               const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
-                                              uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err) {
+                                              uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
+                                              uint32_t width, uint32_t *__restrict__ pair_gain) {
   __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap], s_fo_off[kCoverCap];
   __shared__ uint16_t s_nfo[kCoverCap];
   __shared__ uint8_t s_flags[kCoverCap];  // bit0 po, bit1 INPT, bit2 new
@@ -525,9 +531,9 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
   const uint64_t below = (1ULL << lane) - 1;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
   uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
-  const uint32_t nact = *act_count;
+  const uint32_t nact = ALL_NEW ? width : *act_count;
   for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
-    const uint32_t rel = act_list[k];
+    const uint32_t rel = ALL_NEW ? k : act_list[k];
     const uint32_t w = rel >> 6, j = rel & 63;
     const uint32_t p = start + rel;
     const uint32_t len = list_cnt[w];
@@ -562,7 +568,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
           s_nfo[idx] = r.nfo;
           s_fo_off[idx] = r.fout_off;
           s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
-                                   ((first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
+                                   ((ALL_NEW || first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
           uint32_t h = hash_slot(g);
           while (atomicCAS(&h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
           h_val[h] = (uint16_t)idx;
@@ -639,8 +645,10 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       for (uint32_t idx = lane; idx < K; idx += 64) {
         const uint64_t c = s_c[idx];
         if (!c) continue;
-        const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[s_node[idx]], (unsigned long long)c);
-        ovf |= old + c < old;
+        if (!ALL_NEW) {
+          const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[s_node[idx]], (unsigned long long)c);
+          ovf |= old + c < old;
+        }
         if (s_flags[idx] & 2) ovf |= add_ovf(tot, c);
       }
     } else {
@@ -664,13 +672,15 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
             }
           }
           uint64_t c = C, h = H;
-          if (first_t0[g] == p || first_t1[g] == p) { ovf |= add_ovf(c, H); h = 0; }
+          if (ALL_NEW || first_t0[g] == p || first_t1[g] == p) { ovf |= add_ovf(c, H); h = 0; }
           if (r.type == PDF_INPT) h = 0;
           c_s[g] = c;
           h_s[g] = h;
           if (c) {
-            const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
-            ovf |= old + c < old;
+            if (!ALL_NEW) {
+              const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
+              ovf |= old + c < old;
+            }
             if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
           }
         }
@@ -684,7 +694,9 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       const uint64_t o = __shfl_down(tot, off);
       ovf |= add_ovf(tot, o);
     }
-    if (lane == 0 && tot) {
+    if (ALL_NEW) {
+      if (lane == 0) pair_gain[p] = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
+    } else if (lane == 0 && tot) {
       const uint64_t o0 = atomicAdd((unsigned long long *)chunk_total, (unsigned long long)tot);
       const uint64_t o1 = atomicAdd((unsigned long long *)grand_total, (unsigned long long)tot);
       ovf |= (o0 + tot < o0) || (o1 + tot < o1);
@@ -693,6 +705,109 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
     __syncthreads();
   }
 }
+
+// ------------------------------------------------------------ pair features
+// 512 features per pair, BF16 (no reference counterpart: inputs of the
+// learned pair ranking):
+//   [0, 256)    fraction of the nodes of level f with a transition (levels up
+//               to the caller's max_level; zero beyond)
+//   [256, 384)  fraction of the PIs of bucket b = f - 256 that toggle (PI i
+//               is in bucket i * 128 / num_pi)
+//   [384, 512)  fanout-weighted share of bucket b = f - 384 that toggles:
+//               sum of the fanouts of toggling PIs / sum of the bucket's fanouts
+constexpr int kFeat = 512;
+constexpr int kCountBits = 24;  // per-lane counters up to 2^24 - 1
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return (uint16_t)(u >> 16);
+}
+
+// 64 per-lane counters held bit-sliced: bit j of c[k] is bit k of counter j.
+// Adds x << k0 (x: one bit per lane).
+__device__ __forceinline__ void vc_add(uint64_t (&c)[kCountBits], uint64_t x, int k0) {
+#pragma unroll
+  for (int k = 0; k < kCountBits; ++k) {
+    if (k < k0) continue;
+    const uint64_t t = c[k] & x;
+    c[k] ^= x;
+    x = t;
+  }
+}
+__device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint32_t j) {
+  uint32_t v = 0;
+#pragma unroll
+  for (int k = 0; k < kCountBits; ++k) v |= (uint32_t)((c[k] >> j) & 1) << k;
+  return v;
+}
+
+// Block = one word of the chunk (64 pairs); thread = two features.
+template <bool SEQ>
+__global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                  const uint32_t *__restrict__ level_off, uint32_t levels_used,
+                                                  uint32_t num_pi, uint64_t first_pair, uint64_t num_pairs,
+                                                  uint16_t *__restrict__ feat) {
+  const uint32_t w = blockIdx.x;
+  for (uint32_t f = threadIdx.x; f < (uint32_t)kFeat; f += blockDim.x) {
+    uint64_t c[kCountBits];
+#pragma unroll
+    for (int k = 0; k < kCountBits; ++k) c[k] = 0;
+    float denom = 0;
+    if (f < 256) {
+      if (f < levels_used) {
+        const uint32_t lo = level_off[f], hi = level_off[f + 1];
+        for (uint32_t g = lo; g < hi; ++g) {
+          uint64_t a, b;
+          frames<SEQ>(pl, g, w, a, b);
+          vc_add(c, a ^ b, 0);
+        }
+        denom = (float)(hi - lo);
+      }
+    } else {
+      const uint32_t bkt = (f - 256) & 127;
+      const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
+      const bool weighted = f >= 384;
+      for (uint32_t g = lo; g < hi; ++g) {
+        uint64_t a, b;
+        frames<SEQ>(pl, g, w, a, b);
+        const uint32_t wt = weighted ? recs[g].nfo : 1u;
+        denom += (float)wt;
+        for (int k = 0; k < 17; ++k)  // weights are at most 65535
+          if ((wt >> k) & 1) vc_add(c, a ^ b, k);
+      }
+    }
+    const float scale = denom > 0 ? 1.0f / denom : 0.0f;
+    const uint64_t base = first_pair + 64ull * w;
+    for (uint32_t j = 0; j < 64 && base + j < num_pairs; ++j)
+      feat[(base + j) * kFeat + f] = to_bf16((float)vc_get(c, j) * scale);
+  }
+}
+
+// Per-pair permutation of PI-major packed bits: pair q of dst = pair perm[q]
+// of src.  Thread = (PI row, output word).
+__global__ void k_permute_pairs(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                                const uint32_t *__restrict__ perm, uint32_t num_pi, uint64_t words, uint64_t num) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)num_pi * words) return;
+  const uint64_t i = t / words, k = t - i * words;
+  const uint64_t *row = src + i * words;
+  uint64_t out = 0;
+  for (uint32_t j = 0; j < 64; ++j) {
+    const uint64_t q = 64 * k + j;
+    if (q >= num) break;
+    const uint32_t s = perm[q];
+    out |= ((row[s >> 6] >> (s & 63)) & 1ULL) << j;
+  }
+  dst[t] = out;
+}
+
+__global__ void k_iota(uint32_t *p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+__global__ void k_copy_u64(const uint64_t *src, uint64_t *dst) { *dst = *src; }
 
 }  // namespace
 
@@ -707,6 +822,7 @@ struct pdf_grader {
   GateRec *d_rec = nullptr;
   uint8_t *d_type = nullptr, *d_po = nullptr;
   uint32_t *d_fin_off = nullptr, *d_fin = nullptr, *d_fout_off = nullptr, *d_fout = nullptr, *d_level = nullptr;
+  uint32_t *d_level_off = nullptr;
   // vectors: sequence mode keeps vwords + 1 words per PI row (positions
   // wrapped mod num_pairs, so the word after the last is available)
   uint64_t num_pairs = 0, vwords = 0, vstride = 0;
@@ -855,11 +971,11 @@ void fwd_one(pdf_grader *g, uint32_t grid, const DevNet &net, const Planes &pl, 
 // clear_state: the last level's extra-column threads zero the per-chunk
 // merge/cover counters (sequence-mode simulation only).
 template <bool SEQ, bool S, bool M>
-void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state) {
+void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state, uint32_t last_level) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
   const uint32_t nw = words_per_lane(g->tune_words_per_lane, SEQ, S, M, width);
-  for (uint32_t k = 0; k <= g->nl.L; ++k) {
+  for (uint32_t k = 0; k <= std::min(g->nl.L, last_level); ++k) {
     uint32_t grid;
     const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
     LevelMap mp = level_map(pl.words, lo, count, SEQ && S, &grid);
@@ -868,7 +984,7 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
       mp.main_blocks = mp.T * ((count + (kBlock >> 6) - 1) / (kBlock >> 6));
       grid = mp.main_blocks + grid_for(count, kBlock);
     }
-    const bool last = k == g->nl.L && SEQ && S && clear_state;
+    const bool last = k == std::min(g->nl.L, last_level) && SEQ && S && clear_state;
     uint32_t *clear = last ? g->d_cstate : nullptr;
     const uint32_t nclear = last ? g->n_cstate : 0;
     const uint64_t wbase = start / 64;
@@ -888,9 +1004,10 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
 }
 
 template <bool S, bool M>
-int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state = false) {
-  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust, clear_state);
-  else launch_fwd_t<false, S, M>(g, start, width, robust, false);
+int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state = false,
+               uint32_t last_level = 0xFFFFFFFFu) {
+  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust, clear_state, last_level);
+  else launch_fwd_t<false, S, M>(g, start, width, robust, false, last_level);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
@@ -937,9 +1054,10 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
                                                          (uint32_t)width, g->d_actmap, g->d_act_list, g->d_act_count);
   tm_end(g);
   tm_begin(g, kCover);
-  k_cover<<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt, g->nl.n,
-                                                g->d_act_list, g->d_act_count, g->d_first0, g->d_first1, g->d_cs,
-                                                g->d_hs, g->d_gate, g->d_chunk_total, g->d_grand, g->d_err);
+  k_cover<false><<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt,
+                                                       g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
+                                                       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
+                                                       g->d_grand, g->d_err, (uint32_t)width, nullptr);
   tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
@@ -1065,6 +1183,7 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   up(g->d_fout_off, nl.fout_off);
   up(g->d_fout, nl.fout);
   up(g->d_level, nl.level);
+  up(g->d_level_off, nl.level_off);
   if (!rc) rc = dalloc(&g->d_first0, nl.n);
   if (!rc) rc = dalloc(&g->d_first1, nl.n);
   if (!rc) rc = dalloc(&g->d_gate, nl.n);
@@ -1089,6 +1208,7 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_fout_off);
   dfree(g->d_fout);
   dfree(g->d_level);
+  dfree(g->d_level_off);
   dfree(g->d_va);
   dfree(g->d_vb);
   dfree(g->d_first0);
@@ -1479,6 +1599,163 @@ int pdf_get_total(const pdf_grader *g, uint64_t *total) {
   if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
   HIPCHK(hipMemcpy(total, g->d_grand, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
+// ------------------------------------------------------- learned proposal
+
+int pdf_pair_features(pdf_grader *g, uint32_t max_level, uint16_t *feat_dev) {
+  if (!g || !feat_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_pair_features: load pairs first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  const uint32_t last = std::min(std::min(max_level, g->nl.L), 255u);
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  int rc;
+  for (uint64_t c = 0; c < nchunks; ++c) {
+    const uint64_t start = c * g->chunk_pairs;
+    const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
+    if ((rc = launch_fwd<true, false>(g, start, width, 0, false, last))) return rc;
+    const Planes pl = planes(g, width);
+    if (g->seq)
+      k_features<true><<<pl.words, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
+                                                       g->nl.num_pi, start, g->num_pairs, feat_dev);
+    else
+      k_features<false><<<pl.words, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
+                                                        g->nl.num_pi, start, g->num_pairs, feat_dev);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(g->stream));
+  g->stage = 0;  // the planes hold a partial simulation
+  g->stage_chunk = -1;
+  return PDF_OK;
+}
+
+int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev) {
+  if (!g || !gain_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_pair_gains: load pairs first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  int rc = queue_reset(g);
+  if (rc) return rc;
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  for (uint64_t c = 0; c < nchunks; ++c) {
+    const uint64_t start = c * g->chunk_pairs;
+    const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
+    if ((rc = launch_fwd<true, true>(g, start, width, robust, true))) return rc;
+    if (!g->seq && (rc = reset_chunk(g))) return rc;
+    if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
+    const Planes pl = planes(g, width);
+    k_cover<true><<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt,
+                                                      g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
+                                                      g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
+                                                      g->d_grand, g->d_err, (uint32_t)width, gain_dev);
+    HIPCHK(hipGetLastError());
+  }
+  uint32_t err = 0;
+  HIPCHK(hipMemcpyAsync(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  g->stage = 0;
+  g->stage_chunk = -1;
+  if (err) { set_error("pdf_pair_gains: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  return PDF_OK;
+}
+
+int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_dev) {
+  if (!g || !score_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_grader_rank_pairs: load pairs first"); return PDF_ERR_STATE; }
+  if (g->seq) {
+    set_error("pdf_grader_rank_pairs: needs explicit pairs (reordering consecutive vectors changes the pairs)");
+    return PDF_ERR_STATE;
+  }
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  const uint64_t num = g->num_pairs;
+  float *keys_out = nullptr;
+  uint32_t *idx_in = nullptr, *idx_out = nullptr;
+  uint64_t *va = nullptr, *vb = nullptr;
+  void *temp = nullptr;
+  size_t temp_bytes = 0;
+  const size_t vsize = (size_t)g->nl.num_pi * g->vstride;
+  int rc = dalloc(&keys_out, num);
+  if (!rc) rc = dalloc(&idx_in, num);
+  if (!rc) rc = dalloc(&idx_out, num);
+  if (!rc) rc = dalloc(&va, vsize);
+  if (!rc) rc = dalloc(&vb, vsize);
+  auto cleanup = [&]() {
+    dfree(keys_out);
+    dfree(idx_in);
+    dfree(idx_out);
+    dfree(va);
+    dfree(vb);
+    if (temp) (void)hipFree(temp);
+  };
+  if (!rc) {
+    k_iota<<<grid_for(num, 256), 256, 0, g->stream>>>(idx_in, num);
+    // stable descending radix sort: equal scores keep their pair order
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, score_dev, keys_out, idx_in, idx_out,
+                                                     (int)num, 0, 32, g->stream) != hipSuccess ||
+        hipMalloc(&temp, temp_bytes) != hipSuccess ||
+        hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, score_dev, keys_out, idx_in, idx_out,
+                                                     (int)num, 0, 32, g->stream) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("pdf_grader_rank_pairs: radix sort failed");
+      rc = PDF_ERR_HIP;
+    }
+  }
+  if (!rc) {
+    const uint64_t items = (uint64_t)g->nl.num_pi * g->vstride;
+    k_permute_pairs<<<grid_for(items, 256), 256, 0, g->stream>>>(g->d_va, va, idx_out, g->nl.num_pi, g->vstride, num);
+    k_permute_pairs<<<grid_for(items, 256), 256, 0, g->stream>>>(g->d_vb, vb, idx_out, g->nl.num_pi, g->vstride, num);
+    if (perm_dev) (void)hipMemcpyAsync(perm_dev, idx_out, num * 4, hipMemcpyDeviceToDevice, g->stream);
+    if (!hip_ok(hipStreamSynchronize(g->stream), "rank_pairs")) rc = PDF_ERR_HIP;
+  }
+  if (!rc) {
+    std::swap(g->d_va, va);
+    std::swap(g->d_vb, vb);
+    drop_graph(g);
+    g->stage = 0;
+    g->stage_chunk = -1;
+  }
+  cleanup();
+  return rc;
+}
+
+int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms) {
+  if (!g || !cum_total) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_grade_curve: load vectors first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  std::vector<hipEvent_t> ev(nchunks + 1, nullptr);
+  uint64_t *curve = nullptr;
+  int rc = dalloc(&curve, nchunks);
+  for (auto &e : ev)
+    if (!rc && hipEventCreate(&e) != hipSuccess) { set_error("hipEventCreate failed"); rc = PDF_ERR_HIP; }
+  if (!rc) {
+    (void)hipEventRecord(ev[0], g->stream);
+    rc = queue_reset(g);
+  }
+  for (uint64_t c = 0; c < nchunks && !rc; ++c) {
+    const uint64_t start = c * g->chunk_pairs;
+    const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
+    if ((rc = launch_fwd<true, true>(g, start, width, robust, true))) break;
+    if (!g->seq && (rc = reset_chunk(g))) break;
+    if ((rc = launch_reverse<true, true>(g, start, width))) break;
+    if ((rc = launch_cover(g, start, width))) break;
+    k_copy_u64<<<1, 1, 0, g->stream>>>(g->d_grand, curve + c);
+    (void)hipEventRecord(ev[c + 1], g->stream);
+  }
+  uint32_t err = 0;
+  if (!rc && !hip_ok(hipStreamSynchronize(g->stream), "pdf_grade_curve")) rc = PDF_ERR_HIP;
+  if (!rc && !hip_ok(hipMemcpy(cum_total, curve, nchunks * 8, hipMemcpyDeviceToHost), "hipMemcpy")) rc = PDF_ERR_HIP;
+  if (!rc && !hip_ok(hipMemcpy(&err, g->d_err, 4, hipMemcpyDeviceToHost), "hipMemcpy")) rc = PDF_ERR_HIP;
+  for (uint64_t c = 0; c < nchunks && !rc && cum_ms; ++c)
+    if (!hip_ok(hipEventElapsedTime(&cum_ms[c], ev[0], ev[c + 1]), "hipEventElapsedTime")) rc = PDF_ERR_HIP;
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  dfree(curve);
+  if (rc) return rc;
+  g->stage = 4;
+  g->stage_chunk = (int64_t)nchunks - 1;
+  if (err) { set_error("pdf_grade_curve: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   return PDF_OK;
 }
 

@@ -104,6 +104,10 @@ SIGNATURES = {
     "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_gate_counts": ([_P, C.c_void_p], C.c_int),
     "pdf_get_total": ([_P, _U64P], C.c_int),
+    "pdf_pair_features": ([_P, C.c_uint32, C.c_void_p], C.c_int),
+    "pdf_pair_gains": ([_P, C.c_int, C.c_void_p], C.c_int),
+    "pdf_grader_rank_pairs": ([_P, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_grade_curve": ([_P, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
 }
 
 _lib = None
@@ -375,6 +379,48 @@ class Grader:
         t = C.c_uint64()
         _check(lib().pdf_get_total(self._h, C.byref(t)))
         return t.value
+
+    # -- learned pair ranking (device buffers: torch tensors on the grader's GPU)
+    def pair_features(self, max_level: int, feat):
+        """Write the 512 BF16 features of every loaded pair into `feat`, a
+        contiguous int16/bfloat16 device tensor of shape (num_pairs, 512)."""
+        _need_device(feat, self._num_pairs * FEATURES, 2)
+        _check(lib().pdf_pair_features(self._h, max_level, C.c_void_p(feat.data_ptr())))
+
+    def pair_gains(self, robust: bool, gains):
+        """Each pair's marked PI->PO path count graded alone (int32 device
+        tensor of num_pairs entries, read as uint32)."""
+        _need_device(gains, self._num_pairs, 4)
+        _check(lib().pdf_pair_gains(self._h, int(robust), C.c_void_p(gains.data_ptr())))
+
+    def rank_pairs(self, scores, perm=None):
+        """Reorder the loaded explicit pairs by descending float32 score;
+        `perm` (optional int32 device tensor) receives the old index of each
+        new position."""
+        _need_device(scores, self._num_pairs, 4)
+        if perm is not None:
+            _need_device(perm, self._num_pairs, 4)
+        _check(lib().pdf_grader_rank_pairs(self._h, C.c_void_p(scores.data_ptr()),
+                                           C.c_void_p(perm.data_ptr() if perm is not None else 0)))
+
+    def grade_curve(self, robust: bool):
+        """Grade the loaded pairs in order; per chunk the running total and the
+        milliseconds since the start: (uint64 array, float32 array)."""
+        nch = self.num_chunks()
+        tot = np.zeros(nch, np.uint64)
+        ms = np.zeros(nch, np.float32)
+        _check(lib().pdf_grade_curve(self._h, int(robust), _ptr(tot), _ptr(ms)))
+        return tot, ms
+
+
+FEATURES = 512
+
+
+def _need_device(t, count: int, itemsize: int):
+    if not getattr(t, "is_cuda", False) or not t.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    if t.numel() * t.element_size() < count * itemsize:
+        raise ValueError(f"device tensor too small: need {count} x {itemsize} bytes")
 
 
 def planes_to_codes(a: np.ndarray, b: np.ndarray, width: Optional[int] = None) -> np.ndarray:

@@ -228,6 +228,30 @@ int pdf_get_gate_counts(const pdf_grader *g, uint64_t *counts);
 /* Exact total over all chunks graded so far. */
 int pdf_get_total(const pdf_grader *g, uint64_t *total);
 
+/* ------------------------------------------------------------------------ */
+/* Learned pair ranking (no reference counterpart).  Device buffers are      */
+/* caller-allocated device pointers on the grader's device.                  */
+/* ------------------------------------------------------------------------ */
+#define PDF_FEATURES 512
+
+/* Per-pair features of the loaded pairs, BF16 row-major [num_pairs][512]:
+ * [0,256) fraction of the nodes of each level (up to max_level) with a
+ * transition; [256,384) fraction of each of 128 PI buckets that toggles;
+ * [384,512) fanout-weighted toggling share of each PI bucket.  Simulates only
+ * levels 0..max_level. */
+int pdf_pair_features(pdf_grader *g, uint32_t max_level, uint16_t *feat_dev);
+/* Exact per-pair new-path gain against an empty detection history (each
+ * pair graded alone): its number of marked PI->PO paths, saturated to u32. */
+int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev);
+/* Reorder the loaded explicit pairs by descending score (stable: equal
+ * scores keep their order); perm_dev (may be NULL) receives, for each new
+ * position, the pair's old index. */
+int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_dev);
+/* Exact grading of the loaded pairs in order, chunk by chunk: cum_total[c]
+ * and cum_ms[c] (may be NULL) hold the running total and the wall time since
+ * the start after chunk c. */
+int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950",
              # packed FP32 instructions stay off in every HIP build here
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-result", f"-I{INCLUDE}"]
-LIB_SOURCES = [os.path.join(CSRC, f) for f in ("netlist.cpp", "grader.hip")]
+LIB_SOURCES = [os.path.join(CSRC, f) for f in ("netlist.cpp", "grader.hip", "mlp.hip")]
 
 
 def _run(cmd):

@@ -711,8 +711,8 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
 // learned pair ranking):
 //   [0, 256)    fraction of the nodes of level f with a transition (levels up
 //               to the caller's max_level; zero beyond)
-//   [256, 384)  fraction of the PIs of bucket b = f - 256 that toggle (PI i
-//               is in bucket i * 128 / num_pi)
+//   [256, 384)  fraction of the PIs of bucket b = f - 256 that toggle
+//               (bucket b = PIs [floor(b I / 128), floor((b + 1) I / 128)))
 //   [384, 512)  fanout-weighted share of bucket b = f - 384 that toggles:
 //               sum of the fanouts of toggling PIs / sum of the bucket's fanouts
 constexpr int kFeat = 512;

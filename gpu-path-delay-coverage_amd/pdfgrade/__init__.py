@@ -108,6 +108,15 @@ SIGNATURES = {
     "pdf_pair_gains": ([_P, C.c_int, C.c_void_p], C.c_int),
     "pdf_grader_rank_pairs": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grade_curve": ([_P, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_mlp_create": ([C.c_int, _U64, C.POINTER(_P)], C.c_int),
+    "pdf_mlp_destroy": ([_P], None),
+    "pdf_mlp_set_params": ([_P, C.c_void_p], C.c_int),
+    "pdf_mlp_get_params": ([_P, C.c_void_p], C.c_int),
+    "pdf_mlp_forward": ([_P, C.c_void_p, _U64, C.c_void_p], C.c_int),
+    "pdf_mlp_gradients": ([_P, C.c_void_p, C.c_void_p, _U64, _FP, C.c_void_p], C.c_int),
+    "pdf_mlp_adam": ([_P, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
+    "pdf_mlp_train_step": ([_P, C.c_void_p, C.c_void_p, _U64, C.c_float, _FP], C.c_int),
+    "pdf_mlp_sync": ([_P], C.c_int),
 }
 
 _lib = None
@@ -414,13 +423,83 @@ class Grader:
 
 
 FEATURES = 512
+MLP_PARAMS = 197377
+
+
+class Mlp:
+    """The BF16 MFMA pair scorer (512 -> 256 -> 256 -> 1) of libpdfgrade.so.
+    Device buffers are torch tensors on the scorer's GPU; features are BF16
+    bits in an int16 (or bfloat16) tensor of shape (rows, 512)."""
+
+    def __init__(self, device: int = 0, seed: int = 1):
+        self._h = None
+        h = C.c_void_p()
+        _check(lib().pdf_mlp_create(device, seed, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().pdf_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().pdf_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, flat: np.ndarray):
+        flat = np.ascontiguousarray(flat, dtype=np.float32)
+        if flat.size != MLP_PARAMS:
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        _check(lib().pdf_mlp_set_params(self._h, _ptr(flat)))
+
+    def forward(self, feat, scores):
+        rows = feat.shape[0]
+        _need_device(feat, rows * FEATURES, 2)
+        _need_device(scores, rows, 4)
+        _check(lib().pdf_mlp_forward(self._h, C.c_void_p(feat.data_ptr()), rows, C.c_void_p(scores.data_ptr())))
+
+    def gradients(self, feat, target):
+        """(loss, flat float32 gradient) of the mean squared error; no update."""
+        rows = feat.shape[0]
+        _need_device(feat, rows * FEATURES, 2)
+        _need_device(target, rows, 4)
+        loss = C.c_float()
+        grad = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().pdf_mlp_gradients(self._h, C.c_void_p(feat.data_ptr()), C.c_void_p(target.data_ptr()), rows,
+                                       C.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
+    def adam(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        _check(lib().pdf_mlp_adam(self._h, lr, beta1, beta2, eps))
+
+    def train_step(self, feat, target, lr: float, want_loss: bool = False):
+        rows = feat.shape[0]
+        _need_device(feat, rows * FEATURES, 2)
+        _need_device(target, rows, 4)
+        loss = C.c_float()
+        _check(lib().pdf_mlp_train_step(self._h, C.c_void_p(feat.data_ptr()), C.c_void_p(target.data_ptr()), rows,
+                                        lr, C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def sync(self):
+        _check(lib().pdf_mlp_sync(self._h))
 
 
 def _need_device(t, count: int, itemsize: int):
+    """Check a device-tensor argument and wait for torch's pending work on it
+    (the library runs on its own streams)."""
     if not getattr(t, "is_cuda", False) or not t.is_contiguous():
         raise ValueError("expected a contiguous device tensor")
     if t.numel() * t.element_size() < count * itemsize:
         raise ValueError(f"device tensor too small: need {count} x {itemsize} bytes")
+    if t.data_ptr() % 16:
+        raise ValueError("device tensor must be 16-byte aligned")
+    import torch
+    torch.cuda.current_stream(t.device).synchronize()
 
 
 def planes_to_codes(a: np.ndarray, b: np.ndarray, width: Optional[int] = None) -> np.ndarray:

@@ -236,7 +236,8 @@ int pdf_get_total(const pdf_grader *g, uint64_t *total);
 
 /* Per-pair features of the loaded pairs, BF16 row-major [num_pairs][512]:
  * [0,256) fraction of the nodes of each level (up to max_level) with a
- * transition; [256,384) fraction of each of 128 PI buckets that toggles;
+ * transition; [256,384) fraction of each of 128 PI buckets that toggles
+ * (bucket b = PIs [floor(b I / 128), floor((b + 1) I / 128)));
  * [384,512) fanout-weighted toggling share of each PI bucket.  Simulates only
  * levels 0..max_level. */
 int pdf_pair_features(pdf_grader *g, uint32_t max_level, uint16_t *feat_dev);
@@ -251,6 +252,31 @@ int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_
  * and cum_ms[c] (may be NULL) hold the running total and the wall time since
  * the start after chunk c. */
 int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms);
+
+/* BF16 MFMA scorer for the pair ranking: 512 -> 256 -> 256 -> 1 with ReLU,
+ * FP32 accumulation, trained with Adam by the library's own forward and
+ * backward kernels.  Parameters (FP32 master copy) are one flat array:
+ * W1[256][512] b1[256] W2[256][256] b2[256] w3[256] b3[1].  Every call is
+ * synchronous on the scorer's own stream except pdf_mlp_train_step with
+ * loss == NULL (finish with pdf_mlp_sync). */
+#define PDF_MLP_PARAMS 197377
+typedef struct pdf_mlp pdf_mlp;
+int pdf_mlp_create(int device, uint64_t seed, pdf_mlp **out);
+void pdf_mlp_destroy(pdf_mlp *m);
+int pdf_mlp_set_params(pdf_mlp *m, const float *host); /* also clears the Adam state */
+int pdf_mlp_get_params(pdf_mlp *m, float *host);
+/* score_dev[r] = network output for feature row r (BF16 [rows][512]). */
+int pdf_mlp_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t rows, float *score_dev);
+/* Mean squared error against target_dev and its gradient (flat layout, to
+ * grad_host when not NULL); no update. */
+int pdf_mlp_gradients(pdf_mlp *m, const uint16_t *feat_dev, const float *target_dev, uint64_t rows, float *loss,
+                      float *grad_host);
+/* One Adam step with the last computed gradient. */
+int pdf_mlp_adam(pdf_mlp *m, float lr, float beta1, float beta2, float eps);
+/* Gradient + Adam (betas 0.9 / 0.999, eps 1e-8) on one batch. */
+int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, const float *target_dev, uint64_t rows, float lr,
+                       float *loss);
+int pdf_mlp_sync(pdf_mlp *m);
 
 #ifdef __cplusplus
 }

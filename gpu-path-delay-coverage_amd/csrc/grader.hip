@@ -509,7 +509,9 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 26544357
 // this wave's slice of cs/hs and reads everything from global memory.
 // ALL_NEW: every pair of the chunk, every marked node counted as new (an
 // empty detection history), so a pair's PI total is its number of marked
-// PI->PO paths; written to pair_gain instead of the running totals.
+// PI->PO paths; written to pair_gain instead of the running totals.  In the
+// normal mode a non-null pair_gain receives each processed pair's total
+// (saturated to uint32; the caller zeroes it for the pairs not processed).
 template <bool ALL_NEW>
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
@@ -694,9 +696,8 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
       const uint64_t o = __shfl_down(tot, off);
       ovf |= add_ovf(tot, o);
     }
-    if (ALL_NEW) {
-      if (lane == 0) pair_gain[p] = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
-    } else if (lane == 0 && tot) {
+    if (pair_gain && lane == 0) pair_gain[p] = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
+    if (!ALL_NEW && lane == 0 && tot) {
       const uint64_t o0 = atomicAdd((unsigned long long *)chunk_total, (unsigned long long)tot);
       const uint64_t o1 = atomicAdd((unsigned long long *)grand_total, (unsigned long long)tot);
       ovf |= (o0 + tot < o0) || (o1 + tot < o1);
@@ -708,15 +709,20 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
 
 // ------------------------------------------------------------ pair features
 // 512 features per pair, BF16 (no reference counterpart: inputs of the
-// learned pair ranking):
-//   [0, 256)    fraction of the nodes of level f with a transition (levels up
-//               to the caller's max_level; zero beyond)
+// learned pair ranking), stored feature-major: feat[f * ld + pair]:
+//   [0, 128)    fraction of the nodes of level f with a transition
+//   [128, 256)  fraction of the nodes of level f - 128 marked by the forward
+//               pass (robust or non-robust sensitization from a PI, before
+//               the reverse removal); both for levels up to the caller's
+//               max_level, zero beyond
 //   [256, 384)  fraction of the PIs of bucket b = f - 256 that toggle
 //               (bucket b = PIs [floor(b I / 128), floor((b + 1) I / 128)))
 //   [384, 512)  fanout-weighted share of bucket b = f - 384 that toggles:
 //               sum of the fanouts of toggling PIs / sum of the bucket's fanouts
+// Each value is float(count) * (1.0f / float(denominator)), rounded to BF16.
 constexpr int kFeat = 512;
-constexpr int kCountBits = 24;  // per-lane counters up to 2^24 - 1
+constexpr int kCountBits = 24;  // wide per-lane counters, up to 2^24 - 1
+constexpr int kSmallBits = 6;   // narrow counters flushed every 63 additions
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
   uint32_t u = __float_as_uint(f);
@@ -725,14 +731,26 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 }
 
 // 64 per-lane counters held bit-sliced: bit j of c[k] is bit k of counter j.
-// Adds x << k0 (x: one bit per lane).
-__device__ __forceinline__ void vc_add(uint64_t (&c)[kCountBits], uint64_t x, int k0) {
+// vc_inc adds x (one bit per lane) at bit k0; vc_add_sliced adds a
+// bit-sliced kSmallBits-wide operand.
+template <int B>
+__device__ __forceinline__ void vc_inc(uint64_t (&c)[B], uint64_t x, int k0) {
 #pragma unroll
-  for (int k = 0; k < kCountBits; ++k) {
+  for (int k = 0; k < B; ++k) {
     if (k < k0) continue;
     const uint64_t t = c[k] & x;
     c[k] ^= x;
     x = t;
+  }
+}
+__device__ __forceinline__ void vc_add_sliced(uint64_t (&c)[kCountBits], const uint64_t (&s)[kSmallBits]) {
+  uint64_t carry = 0;
+#pragma unroll
+  for (int k = 0; k < kCountBits; ++k) {
+    const uint64_t y = k < kSmallBits ? s[k] : 0;
+    const uint64_t sum = c[k] ^ y ^ carry;
+    carry = (c[k] & y) | (carry & (c[k] ^ y));
+    c[k] = sum;
   }
 }
 __device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint32_t j) {
@@ -742,45 +760,89 @@ __device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint
   return v;
 }
 
-// Block = one word of the chunk (64 pairs); thread = two features.
+// Thread = (word w of the chunk, feature f = blockIdx.y): consecutive lanes
+// read consecutive words of each gate row and write 128 contiguous bytes.
 template <bool SEQ>
 __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__restrict__ recs, Planes pl,
                                                   const uint32_t *__restrict__ level_off, uint32_t levels_used,
                                                   uint32_t num_pi, uint64_t first_pair, uint64_t num_pairs,
-                                                  uint16_t *__restrict__ feat) {
-  const uint32_t w = blockIdx.x;
-  for (uint32_t f = threadIdx.x; f < (uint32_t)kFeat; f += blockDim.x) {
-    uint64_t c[kCountBits];
+                                                  uint16_t *__restrict__ feat, uint64_t ld) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y;
+  if (w >= pl.words) return;
+  uint64_t c[kCountBits];
 #pragma unroll
-    for (int k = 0; k < kCountBits; ++k) c[k] = 0;
-    float denom = 0;
-    if (f < 256) {
-      if (f < levels_used) {
-        const uint32_t lo = level_off[f], hi = level_off[f + 1];
-        for (uint32_t g = lo; g < hi; ++g) {
-          uint64_t a, b;
-          frames<SEQ>(pl, g, w, a, b);
-          vc_add(c, a ^ b, 0);
+  for (int k = 0; k < kCountBits; ++k) c[k] = 0;
+  float denom = 0;
+  if (f < 256) {
+    const uint32_t lv = f & 127;
+    const bool marks = f >= 128;
+    if (lv < levels_used) {
+      const uint32_t lo = level_off[lv], hi = level_off[lv + 1];
+      for (uint32_t g0 = lo; g0 < hi; g0 += 63) {
+        uint64_t s[kSmallBits];
+#pragma unroll
+        for (int k = 0; k < kSmallBits; ++k) s[k] = 0;
+        const uint32_t g1 = min(hi, g0 + 63);
+        uint32_t g = g0;
+        for (; g + 8 <= g1; g += 8) {  // eight rows' loads in flight
+          uint64_t x[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (marks) {
+              x[u] = mark_of(pl, g + u, w);
+            } else {
+              uint64_t a, b;
+              frames<SEQ>(pl, g + u, w, a, b);
+              x[u] = a ^ b;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) vc_inc<kSmallBits>(s, x[u], 0);
         }
-        denom = (float)(hi - lo);
+        for (; g < g1; ++g) {
+          uint64_t x;
+          if (marks) {
+            x = mark_of(pl, g, w);
+          } else {
+            uint64_t a, b;
+            frames<SEQ>(pl, g, w, a, b);
+            x = a ^ b;
+          }
+          vc_inc<kSmallBits>(s, x, 0);
+        }
+        vc_add_sliced(c, s);
       }
-    } else {
-      const uint32_t bkt = (f - 256) & 127;
-      const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
-      const bool weighted = f >= 384;
-      for (uint32_t g = lo; g < hi; ++g) {
-        uint64_t a, b;
-        frames<SEQ>(pl, g, w, a, b);
-        const uint32_t wt = weighted ? recs[g].nfo : 1u;
-        denom += (float)wt;
-        for (int k = 0; k < 17; ++k)  // weights are at most 65535
-          if ((wt >> k) & 1) vc_add(c, a ^ b, k);
-      }
+      denom = (float)(hi - lo);
     }
-    const float scale = denom > 0 ? 1.0f / denom : 0.0f;
-    const uint64_t base = first_pair + 64ull * w;
-    for (uint32_t j = 0; j < 64 && base + j < num_pairs; ++j)
-      feat[(base + j) * kFeat + f] = to_bf16((float)vc_get(c, j) * scale);
+  } else {
+    const uint32_t bkt = (f - 256) & 127;
+    const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
+    const bool weighted = f >= 384;
+    for (uint32_t g = lo; g < hi; ++g) {
+      uint64_t a, b;
+      frames<SEQ>(pl, g, w, a, b);
+      const uint32_t wt = weighted ? recs[g].nfo : 1u;
+      denom += (float)wt;
+      for (int k = 0; k < 17; ++k)  // weights are at most 65535
+        if ((wt >> k) & 1) vc_inc<kCountBits>(c, a ^ b, k);
+    }
+  }
+  const float scale = denom > 0 ? 1.0f / denom : 0.0f;
+  const uint64_t base = first_pair + 64ull * w;
+  uint16_t *dst = feat + (uint64_t)f * ld + base;
+#pragma unroll
+  for (uint32_t j0 = 0; j0 < 64; j0 += 8) {
+    uint32_t v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      v[e] = (uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e) * scale) |
+             ((uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e + 1) * scale) << 16);
+    if (base + j0 + 8 <= num_pairs) {
+      *reinterpret_cast<uint4 *>(dst + j0) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+      for (uint32_t e = 0; e < 8 && base + j0 + e < num_pairs; ++e)
+        dst[j0 + e] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+    }
   }
 }
 
@@ -852,6 +914,12 @@ struct pdf_grader {
   // tuning knobs (pdf_grader_set_tuning)
   uint32_t tune_words_per_lane = 1;
   uint32_t tune_use_graph = 1;
+  // ranking scratch (pdf_grader_rank_pairs), kept between calls
+  float *d_rk_keys = nullptr;
+  uint32_t *d_rk_idx = nullptr, *d_rk_perm = nullptr;
+  uint64_t *d_rk_va = nullptr, *d_rk_vb = nullptr;
+  void *d_rk_temp = nullptr;
+  size_t rk_pairs = 0, rk_vsize = 0, rk_temp_bytes = 0;
   // captured fused grade (pdf_grade), rebuilt when its shape changes
   hipGraphExec_t graph = nullptr;
   int graph_robust = -1;
@@ -1047,7 +1115,7 @@ int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width) {
   return PDF_OK;
 }
 
-int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
+int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_total = nullptr) {
   const Planes pl = planes(g, width);
   tm_begin(g, kEvents);
   k_events<<<grid_for(g->nl.n, 256), 256, 0, g->stream>>>(g->nl.n, g->d_first0, g->d_first1, (uint32_t)start,
@@ -1057,10 +1125,21 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width) {
   k_cover<false><<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt,
                                                        g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                        g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
-                                                       g->d_grand, g->d_err, (uint32_t)width, nullptr);
+                                                       g->d_grand, g->d_err, (uint32_t)width, pair_total);
   tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
+}
+
+void free_rank(pdf_grader *g) {
+  dfree(g->d_rk_keys);
+  dfree(g->d_rk_idx);
+  dfree(g->d_rk_perm);
+  dfree(g->d_rk_va);
+  dfree(g->d_rk_vb);
+  if (g->d_rk_temp) (void)hipFree(g->d_rk_temp);
+  g->d_rk_temp = nullptr;
+  g->rk_pairs = g->rk_vsize = g->rk_temp_bytes = 0;
 }
 
 void drop_graph(pdf_grader *g) {
@@ -1211,6 +1290,7 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_level_off);
   dfree(g->d_va);
   dfree(g->d_vb);
+  free_rank(g);
   dfree(g->d_first0);
   dfree(g->d_first1);
   dfree(g->d_gate);
@@ -1604,24 +1684,29 @@ int pdf_get_total(const pdf_grader *g, uint64_t *total) {
 
 // ------------------------------------------------------- learned proposal
 
-int pdf_pair_features(pdf_grader *g, uint32_t max_level, uint16_t *feat_dev) {
+int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *feat_dev, uint64_t ld) {
   if (!g || !feat_dev) { set_error("null argument"); return PDF_ERR_ARG; }
   if (!g->chunk_pairs) { set_error("pdf_pair_features: load pairs first"); return PDF_ERR_STATE; }
+  if (ld % 64 || ld < g->num_pairs || (uintptr_t)feat_dev % 16) {
+    set_error("pdf_pair_features: ld must be a multiple of 64 and >= the pair count, feat_dev 16-byte aligned");
+    return PDF_ERR_ARG;
+  }
   HIPCHK(hipSetDevice(g->device));
-  const uint32_t last = std::min(std::min(max_level, g->nl.L), 255u);
+  const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   int rc;
   for (uint64_t c = 0; c < nchunks; ++c) {
     const uint64_t start = c * g->chunk_pairs;
     const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
-    if ((rc = launch_fwd<true, false>(g, start, width, 0, false, last))) return rc;
+    if ((rc = launch_fwd<true, true>(g, start, width, robust, false, last))) return rc;
     const Planes pl = planes(g, width);
+    const dim3 grid(grid_for(pl.words, 256), kFeat);
     if (g->seq)
-      k_features<true><<<pl.words, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
-                                                       g->nl.num_pi, start, g->num_pairs, feat_dev);
+      k_features<true><<<grid, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
+                                                   g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
     else
-      k_features<false><<<pl.words, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
-                                                        g->nl.num_pi, start, g->num_pairs, feat_dev);
+      k_features<false><<<grid, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
+                                                    g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(g->stream));
@@ -1667,62 +1752,60 @@ int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_
     return PDF_ERR_STATE;
   }
   HIPCHK(hipSetDevice(g->device));
-  HIPCHK(hipStreamSynchronize(g->stream));
   const uint64_t num = g->num_pairs;
-  float *keys_out = nullptr;
-  uint32_t *idx_in = nullptr, *idx_out = nullptr;
-  uint64_t *va = nullptr, *vb = nullptr;
-  void *temp = nullptr;
-  size_t temp_bytes = 0;
   const size_t vsize = (size_t)g->nl.num_pi * g->vstride;
-  int rc = dalloc(&keys_out, num);
-  if (!rc) rc = dalloc(&idx_in, num);
-  if (!rc) rc = dalloc(&idx_out, num);
-  if (!rc) rc = dalloc(&va, vsize);
-  if (!rc) rc = dalloc(&vb, vsize);
-  auto cleanup = [&]() {
-    dfree(keys_out);
-    dfree(idx_in);
-    dfree(idx_out);
-    dfree(va);
-    dfree(vb);
-    if (temp) (void)hipFree(temp);
-  };
-  if (!rc) {
-    k_iota<<<grid_for(num, 256), 256, 0, g->stream>>>(idx_in, num);
-    // stable descending radix sort: equal scores keep their pair order
-    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, score_dev, keys_out, idx_in, idx_out,
-                                                     (int)num, 0, 32, g->stream) != hipSuccess ||
-        hipMalloc(&temp, temp_bytes) != hipSuccess ||
-        hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, score_dev, keys_out, idx_in, idx_out,
-                                                     (int)num, 0, 32, g->stream) != hipSuccess) {
+  size_t temp_bytes = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, score_dev, g->d_rk_keys, g->d_rk_idx,
+                                                      g->d_rk_perm, (int)num, 0, 32, g->stream));
+  if (g->rk_pairs < num || g->rk_vsize != vsize || g->rk_temp_bytes < temp_bytes) {
+    HIPCHK(hipStreamSynchronize(g->stream));
+    free_rank(g);
+    int rc = dalloc(&g->d_rk_keys, num);
+    if (!rc) rc = dalloc(&g->d_rk_idx, num);
+    if (!rc) rc = dalloc(&g->d_rk_perm, num);
+    if (!rc) rc = dalloc(&g->d_rk_va, vsize);
+    if (!rc) rc = dalloc(&g->d_rk_vb, vsize);
+    if (!rc && hipMalloc(&g->d_rk_temp, temp_bytes) != hipSuccess) {
       (void)hipGetLastError();
-      set_error("pdf_grader_rank_pairs: radix sort failed");
-      rc = PDF_ERR_HIP;
+      g->d_rk_temp = nullptr;
+      set_error("pdf_grader_rank_pairs: hipMalloc of sort scratch failed");
+      rc = PDF_ERR_NOMEM;
     }
+    if (rc) {
+      free_rank(g);
+      return rc;
+    }
+    g->rk_pairs = num;
+    g->rk_vsize = vsize;
+    g->rk_temp_bytes = temp_bytes;
   }
-  if (!rc) {
-    const uint64_t items = (uint64_t)g->nl.num_pi * g->vstride;
-    k_permute_pairs<<<grid_for(items, 256), 256, 0, g->stream>>>(g->d_va, va, idx_out, g->nl.num_pi, g->vstride, num);
-    k_permute_pairs<<<grid_for(items, 256), 256, 0, g->stream>>>(g->d_vb, vb, idx_out, g->nl.num_pi, g->vstride, num);
-    if (perm_dev) (void)hipMemcpyAsync(perm_dev, idx_out, num * 4, hipMemcpyDeviceToDevice, g->stream);
-    if (!hip_ok(hipStreamSynchronize(g->stream), "rank_pairs")) rc = PDF_ERR_HIP;
-  }
-  if (!rc) {
-    std::swap(g->d_va, va);
-    std::swap(g->d_vb, vb);
-    drop_graph(g);
-    g->stage = 0;
-    g->stage_chunk = -1;
-  }
-  cleanup();
-  return rc;
+  k_iota<<<grid_for(num, 256), 256, 0, g->stream>>>(g->d_rk_idx, num);
+  // stable descending radix sort: equal scores keep their pair order
+  temp_bytes = g->rk_temp_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(g->d_rk_temp, temp_bytes, score_dev, g->d_rk_keys,
+                                                      g->d_rk_idx, g->d_rk_perm, (int)num, 0, 32, g->stream));
+  k_permute_pairs<<<grid_for(vsize, 256), 256, 0, g->stream>>>(g->d_va, g->d_rk_va, g->d_rk_perm, g->nl.num_pi,
+                                                              g->vstride, num);
+  k_permute_pairs<<<grid_for(vsize, 256), 256, 0, g->stream>>>(g->d_vb, g->d_rk_vb, g->d_rk_perm, g->nl.num_pi,
+                                                              g->vstride, num);
+  HIPCHK(hipGetLastError());
+  if (perm_dev) HIPCHK(hipMemcpyAsync(perm_dev, g->d_rk_perm, num * 4, hipMemcpyDeviceToDevice, g->stream));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  // the permuted vectors become the loaded ones; the old buffers are the
+  // next call's scratch
+  std::swap(g->d_va, g->d_rk_va);
+  std::swap(g->d_vb, g->d_rk_vb);
+  drop_graph(g);
+  g->stage = 0;
+  g->stage_chunk = -1;
+  return PDF_OK;
 }
 
-int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms) {
+int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms, uint32_t *pair_total_dev) {
   if (!g || !cum_total) { set_error("null argument"); return PDF_ERR_ARG; }
   if (!g->chunk_pairs) { set_error("pdf_grade_curve: load vectors first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
+  if (pair_total_dev) HIPCHK(hipMemsetAsync(pair_total_dev, 0, g->num_pairs * 4, g->stream));
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   std::vector<hipEvent_t> ev(nchunks + 1, nullptr);
   uint64_t *curve = nullptr;
@@ -1739,7 +1822,7 @@ int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_m
     if ((rc = launch_fwd<true, true>(g, start, width, robust, true))) break;
     if (!g->seq && (rc = reset_chunk(g))) break;
     if ((rc = launch_reverse<true, true>(g, start, width))) break;
-    if ((rc = launch_cover(g, start, width))) break;
+    if ((rc = launch_cover(g, start, width, pair_total_dev))) break;
     k_copy_u64<<<1, 1, 0, g->stream>>>(g->d_grand, curve + c);
     (void)hipEventRecord(ev[c + 1], g->stream);
   }

@@ -109,8 +109,12 @@ struct Tile {
 // B(k, j) = TB ? B[j * ldb + k] : B[k * ldb + j].
 // Block = BM x BN tile of C, 4 waves as 2 x 2, each wave (BM/2) x (BN/2) in
 // 16 x 16 MFMA tiles.  Split z covers k in [z * kchunk, (z + 1) * kchunk).
-// Host-checked: the contiguous dimension of each operand is a multiple of 8,
-// kchunk a multiple of kBK, pointers 16-byte aligned.
+// Host-checked: leading dimensions are multiples of 8, kchunk a multiple of
+// kBK, pointers 16-byte aligned.  Loads move 8 elements along an operand's
+// contiguous dimension and are taken whole when their first element is in
+// range, so each operand must be readable (and finite) up to the next
+// multiple of 8 of that dimension; the extra products meet zeros of the
+// other operand (k) or land in rows and columns the epilogue drops (i, j).
 template <bool TA, bool TB, int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, uint32_t lda,
                                               const uint16_t *__restrict__ B, uint32_t ldb, uint32_t M, uint32_t N,
@@ -383,8 +387,7 @@ template <bool TA, bool TB, int EPI>
 int gemm(pdf_mlp *m, const uint16_t *A, uint32_t lda, const uint16_t *B, uint32_t ldb, uint32_t M, uint32_t N,
          uint32_t K, uint32_t splits, const EpiArgs &ep) {
   constexpr int BM = 128, BN = 128;
-  // whole 16-byte loads along each operand's contiguous dimension
-  if ((TA ? M : K) % 8 || (TB ? K : N) % 8 || lda % 8 || ldb % 8 || ((uintptr_t)A | (uintptr_t)B) % 16) {
+  if (lda % 8 || ldb % 8 || ((uintptr_t)A | (uintptr_t)B) % 16) {
     pdf::set_error("mlp gemm: operand not 16-byte granular");
     return PDF_ERR_ARG;
   }
@@ -399,11 +402,11 @@ int gemm(pdf_mlp *m, const uint16_t *A, uint32_t lda, const uint16_t *B, uint32_
   return PDF_OK;
 }
 
-// H1, H2 (and y) of `rows` rows of X.
-int forward(pdf_mlp *m, const uint16_t *X, uint32_t rows, float *y, const float *target) {
+// H1, H2 (and y) of `rows` rows of X (feature-major: X[f * ld + r]).
+int forward(pdf_mlp *m, const uint16_t *X, uint64_t ld, uint32_t rows, float *y, const float *target) {
   int rc;
   EpiArgs e1{m->w + kB1, nullptr, m->h1, nullptr, kHid};
-  if ((rc = gemm<false, true, EPI_BIAS_RELU>(m, X, kIn, m->w1b, kIn, rows, kHid, kIn, 1, e1))) return rc;
+  if ((rc = gemm<true, true, EPI_BIAS_RELU>(m, X, (uint32_t)ld, m->w1b, kIn, rows, kHid, kIn, 1, e1))) return rc;
   EpiArgs e2{m->w + kB2, nullptr, m->h2, nullptr, kHid};
   if ((rc = gemm<false, true, EPI_BIAS_RELU>(m, m->h1, kHid, m->w2b, kHid, rows, kHid, kHid, 1, e2))) return rc;
   k_head_fwd<<<blocks(rows, 4), 256, 0, m->stream>>>(m->h2, m->w, rows, y, target, m->dy, m->sq);
@@ -412,13 +415,13 @@ int forward(pdf_mlp *m, const uint16_t *X, uint32_t rows, float *y, const float 
 }
 
 // Forward + backward of one batch; the gradient lands in m->g.
-int gradients(pdf_mlp *m, const uint16_t *X, const float *target, uint32_t rows) {
+int gradients(pdf_mlp *m, const uint16_t *X, uint64_t ld, const float *target, uint32_t rows) {
   int rc;
   if ((rc = ensure_rows(m, rows))) return rc;
   const uint32_t nblk = blocks(rows, kHeadRows);
   const uint32_t splits = std::max(1u, std::min(kMaxSplits, rows / 256));
   if ((rc = ensure_part(m, std::max<size_t>((size_t)splits * kHid * kIn, (size_t)3 * nblk * kHid)))) return rc;
-  if ((rc = forward(m, X, rows, m->y, target))) return rc;
+  if ((rc = forward(m, X, ld, rows, m->y, target))) return rc;
   k_mean<<<1, 1024, 0, m->stream>>>(m->sq, rows, m->loss);
   // output layer: dZ2, dw3, db2, db3
   k_head_bwd<<<nblk, 256, 0, m->stream>>>(m->h2, m->w, m->dy, rows, m->dz2, m->part);
@@ -438,7 +441,7 @@ int gradients(pdf_mlp *m, const uint16_t *X, const float *target, uint32_t rows)
   k_reduce<<<1, 256, 0, m->stream>>>(m->part, nblk, kHid, kHid, m->g + kB1);
   // dW1 = dZ1^T X
   EpiArgs ew{nullptr, nullptr, nullptr, m->part, kIn};
-  if ((rc = gemm<true, false, EPI_PARTIAL>(m, m->dz1, kHid, X, kIn, kHid, kIn, rows, splits, ew))) return rc;
+  if ((rc = gemm<true, true, EPI_PARTIAL>(m, m->dz1, kHid, X, (uint32_t)ld, kHid, kIn, rows, splits, ew))) return rc;
   k_reduce<<<blocks((size_t)kHid * kIn, 256), 256, 0, m->stream>>>(m->part, splits, (uint64_t)kHid * kIn,
                                                                   (uint64_t)kHid * kIn, m->g + kW1);
   MLPCHK(hipGetLastError());
@@ -456,13 +459,13 @@ int adam(pdf_mlp *m, float lr, float beta1, float beta2, float eps) {
   return PDF_OK;
 }
 
-int check_rows(const pdf_mlp *m, const void *X, uint64_t rows) {
+int check_rows(const pdf_mlp *m, const void *X, uint64_t ld, uint64_t rows) {
   if (!m || !X) {
     pdf::set_error("null argument");
     return PDF_ERR_ARG;
   }
-  if (rows == 0 || rows > 0xFFFFFFFFull) {
-    pdf::set_error("mlp: rows must be in [1, 2^32)");
+  if (rows == 0 || rows > 0xFFFFFFFFull || ld % 64 || ld < rows || ld > 0xFFFFFFFFull || (uintptr_t)X % 16) {
+    pdf::set_error("mlp: need 1 <= rows <= ld < 2^32, ld a multiple of 64, features 16-byte aligned");
     return PDF_ERR_ARG;
   }
   return PDF_OK;
@@ -557,8 +560,8 @@ int pdf_mlp_get_params(pdf_mlp *m, float *host) {
   return PDF_OK;
 }
 
-int pdf_mlp_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t rows, float *score_dev) {
-  int rc = check_rows(m, feat_dev, rows);
+int pdf_mlp_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t rows, float *score_dev) {
+  int rc = check_rows(m, feat_dev, ld, rows);
   if (rc) return rc;
   if (!score_dev) {
     pdf::set_error("null argument");
@@ -568,22 +571,22 @@ int pdf_mlp_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t rows, float *
   if ((rc = ensure_rows(m, std::min<uint64_t>(rows, kSlab)))) return rc;
   for (uint64_t r0 = 0; r0 < rows; r0 += kSlab) {
     const uint32_t r = (uint32_t)std::min<uint64_t>(kSlab, rows - r0);
-    if ((rc = forward(m, feat_dev + r0 * kIn, r, score_dev + r0, nullptr))) return rc;
+    if ((rc = forward(m, feat_dev + r0, ld, r, score_dev + r0, nullptr))) return rc;
   }
   MLPCHK(hipStreamSynchronize(m->stream));
   return PDF_OK;
 }
 
-int pdf_mlp_gradients(pdf_mlp *m, const uint16_t *feat_dev, const float *target_dev, uint64_t rows, float *loss,
-                      float *grad_host) {
-  int rc = check_rows(m, feat_dev, rows);
+int pdf_mlp_gradients(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const float *target_dev, uint64_t rows,
+                      float *loss, float *grad_host) {
+  int rc = check_rows(m, feat_dev, ld, rows);
   if (rc) return rc;
   if (!target_dev) {
     pdf::set_error("null argument");
     return PDF_ERR_ARG;
   }
   MLPCHK(hipSetDevice(m->device));
-  if ((rc = gradients(m, feat_dev, target_dev, (uint32_t)rows))) return rc;
+  if ((rc = gradients(m, feat_dev, ld, target_dev, (uint32_t)rows))) return rc;
   if (loss) MLPCHK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
   if (grad_host) MLPCHK(hipMemcpyAsync(grad_host, m->g, kNumParams * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   MLPCHK(hipStreamSynchronize(m->stream));
@@ -602,16 +605,16 @@ int pdf_mlp_adam(pdf_mlp *m, float lr, float beta1, float beta2, float eps) {
   return PDF_OK;
 }
 
-int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, const float *target_dev, uint64_t rows, float lr,
-                       float *loss) {
-  int rc = check_rows(m, feat_dev, rows);
+int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const float *target_dev, uint64_t rows,
+                       float lr, float *loss) {
+  int rc = check_rows(m, feat_dev, ld, rows);
   if (rc) return rc;
   if (!target_dev) {
     pdf::set_error("null argument");
     return PDF_ERR_ARG;
   }
   MLPCHK(hipSetDevice(m->device));
-  if ((rc = gradients(m, feat_dev, target_dev, (uint32_t)rows))) return rc;
+  if ((rc = gradients(m, feat_dev, ld, target_dev, (uint32_t)rows))) return rc;
   if ((rc = adam(m, lr, 0.9f, 0.999f, 1e-8f))) return rc;
   if (loss) {
     MLPCHK(hipMemcpyAsync(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));

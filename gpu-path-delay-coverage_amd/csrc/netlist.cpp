@@ -328,6 +328,35 @@ int pdf_gen_vectors(uint64_t seed, uint32_t num_pi, uint64_t num_vectors, uint64
   return PDF_OK;
 }
 
+int pdf_gen_toggle_pairs(uint64_t seed, uint32_t num_pi, uint64_t count, double q_lo, double q_hi,
+                         uint64_t *a_bits, uint64_t *b_bits) {
+  if ((!a_bits || !b_bits) && num_pi && count) { set_error("pdf_gen_toggle_pairs: null output"); return PDF_ERR_ARG; }
+  if (!(q_lo > 0.0) || !(q_lo <= q_hi) || !(q_hi <= 1.0)) {
+    set_error("pdf_gen_toggle_pairs: need 0 < q_lo <= q_hi <= 1");
+    return PDF_ERR_ARG;
+  }
+  const uint64_t words = (count + 63) / 64;
+  int rc = pdf_gen_vectors(seed, num_pi, count, a_bits);
+  if (rc) return rc;
+  std::copy(a_bits, a_bits + (size_t)num_pi * words, b_bits);
+  // per pair: q log-uniform in [q_lo, q_hi]; the flipped PIs are found by
+  // geometric skipping (gap ~ Geometric(q)), O(flips) per pair
+  Rng rng(seed ^ 0x5851F42D4C957F2DULL);
+  const double l0 = std::log(q_lo), l1 = std::log(q_hi);
+  for (uint64_t p = 0; p < count; ++p) {
+    const double q = std::exp(l0 + rng.uniform() * (l1 - l0));
+    const double lq = std::log1p(-std::min(q, 1.0 - 1e-16));
+    for (int64_t i = -1;;) {
+      const double gap = std::floor(std::log1p(-rng.uniform()) / lq);
+      if (!(gap < (double)num_pi)) break;
+      i += 1 + (int64_t)gap;
+      if (i >= (int64_t)num_pi) break;
+      b_bits[(uint64_t)i * words + (p >> 6)] ^= 1ULL << (p & 63);
+    }
+  }
+  return PDF_OK;
+}
+
 // ------------------------------------------------------------- file formats
 
 static std::string trim(const std::string &s) {

@@ -84,6 +84,7 @@ SIGNATURES = {
     "pdf_netlist_export": ([_P] + [C.c_void_p] * 8, C.c_int),
     "pdf_netlist_generate": ([C.POINTER(_GenParams), C.POINTER(_P)], C.c_int),
     "pdf_gen_vectors": ([_U64, C.c_uint32, _U64, C.c_void_p], C.c_int),
+    "pdf_gen_toggle_pairs": ([_U64, C.c_uint32, _U64, C.c_double, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grader_create": ([C.c_int, _P, C.POINTER(_P)], C.c_int),
     "pdf_grader_destroy": ([_P], None),
     "pdf_grader_set_vectors": ([_P, C.c_void_p, _U64], C.c_int),
@@ -104,18 +105,18 @@ SIGNATURES = {
     "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_gate_counts": ([_P, C.c_void_p], C.c_int),
     "pdf_get_total": ([_P, _U64P], C.c_int),
-    "pdf_pair_features": ([_P, C.c_uint32, C.c_void_p], C.c_int),
+    "pdf_pair_features": ([_P, C.c_uint32, C.c_int, C.c_void_p, _U64], C.c_int),
     "pdf_pair_gains": ([_P, C.c_int, C.c_void_p], C.c_int),
     "pdf_grader_rank_pairs": ([_P, C.c_void_p, C.c_void_p], C.c_int),
-    "pdf_grade_curve": ([_P, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_grade_curve": ([_P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_mlp_create": ([C.c_int, _U64, C.POINTER(_P)], C.c_int),
     "pdf_mlp_destroy": ([_P], None),
     "pdf_mlp_set_params": ([_P, C.c_void_p], C.c_int),
     "pdf_mlp_get_params": ([_P, C.c_void_p], C.c_int),
-    "pdf_mlp_forward": ([_P, C.c_void_p, _U64, C.c_void_p], C.c_int),
-    "pdf_mlp_gradients": ([_P, C.c_void_p, C.c_void_p, _U64, _FP, C.c_void_p], C.c_int),
+    "pdf_mlp_forward": ([_P, C.c_void_p, _U64, _U64, C.c_void_p], C.c_int),
+    "pdf_mlp_gradients": ([_P, C.c_void_p, _U64, C.c_void_p, _U64, _FP, C.c_void_p], C.c_int),
     "pdf_mlp_adam": ([_P, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
-    "pdf_mlp_train_step": ([_P, C.c_void_p, C.c_void_p, _U64, C.c_float, _FP], C.c_int),
+    "pdf_mlp_train_step": ([_P, C.c_void_p, _U64, C.c_void_p, _U64, C.c_float, _FP], C.c_int),
     "pdf_mlp_sync": ([_P], C.c_int),
 }
 
@@ -254,6 +255,16 @@ def gen_vectors(seed: int, num_pi: int, num_vectors: int) -> np.ndarray:
     return out
 
 
+def gen_toggle_pairs(seed: int, num_pi: int, count: int, q_lo: float, q_hi: float):
+    """Explicit pairs (A, B): A uniform, B = A with each PI flipped with a
+    per-pair probability log-uniform in [q_lo, q_hi]."""
+    words = (count + 63) // 64
+    a = np.zeros((num_pi, words), np.uint64)
+    b = np.zeros((num_pi, words), np.uint64)
+    _check(lib().pdf_gen_toggle_pairs(seed, num_pi, count, q_lo, q_hi, _ptr(a), _ptr(b)))
+    return a, b
+
+
 class Grader:
     """One netlist resident in HBM plus the chunk bit-planes of one vector set."""
 
@@ -390,11 +401,14 @@ class Grader:
         return t.value
 
     # -- learned pair ranking (device buffers: torch tensors on the grader's GPU)
-    def pair_features(self, max_level: int, feat):
+    def pair_features(self, max_level: int, robust: bool, feat):
         """Write the 512 BF16 features of every loaded pair into `feat`, a
-        contiguous int16/bfloat16 device tensor of shape (num_pairs, 512)."""
-        _need_device(feat, self._num_pairs * FEATURES, 2)
-        _check(lib().pdf_pair_features(self._h, max_level, C.c_void_p(feat.data_ptr())))
+        feature-major int16/bfloat16 device tensor (512, ld) or a column
+        slice of one (row stride ld a multiple of 64, >= num_pairs columns)."""
+        ptr, ld, cols = _feat_arg(feat)
+        if cols < self._num_pairs:
+            raise ValueError("feature tensor has fewer columns than pairs")
+        _check(lib().pdf_pair_features(self._h, max_level, int(robust), C.c_void_p(ptr), ld))
 
     def pair_gains(self, robust: bool, gains):
         """Each pair's marked PI->PO path count graded alone (int32 device
@@ -412,13 +426,18 @@ class Grader:
         _check(lib().pdf_grader_rank_pairs(self._h, C.c_void_p(scores.data_ptr()),
                                            C.c_void_p(perm.data_ptr() if perm is not None else 0)))
 
-    def grade_curve(self, robust: bool):
+    def grade_curve(self, robust: bool, pair_totals=None):
         """Grade the loaded pairs in order; per chunk the running total and the
-        milliseconds since the start: (uint64 array, float32 array)."""
+        milliseconds since the start: (uint64 array, float32 array).  With
+        `pair_totals` (int32 device tensor of num_pairs entries) each pair's
+        contribution to the total is written there too."""
         nch = self.num_chunks()
         tot = np.zeros(nch, np.uint64)
         ms = np.zeros(nch, np.float32)
-        _check(lib().pdf_grade_curve(self._h, int(robust), _ptr(tot), _ptr(ms)))
+        if pair_totals is not None:
+            _need_device(pair_totals, self._num_pairs, 4)
+        _check(lib().pdf_grade_curve(self._h, int(robust), _ptr(tot), _ptr(ms),
+                                     C.c_void_p(pair_totals.data_ptr() if pair_totals is not None else 0)))
         return tot, ms
 
 
@@ -457,19 +476,19 @@ class Mlp:
         _check(lib().pdf_mlp_set_params(self._h, _ptr(flat)))
 
     def forward(self, feat, scores):
-        rows = feat.shape[0]
-        _need_device(feat, rows * FEATURES, 2)
+        """scores[r] = output for column r of `feat` (feature-major (512, rows)
+        tensor or column slice)."""
+        ptr, ld, rows = _feat_arg(feat)
         _need_device(scores, rows, 4)
-        _check(lib().pdf_mlp_forward(self._h, C.c_void_p(feat.data_ptr()), rows, C.c_void_p(scores.data_ptr())))
+        _check(lib().pdf_mlp_forward(self._h, C.c_void_p(ptr), ld, rows, C.c_void_p(scores.data_ptr())))
 
     def gradients(self, feat, target):
         """(loss, flat float32 gradient) of the mean squared error; no update."""
-        rows = feat.shape[0]
-        _need_device(feat, rows * FEATURES, 2)
+        ptr, ld, rows = _feat_arg(feat)
         _need_device(target, rows, 4)
         loss = C.c_float()
         grad = np.zeros(MLP_PARAMS, np.float32)
-        _check(lib().pdf_mlp_gradients(self._h, C.c_void_p(feat.data_ptr()), C.c_void_p(target.data_ptr()), rows,
+        _check(lib().pdf_mlp_gradients(self._h, C.c_void_p(ptr), ld, C.c_void_p(target.data_ptr()), rows,
                                        C.byref(loss), _ptr(grad)))
         return loss.value, grad
 
@@ -477,16 +496,31 @@ class Mlp:
         _check(lib().pdf_mlp_adam(self._h, lr, beta1, beta2, eps))
 
     def train_step(self, feat, target, lr: float, want_loss: bool = False):
-        rows = feat.shape[0]
-        _need_device(feat, rows * FEATURES, 2)
+        """Gradient + Adam on one batch (a column slice of the feature matrix).
+        Without want_loss the call returns before the GPU finishes: keep the
+        tensors alive and call sync() before reading or changing them."""
+        ptr, ld, rows = _feat_arg(feat)
         _need_device(target, rows, 4)
         loss = C.c_float()
-        _check(lib().pdf_mlp_train_step(self._h, C.c_void_p(feat.data_ptr()), C.c_void_p(target.data_ptr()), rows,
+        _check(lib().pdf_mlp_train_step(self._h, C.c_void_p(ptr), ld, C.c_void_p(target.data_ptr()), rows,
                                         lr, C.byref(loss) if want_loss else None))
         return loss.value if want_loss else None
 
     def sync(self):
         _check(lib().pdf_mlp_sync(self._h))
+
+
+def _feat_arg(feat):
+    """(pointer, ld, columns) of a feature-major (512, columns) device tensor
+    whose rows are contiguous with a stride that is a multiple of 64."""
+    if not getattr(feat, "is_cuda", False) or feat.dim() != 2 or feat.shape[0] != FEATURES:
+        raise ValueError("expected a (512, columns) device tensor")
+    if feat.element_size() != 2 or feat.stride(1) != 1 or feat.stride(0) % 64 or feat.data_ptr() % 16:
+        raise ValueError("features: 2-byte elements, unit column stride, row stride a multiple of 64, "
+                         "16-byte aligned start")
+    import torch
+    torch.cuda.current_stream(feat.device).synchronize()
+    return feat.data_ptr(), feat.stride(0), feat.shape[1]
 
 
 def _need_device(t, count: int, itemsize: int):

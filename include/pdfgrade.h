@@ -139,6 +139,13 @@ int pdf_netlist_generate(const pdf_gen_params *params, pdf_netlist **out);
  * num_pi * ceil(num_vectors/64) words; bits past num_vectors are zero. */
 int pdf_gen_vectors(uint64_t seed, uint32_t num_pi, uint64_t num_vectors,
                     uint64_t *out_bits);
+/* Explicit pairs of varied toggle density: A_p uniform (as pdf_gen_vectors
+ * with `seed`), B_p = A_p with each PI flipped independently with
+ * probability q_p, q_p log-uniform in [q_lo, q_hi] per pair
+ * (0 < q_lo <= q_hi <= 1).  Both outputs PI-major, num_pi x ceil(count/64)
+ * words, bits past `count` zero. */
+int pdf_gen_toggle_pairs(uint64_t seed, uint32_t num_pi, uint64_t count, double q_lo, double q_hi,
+                         uint64_t *a_bits, uint64_t *b_bits);
 
 /* ------------------------------------------------------------------------ */
 /* Grader (GPU)                                                              */
@@ -234,13 +241,16 @@ int pdf_get_total(const pdf_grader *g, uint64_t *total);
 /* ------------------------------------------------------------------------ */
 #define PDF_FEATURES 512
 
-/* Per-pair features of the loaded pairs, BF16 row-major [num_pairs][512]:
- * [0,256) fraction of the nodes of each level (up to max_level) with a
- * transition; [256,384) fraction of each of 128 PI buckets that toggles
+/* Per-pair features of the loaded pairs, BF16 feature-major:
+ * feat_dev[f * ld + pair] (ld a multiple of 64, >= num_pairs):
+ * [0,128) fraction of the nodes of each level with a transition; [128,256)
+ * fraction of the nodes of each level marked by the forward pass (robust
+ * or non-robust as `robust` says), both for levels 0..min(max_level, 127);
+ * [256,384) fraction of each of 128 PI buckets that toggles
  * (bucket b = PIs [floor(b I / 128), floor((b + 1) I / 128)));
- * [384,512) fanout-weighted toggling share of each PI bucket.  Simulates only
- * levels 0..max_level. */
-int pdf_pair_features(pdf_grader *g, uint32_t max_level, uint16_t *feat_dev);
+ * [384,512) fanout-weighted toggling share of each PI bucket.  Simulates and
+ * marks only levels 0..max_level. */
+int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *feat_dev, uint64_t ld);
 /* Exact per-pair new-path gain against an empty detection history (each
  * pair graded alone): its number of marked PI->PO paths, saturated to u32. */
 int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev);
@@ -250,8 +260,9 @@ int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev);
 int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_dev);
 /* Exact grading of the loaded pairs in order, chunk by chunk: cum_total[c]
  * and cum_ms[c] (may be NULL) hold the running total and the wall time since
- * the start after chunk c. */
-int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms);
+ * the start after chunk c; pair_total_dev (device, may be NULL) receives each
+ * pair's contribution to the total (saturated to uint32). */
+int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_ms, uint32_t *pair_total_dev);
 
 /* BF16 MFMA scorer for the pair ranking: 512 -> 256 -> 256 -> 1 with ReLU,
  * FP32 accumulation, trained with Adam by the library's own forward and
@@ -265,17 +276,22 @@ int pdf_mlp_create(int device, uint64_t seed, pdf_mlp **out);
 void pdf_mlp_destroy(pdf_mlp *m);
 int pdf_mlp_set_params(pdf_mlp *m, const float *host); /* also clears the Adam state */
 int pdf_mlp_get_params(pdf_mlp *m, float *host);
-/* score_dev[r] = network output for feature row r (BF16 [rows][512]). */
-int pdf_mlp_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t rows, float *score_dev);
+/* Features are BF16 feature-major, as pdf_pair_features writes them:
+ * feature f of row r at feat_dev[f * ld + r], ld a multiple of 64 with
+ * ld >= rows; feat_dev 16-byte aligned (a column offset that is a multiple
+ * of 8 keeps this).  Columns up to the next multiple of 8 past the last row
+ * must be readable and finite.
+ * score_dev[r] = network output for row r. */
+int pdf_mlp_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t rows, float *score_dev);
 /* Mean squared error against target_dev and its gradient (flat layout, to
  * grad_host when not NULL); no update. */
-int pdf_mlp_gradients(pdf_mlp *m, const uint16_t *feat_dev, const float *target_dev, uint64_t rows, float *loss,
-                      float *grad_host);
+int pdf_mlp_gradients(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const float *target_dev, uint64_t rows,
+                      float *loss, float *grad_host);
 /* One Adam step with the last computed gradient. */
 int pdf_mlp_adam(pdf_mlp *m, float lr, float beta1, float beta2, float eps);
 /* Gradient + Adam (betas 0.9 / 0.999, eps 1e-8) on one batch. */
-int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, const float *target_dev, uint64_t rows, float lr,
-                       float *loss);
+int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const float *target_dev, uint64_t rows,
+                       float lr, float *loss);
 int pdf_mlp_sync(pdf_mlp *m);
 
 #ifdef __cplusplus

@@ -78,7 +78,8 @@ def sensitizes(robust: bool, ty: int, on, off) -> bool:
     return False
 
 
-def marks_for_pattern(types, fanins, fanouts, po, fr, robust):
+def marks_for_pattern(types, fanins, fanouts, po, fr, robust, remove=True):
+    """Marks of one pattern; remove=False stops after the forward pass."""
     n = len(types)
     m = [0] * n
     for g in range(n):
@@ -96,7 +97,7 @@ def marks_for_pattern(types, fanins, fanouts, po, fr, robust):
             m[g] = int(t and m[fi[0]] and ok)
         else:
             m[g] = 0
-    for g in reversed(range(n)):
+    for g in reversed(range(n) if remove else ()):
         if m[g] and not po[g] and not any(m[f] for f in fanouts[g]):
             m[g] = 0
     return m

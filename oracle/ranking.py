@@ -7,8 +7,9 @@ ranking built on top of the grading path).  Parity is pinned by:
   * bf16_round  -- torch's float32 -> bfloat16 conversion (an independent
                    round-to-nearest-even implementation);
   * features    -- closed forms (all-stable pairs give 0; all-toggling pairs
-                   give 1 on every PI bucket and on level 0) and a pure-Python
-                   loop over brute-force frames (brute.frames);
+                   give 1 on every PI bucket and on level 0, forward marks
+                   included) and a pure-Python loop over brute-force frames
+                   and forward marks (brute.frames, brute.sensitizes);
   * pair_gains  -- brute-force enumeration of fully marked PI->PO paths
                    (brute.marks_for_pattern + brute.suffix_paths).
 """
@@ -19,7 +20,7 @@ import numpy as np
 from . import oracle as orc
 
 FEATURES = 512
-LEVEL_FEATURES = 256
+LEVELS = 128          # levels with a transition feature and a forward-mark feature
 PI_BUCKETS = 128
 
 
@@ -40,11 +41,14 @@ def pi_bucket_bounds(num_pi: int, b: int):
     return b * num_pi // PI_BUCKETS, (b + 1) * num_pi // PI_BUCKETS
 
 
-def features(arrs: dict, codes: np.ndarray, max_level: int) -> np.ndarray:
+def features(arrs: dict, codes: np.ndarray, fwd_marks: np.ndarray, max_level: int) -> np.ndarray:
     """Feature bits (uint16, BF16) of shape (pairs, 512) from the node codes
-    (n, pairs) of a simulation (code bit 1 = transition).
+    (n, pairs) of a simulation (code bit 1 = transition) and the forward
+    marks (n, pairs) of the mark pass before removal:
 
-      [0, 256)    transitions / nodes on level f, for f <= min(max_level, L)
+      [0, 128)    transitions / nodes on level f
+      [128, 256)  forward marks / nodes on level f - 128
+                  (both for levels <= min(max_level, L, 127))
       [256, 384)  toggling PIs / PIs of bucket b
       [384, 512)  sum of fanout counts of toggling PIs / that of bucket b
     Each value is float32(count) * float32(1 / float32(denominator)).
@@ -62,14 +66,16 @@ def features(arrs: dict, codes: np.ndarray, max_level: int) -> np.ndarray:
         scale = np.float32(1.0) / np.float32(denom) if denom > 0 else np.float32(0.0)
         out[:, f] = count.astype(np.float32) * scale
 
-    for f in range(min(max_level, L, LEVEL_FEATURES - 1) + 1):
+    m = np.asarray(fwd_marks, dtype=np.int64)
+    for f in range(min(max_level, L, LEVELS - 1) + 1):
         lo, hi = level_off[f], level_off[f + 1]
         put(f, t[lo:hi].sum(axis=0), hi - lo)
+        put(LEVELS + f, m[lo:hi].sum(axis=0), hi - lo)
     for b in range(PI_BUCKETS):
         lo, hi = pi_bucket_bounds(num_pi, b)
-        put(LEVEL_FEATURES + b, t[lo:hi].sum(axis=0), hi - lo)
+        put(256 + b, t[lo:hi].sum(axis=0), hi - lo)
         w = nfo[lo:hi]
-        put(LEVEL_FEATURES + PI_BUCKETS + b, (w[:, None] * t[lo:hi]).sum(axis=0), int(w.sum()))
+        put(384 + b, (w[:, None] * t[lo:hi]).sum(axis=0), int(w.sum()))
     return bf16_round(out)
 
 

@@ -25,22 +25,33 @@ def _explicit(num_pi, P, seed):
     return pg.gen_vectors(seed, num_pi, P), pg.gen_vectors(seed + 1, num_pi, P)
 
 
-def _codes(nl, a_bits, P, b_bits=None):
-    return orc.sim_chunk(orc.OracleNet(nl.arrays()), a_bits, P, 0, P, b_bits=b_bits)
+def _oracle_features(nl, a_bits, P, max_level, b_bits=None, robust=True):
+    net = orc.OracleNet(nl.arrays())
+    codes = orc.sim_chunk(net, a_bits, P, 0, P, b_bits=b_bits)
+    _, fwd = orc.mark_chunk(net, codes, robust)
+    return orank.features(nl.arrays(), codes, fwd, max_level)
 
 
-def _features_gpu(g, P, max_level):
-    feat = torch.zeros((P, pg.FEATURES), dtype=torch.int16, device=DEV)
-    g.pair_features(max_level, feat)
-    return feat.cpu().numpy().view(np.uint16)
+def _features_gpu(g, P, max_level, pad=0, robust=True):
+    """GPU features as (P, 512) uint16, written into a feature-major matrix
+    with `pad` extra columns on each side (left untouched)."""
+    ld = (P + 2 * pad + 63) // 64 * 64 + 64 * (pad > 0)
+    feat = torch.full((pg.FEATURES, ld), 0x7F7F, dtype=torch.int16, device=DEV)
+    g.pair_features(max_level, robust, feat[:, pad:pad + P] if pad else feat)
+    full = feat.cpu().numpy().view(np.uint16)
+    if pad:
+        assert (full[:, :pad] == 0x7F7F).all() and (full[:, pad + P:] == 0x7F7F).all()
+    return full[:, pad:pad + P].T.copy()
 
 
 # ----------------------------------------------------------------- features
 
-@pytest.mark.parametrize("mode,P,chunk,max_level", [("seq", 1024, 1024, 99), ("seq", 2048, 512, 2),
-                                                    ("pairs", 1000, 256, 0), ("pairs", 777, 777, 5)])
-def test_pair_features_match_oracle(mode, P, chunk, max_level):
-    _say("features", mode, P, chunk, max_level)
+@pytest.mark.parametrize("mode,P,chunk,max_level,pad,robust", [("seq", 1024, 1024, 99, 0, True),
+                                                               ("seq", 2048, 512, 2, 64, False),
+                                                               ("pairs", 1000, 256, 0, 0, True),
+                                                               ("pairs", 777, 777, 5, 8, False)])
+def test_pair_features_match_oracle(mode, P, chunk, max_level, pad, robust):
+    _say("features", mode, P, chunk, max_level, pad, robust)
     nl = pg.Netlist.generate(small_config(21, num_pi=40, num_gates=300, levels=9, alpha=2.2))
     I = 40
     g = pg.Grader(nl)
@@ -52,8 +63,8 @@ def test_pair_features_match_oracle(mode, P, chunk, max_level):
         a, b = _explicit(I, P, 33)
         g.set_pairs(a, b, P)
     g.set_chunk(chunk)
-    got = _features_gpu(g, P, max_level)
-    want = orank.features(nl.arrays(), _codes(nl, a, P, b), max_level)
+    got = _features_gpu(g, P, max_level, pad, robust)
+    want = _oracle_features(nl, a, P, max_level, b, robust)
     np.testing.assert_array_equal(got, want)
     assert got[:, :256].any() and got[:, 256:].any()
     # the planes now hold a partial simulation; grading still starts afresh
@@ -73,7 +84,7 @@ def test_pair_features_wide_netlist_buckets():
     g = pg.Grader(nl)
     g.set_pairs(a, b, P)
     got = _features_gpu(g, P, 255)
-    np.testing.assert_array_equal(got, orank.features(nl.arrays(), _codes(nl, a, P, b), 255))
+    np.testing.assert_array_equal(got, _oracle_features(nl, a, P, 255, b))
     g.close()
 
 
@@ -135,6 +146,12 @@ def test_rank_pairs_then_grade_matches_oracle():
     g.rank_pairs(torch.from_numpy(scores).to(DEV), perm)
     order = np.argsort(-scores, kind="stable")
     np.testing.assert_array_equal(perm.cpu().numpy(), order)
+    # a second ranking with reversed scores (reusing the scratch buffers)
+    # composes with the first
+    g.rank_pairs(torch.from_numpy(-scores[order]).to(DEV), perm)
+    order2 = np.argsort(scores[order], kind="stable")
+    np.testing.assert_array_equal(perm.cpu().numpy(), order2)
+    order = order[order2]
     # the ranked pairs graded on the GPU = the oracle on host-permuted pairs
     from helpers import pack, unpack
     pa = pack(unpack(a, P)[order])
@@ -170,8 +187,19 @@ def test_grade_curve_matches_chunk_totals(mode):
         a, b = _explicit(20, P, 82)
         g.set_pairs(a, b, P)
     g.set_chunk(512)
-    cum, ms = g.grade_curve(True)
+    pt = torch.full((P,), 7, dtype=torch.int32, device=DEV)
+    cum, ms = g.grade_curve(True, pt)
     want = orc.grade(orc.OracleNet(nl.arrays()), a, P, 512, True, b_bits=b)
+    per_pair = np.cumsum(pt.cpu().numpy().astype(np.uint64))
+    np.testing.assert_array_equal(per_pair[511::512], cum)
+    # the oracle's per-pair totals: each single-pair cover against the final first[]
+    net = orc.OracleNet(nl.arrays())
+    marks, _ = orc.mark_chunk(net, orc.sim_chunk(net, a, P, 0, P, b_bits=b), True)
+    some = np.random.default_rng(0).choice(P, 64, replace=False)
+    for p in some:
+        tot, _ = orc.cover_chunk(net, np.ascontiguousarray(marks[:, p:p + 1]), int(p), want["first_t0"].copy(),
+                                 want["first_t1"].copy(), np.zeros(net.n, np.uint64))
+        assert int(pt[int(p)]) == tot
     np.testing.assert_array_equal(np.diff(np.concatenate([[0], cum])).astype(np.uint64), want["chunk_totals"])
     assert cum[-1] == want["total"] == g.total()
     assert (np.diff(ms) >= 0).all() and ms[0] > 0
@@ -194,27 +222,33 @@ def _rand_feat(rows, seed):
     return orank.bf16_round(rng.random((rows, 512)).astype(np.float32))
 
 
-def _dev16(x):
-    return torch.from_numpy(x.view(np.int16)).to(DEV)
+def _dev16(x, col0=0):
+    """(rows, 512) uint16 features -> a feature-major device column slice
+    starting at column col0 of a zero matrix with a row stride multiple of 64."""
+    rows = x.shape[0]
+    ld = (col0 + rows + 8 + 63) // 64 * 64
+    m = torch.zeros((512, ld), dtype=torch.int16, device=DEV)
+    m[:, col0:col0 + rows] = torch.from_numpy(np.ascontiguousarray(x.T).view(np.int16)).to(DEV)
+    return m[:, col0:col0 + rows]
 
 
-@pytest.mark.parametrize("rows", [1, 200, 1000, 4096])
-def test_mlp_forward_matches_oracle(rows):
-    _say("mlp forward", rows)
+@pytest.mark.parametrize("rows,col0", [(1, 0), (200, 8), (1000, 0), (4096, 64)])
+def test_mlp_forward_matches_oracle(rows, col0):
+    _say("mlp forward", rows, col0)
     m = pg.Mlp(seed=3)
     flat = _rand_params(rows)
     m.set_params(flat)
     np.testing.assert_array_equal(m.params(), flat)
     feat = _rand_feat(rows, rows + 1)
     out = torch.zeros(rows, dtype=torch.float32, device=DEV)
-    m.forward(_dev16(feat), out)
+    m.forward(_dev16(feat, col0), out)
     got = out.cpu().numpy()
     want, _ = omlp.forward(flat.astype(np.float64), orank.bf16_to_f32(feat), quant=True)
     np.testing.assert_allclose(got, want, rtol=2e-3, atol=2e-3)
     m.close()
 
 
-@pytest.mark.parametrize("rows", [256, 1000, 4096])
+@pytest.mark.parametrize("rows", [256, 1000, 4093])
 def test_mlp_gradients_match_oracle(rows):
     _say("mlp gradients", rows)
     m = pg.Mlp(seed=4)
@@ -223,7 +257,7 @@ def test_mlp_gradients_match_oracle(rows):
     feat = _rand_feat(rows, rows + 7)
     rng = np.random.default_rng(rows)
     t = rng.standard_normal(rows).astype(np.float32)
-    loss, grad = m.gradients(_dev16(feat), torch.from_numpy(t).to(DEV))
+    loss, grad = m.gradients(_dev16(feat, 24), torch.from_numpy(t).to(DEV))
     wl, wg = omlp.loss_and_grad(flat.astype(np.float64), orank.bf16_to_f32(feat), t, quant=True)
     assert abs(loss - wl) <= 1e-4 * wl
     a, b = omlp.unflatten(grad.astype(np.float64)), omlp.unflatten(wg)

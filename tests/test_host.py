@@ -163,3 +163,28 @@ def test_netlist_validation(types, po, fanins, why):
     with pytest.raises(pg.PdfError) as e:
         pg.Netlist.from_arrays(1, types, po, fanins)
     assert e.value.code == pg.ERR_NETLIST and why in str(e.value)
+
+
+def test_gen_toggle_pairs_statistics():
+    """A equals pdf_gen_vectors' output; the flip fraction per pair follows
+    the log-uniform flip probability; bits past the count are zero."""
+    I, P = 256, 20000
+    a, b = pg.gen_toggle_pairs(7, I, P, 1.0 / 256, 0.5)
+    np.testing.assert_array_equal(a, pg.gen_vectors(7, I, P))
+    a2, b2 = pg.gen_toggle_pairs(7, I, P, 1.0 / 256, 0.5)
+    np.testing.assert_array_equal(b, b2)
+    flips = np.unpackbits((a ^ b).view(np.uint8).reshape(I, -1), axis=1, bitorder="little")[:, :P].sum(axis=0)
+    # E[q] for q log-uniform on [lo, hi] is (hi - lo) / ln(hi / lo)
+    eq = (0.5 - 1 / 256) / np.log(128)
+    assert abs(flips.mean() / I - eq) < 0.01
+    # P(no flip) = E[(1 - q)^I] ~ E1(1) / ln(hi / lo) (exponential integral)
+    from scipy.special import exp1
+    assert abs((flips == 0).mean() - (exp1(1.0) - exp1(128.0)) / np.log(128)) < 0.01
+    tail = (a ^ b)[:, -1] >> np.uint64(P % 64)
+    assert not tail.any()
+    # q = 1 flips every PI of every pair
+    a1, b1 = pg.gen_toggle_pairs(3, 8, 100, 1.0, 1.0)
+    want = np.array([[2**64 - 1, (1 << 36) - 1]] * 8, np.uint64)
+    np.testing.assert_array_equal(a1 ^ b1, want)
+    with pytest.raises(pg.PdfError):
+        pg.gen_toggle_pairs(1, 8, 64, 0.0, 0.5)

@@ -30,9 +30,11 @@ def test_bf16_round_matches_torch():
 
 # ---------------------------------------------------------------- features
 
-def _codes(nl, a_bits, P, b_bits=None):
+def _feat(nl, a_bits, P, max_level, b_bits=None, robust=True):
     net = orc.OracleNet(nl.arrays())
-    return orc.sim_chunk(net, a_bits, P, 0, P, b_bits=b_bits)
+    codes = orc.sim_chunk(net, a_bits, P, 0, P, b_bits=b_bits)
+    _, fwd = orc.mark_chunk(net, codes, robust)
+    return orank.features(nl.arrays(), codes, fwd, max_level)
 
 
 def test_features_closed_forms():
@@ -43,12 +45,13 @@ def test_features_closed_forms():
     vals = rng.integers(0, 2, (P, 6)).astype(np.uint8)
     a = pack(vals)
     # stable pairs: every feature is 0
-    f = orank.features(arrs, _codes(nl, a, P, b_bits=a), max_level=10)
+    f = _feat(nl, a, P, 10, b_bits=a)
     assert not f.any()
-    # every PI toggles: PI buckets that hold a PI read 1.0, as does level 0
-    f = orank.features(arrs, _codes(nl, a, P, b_bits=pack(1 - vals)), max_level=10)
+    # every PI toggles: PI buckets that hold a PI read 1.0, as do level 0's
+    # transitions and forward marks
+    f = _feat(nl, a, P, 10, b_bits=pack(1 - vals))
     one = orank.bf16_round(np.float32([1.0]))[0]
-    assert (f[:, 0] == one).all()
+    assert (f[:, 0] == one).all() and (f[:, 128] == one).all()
     for b in range(orank.PI_BUCKETS):
         lo, hi = orank.pi_bucket_bounds(6, b)
         expect = one if hi > lo else 0
@@ -59,9 +62,10 @@ def test_features_closed_forms():
     assert sum(orank.pi_bucket_bounds(300, b)[1] - orank.pi_bucket_bounds(300, b)[0] for b in range(128)) == 300
 
 
-@pytest.mark.parametrize("max_level", [0, 2, 99])
-def test_features_match_python_loop(max_level):
-    """The numpy features against plain loops over brute-force frames."""
+@pytest.mark.parametrize("max_level,robust", [(0, True), (2, False), (99, True)])
+def test_features_match_python_loop(max_level, robust):
+    """The numpy features against plain loops over brute-force frames and
+    forward marks."""
     nl = pg.Netlist.generate(small_config(11, num_pi=9, num_gates=40, levels=6))
     arrs = nl.arrays()
     types, fanins, po = netlist_lists(arrs)
@@ -69,7 +73,11 @@ def test_features_match_python_loop(max_level):
     rng = np.random.default_rng(2)
     va = rng.integers(0, 2, (P, I)).astype(np.uint8)
     vb = rng.integers(0, 2, (P, I)).astype(np.uint8)
-    got = orank.features(arrs, _codes(nl, pack(va), P, b_bits=pack(vb)), max_level)
+    got = _feat(nl, pack(va), P, max_level, b_bits=pack(vb), robust=robust)
+    fanouts = [[] for _ in range(arrs["n"])]
+    for g in range(arrs["n"]):
+        for f in fanins[g]:
+            fanouts[f].append(g)
     level = arrs["level"]
     L = int(level.max())
     nfo = [0] * arrs["n"]
@@ -79,10 +87,13 @@ def test_features_match_python_loop(max_level):
     for p in range(P):
         fr = brute.frames(types, fanins, I, va[p].tolist(), vb[p].tolist())
         t = [a ^ b for a, b in fr]
+        fm = brute.marks_for_pattern(types, fanins, fanouts, po, fr, robust, remove=False)
         want = np.zeros(512, np.float32)
         for f in range(min(max_level, L) + 1):
             nodes = [g for g in range(arrs["n"]) if level[g] == f]
-            want[f] = np.float32(sum(t[g] for g in nodes)) * (np.float32(1) / np.float32(len(nodes)))
+            inv = np.float32(1) / np.float32(len(nodes))
+            want[f] = np.float32(sum(t[g] for g in nodes)) * inv
+            want[128 + f] = np.float32(sum(fm[g] for g in nodes)) * inv
         for b in range(128):
             members = [i for i in range(I) if b * I // 128 <= i < (b + 1) * I // 128]
             if members:

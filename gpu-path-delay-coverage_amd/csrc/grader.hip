@@ -743,11 +743,12 @@ __device__ __forceinline__ void vc_inc(uint64_t (&c)[B], uint64_t x, int k0) {
     x = t;
   }
 }
-__device__ __forceinline__ void vc_add_sliced(uint64_t (&c)[kCountBits], const uint64_t (&s)[kSmallBits]) {
+template <int B>
+__device__ __forceinline__ void vc_add_sliced(uint64_t (&c)[kCountBits], const uint64_t (&s)[B]) {
   uint64_t carry = 0;
 #pragma unroll
   for (int k = 0; k < kCountBits; ++k) {
-    const uint64_t y = k < kSmallBits ? s[k] : 0;
+    const uint64_t y = k < B ? s[k] : 0;
     const uint64_t sum = c[k] ^ y ^ carry;
     carry = (c[k] & y) | (carry & (c[k] ^ y));
     c[k] = sum;
@@ -760,29 +761,42 @@ __device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint
   return v;
 }
 
-// Thread = (word w of the chunk, feature f = blockIdx.y): consecutive lanes
-// read consecutive words of each gate row and write 128 contiguous bytes.
+// Block = kFeatWords words x kFeatSlices slices of one feature f =
+// blockIdx.y; each slice converts and writes 8 of its word's 64 pairs (one
+// 16-byte store).  For a level feature each slice counts its share of the
+// level's gates, consecutive lanes reading consecutive words of each row,
+// and the slices' bit-sliced counters are added in LDS; a PI-bucket feature
+// (a few PIs) is counted whole by every slice; a level past the simulated
+// depth is zero.
+constexpr uint32_t kFeatWords = 32, kFeatSlices = 8;
+
 template <bool SEQ>
 __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__restrict__ recs, Planes pl,
                                                   const uint32_t *__restrict__ level_off, uint32_t levels_used,
                                                   uint32_t num_pi, uint64_t first_pair, uint64_t num_pairs,
                                                   uint16_t *__restrict__ feat, uint64_t ld) {
-  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x, f = blockIdx.y;
-  if (w >= pl.words) return;
+  __shared__ uint64_t red[kFeatSlices][kCountBits][kFeatWords];
+  const uint32_t wl = threadIdx.x % kFeatWords, sl = threadIdx.x / kFeatWords;
+  const uint32_t w = blockIdx.x * kFeatWords + wl, f = blockIdx.y;
+  const bool live = w < pl.words;
   uint64_t c[kCountBits];
 #pragma unroll
   for (int k = 0; k < kCountBits; ++k) c[k] = 0;
   float denom = 0;
-  if (f < 256) {
+  const bool level_feature = f < 256 && (f & 127) < levels_used;  // block-uniform
+  if (level_feature) {
     const uint32_t lv = f & 127;
     const bool marks = f >= 128;
-    if (lv < levels_used) {
+    {
       const uint32_t lo = level_off[lv], hi = level_off[lv + 1];
-      for (uint32_t g0 = lo; g0 < hi; g0 += 63) {
+      denom = (float)(hi - lo);
+      const uint32_t gb = lo + (uint32_t)((uint64_t)(hi - lo) * sl / kFeatSlices);
+      const uint32_t ge = lo + (uint32_t)((uint64_t)(hi - lo) * (sl + 1) / kFeatSlices);
+      for (uint32_t g0 = gb; live && g0 < ge; g0 += 63) {
         uint64_t s[kSmallBits];
 #pragma unroll
         for (int k = 0; k < kSmallBits; ++k) s[k] = 0;
-        const uint32_t g1 = min(hi, g0 + 63);
+        const uint32_t g1 = min(ge, g0 + 63);
         uint32_t g = g0;
         for (; g + 8 <= g1; g += 8) {  // eight rows' loads in flight
           uint64_t x[8];
@@ -812,56 +826,105 @@ __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__r
         }
         vc_add_sliced(c, s);
       }
-      denom = (float)(hi - lo);
     }
-  } else {
+  } else if (f >= 256) {
     const uint32_t bkt = (f - 256) & 127;
     const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
     const bool weighted = f >= 384;
     for (uint32_t g = lo; g < hi; ++g) {
-      uint64_t a, b;
-      frames<SEQ>(pl, g, w, a, b);
       const uint32_t wt = weighted ? recs[g].nfo : 1u;
       denom += (float)wt;
+      if (!live) continue;
+      uint64_t a, b;
+      frames<SEQ>(pl, g, w, a, b);
       for (int k = 0; k < 17; ++k)  // weights are at most 65535
         if ((wt >> k) & 1) vc_inc<kCountBits>(c, a ^ b, k);
     }
   }
+  if (level_feature) {  // add the slices' counters
+#pragma unroll
+    for (int k = 0; k < kCountBits; ++k) red[sl][k][wl] = c[k];
+    __syncthreads();
+    if (sl == 0) {
+      for (uint32_t o = 1; o < kFeatSlices; ++o) {
+        uint64_t y[kCountBits];
+#pragma unroll
+        for (int k = 0; k < kCountBits; ++k) y[k] = red[o][k][wl];
+        vc_add_sliced(c, y);
+      }
+#pragma unroll
+      for (int k = 0; k < kCountBits; ++k) red[0][k][wl] = c[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kCountBits; ++k) c[k] = red[0][k][wl];
+  }
+  if (!live) return;
   const float scale = denom > 0 ? 1.0f / denom : 0.0f;
   const uint64_t base = first_pair + 64ull * w;
+  const uint32_t j0 = 8 * sl;
   uint16_t *dst = feat + (uint64_t)f * ld + base;
+  uint32_t v[4];
 #pragma unroll
-  for (uint32_t j0 = 0; j0 < 64; j0 += 8) {
-    uint32_t v[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e)
-      v[e] = (uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e) * scale) |
-             ((uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e + 1) * scale) << 16);
-    if (base + j0 + 8 <= num_pairs) {
-      *reinterpret_cast<uint4 *>(dst + j0) = make_uint4(v[0], v[1], v[2], v[3]);
-    } else {
-      for (uint32_t e = 0; e < 8 && base + j0 + e < num_pairs; ++e)
-        dst[j0 + e] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
-    }
+  for (int e = 0; e < 4; ++e)
+    v[e] = (uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e) * scale) |
+           ((uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e + 1) * scale) << 16);
+  if (base + j0 + 8 <= num_pairs) {
+    *reinterpret_cast<uint4 *>(dst + j0) = make_uint4(v[0], v[1], v[2], v[3]);
+  } else {
+    for (uint32_t e = 0; e < 8 && base + j0 + e < num_pairs; ++e)
+      dst[j0 + e] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
   }
 }
 
-// Per-pair permutation of PI-major packed bits: pair q of dst = pair perm[q]
-// of src.  Thread = (PI row, output word).
-__global__ void k_permute_pairs(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
-                                const uint32_t *__restrict__ perm, uint32_t num_pi, uint64_t words, uint64_t num) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (uint64_t)num_pi * words) return;
-  const uint64_t i = t / words, k = t - i * words;
-  const uint64_t *row = src + i * words;
-  uint64_t out = 0;
-  for (uint32_t j = 0; j < 64; ++j) {
-    const uint64_t q = 64 * k + j;
-    if (q >= num) break;
-    const uint32_t s = perm[q];
-    out |= ((row[s >> 6] >> (s & 63)) & 1ULL) << j;
+// Pair permutation of PI-major packed bits by 64 x 64 bit transposes.  A
+// wave per (PI block ib, word w); lane l holds one 64-bit word and 64
+// ballots transpose the tile.  Pair-major layout: pm[p * nb + ib] holds PIs
+// 64 ib .. 64 ib + 63 of pair p.
+__global__ __launch_bounds__(256) void k_pi_to_pair(const uint64_t *__restrict__ src, uint64_t stride,
+                                                    uint32_t num_pi, uint64_t words, uint32_t nb,
+                                                    uint64_t *__restrict__ pm) {
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  if (wave >= words * nb) return;  // wave-uniform
+  const uint64_t w = wave / nb;
+  const uint32_t ib = (uint32_t)(wave - w * nb);
+  const uint32_t i = 64 * ib + lane;
+  const uint64_t x = i < num_pi ? src[(uint64_t)i * stride + w] : 0;
+  uint64_t mine = 0;
+#pragma unroll
+  for (int j = 0; j < 64; ++j) {
+    const uint64_t m = __ballot((x >> j) & 1);
+    if (lane == (uint32_t)j) mine = m;
   }
-  dst[t] = out;
+  pm[(64 * w + lane) * nb + ib] = mine;
+}
+
+// dst pair q = src pair perm[q] (q < num), zero beyond.
+__global__ void k_gather_pairs(const uint64_t *__restrict__ src, uint64_t *__restrict__ dst,
+                               const uint32_t *__restrict__ perm, uint64_t num, uint32_t nb, uint64_t total) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint64_t q = t / nb;
+  const uint32_t ib = (uint32_t)(t - q * nb);
+  dst[t] = q < num ? src[(uint64_t)perm[q] * nb + ib] : 0;
+}
+
+__global__ __launch_bounds__(256) void k_pair_to_pi(const uint64_t *__restrict__ pm, uint32_t nb, uint32_t num_pi,
+                                                    uint64_t words, uint64_t *__restrict__ dst, uint64_t stride) {
+  const uint64_t wave = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  if (wave >= words * nb) return;  // wave-uniform
+  const uint64_t w = wave / nb;
+  const uint32_t ib = (uint32_t)(wave - w * nb);
+  const uint64_t y = pm[(64 * w + lane) * nb + ib];
+  uint64_t mine = 0;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    const uint64_t m = __ballot((y >> i) & 1);
+    if (lane == (uint32_t)i) mine = m;
+  }
+  if (64 * ib + lane < num_pi) dst[(uint64_t)(64 * ib + lane) * stride + w] = mine;
 }
 
 __global__ void k_iota(uint32_t *p, uint64_t n) {
@@ -917,7 +980,7 @@ struct pdf_grader {
   // ranking scratch (pdf_grader_rank_pairs), kept between calls
   float *d_rk_keys = nullptr;
   uint32_t *d_rk_idx = nullptr, *d_rk_perm = nullptr;
-  uint64_t *d_rk_va = nullptr, *d_rk_vb = nullptr;
+  uint64_t *d_rk_va = nullptr, *d_rk_vb = nullptr, *d_rk_pm0 = nullptr, *d_rk_pm1 = nullptr;
   void *d_rk_temp = nullptr;
   size_t rk_pairs = 0, rk_vsize = 0, rk_temp_bytes = 0;
   // captured fused grade (pdf_grade), rebuilt when its shape changes
@@ -1137,6 +1200,8 @@ void free_rank(pdf_grader *g) {
   dfree(g->d_rk_perm);
   dfree(g->d_rk_va);
   dfree(g->d_rk_vb);
+  dfree(g->d_rk_pm0);
+  dfree(g->d_rk_pm1);
   if (g->d_rk_temp) (void)hipFree(g->d_rk_temp);
   g->d_rk_temp = nullptr;
   g->rk_pairs = g->rk_vsize = g->rk_temp_bytes = 0;
@@ -1700,7 +1765,7 @@ int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *f
     const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
     if ((rc = launch_fwd<true, true>(g, start, width, robust, false, last))) return rc;
     const Planes pl = planes(g, width);
-    const dim3 grid(grid_for(pl.words, 256), kFeat);
+    const dim3 grid(grid_for(pl.words, kFeatWords), kFeat);
     if (g->seq)
       k_features<true><<<grid, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
                                                    g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
@@ -1754,6 +1819,8 @@ int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_
   HIPCHK(hipSetDevice(g->device));
   const uint64_t num = g->num_pairs;
   const size_t vsize = (size_t)g->nl.num_pi * g->vstride;
+  const uint32_t nb = (g->nl.num_pi + 63) / 64;
+  const uint64_t pm_words = g->vstride * 64 * nb;  // pair-major copy of one vector set
   size_t temp_bytes = 0;
   HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, score_dev, g->d_rk_keys, g->d_rk_idx,
                                                       g->d_rk_perm, (int)num, 0, 32, g->stream));
@@ -1765,6 +1832,8 @@ int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_
     if (!rc) rc = dalloc(&g->d_rk_perm, num);
     if (!rc) rc = dalloc(&g->d_rk_va, vsize);
     if (!rc) rc = dalloc(&g->d_rk_vb, vsize);
+    if (!rc) rc = dalloc(&g->d_rk_pm0, pm_words);
+    if (!rc) rc = dalloc(&g->d_rk_pm1, pm_words);
     if (!rc && hipMalloc(&g->d_rk_temp, temp_bytes) != hipSuccess) {
       (void)hipGetLastError();
       g->d_rk_temp = nullptr;
@@ -1784,10 +1853,18 @@ int pdf_grader_rank_pairs(pdf_grader *g, const float *score_dev, uint32_t *perm_
   temp_bytes = g->rk_temp_bytes;
   HIPCHK(hipcub::DeviceRadixSort::SortPairsDescending(g->d_rk_temp, temp_bytes, score_dev, g->d_rk_keys,
                                                       g->d_rk_idx, g->d_rk_perm, (int)num, 0, 32, g->stream));
-  k_permute_pairs<<<grid_for(vsize, 256), 256, 0, g->stream>>>(g->d_va, g->d_rk_va, g->d_rk_perm, g->nl.num_pi,
-                                                              g->vstride, num);
-  k_permute_pairs<<<grid_for(vsize, 256), 256, 0, g->stream>>>(g->d_vb, g->d_rk_vb, g->d_rk_perm, g->nl.num_pi,
-                                                              g->vstride, num);
+  // each vector set: to pair-major, gather, back to PI-major
+  const uint64_t tiles = g->vstride * nb;  // one wave each
+  for (int set = 0; set < 2; ++set) {
+    const uint64_t *src = set ? g->d_vb : g->d_va;
+    uint64_t *dst = set ? g->d_rk_vb : g->d_rk_va;
+    k_pi_to_pair<<<grid_for(tiles * 64, 256), 256, 0, g->stream>>>(src, g->vstride, g->nl.num_pi, g->vstride, nb,
+                                                                   g->d_rk_pm0);
+    k_gather_pairs<<<grid_for(pm_words, 256), 256, 0, g->stream>>>(g->d_rk_pm0, g->d_rk_pm1, g->d_rk_perm, num, nb,
+                                                                   pm_words);
+    k_pair_to_pi<<<grid_for(tiles * 64, 256), 256, 0, g->stream>>>(g->d_rk_pm1, nb, g->nl.num_pi, g->vstride,
+                                                                   dst, g->vstride);
+  }
   HIPCHK(hipGetLastError());
   if (perm_dev) HIPCHK(hipMemcpyAsync(perm_dev, g->d_rk_perm, num * 4, hipMemcpyDeviceToDevice, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));

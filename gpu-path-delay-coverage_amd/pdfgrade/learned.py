@@ -107,14 +107,21 @@ def log_records(cfg: LearnedConfig, device: str = "cuda:0"):
     return feat, gains
 
 
-def gain_target(gains):
-    """Regression target: log(1 + gain), standardised.  Returns (target,
-    mean, std)."""
+def gain_target(gains, per_circuit: int = 0):
+    """Regression target: log(1 + gain), standardised over all records, or
+    within each circuit's block of `per_circuit` records.  Returns (target,
+    mean, std) of the unstandardised values.  (Per-circuit standardising,
+    shuffled batches, binary targets and more epochs were measured on
+    held-out circuits and did not rank better.)"""
     torch = _torch()
     g = gains.to(torch.int64) & 0xFFFFFFFF
     t = torch.log1p(g.to(torch.float64))
-    mean, std = float(t.mean()), float(t.std()) or 1.0
-    return ((t - mean) / std).to(torch.float32).contiguous(), mean, std
+    mean, std = float(t.mean()), float(t.std())
+    blocks = t.view(-1, per_circuit) if per_circuit else t.view(1, -1)
+    m = blocks.mean(dim=1, keepdim=True)
+    sd = blocks.std(dim=1, keepdim=True).clamp_min(1e-6)
+    return ((blocks - m) / sd).reshape(-1).to(torch.float32).contiguous(), mean, std
+
 
 
 # ----------------------------------------------------------------- training

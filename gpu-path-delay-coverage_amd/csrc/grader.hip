@@ -761,30 +761,31 @@ __device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint
   return v;
 }
 
-// Block = kFeatWords words x kFeatSlices slices of one feature f =
-// blockIdx.y; each slice converts and writes 8 of its word's 64 pairs (one
-// 16-byte store).  For a level feature each slice counts its share of the
-// level's gates, consecutive lanes reading consecutive words of each row,
-// and the slices' bit-sliced counters are added in LDS; a PI-bucket feature
-// (a few PIs) is counted whole by every slice; a level past the simulated
-// depth is zero.
+// Block = kFeatWords words x kFeatSlices slices of one feature; each slice
+// converts and writes 8 of its word's 64 pairs (one 16-byte store).
+// LEVELS: feature f = blockIdx.y mapped to the level features of the
+// simulated levels (transitions, then forward marks); each slice counts its
+// share of the level's gates, consecutive lanes reading consecutive words of
+// each row, and the slices' bit-sliced counters are added in LDS.  Otherwise
+// f = 256 + blockIdx.y, a PI-bucket feature (a few PIs), counted whole by
+// every slice.  Levels past the simulated depth are zero-filled by the host.
 constexpr uint32_t kFeatWords = 32, kFeatSlices = 8;
 
-template <bool SEQ>
+template <bool SEQ, bool LEVELS>
 __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__restrict__ recs, Planes pl,
                                                   const uint32_t *__restrict__ level_off, uint32_t levels_used,
                                                   uint32_t num_pi, uint64_t first_pair, uint64_t num_pairs,
                                                   uint16_t *__restrict__ feat, uint64_t ld) {
-  __shared__ uint64_t red[kFeatSlices][kCountBits][kFeatWords];
   const uint32_t wl = threadIdx.x % kFeatWords, sl = threadIdx.x / kFeatWords;
-  const uint32_t w = blockIdx.x * kFeatWords + wl, f = blockIdx.y;
+  const uint32_t w = blockIdx.x * kFeatWords + wl;
+  const uint32_t f = LEVELS ? (blockIdx.y < levels_used ? blockIdx.y : 128 + blockIdx.y - levels_used)
+                            : 256 + blockIdx.y;
   const bool live = w < pl.words;
   uint64_t c[kCountBits];
 #pragma unroll
   for (int k = 0; k < kCountBits; ++k) c[k] = 0;
   float denom = 0;
-  const bool level_feature = f < 256 && (f & 127) < levels_used;  // block-uniform
-  if (level_feature) {
+  if (LEVELS) {
     const uint32_t lv = f & 127;
     const bool marks = f >= 128;
     {
@@ -827,7 +828,7 @@ __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__r
         vc_add_sliced(c, s);
       }
     }
-  } else if (f >= 256) {
+  } else {
     const uint32_t bkt = (f - 256) & 127;
     const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
     const bool weighted = f >= 384;
@@ -841,7 +842,8 @@ __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__r
         if ((wt >> k) & 1) vc_inc<kCountBits>(c, a ^ b, k);
     }
   }
-  if (level_feature) {  // add the slices' counters
+  if constexpr (LEVELS) {  // add the slices' counters
+    __shared__ uint64_t red[kFeatSlices][kCountBits][kFeatWords];
 #pragma unroll
     for (int k = 0; k < kCountBits; ++k) red[sl][k][wl] = c[k];
     __syncthreads();
@@ -1760,18 +1762,28 @@ int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *f
   const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   int rc;
+  if (last < 127) {  // level features past the simulated depth are zero
+    for (uint64_t f0 : {(uint64_t)last + 1, (uint64_t)128 + last + 1})
+      HIPCHK(hipMemset2DAsync(feat_dev + f0 * ld, ld * 2, 0, g->num_pairs * 2, 127 - last, g->stream));
+  }
   for (uint64_t c = 0; c < nchunks; ++c) {
     const uint64_t start = c * g->chunk_pairs;
     const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
     if ((rc = launch_fwd<true, true>(g, start, width, robust, false, last))) return rc;
     const Planes pl = planes(g, width);
-    const dim3 grid(grid_for(pl.words, kFeatWords), kFeat);
-    if (g->seq)
-      k_features<true><<<grid, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
-                                                   g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
-    else
-      k_features<false><<<grid, 256, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off, last + 1,
-                                                    g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
+    const dim3 lgrid(grid_for(pl.words, kFeatWords), 2 * (last + 1)), pgrid(grid_for(pl.words, kFeatWords), 256);
+    const DevNet net = g->net();
+    if (g->seq) {
+      k_features<true, true><<<lgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1, g->nl.num_pi,
+                                                           start, g->num_pairs, feat_dev, ld);
+      k_features<true, false><<<pgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1,
+                                                            g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
+    } else {
+      k_features<false, true><<<lgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1,
+                                                            g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
+      k_features<false, false><<<pgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1,
+                                                             g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
+    }
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(g->stream));

@@ -96,8 +96,7 @@ struct LevelMap {
   uint32_t main_blocks;
 };
 
-__device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t &g, uint32_t &w) {
-  const uint32_t b = blockIdx.x;
+__device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t b, uint32_t &g, uint32_t &w) {
   const uint32_t grp = b / mp.T, tile = b - grp * mp.T;
   w = (tile << mp.twlog) + (threadIdx.x & ((1u << mp.twlog) - 1));
   const uint32_t gi = grp * (kBlock >> mp.twlog) + (threadIdx.x >> mp.twlog);
@@ -205,18 +204,21 @@ __device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Plane
 // only): words per lane, w and w + 64 of a 128-word tile; every load of both
 // words is issued before any of them is used, so a level's waves fit in one
 // resident round with twice the loads in flight.
+// The body takes its block index `blk` of `nblk` explicitly, so one launch
+// can also run the levels of several circuits (k_fwd_batch).
 template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST, bool WIDE, int NW>
-__global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__restrict__ recs, Planes pl,
-                                                const uint64_t *__restrict__ va,
-                                                const uint64_t *__restrict__ vb, uint64_t vstride, uint64_t wbase,
-                                                LevelMap mp, uint32_t *__restrict__ clear32, uint32_t clear32_n) {
+__device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
+                                         const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
+                                         uint64_t vstride, uint64_t wbase, const LevelMap &mp,
+                                         uint32_t *__restrict__ clear32, uint32_t clear32_n, uint32_t blk,
+                                         uint32_t nblk) {
   static_assert(NW == 1 || WIDE, "two words per lane needs whole-wave tiles");
-  if (SEQ && DO_SIM && blockIdx.x >= mp.main_blocks) {
+  if (SEQ && DO_SIM && blk >= mp.main_blocks) {
     // extra column: the node values of the 64 vectors after the chunk
-    const uint32_t gi = (blockIdx.x - mp.main_blocks) * kBlock + threadIdx.x;
+    const uint32_t gi = (blk - mp.main_blocks) * kBlock + threadIdx.x;
     // these threads also clear a caller-given u32 range (the next pass's
     // counters), striding over it
-    const uint32_t nthreads = (gridDim.x - mp.main_blocks) * kBlock;
+    const uint32_t nthreads = (nblk - mp.main_blocks) * kBlock;
     for (uint32_t i = gi; i < clear32_n; i += nthreads) clear32[i] = 0;
     if (gi >= mp.count) return;
     const uint32_t g = mp.lo + gi;
@@ -228,9 +230,9 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
   }
   uint32_t g, w;
   if (NW == 1) {
-    if (!map_item(mp, g, w)) return;
+    if (!map_item(mp, blk, g, w)) return;
   } else {
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = blk;
     const uint32_t grp = b / mp.T, tile = b - grp * mp.T;
     const uint32_t gi = grp * (kBlock >> 6) + (threadIdx.x >> 6);
     if (gi >= mp.count) return;
@@ -380,6 +382,15 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
   }
 }
 
+template <bool SEQ, bool DO_SIM, bool DO_MARK, bool ROBUST, bool WIDE, int NW>
+__global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                const uint64_t *__restrict__ va,
+                                                const uint64_t *__restrict__ vb, uint64_t vstride, uint64_t wbase,
+                                                LevelMap mp, uint32_t *__restrict__ clear32, uint32_t clear32_n) {
+  fwd_body<SEQ, DO_SIM, DO_MARK, ROBUST, WIDE, NW>(net, recs, pl, va, vb, vstride, wbase, mp, clear32, clear32_n,
+                                                   blockIdx.x, gridDim.x);
+}
+
 // Reverse pass over one level.  DO_REMOVE: m &= po | OR_fanouts m
 // (markkernel.cuh:279-291), gathered by the fanouts: every node whose final
 // mark is non-zero ORs it into ACC of each fanin marked at the same word
@@ -388,13 +399,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
 // between passes.  DO_MERGE: earliest marked T0/T1 pair per node
 // (mergekernel.cuh:141-197) and the per-word list of marked nodes.
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WIDE>
-__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *__restrict__ recs, Planes pl,
-                                                    LevelMap mp, uint32_t start,
-                                                    uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
-                                                    uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
-                                                    uint64_t list_stride) {
+__device__ __forceinline__ void rev_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
+                                         const LevelMap &mp, uint32_t start, uint32_t *__restrict__ first_t0,
+                                         uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                         uint32_t *__restrict__ list_cnt, uint64_t list_stride, uint32_t blk) {
   uint32_t g, w;
-  if (!map_item(mp, g, w)) return;
+  if (!map_item(mp, blk, g, w)) return;
   if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
   // round 1: this node's mark-present word and record
   const uint64_t nzw = pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)];
@@ -446,6 +456,16 @@ __global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *_
   }
 }
 
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                    LevelMap mp, uint32_t start,
+                                                    uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
+                                                    uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                    uint64_t list_stride) {
+  rev_body<SEQ, DO_REMOVE, DO_MERGE, WIDE>(net, recs, pl, mp, start, first_t0, first_t1, list, list_cnt, list_stride,
+                                           blockIdx.x);
+}
+
 // Cross-chunk state to its initial value (one launch per grade).
 __global__ void k_reset_grade(uint32_t n, uint32_t *first_t0, uint32_t *first_t1, uint64_t *gate_counts,
                               uint64_t *grand_total, uint32_t *err) {
@@ -468,11 +488,10 @@ __global__ void k_reset_chunk(uint32_t *state, uint32_t n) {
 }
 
 // Distinct first-detection patterns of this chunk (relative index).
-__global__ void k_events(uint32_t n, const uint32_t *__restrict__ first_t0, const uint32_t *__restrict__ first_t1,
-                         uint32_t start, uint32_t width, uint64_t *__restrict__ actmap,
-                         uint32_t *__restrict__ act_list, uint32_t *__restrict__ act_count) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
+__device__ __forceinline__ void events_one(uint32_t g, const uint32_t *__restrict__ first_t0,
+                                           const uint32_t *__restrict__ first_t1, uint32_t start, uint32_t width,
+                                           uint64_t *__restrict__ actmap, uint32_t *__restrict__ act_list,
+                                           uint32_t *__restrict__ act_count) {
   const uint32_t x[2] = {first_t0[g], first_t1[g]};
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
@@ -482,6 +501,13 @@ __global__ void k_events(uint32_t n, const uint32_t *__restrict__ first_t0, cons
     const uint64_t old = atomicOr((unsigned long long *)&actmap[rel >> 6], (unsigned long long)bit);
     if (!(old & bit)) act_list[atomicAdd(act_count, 1u)] = rel;
   }
+}
+
+__global__ void k_events(uint32_t n, const uint32_t *__restrict__ first_t0, const uint32_t *__restrict__ first_t1,
+                         uint32_t start, uint32_t width, uint64_t *__restrict__ actmap,
+                         uint32_t *__restrict__ act_list, uint32_t *__restrict__ act_count) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < n) events_one(g, first_t0, first_t1, start, width, actmap, act_list, act_count);
 }
 
 __device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
@@ -512,6 +538,197 @@ __device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 26544357
 // PI->PO paths; written to pair_gain instead of the running totals.  In the
 // normal mode a non-null pair_gain receives each processed pair's total
 // (saturated to uint32; the caller zeroes it for the pairs not processed).
+struct CoverLds {
+  uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap], s_fo_off[kCoverCap];
+  uint64_t s_c[kCoverCap], s_h[kCoverCap];
+  uint32_t h_key[kHashSlots];
+  uint32_t s_pool[kPool];
+  uint16_t s_nfo[kCoverCap];
+  uint16_t h_val[kHashSlots];
+  uint8_t s_flags[kCoverCap];  // bit0 po, bit1 INPT, bit2 new
+};
+
+// The DP of one pair (relative index rel in the chunk) by one wave.
+template <bool ALL_NEW>
+__device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint32_t start,
+                          const uint32_t *__restrict__ list, const uint32_t *__restrict__ list_cnt,
+                          uint64_t list_stride, const uint32_t *__restrict__ first_t0,
+                          const uint32_t *__restrict__ first_t1, uint64_t *c_s, uint64_t *h_s,
+                          uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
+                          uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
+                          uint32_t *__restrict__ pair_gain, uint32_t rel) {
+  const uint32_t lane = threadIdx.x;
+  const uint64_t below = (1ULL << lane) - 1;
+  const uint32_t w = rel >> 6, j = rel & 63;
+  const uint32_t p = start + rel;
+  const uint32_t len = list_cnt[w];
+  const uint32_t *L = list + (uint64_t)w * list_stride;
+  // 1. the nodes marked at p, in list order
+  for (uint32_t t = lane; t < kHashSlots; t += 64) S.h_key[t] = kEmpty;
+  uint32_t K = 0;
+  for (uint32_t base = 0; base < len; base += 64) {
+    const uint32_t idx = base + lane;
+    const uint32_t g = idx < len ? L[idx] : 0;
+    const bool marked = idx < len && ((mark_of(pl, g, w) >> j) & 1);
+    const uint64_t bal = __ballot(marked);
+    const uint32_t pos = K + (uint32_t)__builtin_popcountll(bal & below);
+    if (marked && pos < kCoverCap) S.s_node[pos] = g;
+    K += (uint32_t)__builtin_popcountll(bal);
+  }
+  __syncthreads();
+  const bool in_lds = K <= kCoverCap;
+  uint64_t tot = 0;
+  bool ovf = false;
+  if (in_lds) {
+    // 2. records, hash, and the fanout-id pool offsets
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < K; base += 64) {
+      const uint32_t idx = base + lane;
+      uint32_t nfo = 0;
+      if (idx < K) {
+        const uint32_t g = S.s_node[idx];
+        const GateRec r = net.rec[g];
+        S.s_level[idx] = net.level[g];
+        nfo = r.nfo;
+        S.s_nfo[idx] = r.nfo;
+        S.s_fo_off[idx] = r.fout_off;
+        S.s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
+                                 ((ALL_NEW || first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
+        uint32_t h = hash_slot(g);
+        while (atomicCAS(&S.h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
+        S.h_val[h] = (uint16_t)idx;
+      }
+      uint32_t incl = nfo;  // inclusive wave scan of the fanout counts
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off);
+        if (lane >= (uint32_t)off) incl += o;
+      }
+      if (idx < K) S.s_pool_off[idx] = carry + incl - nfo;
+      carry += __shfl(incl, 63);
+    }
+    __syncthreads();
+    // 3. fanout ids into the pool, one pool slot per lane and eight loads in
+    // flight (nodes past the pool's end read theirs in the DP)
+    const uint32_t F = carry < kPool ? carry : kPool;
+    for (uint32_t base = 0; base < F; base += 8 * 64) {
+      uint32_t ids[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t slot = base + u * 64 + lane;
+        ids[u] = 0;
+        if (slot < F) {
+          uint32_t lo = 0, hi = K - 1;  // last node whose pool range starts at or before slot
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (S.s_pool_off[mid] <= slot) lo = mid;
+            else hi = mid - 1;
+          }
+          ids[u] = net.fout[S.s_fo_off[lo] + (slot - S.s_pool_off[lo])];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const uint32_t slot = base + u * 64 + lane;
+        if (slot < F) S.s_pool[slot] = ids[u];
+      }
+    }
+    __syncthreads();
+    // 4. the DP over runs of equal level, in LDS
+    for (uint32_t i = 0; i < K;) {
+      const uint32_t idx = i + lane;
+      const bool valid = idx < K;
+      const uint32_t lv = valid ? S.s_level[idx] : 0xFFFFFFFFu;
+      const uint32_t lv0 = __shfl(lv, 0);
+      const uint64_t same = __ballot(valid && lv == lv0);
+      const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;  // contiguous prefix
+      if (lane < run) {
+        const uint8_t fl = S.s_flags[idx];
+        const uint32_t off = S.s_pool_off[idx], nfo = S.s_nfo[idx];
+        const bool pooled = off + nfo <= kPool;
+        const uint32_t fo_off = S.s_fo_off[idx];
+        uint64_t C = 0, H = fl & 1;
+        for (uint32_t e = 0; e < nfo; ++e) {
+          const uint32_t fo = pooled ? S.s_pool[off + e] : net.fout[fo_off + e];
+          uint32_t h = hash_slot(fo);
+          uint32_t key;
+          while ((key = S.h_key[h]) != kEmpty && key != fo) h = (h + 1) & (kHashSlots - 1);
+          if (key == fo) {
+            ovf |= add_ovf(C, S.s_c[S.h_val[h]]);
+            ovf |= add_ovf(H, S.s_h[S.h_val[h]]);
+          }
+        }
+        uint64_t c = C, h = H;
+        if (fl & 4) { ovf |= add_ovf(c, H); h = 0; }
+        if (fl & 2) h = 0;
+        S.s_c[idx] = c;
+        S.s_h[idx] = h;
+      }
+      __syncthreads();
+      i += run;
+    }
+    // 5. per-gate counts and the PI total, all nodes at once
+    for (uint32_t idx = lane; idx < K; idx += 64) {
+      const uint64_t c = S.s_c[idx];
+      if (!c) continue;
+      if (!ALL_NEW) {
+        const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[S.s_node[idx]], (unsigned long long)c);
+        ovf |= old + c < old;
+      }
+      if (S.s_flags[idx] & 2) ovf |= add_ovf(tot, c);
+    }
+  } else {
+    // too many marked nodes for LDS: c/h in global scratch, the whole list
+    for (uint32_t i = 0; i < len;) {
+      const uint32_t idx = i + lane;
+      const bool valid = idx < len;
+      const uint32_t g = valid ? L[idx] : 0;
+      const uint32_t lv = valid ? net.level[g] : 0xFFFFFFFFu;
+      const uint32_t lv0 = __shfl(lv, 0);
+      const uint64_t same = __ballot(valid && lv == lv0);
+      const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
+      if (lane < run && ((mark_of(pl, g, w) >> j) & 1)) {
+        const GateRec r = net.rec[g];
+        uint64_t C = 0, H = r.po == 1 ? 1 : 0;
+        for (uint32_t e = 0; e < r.nfo; ++e) {
+          const uint32_t fo = net.fout[r.fout_off + e];
+          if ((mark_of(pl, fo, w) >> j) & 1) {
+            ovf |= add_ovf(C, c_s[fo]);
+            ovf |= add_ovf(H, h_s[fo]);
+          }
+        }
+        uint64_t c = C, h = H;
+        if (ALL_NEW || first_t0[g] == p || first_t1[g] == p) { ovf |= add_ovf(c, H); h = 0; }
+        if (r.type == PDF_INPT) h = 0;
+        c_s[g] = c;
+        h_s[g] = h;
+        if (c) {
+          if (!ALL_NEW) {
+            const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
+            ovf |= old + c < old;
+          }
+          if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+      i += run;
+    }
+  }
+  // wave sum of the PI counts of this pattern (kernSumSingle)
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_down(tot, off);
+    ovf |= add_ovf(tot, o);
+  }
+  if (pair_gain && lane == 0) pair_gain[p] = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
+  if (!ALL_NEW && lane == 0 && tot) {
+    const uint64_t o0 = atomicAdd((unsigned long long *)chunk_total, (unsigned long long)tot);
+    const uint64_t o1 = atomicAdd((unsigned long long *)grand_total, (unsigned long long)tot);
+    ovf |= (o0 + tot < o0) || (o1 + tot < o1);
+  }
+  if (__ballot(ovf)) { if (lane == 0) atomicOr(err, 1u); }
+  __syncthreads();
+}
+
 template <bool ALL_NEW>
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
@@ -522,189 +739,13 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
                                               uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
                                               uint32_t width, uint32_t *__restrict__ pair_gain) {
-  __shared__ uint32_t s_node[kCoverCap], s_level[kCoverCap], s_pool_off[kCoverCap], s_fo_off[kCoverCap];
-  __shared__ uint16_t s_nfo[kCoverCap];
-  __shared__ uint8_t s_flags[kCoverCap];  // bit0 po, bit1 INPT, bit2 new
-  __shared__ uint64_t s_c[kCoverCap], s_h[kCoverCap];
-  __shared__ uint32_t h_key[kHashSlots];
-  __shared__ uint16_t h_val[kHashSlots];
-  __shared__ uint32_t s_pool[kPool];
-  const uint32_t lane = threadIdx.x;
-  const uint64_t below = (1ULL << lane) - 1;
+  __shared__ CoverLds S;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
   uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
   const uint32_t nact = ALL_NEW ? width : *act_count;
-  for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x) {
-    const uint32_t rel = ALL_NEW ? k : act_list[k];
-    const uint32_t w = rel >> 6, j = rel & 63;
-    const uint32_t p = start + rel;
-    const uint32_t len = list_cnt[w];
-    const uint32_t *L = list + (uint64_t)w * list_stride;
-    // 1. the nodes marked at p, in list order
-    for (uint32_t t = lane; t < kHashSlots; t += 64) h_key[t] = kEmpty;
-    uint32_t K = 0;
-    for (uint32_t base = 0; base < len; base += 64) {
-      const uint32_t idx = base + lane;
-      const uint32_t g = idx < len ? L[idx] : 0;
-      const bool marked = idx < len && ((mark_of(pl, g, w) >> j) & 1);
-      const uint64_t bal = __ballot(marked);
-      const uint32_t pos = K + (uint32_t)__builtin_popcountll(bal & below);
-      if (marked && pos < kCoverCap) s_node[pos] = g;
-      K += (uint32_t)__builtin_popcountll(bal);
-    }
-    __syncthreads();
-    const bool in_lds = K <= kCoverCap;
-    uint64_t tot = 0;
-    bool ovf = false;
-    if (in_lds) {
-      // 2. records, hash, and the fanout-id pool offsets
-      uint32_t carry = 0;
-      for (uint32_t base = 0; base < K; base += 64) {
-        const uint32_t idx = base + lane;
-        uint32_t nfo = 0;
-        if (idx < K) {
-          const uint32_t g = s_node[idx];
-          const GateRec r = net.rec[g];
-          s_level[idx] = net.level[g];
-          nfo = r.nfo;
-          s_nfo[idx] = r.nfo;
-          s_fo_off[idx] = r.fout_off;
-          s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
-                                   ((ALL_NEW || first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
-          uint32_t h = hash_slot(g);
-          while (atomicCAS(&h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
-          h_val[h] = (uint16_t)idx;
-        }
-        uint32_t incl = nfo;  // inclusive wave scan of the fanout counts
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t o = __shfl_up(incl, off);
-          if (lane >= (uint32_t)off) incl += o;
-        }
-        if (idx < K) s_pool_off[idx] = carry + incl - nfo;
-        carry += __shfl(incl, 63);
-      }
-      __syncthreads();
-      // 3. fanout ids into the pool, one pool slot per lane and eight loads in
-      // flight (nodes past the pool's end read theirs in the DP)
-      const uint32_t F = carry < kPool ? carry : kPool;
-      for (uint32_t base = 0; base < F; base += 8 * 64) {
-        uint32_t ids[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t slot = base + u * 64 + lane;
-          ids[u] = 0;
-          if (slot < F) {
-            uint32_t lo = 0, hi = K - 1;  // last node whose pool range starts at or before slot
-            while (lo < hi) {
-              const uint32_t mid = (lo + hi + 1) >> 1;
-              if (s_pool_off[mid] <= slot) lo = mid;
-              else hi = mid - 1;
-            }
-            ids[u] = net.fout[s_fo_off[lo] + (slot - s_pool_off[lo])];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const uint32_t slot = base + u * 64 + lane;
-          if (slot < F) s_pool[slot] = ids[u];
-        }
-      }
-      __syncthreads();
-      // 4. the DP over runs of equal level, in LDS
-      for (uint32_t i = 0; i < K;) {
-        const uint32_t idx = i + lane;
-        const bool valid = idx < K;
-        const uint32_t lv = valid ? s_level[idx] : 0xFFFFFFFFu;
-        const uint32_t lv0 = __shfl(lv, 0);
-        const uint64_t same = __ballot(valid && lv == lv0);
-        const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;  // contiguous prefix
-        if (lane < run) {
-          const uint8_t fl = s_flags[idx];
-          const uint32_t off = s_pool_off[idx], nfo = s_nfo[idx];
-          const bool pooled = off + nfo <= kPool;
-          const uint32_t fo_off = s_fo_off[idx];
-          uint64_t C = 0, H = fl & 1;
-          for (uint32_t e = 0; e < nfo; ++e) {
-            const uint32_t fo = pooled ? s_pool[off + e] : net.fout[fo_off + e];
-            uint32_t h = hash_slot(fo);
-            uint32_t key;
-            while ((key = h_key[h]) != kEmpty && key != fo) h = (h + 1) & (kHashSlots - 1);
-            if (key == fo) {
-              ovf |= add_ovf(C, s_c[h_val[h]]);
-              ovf |= add_ovf(H, s_h[h_val[h]]);
-            }
-          }
-          uint64_t c = C, h = H;
-          if (fl & 4) { ovf |= add_ovf(c, H); h = 0; }
-          if (fl & 2) h = 0;
-          s_c[idx] = c;
-          s_h[idx] = h;
-        }
-        __syncthreads();
-        i += run;
-      }
-      // 5. per-gate counts and the PI total, all nodes at once
-      for (uint32_t idx = lane; idx < K; idx += 64) {
-        const uint64_t c = s_c[idx];
-        if (!c) continue;
-        if (!ALL_NEW) {
-          const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[s_node[idx]], (unsigned long long)c);
-          ovf |= old + c < old;
-        }
-        if (s_flags[idx] & 2) ovf |= add_ovf(tot, c);
-      }
-    } else {
-      // too many marked nodes for LDS: c/h in global scratch, the whole list
-      for (uint32_t i = 0; i < len;) {
-        const uint32_t idx = i + lane;
-        const bool valid = idx < len;
-        const uint32_t g = valid ? L[idx] : 0;
-        const uint32_t lv = valid ? net.level[g] : 0xFFFFFFFFu;
-        const uint32_t lv0 = __shfl(lv, 0);
-        const uint64_t same = __ballot(valid && lv == lv0);
-        const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;
-        if (lane < run && ((mark_of(pl, g, w) >> j) & 1)) {
-          const GateRec r = net.rec[g];
-          uint64_t C = 0, H = r.po == 1 ? 1 : 0;
-          for (uint32_t e = 0; e < r.nfo; ++e) {
-            const uint32_t fo = net.fout[r.fout_off + e];
-            if ((mark_of(pl, fo, w) >> j) & 1) {
-              ovf |= add_ovf(C, c_s[fo]);
-              ovf |= add_ovf(H, h_s[fo]);
-            }
-          }
-          uint64_t c = C, h = H;
-          if (ALL_NEW || first_t0[g] == p || first_t1[g] == p) { ovf |= add_ovf(c, H); h = 0; }
-          if (r.type == PDF_INPT) h = 0;
-          c_s[g] = c;
-          h_s[g] = h;
-          if (c) {
-            if (!ALL_NEW) {
-              const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
-              ovf |= old + c < old;
-            }
-            if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
-          }
-        }
-        __threadfence_block();
-        __syncthreads();
-        i += run;
-      }
-    }
-    // wave sum of the PI counts of this pattern (kernSumSingle)
-    for (int off = 32; off > 0; off >>= 1) {
-      const uint64_t o = __shfl_down(tot, off);
-      ovf |= add_ovf(tot, o);
-    }
-    if (pair_gain && lane == 0) pair_gain[p] = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
-    if (!ALL_NEW && lane == 0 && tot) {
-      const uint64_t o0 = atomicAdd((unsigned long long *)chunk_total, (unsigned long long)tot);
-      const uint64_t o1 = atomicAdd((unsigned long long *)grand_total, (unsigned long long)tot);
-      ovf |= (o0 + tot < o0) || (o1 + tot < o1);
-    }
-    if (__ballot(ovf)) { if (lane == 0) atomicOr(err, 1u); }
-    __syncthreads();
-  }
+  for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x)
+    cover_one<ALL_NEW>(S, net, pl, start, list, list_cnt, list_stride, first_t0, first_t1, c_s, h_s, gate_counts,
+                       chunk_total, grand_total, err, pair_gain, ALL_NEW ? k : act_list[k]);
 }
 
 // ------------------------------------------------------------ pair features
@@ -936,6 +977,131 @@ __global__ void k_iota(uint32_t *p, uint64_t n) {
 
 __global__ void k_copy_u64(const uint64_t *src, uint64_t *dst) { *dst = *src; }
 
+// ------------------------------------------------------------ batched grading
+// Many circuits, one chunk each, graded by level launches over the union of
+// their levels: launch k runs level k of every circuit that has one.  A
+// launch's blocks are split into segments, one per circuit, each with that
+// circuit's level map; a block finds its segment by binary search.
+struct BatchCircuit {
+  DevNet net;
+  const GateRec *recs;
+  Planes pl;
+  const uint64_t *va;
+  uint64_t vstride;
+  uint32_t *first0, *first1, *list, *list_cnt;
+  uint64_t list_stride;
+  uint32_t *cstate;  // per-chunk counters, cleared by the last forward level
+  uint32_t n_cstate;
+  uint64_t *actmap;
+  uint32_t *act_list, *act_count;
+  uint64_t *gate, *chunk_total, *grand;
+  uint32_t *err;
+  uint32_t width;  // pairs
+};
+
+struct BatchSeg {
+  LevelMap mp;
+  uint32_t circuit, block0, nblocks, clear;
+};
+
+__device__ __forceinline__ uint32_t find_seg(const BatchSeg *__restrict__ segs, uint32_t nseg, uint32_t b) {
+  uint32_t lo = 0, hi = nseg - 1;  // last segment starting at or before b
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (segs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool ROBUST, bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_fwd_batch(const BatchCircuit *__restrict__ circ,
+                                                      const BatchSeg *__restrict__ segs, uint32_t nseg) {
+  const BatchSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+  const BatchCircuit &c = circ[sg.circuit];
+  const DevNet net = c.net;
+  const Planes pl = c.pl;
+  fwd_body<true, true, true, ROBUST, WIDE, 1>(net, c.recs, pl, c.va, nullptr, c.vstride, 0, sg.mp,
+                                              sg.clear ? c.cstate : nullptr, sg.clear ? c.n_cstate : 0,
+                                              blockIdx.x - sg.block0, sg.nblocks);
+}
+
+template <bool WIDE>
+__global__ __launch_bounds__(kBlock) void k_rev_batch(const BatchCircuit *__restrict__ circ,
+                                                      const BatchSeg *__restrict__ segs, uint32_t nseg) {
+  const BatchSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
+  const BatchCircuit &c = circ[sg.circuit];
+  const DevNet net = c.net;
+  const Planes pl = c.pl;
+  rev_body<true, true, true, WIDE>(net, c.recs, pl, sg.mp, 0, c.first0, c.first1, c.list, c.list_cnt,
+                                   c.list_stride, blockIdx.x - sg.block0);
+}
+
+// Thread t = node t - node_base[c] of circuit c (node_base: prefix sums).
+__device__ __forceinline__ uint32_t find_circuit(const uint32_t *__restrict__ node_base, uint32_t C, uint32_t t) {
+  uint32_t lo = 0, hi = C - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (node_base[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_reset_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ node_base,
+                              uint32_t C, uint32_t total) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t ci = find_circuit(node_base, C, t);
+  const BatchCircuit &c = circ[ci];
+  const uint32_t g = t - node_base[ci];
+  c.first0[g] = PDF_NONE;
+  c.first1[g] = PDF_NONE;
+  c.gate[g] = 0;
+  if (g == 0) {
+    *c.grand = 0;
+    *c.err = 0;
+  }
+}
+
+__global__ void k_events_batch(const BatchCircuit *__restrict__ circ, const uint32_t *__restrict__ node_base,
+                               uint32_t C, uint32_t total) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t ci = find_circuit(node_base, C, t);
+  const BatchCircuit &c = circ[ci];
+  events_one(t - node_base[ci], c.first0, c.first1, 0, c.width, c.actmap, c.act_list, c.act_count);
+}
+
+// Every circuit's first-detection pairs, dealt round-robin to the waves
+// across circuits; c/h fallback scratch is one scratch_n slice per wave.
+__global__ __launch_bounds__(64) void k_cover_batch(const BatchCircuit *__restrict__ circ, uint32_t C, uint64_t *cs,
+                                                    uint64_t *hs, uint64_t scratch_n) {
+  __shared__ CoverLds S;
+  uint64_t *c_s = cs + (uint64_t)blockIdx.x * scratch_n;
+  uint64_t *h_s = hs + (uint64_t)blockIdx.x * scratch_n;
+  uint64_t dealt = 0;  // pairs of the circuits before this one
+  for (uint32_t ci = 0; ci < C; ++ci) {
+    const BatchCircuit &c = circ[ci];
+    const uint32_t nact = *c.act_count;
+    const DevNet net = c.net;
+    const Planes pl = c.pl;
+    uint32_t k = (uint32_t)((blockIdx.x + gridDim.x - dealt % gridDim.x) % gridDim.x);
+    for (; k < nact; k += gridDim.x)
+      cover_one<false>(S, net, pl, 0, c.list, c.list_cnt, c.list_stride, c.first0, c.first1, c_s, h_s, c.gate,
+                       c.chunk_total, c.grand, c.err, nullptr, c.act_list[k]);
+    dealt += nact;
+  }
+}
+
+__global__ void k_gather_totals(const BatchCircuit *__restrict__ circ, uint32_t C, uint64_t *__restrict__ totals,
+                                uint32_t *__restrict__ errs) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  totals[c] = *circ[c].grand;
+  errs[c] = *circ[c].err;
+}
+
 }  // namespace
 
 // ----------------------------------------------------------------- host side
@@ -971,6 +1137,7 @@ struct pdf_grader {
   uint32_t *d_list_cnt = nullptr, *d_act_count = nullptr;
   uint64_t *d_cs = nullptr, *d_hs = nullptr;
   uint32_t cover_waves = 0;
+  uint32_t max_cover_waves = 512;  // 1 for batch members (the batch has its own scratch)
   // per-launch event timing (pdf_grade_timed)
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_family;
@@ -1456,7 +1623,8 @@ int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
   const uint64_t n = g->nl.n;
   // cover scratch: one n-long c/h slice per wave, about 1 GiB at most
   const uint64_t per_wave = 2 * n * sizeof(uint64_t);
-  g->cover_waves = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(512, (1ull << 30) / per_wave));
+  g->cover_waves =
+      (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g->max_cover_waves, (1ull << 30) / per_wave));
   int rc = dalloc(&g->d_V, n * g->pitch);
   if (!rc && !g->seq) rc = dalloc(&g->d_VB, n * g->pitch);
   if (!rc) rc = dalloc(&g->d_M, n * g->pitch);
@@ -1928,6 +2096,359 @@ int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_m
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   if (err) { set_error("pdf_grade_curve: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  return PDF_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------- batched grading
+// A batch owns one grader per circuit (vectors, planes, first[] and lists),
+// the per-circuit descriptors and the per-level segment tables in HBM, a
+// shared cover scratch and one captured graph of the whole grade.
+
+struct pdf_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<pdf_grader *> members;
+  uint32_t C = 0, maxL = 0, total_nodes = 0, max_n = 0;
+  bool wide = true;
+  BatchCircuit *d_circ = nullptr;
+  uint32_t *d_node_base = nullptr;
+  BatchSeg *d_segs = nullptr;
+  std::vector<uint32_t> fwd_off, fwd_cnt, fwd_grid, rev_off, rev_cnt, rev_grid;  // per level
+  uint64_t *d_cs = nullptr, *d_hs = nullptr;
+  uint32_t cover_waves = 0;
+  uint64_t *d_totals = nullptr;
+  uint32_t *d_errs = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_robust = -1;
+  // tuning knobs (pdf_batch_set_tuning / pdf_batch_autotune)
+  uint32_t tune_cover_waves = 512, tune_order = 1, tune_use_graph = 1;
+};
+
+namespace {
+
+void batch_drop_graph(pdf_batch *b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_robust = -1;
+}
+
+// Segment tables of every level.  order 0: circuits in index order; order 1:
+// the segments with most blocks first (the longest-running blocks are
+// dispatched first, so the launch's tail is short).
+int batch_build_segments(pdf_batch *b) {
+  std::vector<BatchSeg> all;
+  b->fwd_off.assign(b->maxL + 1, 0);
+  b->fwd_cnt.assign(b->maxL + 1, 0);
+  b->fwd_grid.assign(b->maxL + 1, 0);
+  b->rev_off = b->fwd_off;
+  b->rev_cnt = b->fwd_cnt;
+  b->rev_grid = b->fwd_grid;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (uint32_t k = 0; k <= b->maxL; ++k) {
+      std::vector<BatchSeg> lv;
+      for (uint32_t c = 0; c < b->C; ++c) {
+        const pdf_grader *g = b->members[c];
+        if (k > g->nl.L) continue;
+        const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
+        const uint32_t words = (uint32_t)((g->num_pairs + 63) / 64);
+        BatchSeg sg{};
+        sg.mp = level_map(words, lo, count, pass == 0, &sg.nblocks);
+        sg.circuit = c;
+        sg.clear = pass == 0 && k == g->nl.L;
+        lv.push_back(sg);
+      }
+      if (b->tune_order == 1)
+        std::stable_sort(lv.begin(), lv.end(), [](const BatchSeg &x, const BatchSeg &y) { return x.nblocks > y.nblocks; });
+      uint32_t blk = 0;
+      for (auto &sg : lv) {
+        sg.block0 = blk;
+        blk += sg.nblocks;
+      }
+      (pass == 0 ? b->fwd_off : b->rev_off)[k] = (uint32_t)all.size();
+      (pass == 0 ? b->fwd_cnt : b->rev_cnt)[k] = (uint32_t)lv.size();
+      (pass == 0 ? b->fwd_grid : b->rev_grid)[k] = blk;
+      all.insert(all.end(), lv.begin(), lv.end());
+    }
+  }
+  dfree(b->d_segs);
+  int rc = dalloc(&b->d_segs, all.size());
+  if (!rc && !hip_ok(hipMemcpy(b->d_segs, all.data(), all.size() * sizeof(BatchSeg), hipMemcpyHostToDevice),
+                     "hipMemcpy"))
+    rc = PDF_ERR_HIP;
+  return rc;
+}
+
+int batch_alloc_scratch(pdf_batch *b) {
+  dfree(b->d_cs);
+  dfree(b->d_hs);
+  b->cover_waves = std::max(1u, b->tune_cover_waves);
+  int rc = dalloc(&b->d_cs, (size_t)b->cover_waves * b->max_n);
+  if (!rc) rc = dalloc(&b->d_hs, (size_t)b->cover_waves * b->max_n);
+  return rc;
+}
+
+template <bool R>
+void batch_queue_t(pdf_batch *b) {
+  hipStream_t st = b->stream;
+  k_reset_batch<<<grid_for(b->total_nodes, 256), 256, 0, st>>>(b->d_circ, b->d_node_base, b->C, b->total_nodes);
+  for (uint32_t k = 0; k <= b->maxL; ++k) {
+    if (b->wide)
+      k_fwd_batch<R, true><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->fwd_off[k], b->fwd_cnt[k]);
+    else
+      k_fwd_batch<R, false><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->fwd_off[k], b->fwd_cnt[k]);
+  }
+  for (uint32_t k = b->maxL + 1; k-- > 0;) {
+    if (b->wide)
+      k_rev_batch<true><<<b->rev_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->rev_off[k], b->rev_cnt[k]);
+    else
+      k_rev_batch<false><<<b->rev_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->rev_off[k], b->rev_cnt[k]);
+  }
+  k_events_batch<<<grid_for(b->total_nodes, 256), 256, 0, st>>>(b->d_circ, b->d_node_base, b->C, b->total_nodes);
+  k_cover_batch<<<b->cover_waves, 64, 0, st>>>(b->d_circ, b->C, b->d_cs, b->d_hs, b->max_n);
+  k_gather_totals<<<grid_for(b->C, 256), 256, 0, st>>>(b->d_circ, b->C, b->d_totals, b->d_errs);
+}
+
+int batch_queue(pdf_batch *b, int robust) {
+  if (robust) batch_queue_t<true>(b);
+  else batch_queue_t<false>(b);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+// One grade of the whole batch on b->stream (graph replay or direct launches).
+int batch_run(pdf_batch *b, int robust) {
+  if (b->tune_use_graph) {
+    if (!b->graph || b->graph_robust != (robust != 0)) {
+      batch_drop_graph(b);
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = batch_queue(b, robust);
+      const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      HIPCHK(e);
+      const hipError_t ie = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        b->graph = nullptr;
+        HIPCHK(ie);
+      }
+      b->graph_robust = robust != 0;
+    }
+    HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    return PDF_OK;
+  }
+  return batch_queue(b, robust);
+}
+
+int batch_results(pdf_batch *b, uint64_t *totals) {
+  std::vector<uint32_t> errs(b->C);
+  HIPCHK(hipMemcpyAsync(errs.data(), b->d_errs, b->C * 4, hipMemcpyDeviceToHost, b->stream));
+  if (totals) HIPCHK(hipMemcpyAsync(totals, b->d_totals, b->C * 8, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (uint32_t c = 0; c < b->C; ++c)
+    if (errs[c]) {
+      set_error("pdf_batch_grade: uint64 path count overflow in circuit " + std::to_string(c));
+      return PDF_ERR_OVERFLOW;
+    }
+  return PDF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdf_batch_create(int device, uint32_t count, const pdf_netlist *const *nets, const uint64_t *const *vec_bits,
+                     const uint64_t *num_vectors, pdf_batch **out) {
+  if (!out || !nets || !vec_bits || !num_vectors || count == 0) { set_error("pdf_batch_create: bad argument"); return PDF_ERR_ARG; }
+  *out = nullptr;
+  auto *b = new (std::nothrow) pdf_batch;
+  if (!b) { set_error("out of memory"); return PDF_ERR_NOMEM; }
+  b->device = device;
+  int rc = PDF_OK;
+  for (uint32_t c = 0; c < count && !rc; ++c) {
+    pdf_grader *g = nullptr;
+    if ((rc = pdf_grader_create(device, nets[c], &g))) break;
+    b->members.push_back(g);
+    g->max_cover_waves = 1;  // the batch's own scratch serves the cover
+    if ((rc = pdf_grader_set_vectors(g, vec_bits[c], num_vectors[c]))) break;
+    if (g->chunk_pairs != g->num_pairs) {
+      set_error("pdf_batch_create: circuit " + std::to_string(c) + " does not fit one chunk in HBM");
+      rc = PDF_ERR_NOMEM;
+    }
+  }
+  if (!rc && (!hip_ok(hipSetDevice(device), "hipSetDevice") ||
+              !hip_ok(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+              !hip_ok(hipEventCreate(&b->ev0), "hipEventCreate") || !hip_ok(hipEventCreate(&b->ev1), "hipEventCreate")))
+    rc = PDF_ERR_HIP;
+  if (rc) {
+    pdf_batch_destroy(b);
+    return rc;
+  }
+  b->C = count;
+  std::vector<BatchCircuit> circ(count);
+  std::vector<uint32_t> base(count);
+  uint64_t nodes = 0;
+  for (uint32_t c = 0; c < count; ++c) {
+    pdf_grader *g = b->members[c];
+    base[c] = (uint32_t)nodes;
+    nodes += g->nl.n;
+    b->maxL = std::max(b->maxL, g->nl.L);
+    b->max_n = std::max(b->max_n, g->nl.n);
+    if (g->num_pairs < 4096) b->wide = false;  // fewer than 64 words: narrow tiles
+    BatchCircuit &d = circ[c];
+    d.net = g->net();
+    d.recs = g->d_rec;
+    d.pl = planes(g, g->num_pairs);
+    d.va = g->d_va;
+    d.vstride = g->vstride;
+    d.first0 = g->d_first0;
+    d.first1 = g->d_first1;
+    d.list = g->d_list;
+    d.list_cnt = g->d_list_cnt;
+    d.list_stride = g->nl.n;
+    d.cstate = g->d_cstate;
+    d.n_cstate = g->n_cstate;
+    d.actmap = g->d_actmap;
+    d.act_list = g->d_act_list;
+    d.act_count = g->d_act_count;
+    d.gate = g->d_gate;
+    d.chunk_total = g->d_chunk_total;
+    d.grand = g->d_grand;
+    d.err = g->d_err;
+    d.width = (uint32_t)g->num_pairs;
+  }
+  if (nodes >= 0xFFFFFFFFull) {
+    set_error("pdf_batch_create: more than 2^32 - 1 nodes in the batch");
+    pdf_batch_destroy(b);
+    return PDF_ERR_ARG;
+  }
+  b->total_nodes = (uint32_t)nodes;
+  rc = dalloc(&b->d_circ, count);
+  if (!rc) rc = dalloc(&b->d_node_base, count);
+  if (!rc) rc = dalloc(&b->d_totals, count);
+  if (!rc) rc = dalloc(&b->d_errs, count);
+  if (!rc && (!hip_ok(hipMemcpy(b->d_circ, circ.data(), count * sizeof(BatchCircuit), hipMemcpyHostToDevice), "hipMemcpy") ||
+              !hip_ok(hipMemcpy(b->d_node_base, base.data(), count * 4, hipMemcpyHostToDevice), "hipMemcpy")))
+    rc = PDF_ERR_HIP;
+  if (!rc) rc = batch_build_segments(b);
+  if (!rc) rc = batch_alloc_scratch(b);
+  if (rc) {
+    pdf_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return PDF_OK;
+}
+
+void pdf_batch_destroy(pdf_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  dfree(b->d_circ);
+  dfree(b->d_node_base);
+  dfree(b->d_segs);
+  dfree(b->d_cs);
+  dfree(b->d_hs);
+  dfree(b->d_totals);
+  dfree(b->d_errs);
+  for (pdf_grader *g : b->members) pdf_grader_destroy(g);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int pdf_batch_grade(pdf_batch *b, int robust, uint64_t *totals, float *ms) {
+  if (!b) { set_error("null batch"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(b->device));
+  if (ms) HIPCHK(hipEventRecord(b->ev0, b->stream));
+  int rc = batch_run(b, robust);
+  if (rc) return rc;
+  if (ms) HIPCHK(hipEventRecord(b->ev1, b->stream));
+  rc = batch_results(b, totals);
+  if (ms) HIPCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+  for (pdf_grader *g : b->members) {  // the members' getters read this grade
+    g->stage = 4;
+    g->stage_chunk = 0;
+  }
+  return rc;
+}
+
+int pdf_batch_member(pdf_batch *b, uint32_t circuit, pdf_grader **out) {
+  if (!b || !out || circuit >= b->C) { set_error("pdf_batch_member: bad argument"); return PDF_ERR_ARG; }
+  *out = b->members[circuit];
+  return PDF_OK;
+}
+
+int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value) {
+  if (!b) { set_error("null batch"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(b->device));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  switch (key) {
+    case PDF_TUNE_USE_GRAPH:
+      if (value != 0 && value != 1) break;
+      b->tune_use_graph = (uint32_t)value;
+      batch_drop_graph(b);
+      return PDF_OK;
+    case PDF_TUNE_COVER_WAVES:
+      if (value < 1 || value > 65535) break;
+      b->tune_cover_waves = (uint32_t)value;
+      batch_drop_graph(b);
+      return batch_alloc_scratch(b);
+    case PDF_TUNE_SEGMENT_ORDER:
+      if (value != 0 && value != 1) break;
+      b->tune_order = (uint32_t)value;
+      batch_drop_graph(b);
+      return batch_build_segments(b);
+    default:
+      set_error("pdf_batch_set_tuning: unknown key");
+      return PDF_ERR_ARG;
+  }
+  set_error("pdf_batch_set_tuning: value out of range");
+  return PDF_ERR_ARG;
+}
+
+int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best) {
+  if (!b || reps == 0) { set_error("pdf_batch_autotune: bad argument"); return PDF_ERR_ARG; }
+  const uint32_t waves[] = {256, 512, 1024, 2048};
+  std::vector<uint64_t> ref(b->C), tot(b->C);
+  bool have_ref = false;
+  pdf_batch_tuning bt{};
+  bt.ms = 1e30f;
+  int rc;
+  for (uint32_t order = 0; order < 2; ++order)
+    for (uint32_t wv : waves)
+      for (uint32_t graph = 0; graph < 2; ++graph) {
+        if ((rc = pdf_batch_set_tuning(b, PDF_TUNE_SEGMENT_ORDER, order)) ||
+            (rc = pdf_batch_set_tuning(b, PDF_TUNE_COVER_WAVES, wv)) ||
+            (rc = pdf_batch_set_tuning(b, PDF_TUNE_USE_GRAPH, graph)))
+          return rc;
+        std::vector<float> t(reps + 1);
+        for (uint32_t r = 0; r <= reps; ++r)  // the first run warms up (and captures)
+          if ((rc = pdf_batch_grade(b, robust, tot.data(), &t[r]))) return rc;
+        if (!have_ref) {
+          ref = tot;
+          have_ref = true;
+        } else if (tot != ref) {
+          set_error("pdf_batch_autotune: totals differ between configurations");
+          return PDF_ERR_STATE;
+        }
+        std::sort(t.begin() + 1, t.end());
+        const float med = t[1 + reps / 2];
+        if (med < bt.ms) bt = pdf_batch_tuning{order, wv, graph, med};
+      }
+  if ((rc = pdf_batch_set_tuning(b, PDF_TUNE_SEGMENT_ORDER, bt.segment_order)) ||
+      (rc = pdf_batch_set_tuning(b, PDF_TUNE_COVER_WAVES, bt.cover_waves)) ||
+      (rc = pdf_batch_set_tuning(b, PDF_TUNE_USE_GRAPH, bt.use_graph)))
+    return rc;
+  if (best) *best = bt;
   return PDF_OK;
 }
 

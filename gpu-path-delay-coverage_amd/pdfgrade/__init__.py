@@ -29,7 +29,7 @@ UNKN, INPT, AND, NAND, OR, NOR, XOR, XNOR, BUFF, NOT, FROM, DFF = range(12)
 TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF", "NOT", "FROM", "DFF"]
 S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
-TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH = 1, 2
+TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER = 1, 2, 3, 4
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
@@ -118,6 +118,13 @@ SIGNATURES = {
     "pdf_mlp_adam": ([_P, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
     "pdf_mlp_train_step": ([_P, C.c_void_p, _U64, C.c_void_p, _U64, C.c_float, _FP], C.c_int),
     "pdf_mlp_sync": ([_P], C.c_int),
+    "pdf_batch_create": ([C.c_int, C.c_uint32, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_uint64), C.POINTER(_P)],
+                         C.c_int),
+    "pdf_batch_destroy": ([_P], None),
+    "pdf_batch_grade": ([_P, C.c_int, C.c_void_p, _FP], C.c_int),
+    "pdf_batch_member": ([_P, C.c_uint32, C.POINTER(_P)], C.c_int),
+    "pdf_batch_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
+    "pdf_batch_autotune": ([_P, C.c_int, C.c_uint32, C.c_void_p], C.c_int),
 }
 
 _lib = None
@@ -270,15 +277,23 @@ class Grader:
 
     def __init__(self, netlist: Netlist, device: int = 0):
         self._h = None
+        self._owned = True
         h = C.c_void_p()
         _check(lib().pdf_grader_create(device, netlist.handle, C.byref(h)))
         self._h = h
         self.n = netlist.info()["n"]
 
+    @classmethod
+    def _view(cls, handle, n: int, num_pairs: int, keepalive) -> "Grader":
+        """A non-owning view of a grader owned by a batch."""
+        g = cls.__new__(cls)
+        g._h, g._owned, g.n, g._num_pairs, g._keepalive = handle, False, n, num_pairs, keepalive
+        return g
+
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             lib().pdf_grader_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -443,6 +458,61 @@ class Grader:
 
 FEATURES = 512
 MLP_PARAMS = 197377
+
+
+class _BatchTuning(C.Structure):
+    _fields_ = [("segment_order", C.c_uint32), ("cover_waves", C.c_uint32), ("use_graph", C.c_uint32),
+                ("ms", C.c_float)]
+
+
+class Batch:
+    """Many circuits (one chunk of sequence-mode vectors each) graded by one
+    captured graph of level launches over the union of their levels."""
+
+    def __init__(self, netlists, vectors, num_vectors, device: int = 0):
+        self._h = None
+        count = len(netlists)
+        if not (count == len(vectors) == len(num_vectors)) or count == 0:
+            raise ValueError("one vector set per netlist")
+        self._bits = [np.ascontiguousarray(v, dtype=np.uint64) for v in vectors]
+        nets = (_P * count)(*[nl.handle for nl in netlists])
+        bits = (_P * count)(*[b.ctypes.data for b in self._bits])
+        nv = (C.c_uint64 * count)(*[int(x) for x in num_vectors])
+        h = C.c_void_p()
+        _check(lib().pdf_batch_create(device, count, nets, bits, nv, C.byref(h)))
+        self._h = h
+        self.count = count
+        self._n = [nl.info()["n"] for nl in netlists]
+        self._pairs = [int(x) for x in num_vectors]
+
+    def close(self):
+        if self._h:
+            lib().pdf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def grade(self, robust: bool):
+        """(per-circuit totals as uint64 array, GPU milliseconds)."""
+        tot = np.zeros(self.count, np.uint64)
+        ms = C.c_float()
+        _check(lib().pdf_batch_grade(self._h, int(robust), _ptr(tot), C.byref(ms)))
+        return tot, ms.value
+
+    def member(self, c: int) -> Grader:
+        """Circuit c's grader (first(), gate_counts(), total() of the last grade)."""
+        h = C.c_void_p()
+        _check(lib().pdf_batch_member(self._h, c, C.byref(h)))
+        return Grader._view(h, self._n[c], self._pairs[c], self)
+
+    def set_tuning(self, key: int, value: int):
+        _check(lib().pdf_batch_set_tuning(self._h, key, value))
+
+    def autotune(self, robust: bool, reps: int = 3) -> dict:
+        t = _BatchTuning()
+        _check(lib().pdf_batch_autotune(self._h, int(robust), reps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _BatchTuning._fields_}
 
 
 class Mlp:

@@ -47,11 +47,42 @@ GATES_2M = Workload(
     vector_seed=104, pairs=1_048_576, chunk=32_768)
 
 
-def batch_member(i: int, rng_gates: int, levels: int) -> GenConfig:
-    """Circuit i of the 256-circuit batch: the gate count and levels are drawn by
-    the caller (log-uniform in [1k, 200k], levels = round(8 log2 gates))."""
-    return GenConfig(seed=3000 + i, num_pi=256, num_po=128, num_gates=rng_gates, levels=levels,
-                     fanin_p=WIDE_FANIN, prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+@dataclass(frozen=True)
+class BatchWorkload:
+    """Many circuits graded together: circuit i has netlist seed 3000 + i and
+    vector seed 13000 + i; its gate count (log-uniform in [min_gates,
+    max_gates]) and pair count (uniform in [min_pairs, max_pairs], rounded to
+    a multiple of 64) come from a numpy Generator seeded with draw_seed;
+    levels = round(8 log2 gates)."""
+    name: str
+    count: int
+    min_gates: int = 1000
+    max_gates: int = 200_000
+    min_pairs: int = 4096
+    max_pairs: int = 65536
+    draw_seed: int = 3
+    robust: bool = True
 
+    def members(self):
+        """[(GenConfig, vector_seed, pairs)] of every circuit."""
+        import numpy as np
+
+        rng = np.random.default_rng(self.draw_seed)
+        gates = np.exp(rng.uniform(np.log(self.min_gates), np.log(self.max_gates), self.count)).round().astype(int)
+        pairs = (np.round(rng.integers(self.min_pairs, self.max_pairs + 1, self.count) / 64) * 64).astype(int)
+        out = []
+        for i in range(self.count):
+            g = int(gates[i])
+            levels = int(round(8 * np.log2(g)))
+            # above ~100k gates and 130 levels the structural path count passes
+            # 2^63; only marked paths are counted (overflow-checked), so no limit
+            cfg = GenConfig(seed=3000 + i, num_pi=256, num_po=128, num_gates=g, levels=levels, fanin_p=WIDE_FANIN,
+                            prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64, path_limit_log2=0)
+            out.append((cfg, 13000 + i, int(pairs[i])))
+        return out
+
+
+BATCH_256 = BatchWorkload(name="batch-256-circuits", count=256)
 
 WORKLOADS = {w.name: w for w in (TINY, GATES_50K, GATES_2M)}
+BATCH_WORKLOADS = {BATCH_256.name: BATCH_256}

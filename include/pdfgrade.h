@@ -207,7 +207,9 @@ int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
  * simkernel.cuh:29-30, markkernel.cuh:7-8, mergekernel.cuh:49-52).
  * Results never depend on them. */
 #define PDF_TUNE_WORDS_PER_LANE 1 /* 1 or 2 words per lane in the fused forward pass */
-#define PDF_TUNE_USE_GRAPH 2      /* 1: pdf_grade replays a captured HIP graph */
+#define PDF_TUNE_USE_GRAPH 2
+#define PDF_TUNE_COVER_WAVES 3   /* batch: waves of the cover launch */
+#define PDF_TUNE_SEGMENT_ORDER 4 /* batch: 0 circuit order, 1 largest segments first */      /* 1: pdf_grade replays a captured HIP graph */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same
@@ -293,6 +295,35 @@ int pdf_mlp_adam(pdf_mlp *m, float lr, float beta1, float beta2, float eps);
 int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const float *target_dev, uint64_t rows,
                        float lr, float *loss);
 int pdf_mlp_sync(pdf_mlp *m);
+
+/* ------------------------------------------------------------------------ */
+/* Batched grading (no reference counterpart): many circuits, each with its  */
+/* own sequence-mode vectors graded as one chunk, in one captured graph of   */
+/* level launches over the union of the circuits' levels.  Results equal     */
+/* grading each circuit alone with pdf_grade.                                */
+/* ------------------------------------------------------------------------ */
+typedef struct pdf_batch pdf_batch;
+typedef struct {
+  uint32_t segment_order, cover_waves, use_graph;
+  float ms; /* median grade time of this configuration */
+} pdf_batch_tuning;
+
+/* vec_bits[c]: PI-major packed vectors of circuit c (as pdf_grader_set_vectors),
+ * num_vectors[c] of them. */
+int pdf_batch_create(int device, uint32_t count, const pdf_netlist *const *nets, const uint64_t *const *vec_bits,
+                     const uint64_t *num_vectors, pdf_batch **out);
+void pdf_batch_destroy(pdf_batch *b);
+/* Grade every circuit; totals[c] (may be NULL) = circuit c's path total;
+ * ms (may be NULL) = GPU time of the grade. */
+int pdf_batch_grade(pdf_batch *b, int robust, uint64_t *totals, float *ms);
+/* Circuit c's grader, for the getters (pdf_get_first, pdf_get_gate_counts,
+ * pdf_get_total) after pdf_batch_grade; owned by the batch. */
+int pdf_batch_member(pdf_batch *b, uint32_t circuit, pdf_grader **out);
+/* PDF_TUNE_USE_GRAPH, PDF_TUNE_COVER_WAVES, PDF_TUNE_SEGMENT_ORDER. */
+int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value);
+/* Time every configuration of the knobs (reps runs each, median), check that
+ * all give the same totals, keep the fastest and report it in *best. */
+int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best);
 
 #ifdef __cplusplus
 }

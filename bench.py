@@ -5,6 +5,10 @@ chunk, the reference's chunk loop main.cu:171-231) of 65,536 pairs on the
 seeded 50,000-gate / 60-level netlist, with the netlist and vectors already
 resident in HBM.  Prints one JSON line.
 
+--workload batch-256-circuits instead times one batched grade of 256
+circuits (1k-200k gates, 4k-64k pairs each) against grading them one after
+another with per-circuit launches.
+
 The roofline object is for the dominant kernel (the fused sim + forward-mark
 level kernel), timed live with HIP events around each of its launches on the
 grader's stream.  The CPU baseline is the serial C oracle on a bounded sample
@@ -32,6 +36,7 @@ def parse():
     ap.add_argument("--workload", default="single-50k-gates")
     ap.add_argument("--cpu-sample", type=int, default=4096, help="pairs graded by the CPU baseline (0 = skip)")
     ap.add_argument("--robust", type=int, default=1)
+    ap.add_argument("--autotune", type=int, default=1, help="batch workloads: run the autotuner first")
     return ap.parse_args()
 
 
@@ -65,6 +70,63 @@ def cpu_baseline(nl, bits, pairs, sample, robust):
     return sample / dt
 
 
+def bench_batch(args, wl):
+    """One batched grade of every circuit of the workload against per-circuit
+    plain grading (one pdf_grade per circuit, one after another)."""
+    import numpy as np
+
+    import pdfgrade as pg
+
+    robust = bool(args.robust)
+    members = wl.members()
+    nets, vecs, pairs = [], [], []
+    for cfg, vseed, p in members:
+        nets.append(pg.Netlist.generate(cfg))
+        vecs.append(pg.gen_vectors(vseed, cfg.num_pi, p))
+        pairs.append(p)
+    infos = [nl.info() for nl in nets]
+    gate_pairs = sum((i["n"] - i["num_pi"]) * p for i, p in zip(infos, pairs))
+    algo_bytes = sum((10 * i["n"] + i["num_pi"]) / 8.0 * p for i, p in zip(infos, pairs))
+    # per-circuit plain grading, one circuit at a time (each warmed up first)
+    plain_ms, plain_tot = 0.0, []
+    for nl, v, p in zip(nets, vecs, pairs):
+        g = pg.Grader(nl)
+        g.set_vectors(v, p)
+        g.set_chunk(p)
+        g.grade(robust)
+        t0 = time.perf_counter()
+        plain_tot.append(g.grade(robust))
+        plain_ms += (time.perf_counter() - t0) * 1e3
+        g.close()
+    b = pg.Batch(nets, vecs, pairs)
+    tune = b.autotune(robust, reps=2) if args.autotune else None
+    for _ in range(args.warmup):
+        b.grade(robust)
+    t0 = time.perf_counter()
+    gpu_ms = []
+    for _ in range(args.steps):
+        tot, ms = b.grade(robust)
+        gpu_ms.append(ms)
+    dt = (time.perf_counter() - t0) / args.steps
+    assert list(tot) == plain_tot, "batched totals differ from per-circuit grading"
+    out = {
+        "metric": "vector-pairs graded/s, 256-circuit batch", "value": sum(pairs) / dt, "unit": "pairs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (seeded netlists and vectors)",
+        "config": {"workload": wl.name, "circuits": wl.count, "pairs": sum(pairs), "gate_pairs": gate_pairs,
+                   "gates_min": min(i["n"] - i["num_pi"] for i in infos),
+                   "gates_max": max(i["n"] - i["num_pi"] for i in infos), "mode": "robust" if robust else "non-robust",
+                   "tuning": tune},
+        "gate_evals_per_s": gate_pairs / dt,
+        "gpu_ms_per_step": float(np.median(gpu_ms)),
+        "per_circuit_plain_ms": plain_ms,
+        "speedup_vs_per_circuit": plain_ms / (dt * 1e3),
+        "end_to_end": {"algorithmic_GBps": algo_bytes / dt / 1e9, "frac": algo_bytes / dt / 1e9 / HBM_PEAK_GBS},
+    }
+    print(json.dumps(out))
+    b.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -72,7 +134,10 @@ def main():
                                    "design only, see DESIGN.md)"}))
         sys.exit(2)
     import pdfgrade as pg
-    from pdfgrade.configs import WORKLOADS
+    from pdfgrade.configs import BATCH_WORKLOADS, WORKLOADS
+
+    if args.workload in BATCH_WORKLOADS:
+        return bench_batch(args, BATCH_WORKLOADS[args.workload])
 
     wl = WORKLOADS[args.workload]
     nl = pg.Netlist.generate(wl.netlist)

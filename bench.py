@@ -42,12 +42,15 @@ def parse():
 
 def measured_traffic(workload: str, family: str):
     """HBM bytes per launch of the kernel family, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (profiles/traffic_<workload>.json), or None."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<workload>.md, its json
+    block), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     try:
         with open(path) as f:
-            return json.load(f)["bytes_per_launch"].get(f"k_{family}")
-    except (OSError, KeyError, ValueError):
+            text = f.read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)["bytes_per_launch"].get(f"k_{family}")
+    except (OSError, IndexError, KeyError, ValueError):
         return None
 
 

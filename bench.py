@@ -36,7 +36,10 @@ def parse():
     ap.add_argument("--workload", default="single-50k-gates")
     ap.add_argument("--cpu-sample", type=int, default=4096, help="pairs graded by the CPU baseline (0 = skip)")
     ap.add_argument("--robust", type=int, default=1)
-    ap.add_argument("--autotune", type=int, default=1, help="batch workloads: run the autotuner first")
+    ap.add_argument("--graph", type=int, default=None, help="batch workloads: force graph replay on (1) or off (0)")
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="run the autotuner before the timed steps (default: no for single workloads, yes for "
+                         "batches)")
     return ap.parse_args()
 
 
@@ -96,13 +99,17 @@ def bench_batch(args, wl):
         g = pg.Grader(nl)
         g.set_vectors(v, p)
         g.set_chunk(p)
+        if args.graph is not None:
+            g.set_tuning(pg.TUNE_USE_GRAPH, args.graph)
         g.grade(robust)
         t0 = time.perf_counter()
         plain_tot.append(g.grade(robust))
         plain_ms += (time.perf_counter() - t0) * 1e3
         g.close()
     b = pg.Batch(nets, vecs, pairs)
-    tune = b.autotune(robust, reps=2) if args.autotune else None
+    tune = b.autotune(robust, reps=2) if args.autotune in (None, 1) else None
+    if args.graph is not None:
+        b.set_tuning(pg.TUNE_USE_GRAPH, args.graph)
     for _ in range(args.warmup):
         b.grade(robust)
     t0 = time.perf_counter()
@@ -150,7 +157,12 @@ def main():
     robust = bool(args.robust)
     g = pg.Grader(nl, device=0)
     g.set_vectors(bits, wl.pairs)
+    # the workload's recorded tuning (what pdf_grader_autotune chose on an
+    # MI355X); --autotune 1 runs the autotuner again, untimed
+    g.set_tuning(pg.TUNE_LANES, wl.lanes)
     g.set_chunk(wl.chunk)
+    tune = g.autotune(robust, reps=5) if args.autotune else {"lanes": wl.lanes, "chunk_pairs": wl.chunk,
+                                                              "use_graph": 1, "source": "recorded"}
     total = None
     for _ in range(args.warmup):
         total = g.grade(robust)
@@ -163,7 +175,9 @@ def main():
     pairs_per_s = wl.pairs * args.steps / dt
     ms_per_step = dt * 1e3 / args.steps
 
-    # live per-launch timing of the dominant kernel family
+    # live per-launch timing of the dominant kernel family: the same chunk
+    # shape with the launches serialized (lanes overlap kernels, which would
+    # blur one kernel's duration)
     t_total, kt = g.grade_timed(robust)
     assert t_total == total
     fam = max(("fwd", "reverse", "cover", "events"), key=lambda k: kt[f"{k}_ms"])
@@ -203,7 +217,7 @@ def main():
         "data": "synthetic (seeded netlist and vectors)",
         "config": {"workload": wl.name, "gates": info["n"] - I, "levels": info["levels"], "pis": I,
                    "pos": info["num_po"], "pairs": wl.pairs, "chunk": wl.chunk,
-                   "mode": "robust" if robust else "non-robust", "total_paths": total},
+                   "mode": "robust" if robust else "non-robust", "total_paths": total, "tuning": tune},
         "gate_evals_per_s": pairs_per_s * (n - I),
         "roofline": {"bound": "hbm", "kernel": f"k_{fam}", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name, fam),

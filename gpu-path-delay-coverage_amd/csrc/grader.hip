@@ -1146,6 +1146,25 @@ struct pdf_grader {
   // tuning knobs (pdf_grader_set_tuning)
   uint32_t tune_words_per_lane = 1;
   uint32_t tune_use_graph = 1;
+  // chunk lanes (PDF_TUNE_LANES): the fused grade runs consecutive chunks on
+  // `lanes` streams, each lane with its own chunk buffers; lane 0 is the set
+  // of fields above, lanes 1.. are kept here
+  struct Lane {
+    hipStream_t stream = nullptr;
+    uint64_t *V = nullptr, *VB = nullptr, *M = nullptr, *NZ = nullptr, *ACC = nullptr;
+    uint32_t *list = nullptr, *cstate = nullptr, *act_list = nullptr;
+    uint64_t *actmap = nullptr, *chunk_total = nullptr;
+    uint32_t *list_cnt = nullptr, *act_count = nullptr;
+    uint64_t *cs = nullptr, *hs = nullptr;
+    hipEvent_t done = nullptr;  // merges of this lane's latest chunk and all before it
+  };
+  std::vector<Lane> lanes;      // lanes 1..; empty for one lane
+  // with lanes and graphs: each chunk's forward and reverse levels captured
+  // on its lane's stream (a graph per chunk; covers are launched directly)
+  std::vector<hipGraphExec_t> chunk_graphs;
+  int chunk_graphs_robust = -1;
+  hipEvent_t lane0_done = nullptr, lane_start = nullptr;
+  uint32_t tune_lanes = 1;
   // ranking scratch (pdf_grader_rank_pairs), kept between calls
   float *d_rk_keys = nullptr;
   uint32_t *d_rk_idx = nullptr, *d_rk_perm = nullptr;
@@ -1271,11 +1290,12 @@ void fwd_one(pdf_grader *g, uint32_t grid, const DevNet &net, const Planes &pl, 
 // clear_state: the last level's extra-column threads zero the per-chunk
 // merge/cover counters (sequence-mode simulation only).
 template <bool SEQ, bool S, bool M>
-void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state, uint32_t last_level) {
+void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state, uint32_t last_level,
+                  uint32_t k_lo = 0, uint32_t k_hi = 0xFFFFFFFFu) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
   const uint32_t nw = words_per_lane(g->tune_words_per_lane, SEQ, S, M, width);
-  for (uint32_t k = 0; k <= std::min(g->nl.L, last_level); ++k) {
+  for (uint32_t k = k_lo; k <= std::min(std::min(g->nl.L, last_level), k_hi); ++k) {
     uint32_t grid;
     const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
     LevelMap mp = level_map(pl.words, lo, count, SEQ && S, &grid);
@@ -1305,18 +1325,19 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
 
 template <bool S, bool M>
 int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state = false,
-               uint32_t last_level = 0xFFFFFFFFu) {
-  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust, clear_state, last_level);
-  else launch_fwd_t<false, S, M>(g, start, width, robust, false, last_level);
+               uint32_t last_level = 0xFFFFFFFFu, uint32_t k_lo = 0, uint32_t k_hi = 0xFFFFFFFFu) {
+  if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust, clear_state, last_level, k_lo, k_hi);
+  else launch_fwd_t<false, S, M>(g, start, width, robust, false, last_level, k_lo, k_hi);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
 
 template <bool SEQ, bool R, bool M>
-void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width) {
+void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_lo = 0,
+                      uint32_t k_hi = 0xFFFFFFFFu) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
-  for (uint32_t k = g->nl.L + 1; k-- > 0;) {
+  for (uint32_t k = std::min(g->nl.L, k_hi) + 1; k-- > k_lo;) {
     uint32_t grid;
     const LevelMap mp =
         level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], false, &grid);
@@ -1340,9 +1361,9 @@ int reset_chunk(pdf_grader *g) {
 }
 
 template <bool R, bool M>
-int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width) {
-  if (g->seq) launch_reverse_t<true, R, M>(g, start, width);
-  else launch_reverse_t<false, R, M>(g, start, width);
+int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_lo = 0, uint32_t k_hi = 0xFFFFFFFFu) {
+  if (g->seq) launch_reverse_t<true, R, M>(g, start, width, k_lo, k_hi);
+  else launch_reverse_t<false, R, M>(g, start, width, k_lo, k_hi);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
@@ -1380,10 +1401,97 @@ void drop_graph(pdf_grader *g) {
   if (g->graph) (void)hipGraphExecDestroy(g->graph);
   g->graph = nullptr;
   g->graph_robust = -1;
+  for (hipGraphExec_t e : g->chunk_graphs) (void)hipGraphExecDestroy(e);
+  g->chunk_graphs.clear();
+  g->chunk_graphs_robust = -1;
 }
 
+void lane_save(const pdf_grader *g, pdf_grader::Lane &l) {
+  l.stream = g->stream;
+  l.V = g->d_V;
+  l.VB = g->d_VB;
+  l.M = g->d_M;
+  l.NZ = g->d_NZ;
+  l.ACC = g->d_ACC;
+  l.list = g->d_list;
+  l.cstate = g->d_cstate;
+  l.act_list = g->d_act_list;
+  l.actmap = g->d_actmap;
+  l.chunk_total = g->d_chunk_total;
+  l.list_cnt = g->d_list_cnt;
+  l.act_count = g->d_act_count;
+  l.cs = g->d_cs;
+  l.hs = g->d_hs;
+}
+
+void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
+  g->stream = l.stream;
+  g->d_V = l.V;
+  g->d_VB = l.VB;
+  g->d_M = l.M;
+  g->d_NZ = l.NZ;
+  g->d_ACC = l.ACC;
+  g->d_list = l.list;
+  g->d_cstate = l.cstate;
+  g->d_act_list = l.act_list;
+  g->d_actmap = l.actmap;
+  g->d_chunk_total = l.chunk_total;
+  g->d_list_cnt = l.list_cnt;
+  g->d_act_count = l.act_count;
+  g->d_cs = l.cs;
+  g->d_hs = l.hs;
+}
+
+void free_chunk_buffers(pdf_grader *g);
+int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs);
+
+// Frees every lane's chunk buffers (the extra lanes' streams and events too).
 void free_chunk(pdf_grader *g) {
   drop_graph(g);
+  if (!g->lanes.empty()) {
+    pdf_grader::Lane l0;
+    lane_save(g, l0);
+    for (auto &l : g->lanes) {
+      if (l.stream) (void)hipStreamSynchronize(l.stream);
+      lane_load(g, l);
+      free_chunk_buffers(g);
+      if (l.stream) (void)hipStreamDestroy(l.stream);
+      if (l.done) (void)hipEventDestroy(l.done);
+    }
+    g->lanes.clear();
+    lane_load(g, l0);
+  }
+  free_chunk_buffers(g);
+  g->chunk_pairs = 0;
+}
+
+int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs) {
+  const uint64_t n = g->nl.n;
+  const uint64_t words = (chunk_pairs + 63) / 64;
+  int rc = dalloc(&g->d_V, n * g->pitch);
+  if (!rc && !g->seq) rc = dalloc(&g->d_VB, n * g->pitch);
+  if (!rc) rc = dalloc(&g->d_M, n * g->pitch);
+  if (!rc) rc = dalloc(&g->d_NZ, n * g->ntiles);
+  if (!rc) rc = dalloc(&g->d_ACC, n * g->pitch);
+  // the removal accumulator is zero between passes (each reader clears it)
+  if (!rc && !hip_ok(hipMemset(g->d_ACC, 0, n * g->pitch * 8), "hipMemset")) rc = PDF_ERR_HIP;
+  if (!rc) rc = dalloc(&g->d_list, n * words);
+  const uint32_t ct_off = (uint32_t)((3 * words + 2) & ~1ULL);
+  g->n_cstate = ct_off + 2;
+  if (!rc) rc = dalloc(&g->d_cstate, g->n_cstate);
+  if (!rc) {
+    g->d_actmap = reinterpret_cast<uint64_t *>(g->d_cstate);
+    g->d_list_cnt = g->d_cstate + 2 * words;
+    g->d_act_count = g->d_cstate + 3 * words;
+    g->d_chunk_total = reinterpret_cast<uint64_t *>(g->d_cstate + ct_off);
+  }
+  if (!rc) rc = dalloc(&g->d_act_list, chunk_pairs);
+  if (!rc) rc = dalloc(&g->d_cs, n * g->cover_waves);
+  if (!rc) rc = dalloc(&g->d_hs, n * g->cover_waves);
+  return rc;
+}
+
+void free_chunk_buffers(pdf_grader *g) {
   dfree(g->d_V);
   dfree(g->d_VB);
   dfree(g->d_M);
@@ -1397,7 +1505,6 @@ void free_chunk(pdf_grader *g) {
   dfree(g->d_act_list);
   dfree(g->d_cs);
   dfree(g->d_hs);
-  g->chunk_pairs = 0;
 }
 
 // Queue the reset of the cross-chunk state (no host synchronization).
@@ -1531,6 +1638,8 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_grand);
   dfree(g->d_err);
   for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
+  if (g->lane0_done) (void)hipEventDestroy(g->lane0_done);
+  if (g->lane_start) (void)hipEventDestroy(g->lane_start);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
   if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -1625,30 +1734,34 @@ int pdf_grader_set_chunk(pdf_grader *g, uint64_t chunk_pairs) {
   const uint64_t per_wave = 2 * n * sizeof(uint64_t);
   g->cover_waves =
       (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g->max_cover_waves, (1ull << 30) / per_wave));
-  int rc = dalloc(&g->d_V, n * g->pitch);
-  if (!rc && !g->seq) rc = dalloc(&g->d_VB, n * g->pitch);
-  if (!rc) rc = dalloc(&g->d_M, n * g->pitch);
-  if (!rc) rc = dalloc(&g->d_NZ, n * g->ntiles);
-  if (!rc) rc = dalloc(&g->d_ACC, n * g->pitch);
-  // the removal accumulator is zero between passes (each reader clears it)
-  if (!rc && !hip_ok(hipMemset(g->d_ACC, 0, n * g->pitch * 8), "hipMemset")) rc = PDF_ERR_HIP;
-  if (!rc) rc = dalloc(&g->d_list, n * words);
-  const uint32_t ct_off = (uint32_t)((3 * words + 2) & ~1ULL);
-  g->n_cstate = ct_off + 2;
-  if (!rc) rc = dalloc(&g->d_cstate, g->n_cstate);
-  if (!rc) {
-    g->d_actmap = reinterpret_cast<uint64_t *>(g->d_cstate);
-    g->d_list_cnt = g->d_cstate + 2 * words;
-    g->d_act_count = g->d_cstate + 3 * words;
-    g->d_chunk_total = reinterpret_cast<uint64_t *>(g->d_cstate + ct_off);
+  int rc = alloc_chunk_buffers(g, chunk_pairs);
+  // extra lanes: the same buffers again, with a stream and an event each
+  const uint64_t nchunks = (g->num_pairs + chunk_pairs - 1) / chunk_pairs;
+  const uint32_t nlanes = (uint32_t)std::min<uint64_t>(g->tune_lanes, nchunks);
+  if (!rc && nlanes > 1) {
+    pdf_grader::Lane l0;
+    lane_save(g, l0);
+    for (uint32_t i = 1; i < nlanes && !rc; ++i) {
+      pdf_grader::Lane l;
+      lane_load(g, pdf_grader::Lane{});
+      rc = alloc_chunk_buffers(g, chunk_pairs);
+      lane_save(g, l);
+      l.stream = nullptr;
+      if (!rc && !hip_ok(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking), "hipStreamCreate")) rc = PDF_ERR_HIP;
+      if (!rc && !hip_ok(hipEventCreateWithFlags(&l.done, hipEventDisableTiming), "hipEventCreate")) rc = PDF_ERR_HIP;
+      g->lanes.push_back(l);  // freed by free_chunk even when incomplete
+    }
+    lane_load(g, l0);
   }
-  if (!rc) rc = dalloc(&g->d_act_list, chunk_pairs);
-  if (!rc) rc = dalloc(&g->d_cs, n * g->cover_waves);
-  if (!rc) rc = dalloc(&g->d_hs, n * g->cover_waves);
+  if (!rc && nlanes > 1 && !g->lane0_done &&
+      (!hip_ok(hipEventCreateWithFlags(&g->lane0_done, hipEventDisableTiming), "hipEventCreate") ||
+       !hip_ok(hipEventCreateWithFlags(&g->lane_start, hipEventDisableTiming), "hipEventCreate")))
+    rc = PDF_ERR_HIP;
   if (rc) { free_chunk(g); return rc; }
   g->chunk_pairs = chunk_pairs;
   return reset_state(g);
 }
+
 
 int pdf_grader_num_chunks(const pdf_grader *g, uint64_t *num_chunks) {
   if (!g || !num_chunks) { set_error("null argument"); return PDF_ERR_ARG; }
@@ -1742,7 +1855,149 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
 
 // Every launch of one fused grade (reset, then per chunk: forward levels,
 // reverse levels, events, cover), in stream order.
+// The fused grade on several lanes: chunk c runs on lane c % lanes, a group
+// of `lanes` consecutive chunks at a time, their level launches issued
+// interleaved (level k of every chunk of the group, then level k + 1), so
+// the lanes' kernels overlap whether launched directly or replayed from a
+// graph.  A chunk's cover needs the merges of every earlier chunk, so after
+// its reverse pass the lane waits for the previous chunk's `done` and
+// records its own (done of chunk c then covers all merges up to c).
+// Results equal the one-lane order.
+static int queue_grade_lanes(pdf_grader *g, int robust) {
+  const uint32_t L = 1 + (uint32_t)g->lanes.size();
+  std::vector<pdf_grader::Lane> ln(L);
+  lane_save(g, ln[0]);
+  ln[0].done = g->lane0_done;
+  for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
+  int rc = queue_reset(g);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(g->lane_start, ln[0].stream));
+  for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamWaitEvent(ln[i].stream, g->lane_start, 0));
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  for (uint64_t c0 = 0; c0 < nchunks && !rc; c0 += L) {
+    const uint64_t c1 = std::min<uint64_t>(nchunks, c0 + L);
+    auto chunk = [&](uint64_t c, uint64_t &start, uint64_t &width) {
+      lane_load(g, ln[c % L]);
+      start = c * g->chunk_pairs;
+      width = std::min(g->chunk_pairs, g->num_pairs - start);
+    };
+    uint64_t start, width;
+    if (!g->seq)
+      for (uint64_t c = c0; c < c1 && !rc; ++c) {
+        chunk(c, start, width);
+        rc = reset_chunk(g);
+      }
+    for (uint32_t k = 0; k <= g->nl.L && !rc; ++k)
+      for (uint64_t c = c0; c < c1 && !rc; ++c) {
+        chunk(c, start, width);
+        rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true, 0xFFFFFFFFu, k, k);
+      }
+    for (uint32_t k = g->nl.L + 1; k-- > 0 && !rc;)
+      for (uint64_t c = c0; c < c1 && !rc; ++c) {
+        chunk(c, start, width);
+        rc = launch_reverse<true, true>(g, start, width, k, k);
+      }
+    for (uint64_t c = c0; c < c1 && !rc; ++c) {
+      chunk(c, start, width);
+      const pdf_grader::Lane &l = ln[c % L];
+      if (c > 0 && !hip_ok(hipStreamWaitEvent(l.stream, ln[(c - 1) % L].done, 0), "hipStreamWaitEvent"))
+        rc = PDF_ERR_HIP;
+      if (!rc && !hip_ok(hipEventRecord(l.done, l.stream), "hipEventRecord")) rc = PDF_ERR_HIP;
+      if (!rc) rc = launch_cover(g, start, width);
+    }
+  }
+  lane_load(g, ln[0]);
+  // lane 0 waits for the other lanes' last chunks
+  for (uint32_t i = 1; i < L && !rc; ++i) {
+    if (!hip_ok(hipEventRecord(ln[i].done, ln[i].stream), "hipEventRecord") ||
+        !hip_ok(hipStreamWaitEvent(ln[0].stream, ln[i].done, 0), "hipStreamWaitEvent"))
+      rc = PDF_ERR_HIP;
+  }
+  return rc;
+}
+
+// Lanes with graphs: capture (once per robust flag) each chunk's forward and
+// reverse levels on its lane's stream.
+static int build_chunk_graphs(pdf_grader *g, int robust) {
+  drop_graph(g);
+  const uint32_t L = 1 + (uint32_t)g->lanes.size();
+  std::vector<pdf_grader::Lane> ln(L);
+  lane_save(g, ln[0]);
+  for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  int rc = PDF_OK;
+  for (uint64_t c = 0; c < nchunks && !rc; ++c) {
+    lane_load(g, ln[c % L]);
+    const uint64_t start = c * g->chunk_pairs;
+    const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
+    hipGraph_t graph = nullptr;
+    if (!hip_ok(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) {
+      rc = PDF_ERR_HIP;
+      break;
+    }
+    if (!g->seq) rc = reset_chunk(g);
+    if (!rc) rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true);
+    if (!rc) rc = launch_reverse<true, true>(g, start, width);
+    const hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (!rc && !hip_ok(e, "hipStreamEndCapture")) rc = PDF_ERR_HIP;
+    hipGraphExec_t exec = nullptr;
+    if (!rc && !hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) rc = PDF_ERR_HIP;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (exec) g->chunk_graphs.push_back(exec);
+  }
+  lane_load(g, ln[0]);
+  if (rc) {
+    drop_graph(g);
+    return rc;
+  }
+  g->chunk_graphs_robust = robust != 0;
+  return PDF_OK;
+}
+
+// The fused grade on lanes from the per-chunk graphs: a group of `lanes`
+// chunk graphs launched on their streams, then their covers in chunk order
+// behind the `done` chain (as in queue_grade_lanes).
+static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
+  if (g->chunk_graphs_robust != (robust != 0)) {
+    int rc = build_chunk_graphs(g, robust);
+    if (rc) return rc;
+  }
+  const uint32_t L = 1 + (uint32_t)g->lanes.size();
+  std::vector<pdf_grader::Lane> ln(L);
+  lane_save(g, ln[0]);
+  ln[0].done = g->lane0_done;
+  for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
+  int rc = queue_reset(g);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(g->lane_start, ln[0].stream));
+  for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamWaitEvent(ln[i].stream, g->lane_start, 0));
+  const uint64_t nchunks = g->chunk_graphs.size();
+  for (uint64_t c0 = 0; c0 < nchunks && !rc; c0 += L) {
+    const uint64_t c1 = std::min<uint64_t>(nchunks, c0 + L);
+    for (uint64_t c = c0; c < c1 && !rc; ++c)
+      if (!hip_ok(hipGraphLaunch(g->chunk_graphs[c], ln[c % L].stream), "hipGraphLaunch")) rc = PDF_ERR_HIP;
+    for (uint64_t c = c0; c < c1 && !rc; ++c) {
+      const pdf_grader::Lane &l = ln[c % L];
+      lane_load(g, l);
+      const uint64_t start = c * g->chunk_pairs;
+      const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
+      if (c > 0 && !hip_ok(hipStreamWaitEvent(l.stream, ln[(c - 1) % L].done, 0), "hipStreamWaitEvent"))
+        rc = PDF_ERR_HIP;
+      if (!rc && !hip_ok(hipEventRecord(l.done, l.stream), "hipEventRecord")) rc = PDF_ERR_HIP;
+      if (!rc) rc = launch_cover(g, start, width);
+    }
+  }
+  lane_load(g, ln[0]);
+  for (uint32_t i = 1; i < L && !rc; ++i) {
+    if (!hip_ok(hipEventRecord(ln[i].done, ln[i].stream), "hipEventRecord") ||
+        !hip_ok(hipStreamWaitEvent(ln[0].stream, ln[i].done, 0), "hipStreamWaitEvent"))
+      rc = PDF_ERR_HIP;
+  }
+  return rc;
+}
+
 static int queue_grade(pdf_grader *g, int robust) {
+  if (!g->lanes.empty() && !g->timing) return queue_grade_lanes(g, robust);
   int rc = queue_reset(g);
   if (rc) return rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
@@ -1763,7 +2018,9 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   HIPCHK(hipSetDevice(g->device));
   int rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
-  if (g->tune_use_graph && !g->timing) {
+  if (g->tune_use_graph && !g->timing && !g->lanes.empty()) {
+    if ((rc = queue_grade_lane_graphs(g, robust))) return rc;
+  } else if (g->tune_use_graph && !g->timing) {
     // the launch sequence depends only on the chunk shape and the robust
     // flag, so it is captured once and replayed
     if (!g->graph || g->graph_robust != (robust != 0)) {
@@ -1805,6 +2062,11 @@ int pdf_grade(pdf_grader *g, int robust, uint64_t *total) { return grade_impl(g,
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
   if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
   switch (key) {
+    case PDF_TUNE_LANES: {
+      if (value < 1 || value > 4) { set_error("lanes must be 1..4"); return PDF_ERR_ARG; }
+      g->tune_lanes = (uint32_t)value;
+      return g->chunk_pairs ? pdf_grader_set_chunk(g, g->chunk_pairs) : PDF_OK;
+    }
     case PDF_TUNE_WORDS_PER_LANE:
       if (value != 1 && value != 2) { set_error("words per lane must be 1 or 2"); return PDF_ERR_ARG; }
       g->tune_words_per_lane = (uint32_t)value;
@@ -2100,6 +2362,75 @@ int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_m
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- autotuning
+
+extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf_grader_tuning *best) {
+  if (!g || reps == 0) { set_error("pdf_grader_autotune: bad argument"); return PDF_ERR_ARG; }
+  if (!g->num_pairs) { set_error("pdf_grader_autotune: load vectors first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  const uint64_t P = g->num_pairs;
+  auto round64 = [](uint64_t x) { return (x + 63) / 64 * 64; };
+  struct Cand { uint32_t lanes; uint64_t chunk; };
+  std::vector<Cand> cands;
+  for (uint32_t lanes = 1; lanes <= 3; ++lanes) {
+    const uint64_t chunk = lanes == 1 ? P : round64((P + lanes - 1) / lanes);
+    if (lanes > 1 && chunk >= P) continue;
+    cands.push_back({lanes, chunk});
+  }
+  uint64_t ref = 0;
+  bool have_ref = false;
+  pdf_grader_tuning bt{};
+  bt.ms = 1e30f;
+  int rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  if (!hip_ok(hipEventCreate(&e1), "hipEventCreate")) {
+    (void)hipEventDestroy(e0);
+    return PDF_ERR_HIP;
+  }
+  auto run = [&]() -> int {
+    for (const Cand &c : cands)
+      for (uint32_t graph = 0; graph < 2; ++graph)
+        for (uint32_t wpl = 1; wpl <= 2; ++wpl) {
+          if (wpl == 2 && words_per_lane(2, g->seq, true, true, c.chunk) != 2) continue;  // no effect
+          g->tune_lanes = c.lanes;
+          g->tune_use_graph = graph;
+          g->tune_words_per_lane = wpl;
+          if ((rc = pdf_grader_set_chunk(g, c.chunk))) return rc;
+          std::vector<float> t(reps + 1);
+          uint64_t tot = 0;
+          for (uint32_t r = 0; r <= reps; ++r) {  // the first run warms up (and captures)
+            HIPCHK(hipEventRecord(e0, g->stream));
+            if ((rc = grade_impl(g, robust, &tot))) return rc;
+            HIPCHK(hipEventRecord(e1, g->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            HIPCHK(hipEventElapsedTime(&t[r], e0, e1));
+          }
+          if (!have_ref) {
+            ref = tot;
+            have_ref = true;
+          } else if (tot != ref) {
+            set_error("pdf_grader_autotune: totals differ between configurations");
+            return PDF_ERR_STATE;
+          }
+          std::sort(t.begin() + 1, t.end());
+          const float med = t[1 + reps / 2];
+          if (med < bt.ms) bt = pdf_grader_tuning{c.lanes, c.chunk, graph, wpl, med};
+        }
+    return PDF_OK;
+  };
+  rc = run();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  g->tune_lanes = bt.lanes;
+  g->tune_use_graph = bt.use_graph;
+  g->tune_words_per_lane = bt.words_per_lane;
+  if ((rc = pdf_grader_set_chunk(g, bt.chunk_pairs))) return rc;
+  if (best) *best = bt;
+  return PDF_OK;
+}
 
 // ----------------------------------------------------------- batched grading
 // A batch owns one grader per circuit (vectors, planes, first[] and lists),

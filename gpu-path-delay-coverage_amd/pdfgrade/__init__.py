@@ -29,7 +29,7 @@ UNKN, INPT, AND, NAND, OR, NOR, XOR, XNOR, BUFF, NOT, FROM, DFF = range(12)
 TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF", "NOT", "FROM", "DFF"]
 S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
-TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER = 1, 2, 3, 4
+TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES = 1, 2, 3, 4, 5
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
@@ -58,6 +58,11 @@ class _Desc(C.Structure):
 class _Info(C.Structure):
     _fields_ = [("n", C.c_uint32), ("num_pi", C.c_uint32), ("num_po", C.c_uint32), ("levels", C.c_uint32),
                 ("edges", C.c_uint64), ("structural_paths", C.c_uint64)]
+
+
+class _GraderTuning(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("chunk_pairs", C.c_uint64), ("use_graph", C.c_uint32),
+                ("words_per_lane", C.c_uint32), ("ms", C.c_float)]
 
 
 class KernelTimes(C.Structure):
@@ -118,6 +123,7 @@ SIGNATURES = {
     "pdf_mlp_adam": ([_P, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
     "pdf_mlp_train_step": ([_P, C.c_void_p, _U64, C.c_void_p, _U64, C.c_float, _FP], C.c_int),
     "pdf_mlp_sync": ([_P], C.c_int),
+    "pdf_grader_autotune": ([_P, C.c_int, C.c_uint32, C.c_void_p], C.c_int),
     "pdf_batch_create": ([C.c_int, C.c_uint32, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_uint64), C.POINTER(_P)],
                          C.c_int),
     "pdf_batch_destroy": ([_P], None),
@@ -333,8 +339,15 @@ class Grader:
         _check(lib().pdf_grader_reset(self._h))
 
     def set_tuning(self, key: int, value: int):
-        """TUNE_WORDS_PER_LANE (1/2) or TUNE_USE_GRAPH (0/1); results never change."""
+        """TUNE_WORDS_PER_LANE (1/2), TUNE_USE_GRAPH (0/1) or TUNE_LANES (1..4);
+        results never change."""
         _check(lib().pdf_grader_set_tuning(self._h, key, value))
+
+    def autotune(self, robust: bool, reps: int = 5) -> dict:
+        """Time the knob configurations, keep the fastest; returns it."""
+        t = _GraderTuning()
+        _check(lib().pdf_grader_autotune(self._h, int(robust), reps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _GraderTuning._fields_}
 
     # -- the four stages (reference host wrappers)
     def sim(self, chunk: int, start: int) -> float:

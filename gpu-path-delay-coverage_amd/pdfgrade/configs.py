@@ -23,6 +23,7 @@ class Workload:
     pairs: int
     chunk: int
     robust: bool = True
+    lanes: int = 1  # chunks graded concurrently (pdf_grader_autotune's choice, see bench.py)
 
 
 TINY = Workload(
@@ -35,7 +36,7 @@ GATES_50K = Workload(
     name="single-50k-gates",
     netlist=GenConfig(seed=2, num_pi=256, num_po=128, num_gates=50_000, levels=60, fanin_p=WIDE_FANIN,
                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64),
-    vector_seed=102, pairs=65_536, chunk=65_536)
+    vector_seed=102, pairs=65_536, chunk=32_768, lanes=2)
 
 GATES_2M = Workload(
     name="single-2m-gates",

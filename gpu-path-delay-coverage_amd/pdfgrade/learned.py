@@ -67,7 +67,7 @@ class LearnedConfig:
     levels: int = 60
     num_pi: int = 256
     num_po: int = 128
-    max_level: int = 12              # feature simulation depth
+    max_level: int = 8               # feature simulation depth
     batch: int = 8192
     epochs: int = 4
     lr: float = 1e-3
@@ -334,7 +334,7 @@ def main(argv=None):
     ap.add_argument("--candidates", type=int, default=1 << 20)
     ap.add_argument("--gates", type=int, default=50_000)
     ap.add_argument("--levels", type=int, default=60)
-    ap.add_argument("--max-level", type=int, default=12)
+    ap.add_argument("--max-level", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)

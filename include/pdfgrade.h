@@ -209,6 +209,7 @@ int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
 #define PDF_TUNE_WORDS_PER_LANE 1 /* 1 or 2 words per lane in the fused forward pass */
 #define PDF_TUNE_USE_GRAPH 2
 #define PDF_TUNE_COVER_WAVES 3   /* batch: waves of the cover launch */
+#define PDF_TUNE_LANES 5         /* grader: chunks graded concurrently on 1..4 streams */
 #define PDF_TUNE_SEGMENT_ORDER 4 /* batch: 0 circuit order, 1 largest segments first */      /* 1: pdf_grade replays a captured HIP graph */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
@@ -295,6 +296,18 @@ int pdf_mlp_adam(pdf_mlp *m, float lr, float beta1, float beta2, float eps);
 int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const float *target_dev, uint64_t rows,
                        float lr, float *loss);
 int pdf_mlp_sync(pdf_mlp *m);
+
+/* Autotuner (no reference counterpart): times the grade (reps runs each,
+ * median) over chunk lanes 1..3 (chunk = pairs / lanes), graph replay on and
+ * off and words per lane, checks that every configuration gives the same
+ * total, and leaves the grader in the fastest one. */
+typedef struct {
+  uint32_t lanes;
+  uint64_t chunk_pairs;
+  uint32_t use_graph, words_per_lane;
+  float ms; /* median grade time of this configuration */
+} pdf_grader_tuning;
+int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf_grader_tuning *best);
 
 /* ------------------------------------------------------------------------ */
 /* Batched grading (no reference counterpart): many circuits, each with its  */

@@ -5,6 +5,12 @@ chunk, the reference's chunk loop main.cu:171-231) of 65,536 pairs on the
 seeded 50,000-gate / 60-level netlist, with the netlist and vectors already
 resident in HBM.  Prints one JSON line.
 
+The second half of the metric, time to 95% of full path coverage with the
+learned pair ranking against plain in-order grading, comes from the held-out
+experiment of pdfgrade.learned (32 training and 8 held-out 50k-gate circuits,
+1M candidate pairs each), run after the timed steps in both marking modes
+(--learned 0 skips it).
+
 --workload batch-256-circuits instead times one batched grade of 256
 circuits (1k-200k gates, 4k-64k pairs each) against grading them one after
 another with per-circuit launches.
@@ -37,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=4096, help="pairs graded by the CPU baseline (0 = skip)")
     ap.add_argument("--robust", type=int, default=1)
     ap.add_argument("--graph", type=int, default=None, help="batch workloads: force graph replay on (1) or off (0)")
+    ap.add_argument("--learned", type=int, default=1, help="50k workload: run the learned-ranking experiment")
     ap.add_argument("--autotune", type=int, default=None,
                     help="run the autotuner before the timed steps (default: no for single workloads, yes for "
                          "batches)")
@@ -137,6 +144,26 @@ def bench_batch(args, wl):
     b.close()
 
 
+def learned_vs_plain(robust: bool):
+    """Held-out learned ranking against plain order (pdfgrade.learned.run)."""
+    from pdfgrade import learned
+
+    cfg = learned.LearnedConfig(robust=robust)
+    res = learned.run(cfg, log=lambda s: None)
+    s95, g95 = res["summary"]["paths_95"], res["summary"]["segments_95"]
+    per = [c["learned"] for c in res["circuits"]]
+    return {
+        "mode": "robust" if robust else "non-robust",
+        "time_to_95pct_paths_learned_over_plain_full": s95["learned_over_plain_full"],
+        "time_to_95pct_paths_learned_over_plain_same": s95["learned_over_plain_same_fraction"],
+        "pairs_to_95pct_paths": {"learned": s95["pairs_learned_mean"], "plain": s95["pairs_plain_mean"]},
+        "pairs_to_95pct_segments": {"learned": g95["pairs_learned_mean"], "plain": g95["pairs_plain_mean"]},
+        "overhead_ms": {k: sum(p[k] for p in per) / len(per) for k in ("features_ms", "score_ms", "rank_ms")},
+        "plain_full_ms": sum(c["plain_ms"] for c in res["circuits"]) / len(res["circuits"]),
+        "heldout_circuits": len(res["circuits"]), "candidates": cfg.candidates, "records": res["records"],
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -202,8 +229,13 @@ def main():
                "kind": "port",
                "sample": f"first {sample} of {wl.pairs} pairs of the {wl.name} netlist, one chunk, serial C oracle"}
 
+    learned_part = None
+    if args.learned and wl.name == "single-50k-gates":
+        learned_part = [learned_vs_plain(robust), learned_vs_plain(not robust)]
+
     out = {
-        "metric": "vector-pairs graded/s at 50k gates",
+        "metric": ("vector-pairs graded/s at 50k gates; time-to-95%-coverage learned vs plain"
+                   if wl.name == "single-50k-gates" else f"vector-pairs graded/s ({wl.name})"),
         "value": pairs_per_s,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -225,6 +257,7 @@ def main():
         "end_to_end": {"algorithmic_GBps": e2e_gbs, "frac": e2e_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "wall_ms")},
         "cpu_baseline": cpu,
+        "learned_vs_plain": learned_part,
     }
     print(json.dumps(out))
     g.close()

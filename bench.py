@@ -145,11 +145,21 @@ def bench_batch(args, wl):
 
 
 def learned_vs_plain(robust: bool):
-    """Held-out learned ranking against plain order (pdfgrade.learned.run)."""
-    from pdfgrade import learned
+    """Held-out learned ranking against plain order: `python -m
+    pdfgrade.learned` in a child process (it shares torch's HIP runtime,
+    which this process does not load)."""
+    import subprocess
+    import tempfile
 
-    cfg = learned.LearnedConfig(robust=robust)
-    res = learned.run(cfg, log=lambda s: None)
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "learned.json")
+        cmd = [sys.executable, "-m", "pdfgrade.learned", "--out", out] + ([] if robust else ["--non-robust"])
+        r = subprocess.run(cmd, cwd=os.path.join(ROOT, "gpu-path-delay-coverage_amd"), stdout=subprocess.DEVNULL,
+                           stderr=subprocess.PIPE, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"learned experiment failed ({r.returncode}): {r.stderr[-2000:]}")
+        with open(out) as f:
+            res = json.load(f)
     s95, g95 = res["summary"]["paths_95"], res["summary"]["segments_95"]
     per = [c["learned"] for c in res["circuits"]]
     return {
@@ -160,7 +170,8 @@ def learned_vs_plain(robust: bool):
         "pairs_to_95pct_segments": {"learned": g95["pairs_learned_mean"], "plain": g95["pairs_plain_mean"]},
         "overhead_ms": {k: sum(p[k] for p in per) / len(per) for k in ("features_ms", "score_ms", "rank_ms")},
         "plain_full_ms": sum(c["plain_ms"] for c in res["circuits"]) / len(res["circuits"]),
-        "heldout_circuits": len(res["circuits"]), "candidates": cfg.candidates, "records": res["records"],
+        "heldout_circuits": len(res["circuits"]), "candidates": res["config"]["candidates"],
+        "records": res["records"],
     }
 
 

@@ -137,7 +137,14 @@ _lib = None
 
 
 def lib():
-    """Load libpdfgrade.so (built by __graft_entry__.build()); raise if absent."""
+    """Load libpdfgrade.so (built by __graft_entry__.build()); raise if absent.
+
+    torch ships its own HIP runtime under the same soname as the one the
+    library links, and whichever is loaded first serves the whole process;
+    torch's device layer only works on its own.  A process that passes torch
+    tensors to the library (the learned ranking) must import torch before the
+    library is loaded; otherwise the library runs on the system ROCm runtime
+    (the faster one for grading)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
@@ -608,7 +615,8 @@ def _feat_arg(feat):
 
 def _need_device(t, count: int, itemsize: int):
     """Check a device-tensor argument and wait for torch's pending work on it
-    (the library runs on its own streams)."""
+    (the library runs on its own streams).  torch must have been imported
+    before the library was loaded (see lib())."""
     if not getattr(t, "is_cuda", False) or not t.is_contiguous():
         raise ValueError("expected a contiguous device tensor")
     if t.numel() * t.element_size() < count * itemsize:

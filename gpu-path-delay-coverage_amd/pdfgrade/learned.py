@@ -35,6 +35,7 @@ import time
 from dataclasses import asdict, dataclass, field
 
 import numpy as np
+import torch  # before libpdfgrade.so is loaded: the library then shares torch's HIP runtime
 
 from . import FEATURES, NONE, GenConfig, Grader, Mlp, Netlist, gen_toggle_pairs
 
@@ -82,7 +83,6 @@ def _round64(x: int) -> int:
 
 
 def _torch():
-    import torch
     return torch
 
 

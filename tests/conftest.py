@@ -9,3 +9,10 @@ for p in (ROOT, os.path.join(ROOT, "gpu-path-delay-coverage_amd")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+# torch ships its own HIP runtime; the ranking tests pass torch tensors to the
+# library, so torch is loaded before the library in every test process
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass

@@ -802,31 +802,28 @@ __device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint
   return v;
 }
 
-// Block = kFeatWords words x kFeatSlices slices of one feature; each slice
-// converts and writes 8 of its word's 64 pairs (one 16-byte store).
-// LEVELS: feature f = blockIdx.y mapped to the level features of the
-// simulated levels (transitions, then forward marks); each slice counts its
-// share of the level's gates, consecutive lanes reading consecutive words of
-// each row, and the slices' bit-sliced counters are added in LDS.  Otherwise
-// f = 256 + blockIdx.y, a PI-bucket feature (a few PIs), counted whole by
-// every slice.  Levels past the simulated depth are zero-filled by the host.
+// Level features: block = kFeatWords words x kFeatSlices slices of one
+// feature f, blockIdx.y mapped to the simulated levels' transition features
+// and then their forward-mark features.  Each slice counts its share of the
+// level's gates, consecutive lanes reading consecutive words of each row; the
+// slices' bit-sliced counters are added in LDS, and each slice converts and
+// writes 8 of its word's 64 pairs (one 16-byte store).  Levels past the
+// simulated depth are zero-filled by the host.
 constexpr uint32_t kFeatWords = 32, kFeatSlices = 8;
 
-template <bool SEQ, bool LEVELS>
-__global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__restrict__ recs, Planes pl,
-                                                  const uint32_t *__restrict__ level_off, uint32_t levels_used,
-                                                  uint32_t num_pi, uint64_t first_pair, uint64_t num_pairs,
+template <bool SEQ>
+__global__ __launch_bounds__(256) void k_features(const Planes pl, const uint32_t *__restrict__ level_off,
+                                                  uint32_t levels_used, uint64_t first_pair, uint64_t num_pairs,
                                                   uint16_t *__restrict__ feat, uint64_t ld) {
   const uint32_t wl = threadIdx.x % kFeatWords, sl = threadIdx.x / kFeatWords;
   const uint32_t w = blockIdx.x * kFeatWords + wl;
-  const uint32_t f = LEVELS ? (blockIdx.y < levels_used ? blockIdx.y : 128 + blockIdx.y - levels_used)
-                            : 256 + blockIdx.y;
+  const uint32_t f = blockIdx.y < levels_used ? blockIdx.y : 128 + blockIdx.y - levels_used;
   const bool live = w < pl.words;
   uint64_t c[kCountBits];
 #pragma unroll
   for (int k = 0; k < kCountBits; ++k) c[k] = 0;
   float denom = 0;
-  if (LEVELS) {
+  {
     const uint32_t lv = f & 127;
     const bool marks = f >= 128;
     {
@@ -869,21 +866,8 @@ __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__r
         vc_add_sliced(c, s);
       }
     }
-  } else {
-    const uint32_t bkt = (f - 256) & 127;
-    const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
-    const bool weighted = f >= 384;
-    for (uint32_t g = lo; g < hi; ++g) {
-      const uint32_t wt = weighted ? recs[g].nfo : 1u;
-      denom += (float)wt;
-      if (!live) continue;
-      uint64_t a, b;
-      frames<SEQ>(pl, g, w, a, b);
-      for (int k = 0; k < 17; ++k)  // weights are at most 65535
-        if ((wt >> k) & 1) vc_inc<kCountBits>(c, a ^ b, k);
-    }
   }
-  if constexpr (LEVELS) {  // add the slices' counters
+  {  // add the slices' counters
     __shared__ uint64_t red[kFeatSlices][kCountBits][kFeatWords];
 #pragma unroll
     for (int k = 0; k < kCountBits; ++k) red[sl][k][wl] = c[k];
@@ -917,6 +901,50 @@ __global__ __launch_bounds__(256) void k_features(DevNet net, const GateRec *__r
   } else {
     for (uint32_t e = 0; e < 8 && base + j0 + e < num_pairs; ++e)
       dst[j0 + e] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+  }
+}
+
+// PI-bucket features f = 256 + blockIdx.y: thread = word.  A bucket holds a
+// few PIs, so each pair's (weighted) count is accumulated directly, 64 float
+// registers (integers below 2^24, exact), then scaled and written as 8
+// 16-byte stores.
+template <bool SEQ>
+__global__ __launch_bounds__(256) void k_features_pi(const GateRec *__restrict__ recs, Planes pl, uint32_t num_pi,
+                                                     uint64_t first_pair, uint64_t num_pairs,
+                                                     uint16_t *__restrict__ feat, uint64_t ld) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x, f = 256 + blockIdx.y;
+  if (w >= pl.words) return;
+  const uint32_t bkt = (f - 256) & 127;
+  const uint32_t lo = (uint32_t)((uint64_t)bkt * num_pi / 128), hi = (uint32_t)((uint64_t)(bkt + 1) * num_pi / 128);
+  const bool weighted = f >= 384;
+  float acc[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) acc[j] = 0.f;
+  float denom = 0;
+  for (uint32_t g = lo; g < hi; ++g) {
+    const float wt = weighted ? (float)recs[g].nfo : 1.f;
+    denom += wt;
+    uint64_t a, b;
+    frames<SEQ>(pl, g, w, a, b);
+    const uint64_t t = a ^ b;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) acc[j] += ((t >> j) & 1) ? wt : 0.f;
+  }
+  const float scale = denom > 0 ? 1.0f / denom : 0.0f;
+  const uint64_t base = first_pair + 64ull * w;
+  uint16_t *dst = feat + (uint64_t)f * ld + base;
+#pragma unroll
+  for (uint32_t j0 = 0; j0 < 64; j0 += 8) {
+    uint32_t v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      v[e] = (uint32_t)to_bf16(acc[j0 + 2 * e] * scale) | ((uint32_t)to_bf16(acc[j0 + 2 * e + 1] * scale) << 16);
+    if (base + j0 + 8 <= num_pairs) {
+      *reinterpret_cast<uint4 *>(dst + j0) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+      for (uint32_t e = 0; e < 8 && base + j0 + e < num_pairs; ++e)
+        dst[j0 + e] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+    }
   }
 }
 
@@ -2191,32 +2219,56 @@ int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *f
   HIPCHK(hipSetDevice(g->device));
   const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
-  int rc;
+  int rc = PDF_OK;
   if (last < 127) {  // level features past the simulated depth are zero
     for (uint64_t f0 : {(uint64_t)last + 1, (uint64_t)128 + last + 1})
       HIPCHK(hipMemset2DAsync(feat_dev + f0 * ld, ld * 2, 0, g->num_pairs * 2, 127 - last, g->stream));
   }
-  for (uint64_t c = 0; c < nchunks; ++c) {
-    const uint64_t start = c * g->chunk_pairs;
-    const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
-    if ((rc = launch_fwd<true, true>(g, start, width, robust, false, last))) return rc;
-    const Planes pl = planes(g, width);
-    const dim3 lgrid(grid_for(pl.words, kFeatWords), 2 * (last + 1)), pgrid(grid_for(pl.words, kFeatWords), 256);
-    const DevNet net = g->net();
-    if (g->seq) {
-      k_features<true, true><<<lgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1, g->nl.num_pi,
-                                                           start, g->num_pairs, feat_dev, ld);
-      k_features<true, false><<<pgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1,
-                                                            g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
-    } else {
-      k_features<false, true><<<lgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1,
-                                                            g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
-      k_features<false, false><<<pgrid, 256, 0, g->stream>>>(net, g->d_rec, pl, g->d_level_off, last + 1,
-                                                             g->nl.num_pi, start, g->num_pairs, feat_dev, ld);
-    }
-    HIPCHK(hipGetLastError());
+  // chunks are independent here: a group of `lanes` chunks at a time, their
+  // level launches interleaved across the lanes' streams
+  const uint32_t L = 1 + (uint32_t)g->lanes.size();
+  std::vector<pdf_grader::Lane> ln(L);
+  lane_save(g, ln[0]);
+  for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
+  if (L > 1) {
+    HIPCHK(hipEventRecord(g->lane_start, ln[0].stream));
+    for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamWaitEvent(ln[i].stream, g->lane_start, 0));
   }
+  for (uint64_t c0 = 0; c0 < nchunks && !rc; c0 += L) {
+    const uint64_t c1 = std::min<uint64_t>(nchunks, c0 + L);
+    uint64_t start = 0, width = 0;
+    auto chunk = [&](uint64_t c) {
+      lane_load(g, ln[c % L]);
+      start = c * g->chunk_pairs;
+      width = std::min(g->chunk_pairs, g->num_pairs - start);
+    };
+    for (uint32_t k = 0; k <= last && !rc; ++k)
+      for (uint64_t c = c0; c < c1 && !rc; ++c) {
+        chunk(c);
+        rc = launch_fwd<true, true>(g, start, width, robust, false, last, k, k);
+      }
+    for (uint64_t c = c0; c < c1 && !rc; ++c) {
+      chunk(c);
+      const Planes pl = planes(g, width);
+      const dim3 lgrid(grid_for(pl.words, kFeatWords), 2 * (last + 1)), pgrid(grid_for(pl.words, 256), 256);
+      if (g->seq) {
+        k_features<true><<<lgrid, 256, 0, g->stream>>>(pl, g->d_level_off, last + 1, start, g->num_pairs, feat_dev,
+                                                       ld);
+        k_features_pi<true><<<pgrid, 256, 0, g->stream>>>(g->d_rec, pl, g->nl.num_pi, start, g->num_pairs, feat_dev,
+                                                         ld);
+      } else {
+        k_features<false><<<lgrid, 256, 0, g->stream>>>(pl, g->d_level_off, last + 1, start, g->num_pairs, feat_dev,
+                                                        ld);
+        k_features_pi<false><<<pgrid, 256, 0, g->stream>>>(g->d_rec, pl, g->nl.num_pi, start, g->num_pairs,
+                                                          feat_dev, ld);
+      }
+      if (!hip_ok(hipGetLastError(), "k_features")) rc = PDF_ERR_HIP;
+    }
+  }
+  lane_load(g, ln[0]);
+  for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamSynchronize(ln[i].stream));
   HIPCHK(hipStreamSynchronize(g->stream));
+  if (rc) return rc;
   g->stage = 0;  // the planes hold a partial simulation
   g->stage_chunk = -1;
   return PDF_OK;

@@ -56,8 +56,16 @@ struct EpiArgs {
   uint32_t ldc;
 };
 
-constexpr int kBK = 32;        // k per LDS stage (one MFMA deep)
-constexpr int kLd = kBK + 8;   // LDS row pitch in bf16 (80 bytes, keeps 16-byte alignment)
+constexpr int kBK = 64;        // k per LDS stage (two MFMAs deep)
+constexpr int kLd = kBK + 8;   // LDS row pitch in bf16 (144 bytes, keeps 16-byte alignment)
+
+// LDS position of k-chunk `chunk` (8 bf16, 16 bytes) of tile row `row`: the
+// chunk index is XORed with bits 3..5 of the row, so the transposed stores
+// (eight rows 8 apart per lane group) and the MFMA operand reads (16 rows,
+// one chunk) spread over the banks.
+__device__ __forceinline__ uint32_t swz(uint32_t row, uint32_t chunk) {
+  return row * kLd + ((chunk ^ ((row >> 3) & 7)) << 3);
+}
 
 // One operand tile of R rows x kBK k into registers.  Rows are the M (or N)
 // index, k the reduction index.  !T: element (r, k) at P[r * ld + k] (k
@@ -75,8 +83,8 @@ struct Tile {
       const uint32_t q = threadIdx.x + 256u * u;
       uint32_t r, k;
       if (!T) {
-        r = q >> 2;
-        k = (q & 3) * 8;
+        r = q / (kBK / 8);
+        k = (q % (kBK / 8)) * 8;
       } else {
         k = q / (R / 8);
         r = (q % (R / 8)) * 8;
@@ -87,18 +95,18 @@ struct Tile {
     }
   }
 
-  // LDS layout: S[r * kLd + k] (k contiguous) for both operands.
+  // LDS layout: row r, k contiguous within 16-byte chunks placed by swz().
   __device__ __forceinline__ void store(uint16_t *S) const {
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const uint32_t q = threadIdx.x + 256u * u;
       if (!T) {
-        *reinterpret_cast<uint4 *>(&S[(q >> 2) * kLd + (q & 3) * 8]) = reg[u];
+        *reinterpret_cast<uint4 *>(&S[swz(q / (kBK / 8), q % (kBK / 8))]) = reg[u];
       } else {
         const uint32_t k = q / (R / 8), r = (q % (R / 8)) * 8;
         const uint32_t w[4] = {reg[u].x, reg[u].y, reg[u].z, reg[u].w};
 #pragma unroll
-        for (int e = 0; e < 8; ++e) S[(r + e) * kLd + k] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+        for (int e = 0; e < 8; ++e) S[swz(r + e, k >> 3) + (k & 7)] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
       }
     }
   }
@@ -119,8 +127,16 @@ template <bool TA, bool TB, int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, uint32_t lda,
                                               const uint16_t *__restrict__ B, uint32_t ldb, uint32_t M, uint32_t N,
                                               uint32_t K, uint32_t kchunk, EpiArgs ep) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM * kLd];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * kLd];
+  static_assert(BM == 128 && BN == 128, "the epilogue stages 64-row halves of a 128 x 128 tile");
+  // operand tiles during the k loop; the epilogue reuses the space for one
+  // half of the C tile in FP32 (64 x (BN + 4) floats)
+  constexpr int kTileBytes = (BM + BN) * kLd * 2;
+  constexpr int kStageLd = BN + 4;
+  constexpr int kStageBytes = 64 * kStageLd * 4;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[kTileBytes > kStageBytes ? kTileBytes : kStageBytes];
+  uint16_t *As = reinterpret_cast<uint16_t *>(lds);
+  uint16_t *Bs = As + BM * kLd;
+  float *stage = reinterpret_cast<float *>(lds);
   constexpr int MI = BM / 32, NI = BN / 32;
   const uint32_t n0 = blockIdx.x * BN, m0 = blockIdx.y * BM;
   const uint32_t kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
@@ -148,41 +164,75 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, ui
       ta.load(A, lda, m0, M, k0 + kBK, kend);
       tb.load(B, ldb, n0, N, k0 + kBK, kend);
     }
-    bf16x8 a[MI], b[NI];
-    // lane l supplies row (l & 15) and k 8 (l >> 4) .. 8 (l >> 4) + 7
 #pragma unroll
-    for (int mi = 0; mi < MI; ++mi)
-      a[mi] = *reinterpret_cast<const bf16x8 *>(&As[(wr * (BM / 2) + mi * 16 + (lane & 15)) * kLd + 8 * (lane >> 4)]);
+    for (int ks = 0; ks < kBK / 32; ++ks) {
+      bf16x8 a[MI], b[NI];
+      // lane l supplies row (l & 15) and k 8 (l >> 4) .. 8 (l >> 4) + 7 of the substep
 #pragma unroll
-    for (int ni = 0; ni < NI; ++ni)
-      b[ni] = *reinterpret_cast<const bf16x8 *>(&Bs[(wc * (BN / 2) + ni * 16 + (lane & 15)) * kLd + 8 * (lane >> 4)]);
-#pragma unroll
-    for (int mi = 0; mi < MI; ++mi)
+      for (int mi = 0; mi < MI; ++mi)
+        a[mi] = *reinterpret_cast<const bf16x8 *>(
+            &As[swz(wr * (BM / 2) + mi * 16 + (lane & 15), 4 * ks + (lane >> 4))]);
 #pragma unroll
       for (int ni = 0; ni < NI; ++ni)
-        acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+        b[ni] = *reinterpret_cast<const bf16x8 *>(
+            &Bs[swz(wc * (BN / 2) + ni * 16 + (lane & 15), 4 * ks + (lane >> 4))]);
+#pragma unroll
+      for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
+    }
   }
 
-  // lane l holds rows 4 (l >> 4) + r, r = 0..3, of column (l & 15)
+  // Epilogue, one 64-row half at a time: the waves holding that half write
+  // their accumulators (lane l: rows 4 (l >> 4) + r of column l & 15 of each
+  // 16 x 16 tile) into the FP32 stage, then every thread stores 8 consecutive
+  // columns of a row with 16-byte accesses.
 #pragma unroll
-  for (int mi = 0; mi < MI; ++mi)
+  for (uint32_t half = 0; half < 2; ++half) {
+    __syncthreads();  // operand tiles (or the previous half) are no longer read
+    if (wr == half) {
 #pragma unroll
-    for (int ni = 0; ni < NI; ++ni)
+      for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t i = m0 + wr * (BM / 2) + mi * 16 + 4 * (lane >> 4) + r;
-        const uint32_t j = n0 + wc * (BN / 2) + ni * 16 + (lane & 15);
-        if (i >= M || j >= N) continue;
-        const float v = acc[mi][ni][r];
-        const uint64_t o = (uint64_t)i * ep.ldc + j;
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            stage[(mi * 16 + 4 * (lane >> 4) + r) * kStageLd + wc * (BN / 2) + ni * 16 + (lane & 15)] =
+                acc[mi][ni][r];
+    }
+    __syncthreads();
+    for (uint32_t item = threadIdx.x; item < 64 * (BN / 8); item += 256) {
+      const uint32_t r = item / (BN / 8), c8 = (item % (BN / 8)) * 8;
+      const uint32_t i = m0 + half * 64 + r, j = n0 + c8;
+      if (i >= M || j >= N) continue;  // N is a multiple of 8: whole groups
+      const float *v = &stage[r * kStageLd + c8];
+      const uint64_t o = (uint64_t)i * ep.ldc + j;
+      if (EPI == EPI_PARTIAL) {
+        float *dst = ep.out32 + (uint64_t)blockIdx.z * M * ep.ldc + o;
+        *reinterpret_cast<float4 *>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(dst + 4) = make_float4(v[4], v[5], v[6], v[7]);
+      } else {
+        uint32_t w[4];
         if (EPI == EPI_BIAS_RELU) {
-          ep.out16[o] = f2bf(fmaxf(v + ep.bias[j], 0.f));
-        } else if (EPI == EPI_RELU_GRAD) {
-          ep.out16[o] = bf_pos(ep.mask[o]) ? f2bf(v) : (uint16_t)0;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            w[e] = (uint32_t)f2bf(fmaxf(v[2 * e] + ep.bias[j + 2 * e], 0.f)) |
+                   ((uint32_t)f2bf(fmaxf(v[2 * e + 1] + ep.bias[j + 2 * e + 1], 0.f)) << 16);
         } else {
-          ep.out32[(uint64_t)blockIdx.z * M * ep.ldc + o] = v;
+          const uint4 mk = *reinterpret_cast<const uint4 *>(ep.mask + o);
+          const uint32_t mw[4] = {mk.x, mk.y, mk.z, mk.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const uint16_t lo = bf_pos((uint16_t)mw[e]) ? f2bf(v[2 * e]) : (uint16_t)0;
+            const uint16_t hi = bf_pos((uint16_t)(mw[e] >> 16)) ? f2bf(v[2 * e + 1]) : (uint16_t)0;
+            w[e] = (uint32_t)lo | ((uint32_t)hi << 16);
+          }
         }
+        *reinterpret_cast<uint4 *>(ep.out16 + o) = make_uint4(w[0], w[1], w[2], w[3]);
       }
+    }
+  }
 }
 
 // ------------------------------------------------------------ head + sums
@@ -387,7 +437,7 @@ template <bool TA, bool TB, int EPI>
 int gemm(pdf_mlp *m, const uint16_t *A, uint32_t lda, const uint16_t *B, uint32_t ldb, uint32_t M, uint32_t N,
          uint32_t K, uint32_t splits, const EpiArgs &ep) {
   constexpr int BM = 128, BN = 128;
-  if (lda % 8 || ldb % 8 || ((uintptr_t)A | (uintptr_t)B) % 16) {
+  if (lda % 8 || ldb % 8 || ((uintptr_t)A | (uintptr_t)B) % 16 || N % 8 || ep.ldc % 8) {
     pdf::set_error("mlp gemm: operand not 16-byte granular");
     return PDF_ERR_ARG;
   }

@@ -7,8 +7,7 @@
 // CIRCUIT: a .bench file when the name contains "bench", otherwise a
 // presorted .level file (main.cu:129-134).  VECTORS: text files with one
 // vector per line as '0'/'1' characters (one per PI, in INPUT order);
-// whitespace is ignored and '#' starts a comment.  The reference's vector
-// reader lives in its absent util/ submodule, so this format is this tool's.
+// whitespace is ignored and '#' starts a comment (pdf_vectors_read).
 //
 // For every vector file one line goes to stdout, as main.cu:233-234 prints it:
 //   file:V,n,K;gpu;sim;mark;merge;cover;total
@@ -43,33 +42,13 @@ void check(int rc, const char *what) {
   if (rc != PDF_OK) die(std::string(what) + ": " + pdf_last_error());
 }
 
-// Reads one vector per line; returns PI-major packed bits.
+// Reads a vector file (pdf_vectors_read's format); returns PI-major packed bits.
 std::vector<uint64_t> read_vectors(const std::string &path, uint32_t num_pi, uint64_t *count) {
-  std::ifstream in(path);
-  if (!in) die("cannot open vector file " + path);
-  std::vector<std::string> rows;
-  std::string line;
-  while (std::getline(in, line)) {
-    const size_t hash = line.find('#');
-    if (hash != std::string::npos) line.resize(hash);
-    std::string bits;
-    for (char c : line) {
-      if (c == '0' || c == '1') bits.push_back(c);
-      else if (!std::isspace((unsigned char)c)) die("bad character in " + path);
-    }
-    if (bits.empty()) continue;
-    if (bits.size() != num_pi)
-      die(path + ": vector of width " + std::to_string(bits.size()) + ", circuit has " + std::to_string(num_pi) +
-          " inputs");
-    rows.push_back(bits);
-  }
-  if (rows.empty()) die("no vectors in " + path);
-  const uint64_t V = rows.size(), words = (V + 63) / 64;
-  std::vector<uint64_t> packed(num_pi * words, 0);
-  for (uint64_t v = 0; v < V; ++v)
-    for (uint32_t i = 0; i < num_pi; ++i)
-      if (rows[v][i] == '1') packed[i * words + v / 64] |= 1ULL << (v % 64);
-  *count = V;
+  uint64_t V = 0;
+  check(pdf_vectors_count(path.c_str(), num_pi, &V), "pdf_vectors_count");
+  if (V == 0) die("no vectors in " + path);
+  std::vector<uint64_t> packed((size_t)num_pi * ((V + 63) / 64), 0);
+  check(pdf_vectors_read(path.c_str(), num_pi, packed.data(), V, count), "pdf_vectors_read");
   return packed;
 }
 

@@ -357,6 +357,71 @@ int pdf_gen_toggle_pairs(uint64_t seed, uint32_t num_pi, uint64_t count, double 
   return PDF_OK;
 }
 
+// ------------------------------------------------------------- vector files
+// One vector per line, one '0'/'1' per PI in INPUT order; whitespace is
+// ignored and '#' starts a comment (the reference's reader lives in its
+// absent util/ submodule, main.cu:146-156, so this format is the library's).
+
+static int scan_vectors(const char *path, uint32_t num_pi, uint64_t *bits, uint64_t capacity, uint64_t *count) {
+  std::ifstream in(path);
+  if (!in) { set_error(std::string("cannot open vector file ") + path); return PDF_ERR_IO; }
+  const uint64_t words = (capacity + 63) / 64;
+  std::string line;
+  uint64_t v = 0, lineno = 0;
+  while (std::getline(in, line)) {
+    ++lineno;
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    uint32_t width = 0;
+    for (char c : line) {
+      if (c == '0' || c == '1') {
+        if (bits && width < num_pi && v < capacity && c == '1')
+          bits[(uint64_t)width * words + v / 64] |= 1ULL << (v % 64);
+        ++width;
+      } else if (!std::isspace((unsigned char)c)) {
+        set_error(std::string(path) + ":" + std::to_string(lineno) + ": bad character");
+        return PDF_ERR_IO;
+      }
+    }
+    if (width == 0) continue;
+    if (width != num_pi) {
+      set_error(std::string(path) + ":" + std::to_string(lineno) + ": vector of width " + std::to_string(width) +
+                ", circuit has " + std::to_string(num_pi) + " inputs");
+      return PDF_ERR_IO;
+    }
+    if (bits && v >= capacity) { set_error("pdf_vectors_read: more vectors than capacity"); return PDF_ERR_ARG; }
+    ++v;
+  }
+  *count = v;
+  return PDF_OK;
+}
+
+int pdf_vectors_count(const char *path, uint32_t num_pi, uint64_t *count) {
+  if (!path || !count) { set_error("null argument"); return PDF_ERR_ARG; }
+  return scan_vectors(path, num_pi, nullptr, 0, count);
+}
+
+int pdf_vectors_read(const char *path, uint32_t num_pi, uint64_t *bits, uint64_t capacity, uint64_t *count) {
+  if (!path || !bits || !count) { set_error("null argument"); return PDF_ERR_ARG; }
+  std::fill(bits, bits + (size_t)num_pi * ((capacity + 63) / 64), 0ULL);
+  return scan_vectors(path, num_pi, bits, capacity, count);
+}
+
+int pdf_vectors_write(const char *path, uint32_t num_pi, const uint64_t *bits, uint64_t count) {
+  if (!path || (!bits && count)) { set_error("null argument"); return PDF_ERR_ARG; }
+  std::ofstream out(path);
+  if (!out) { set_error(std::string("cannot write ") + path); return PDF_ERR_IO; }
+  const uint64_t words = (count + 63) / 64;
+  std::string row(num_pi, '0');
+  out << "# " << count << " vectors x " << num_pi << " inputs\n";
+  for (uint64_t v = 0; v < count; ++v) {
+    for (uint32_t i = 0; i < num_pi; ++i) row[i] = ((bits[(uint64_t)i * words + v / 64] >> (v % 64)) & 1) ? '1' : '0';
+    out << row << '\n';
+  }
+  if (!out) { set_error(std::string("write failed: ") + path); return PDF_ERR_IO; }
+  return PDF_OK;
+}
+
 // ------------------------------------------------------------- file formats
 
 static std::string trim(const std::string &s) {

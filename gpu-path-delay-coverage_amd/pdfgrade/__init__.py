@@ -89,6 +89,9 @@ SIGNATURES = {
     "pdf_netlist_export": ([_P] + [C.c_void_p] * 8, C.c_int),
     "pdf_netlist_generate": ([C.POINTER(_GenParams), C.POINTER(_P)], C.c_int),
     "pdf_gen_vectors": ([_U64, C.c_uint32, _U64, C.c_void_p], C.c_int),
+    "pdf_vectors_count": ([C.c_char_p, C.c_uint32, _U64P], C.c_int),
+    "pdf_vectors_read": ([C.c_char_p, C.c_uint32, C.c_void_p, _U64, _U64P], C.c_int),
+    "pdf_vectors_write": ([C.c_char_p, C.c_uint32, C.c_void_p, _U64], C.c_int),
     "pdf_gen_toggle_pairs": ([_U64, C.c_uint32, _U64, C.c_double, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grader_create": ([C.c_int, _P, C.POINTER(_P)], C.c_int),
     "pdf_grader_destroy": ([_P], None),
@@ -273,6 +276,22 @@ def gen_vectors(seed: int, num_pi: int, num_vectors: int) -> np.ndarray:
     out = np.zeros((num_pi, words), dtype=np.uint64)
     _check(lib().pdf_gen_vectors(seed, num_pi, num_vectors, _ptr(out)))
     return out
+
+
+def read_vectors(path: str, num_pi: int):
+    """(PI-major packed bits, count) from a vector text file."""
+    n = C.c_uint64()
+    _check(lib().pdf_vectors_count(path.encode(), num_pi, C.byref(n)))
+    bits = np.zeros((num_pi, max(1, (n.value + 63) // 64)), np.uint64)
+    got = C.c_uint64()
+    _check(lib().pdf_vectors_read(path.encode(), num_pi, _ptr(bits), n.value, C.byref(got)))
+    return bits, got.value
+
+
+def write_vectors(path: str, bits: np.ndarray, num_vectors: int):
+    """Write PI-major packed vectors as a vector text file."""
+    bits = np.ascontiguousarray(bits, dtype=np.uint64)
+    _check(lib().pdf_vectors_write(path.encode(), bits.shape[0], _ptr(bits), num_vectors))
 
 
 def gen_toggle_pairs(seed: int, num_pi: int, count: int, q_lo: float, q_hi: float):

@@ -147,6 +147,16 @@ int pdf_gen_vectors(uint64_t seed, uint32_t num_pi, uint64_t num_vectors,
 int pdf_gen_toggle_pairs(uint64_t seed, uint32_t num_pi, uint64_t count, double q_lo, double q_hi,
                          uint64_t *a_bits, uint64_t *b_bits);
 
+/* Vector text files (the reference's reader, main.cu:146-156, lives in its
+ * absent util/ submodule, so the format is this library's): one vector per
+ * line, one '0'/'1' per PI in INPUT order; whitespace is ignored and '#'
+ * starts a comment.  pdf_vectors_count gives the number of vectors;
+ * pdf_vectors_read fills PI-major packed bits (num_pi x ceil(capacity/64)
+ * words) with at most `capacity` vectors. */
+int pdf_vectors_count(const char *path, uint32_t num_pi, uint64_t *count);
+int pdf_vectors_read(const char *path, uint32_t num_pi, uint64_t *bits, uint64_t capacity, uint64_t *count);
+int pdf_vectors_write(const char *path, uint32_t num_pi, const uint64_t *bits, uint64_t count);
+
 /* ------------------------------------------------------------------------ */
 /* Grader (GPU)                                                              */
 /* ------------------------------------------------------------------------ */

@@ -1,5 +1,6 @@
 """Bit-exact parity of the HIP grading path (through the C ABI) against the CPU
 oracle, on the same seeded inputs.  Integer work: every output must be equal."""
+import os
 import sys
 
 import numpy as np
@@ -314,3 +315,27 @@ def test_2m_gates_properties():
             g.mark(c, c * 64, True)
             marks, _ = orc.mark_chunk(net, codes, True)
             assert np.array_equal(pg.plane_to_bytes(g.mark_plane(64)), marks), c
+
+
+def test_fcount_cli_matches_oracle(tmp_path):
+    """The fcount CLI (main.cu's flags and result line) on c17 and a vector file."""
+    import subprocess
+    from helpers import C17_BENCH
+    bench = tmp_path / "c17.bench"
+    bench.write_text(C17_BENCH)
+    nl = pg.Netlist.read_bench(str(bench))
+    V = 256
+    bits = pg.gen_vectors(17, 5, V)
+    vec = tmp_path / "c17.vec"
+    pg.write_vectors(str(vec), bits, V)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpu-path-delay-coverage_amd",
+                       "fcount")
+    for robust in (True, False):
+        args = [exe, "--bench", str(bench), str(vec)] + (["--robust"] if robust else [])
+        print("fcount", robust, flush=True)
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        line = [l for l in r.stdout.splitlines() if l.startswith(str(vec))][-1]
+        total = int(line.rsplit(";", 1)[1])
+        want = orc.grade(orc.OracleNet(nl.arrays()), bits, V, V, robust)["total"]
+        assert total == want, (line, want)

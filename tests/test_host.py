@@ -188,3 +188,39 @@ def test_gen_toggle_pairs_statistics():
     np.testing.assert_array_equal(a1 ^ b1, want)
     with pytest.raises(pg.PdfError):
         pg.gen_toggle_pairs(1, 8, 64, 0.0, 0.5)
+
+
+def test_vector_file_round_trip(tmp_path):
+    I, V = 7, 150
+    bits = pg.gen_vectors(11, I, V)
+    path = str(tmp_path / "v.vec")
+    pg.write_vectors(path, bits, V)
+    got, n = pg.read_vectors(path, I)
+    assert n == V
+    np.testing.assert_array_equal(got, bits)
+    # comments, blank lines and spaces are ignored
+    text = "# header\n\n0 1 0 1 0 1 1  # first\n1111111\n"
+    (tmp_path / "w.vec").write_text(text)
+    got, n = pg.read_vectors(str(tmp_path / "w.vec"), I)
+    assert n == 2
+    assert [int(got[i][0] & 1) for i in range(I)] == [0, 1, 0, 1, 0, 1, 1]
+    assert all(int(got[i][0] >> 1) & 1 for i in range(I))
+
+
+def test_vector_file_errors(tmp_path):
+    (tmp_path / "bad.vec").write_text("01x1\n")
+    with pytest.raises(pg.PdfError):
+        pg.read_vectors(str(tmp_path / "bad.vec"), 4)
+    (tmp_path / "wide.vec").write_text("0101\n011\n")
+    with pytest.raises(pg.PdfError):
+        pg.read_vectors(str(tmp_path / "wide.vec"), 4)
+    with pytest.raises(pg.PdfError):
+        pg.read_vectors(str(tmp_path / "missing.vec"), 4)
+    # more vectors than the caller's capacity is an error, with no write past it
+    import ctypes as C
+    (tmp_path / "many.vec").write_text("1111\n" * 70)
+    buf = np.zeros((4, 2), np.uint64)
+    got = C.c_uint64()
+    rc = pg.lib().pdf_vectors_read(str(tmp_path / "many.vec").encode(), 4, buf[:, :1].copy().ctypes.data, 64,
+                                   C.byref(got))
+    assert rc == pg.ERR_ARG

@@ -178,8 +178,8 @@ def learned_vs_plain(robust: bool):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build grades on one GPU; --gpus must be 1 (multi-GPU sharding is a "
-                                   "design only, see DESIGN.md)"}))
+        print(json.dumps({"error": "grading runs on one GPU; --gpus must be 1 (sharding over GPUs is designed, "
+                                   "not built)"}))
         sys.exit(2)
     import pdfgrade as pg
     from pdfgrade.configs import BATCH_WORKLOADS, WORKLOADS

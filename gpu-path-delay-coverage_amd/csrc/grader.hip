@@ -406,12 +406,12 @@ __device__ __forceinline__ void rev_body(const DevNet &net, const GateRec *__res
   uint32_t g, w;
   if (!map_item(mp, blk, g, w)) return;
   if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
-  // round 1: this node's mark-present word and record
+  // first memory trip: this node's mark-present word and record
   const uint64_t nzw = pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)];
   const GateRec r = recs[g];
   if (!((nzw >> (w & 63)) & 1)) return;
   const uint64_t row = (uint64_t)g * pl.pitch + w;
-  // round 2: mark, accumulator, the fanins' mark-present words and, for the
+  // second trip: mark, accumulator, the fanins' mark-present words and, for the
   // merge, both frames
   uint64_t m = pl.M[row];
   const uint64_t acc = DO_REMOVE ? pl.ACC[row] : 0;
@@ -431,7 +431,7 @@ __device__ __forceinline__ void rev_body(const DevNet &net, const GateRec *__res
       pl.M[row] = m;
     }
     if (m) {
-      // round 3: marks of the fanins present in this word, then the pushes
+      // third trip: marks of the fanins present in this word, then the pushes
       auto push = [&](uint32_t f, uint64_t nz) {
         if (!((nz >> (w & 63)) & 1)) return;
         const uint64_t mf = pl.M[(uint64_t)f * pl.pitch + w] & m;

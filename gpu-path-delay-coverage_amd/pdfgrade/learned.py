@@ -3,7 +3,7 @@ library's MFMA kernels, ranking of candidate pairs and exact evaluation.
 
 No reference counterpart: this sits on top of the grading path.  The grader
 produces every label (each pair's exact solo gain, pdf_pair_gains) and
-judges every order (exact in-order grading, pdf_grade_curve); the network
+scores every order (exact in-order grading, pdf_grade_curve); the network
 only proposes the order.  Unranked in-order grading stays the fallback.
 
 Candidate pairs are explicit (A, B): A uniform, B = A with each PI flipped

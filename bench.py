@@ -17,8 +17,10 @@ another with per-circuit launches.
 
 The roofline object is for the dominant kernel (the fused sim + forward-mark
 level kernel), timed live with HIP events around each of its launches on the
-grader's stream.  The CPU baseline is the serial C oracle on a bounded sample
-of the same workload, on this host.
+grader's stream.  The CPU baseline is the bit-sliced C grader
+(oracle/bitsliced.c, 64 pairs per word, OpenMP over the host's threads) over
+the whole workload on this host, with its one-core rate and the serial
+byte-per-pattern oracle's rate on a bounded sample beside it.
 """
 from __future__ import annotations
 
@@ -40,7 +42,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="single-50k-gates")
-    ap.add_argument("--cpu-sample", type=int, default=4096, help="pairs graded by the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=4096,
+                    help="pairs graded by the serial oracle beside the CPU baseline (0 = skip every CPU leg)")
     ap.add_argument("--robust", type=int, default=1)
     ap.add_argument("--graph", type=int, default=None, help="batch workloads: force graph replay on (1) or off (0)")
     ap.add_argument("--learned", type=int, default=1, help="50k workload: run the learned-ranking experiment")
@@ -64,13 +67,18 @@ def measured_traffic(workload: str, family: str):
         return None
 
 
-def cpu_baseline(nl, bits, pairs, sample, robust):
-    """Serial C oracle (one core) on the first `sample` pairs of the workload."""
+def host_threads() -> int:
+    cores = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(cores, env) if env > 0 else cores)
+
+
+def serial_oracle_rate(net, bits, pairs, sample, robust):
+    """Serial byte-per-pattern C oracle (one core) on the first `sample` pairs."""
     import numpy as np
 
     from oracle import oracle as orc
 
-    net = orc.OracleNet(nl.arrays())
     f0 = np.full(net.n, 0xFFFFFFFF, np.uint32)
     f1 = f0.copy()
     gc = np.zeros(net.n, np.uint64)
@@ -79,8 +87,40 @@ def cpu_baseline(nl, bits, pairs, sample, robust):
     marks, _ = orc.mark_chunk(net, codes, robust)
     orc.merge_chunk(net, codes, marks, 0, f0, f1)
     orc.cover_chunk(net, marks, 0, f0, f1, gc)
-    dt = time.perf_counter() - t0
-    return sample / dt
+    return sample / (time.perf_counter() - t0)
+
+
+def cpu_baseline(nl, bits, wl, sample, robust, gpu):
+    """The bit-sliced C grader over the whole workload (same pairs, chunk and
+    mode as the GPU steps) on every host thread, then on one; the serial
+    oracle on `sample` pairs beside it.  `gpu` holds the GPU run's total,
+    first-detection arrays and per-node counts, compared when the CPU leg
+    graded the whole workload."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    net = orc.OracleNet(nl.arrays())
+    threads = host_threads()
+    # whole workload up to 2^32 node-pairs (about 1 s on 16 cores), else a prefix
+    pairs = min(wl.pairs, max(64, (2**32 // net.n) // 64 * 64))
+    chunk = min(wl.chunk, pairs)
+    whole = pairs == wl.pairs
+    what = f"all {pairs}" if whole else f"first {pairs} (graded as a run of their own) of {wl.pairs}"
+    rate = {}
+    for t in sorted({threads, 1}, reverse=True):
+        t0 = time.perf_counter()
+        r = orc.grade_bitsliced(net, bits, pairs, chunk, robust, threads=t)
+        rate[t] = pairs / (time.perf_counter() - t0)
+    return {"value": rate[threads], "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": (f"{what} pairs of the {wl.name} netlist, chunk {chunk}, bit-sliced C grader "
+                       f"(oracle/bitsliced.c), {threads} OpenMP threads"),
+            "one_core": rate[1], "total_paths": r["total"],
+            "matches_gpu": (r["total"] == gpu["total"] and np.array_equal(r["first_t0"], gpu["first"][0])
+                            and np.array_equal(r["first_t1"], gpu["first"][1])
+                            and np.array_equal(r["gate_counts"], gpu["gate_counts"])) if whole else None,
+            "serial_oracle": {"value": serial_oracle_rate(net, bits, wl.pairs, sample, robust), "unit": "pairs/s",
+                              "cores": 1, "sample": f"first {sample} pairs, byte-per-pattern oracle.c, one chunk"}}
 
 
 def bench_batch(args, wl):
@@ -235,10 +275,8 @@ def main():
 
     cpu = None
     if args.cpu_sample > 0:
-        sample = min(args.cpu_sample, wl.pairs)
-        cpu = {"value": cpu_baseline(nl, bits, wl.pairs, sample, robust), "unit": "pairs/s", "cores": 1,
-               "kind": "port",
-               "sample": f"first {sample} of {wl.pairs} pairs of the {wl.name} netlist, one chunk, serial C oracle"}
+        gpu = {"total": total, "first": g.first(), "gate_counts": g.gate_counts()}
+        cpu = cpu_baseline(nl, bits, wl, min(args.cpu_sample, wl.pairs), robust, gpu)
 
     learned_part = None
     if args.learned and wl.name == "single-50k-gates":

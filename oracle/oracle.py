@@ -12,6 +12,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+BSL_PATH = os.path.join(HERE, "libbitsliced.so")
 
 FLAG_EXCEEDS_U16 = 1
 FLAG_EXCEEDS_U32 = 2
@@ -42,6 +43,18 @@ def lib():
         _lib.orc_mark_lut.restype = C.c_uint16
         _lib.orc_mark_lut.argtypes = [C.c_int, C.c_int]
     return _lib
+
+
+_bsl = None
+
+
+def bsl_lib():
+    global _bsl
+    if _bsl is None:
+        if not os.path.exists(BSL_PATH):
+            raise ImportError(f"{BSL_PATH} is not built; run oracle/build.py (or __graft_entry__.build())")
+        _bsl = C.CDLL(BSL_PATH)
+    return _bsl
 
 
 def _p(a):
@@ -132,3 +145,23 @@ def cover_chunk(net: OracleNet, marks, start: int, first_t0, first_t1, gate_coun
     lib().orc_cover_chunk(net.ref, _p(marks), C.c_uint64(marks.shape[1]), C.c_uint64(start), _p(first_t0),
                           _p(first_t1), _p(gate_counts), C.byref(tot), C.byref(flags))
     return tot.value, flags.value
+
+
+def grade_bitsliced(net: OracleNet, a_bits: np.ndarray, num_pairs: int, chunk: int, robust: bool,
+                    b_bits: np.ndarray | None = None, threads: int = 1) -> dict:
+    """The bit-sliced grader of bitsliced.c: same results as grade() with the
+    whole-chunk merge (chunk a multiple of 64, or the whole run); `flags`
+    carries only FLAG_U64_OVERFLOW."""
+    a_bits = np.ascontiguousarray(a_bits, dtype=np.uint64)
+    if b_bits is not None:
+        b_bits = np.ascontiguousarray(b_bits, dtype=np.uint64)
+    f0 = np.full(net.n, 0xFFFFFFFF, np.uint32)
+    f1 = np.full(net.n, 0xFFFFFFFF, np.uint32)
+    gc = np.zeros(net.n, np.uint64)
+    tot = C.c_uint64()
+    flags = C.c_uint32()
+    rc = bsl_lib().bsl_grade(net.ref, _p(a_bits), _p(b_bits), C.c_uint64(num_pairs), C.c_uint64(chunk),
+                             int(robust), int(threads), _p(f0), _p(f1), _p(gc), C.byref(tot), C.byref(flags))
+    if rc:
+        raise RuntimeError(f"bsl_grade failed ({rc})")
+    return dict(first_t0=f0, first_t1=f1, gate_counts=gc, total=tot.value, flags=flags.value)

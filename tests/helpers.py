@@ -78,6 +78,20 @@ def oracle_grade(nl: pg.Netlist, a_bits, num_pairs, chunk, robust, b_bits=None, 
                      merge_window=merge_window)
 
 
+def host_threads() -> int:
+    """Threads for the bit-sliced CPU grader: the cores this process may use,
+    capped by OMP_NUM_THREADS when that is set."""
+    import os
+    cores = len(os.sched_getaffinity(0))
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(cores, env) if env > 0 else cores)
+
+
+def bitsliced_grade(nl: pg.Netlist, a_bits, num_pairs, chunk, robust, b_bits=None):
+    return orc.grade_bitsliced(orc.OracleNet(nl.arrays()), a_bits, num_pairs, chunk, robust, b_bits=b_bits,
+                               threads=host_threads())
+
+
 def edge_netlist():
     """Every gate type with 1 and >= 2 fanins, a 12-input AND/OR (more fanins
     than the mark pass keeps in registers), POs with fanouts, an unused PI."""

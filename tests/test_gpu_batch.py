@@ -1,5 +1,6 @@
 """Batched grading: one captured graph over many circuits equals grading each
-circuit alone (the oracle for small circuits, pdf_grade for larger ones)."""
+circuit alone (the oracle for small circuits, pdf_grade for larger ones, and
+the bit-sliced CPU grader for every circuit of the 256-circuit workload)."""
 from __future__ import annotations
 
 import numpy as np
@@ -7,7 +8,7 @@ import pytest
 
 import pdfgrade as pg
 from oracle import oracle as orc
-from helpers import small_config
+from helpers import bitsliced_grade, small_config
 
 pytestmark = pytest.mark.gpu
 
@@ -108,4 +109,30 @@ def test_batch_non_robust_mixed_widths():
     np.testing.assert_array_equal(tot, np.array([w[0] for w in want], np.uint64))
     for i, w in enumerate(want):
         np.testing.assert_array_equal(b.member(i).gate_counts(), w[3])
+    b.close()
+
+
+def test_batch_256_circuits_against_bitsliced():
+    """The whole 256-circuit workload (1k-200k gates, 4k-64k pairs each), one
+    batched grade, every circuit's total, first-detection arrays and per-node
+    counts bit-exact against the bit-sliced CPU grader."""
+    from pdfgrade.configs import BATCH_256
+    _say("batch 256 vs bit-sliced")
+    members = BATCH_256.members()
+    nets = [pg.Netlist.generate(c) for c, _, _ in members]
+    vecs = [pg.gen_vectors(v, c.num_pi, p) for c, v, p in members]
+    pairs = [p for _, _, p in members]
+    b = pg.Batch(nets, vecs, pairs)
+    tot, _ = b.grade(True)
+    for i, (nl, v, p) in enumerate(zip(nets, vecs, pairs)):
+        ref = bitsliced_grade(nl, v, p, 16384, True)
+        m = b.member(i)
+        f0, f1 = m.first()
+        assert int(tot[i]) == ref["total"], i
+        np.testing.assert_array_equal(f0, ref["first_t0"], err_msg=str(i))
+        np.testing.assert_array_equal(f1, ref["first_t1"], err_msg=str(i))
+        np.testing.assert_array_equal(m.gate_counts(), ref["gate_counts"], err_msg=str(i))
+        if i % 32 == 31:
+            _say("checked", i + 1)
+    assert int(tot.sum()) > 0
     b.close()

@@ -10,7 +10,8 @@ import pdfgrade as pg
 from pdfgrade.configs import GATES_50K, TINY
 from oracle import oracle as orc
 
-from helpers import biased_vectors, edge_netlist, oracle_grade, pack, small_config, unpack
+from helpers import (biased_vectors, bitsliced_grade, edge_netlist, oracle_grade, pack, small_config,
+                     unpack)
 
 pytestmark = pytest.mark.gpu
 
@@ -339,3 +340,38 @@ def test_fcount_cli_matches_oracle(tmp_path):
         total = int(line.rsplit(";", 1)[1])
         want = orc.grade(orc.OracleNet(nl.arrays()), bits, V, V, robust)["total"]
         assert total == want, (line, want)
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_50k_gates_full_size_against_bitsliced(robust):
+    """All 65,536 pairs of the 50k-gate workload with its recorded chunk and
+    lane tuning, bit-exact against the bit-sliced CPU grader (an independent
+    second CPU implementation, itself checked against the oracle)."""
+    say("50k full vs bit-sliced", robust)
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = GATES_50K.pairs
+    bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
+    ref = bitsliced_grade(nl, bits, P, GATES_50K.chunk, robust)
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_tuning(pg.TUNE_LANES, GATES_50K.lanes)
+        g.set_chunk(GATES_50K.chunk)
+        for rep in range(2):
+            got = dict(total=g.grade(robust), gate_counts=g.gate_counts())
+            got["first_t0"], got["first_t1"] = g.first()
+            assert_same(got, ref, f"rep {rep}")
+    if not robust:
+        assert ref["total"] > 0
+
+
+def test_2m_gates_prefix_against_bitsliced():
+    """The 2,000,000-gate / 200-level netlist on 4,096 pairs (as a vector set
+    of its own), bit-exact against the bit-sliced CPU grader, both modes."""
+    from pdfgrade.configs import GATES_2M
+    say("2m gates vs bit-sliced")
+    nl = pg.Netlist.generate(GATES_2M.netlist)
+    P = 4096
+    bits = pg.gen_vectors(GATES_2M.vector_seed, nl.info()["num_pi"], P)
+    for robust in (True, False):
+        ref = bitsliced_grade(nl, bits, P, 2048, robust)
+        assert_same(gpu_grade(nl, bits, P, 2048, robust), ref, f"robust {robust}")

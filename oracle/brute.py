@@ -19,6 +19,12 @@ instead of the reverse DP.  The reference behaviours it restates:
             falling (T0) / rising (T1) transition.
   coverage  coverkernel.cuh:171-230 -- c(g,p) counts marked paths g -> PO that
             contain a node first detected at p; total = sum over PIs.
+  segments  K >= 2 (mergekernel.cuh:91-110, coverkernel.cuh:106-154): a
+            segment (walk of K nodes along fanouts, shorter only where it
+            reaches a node without fanouts) is detected at p when all its
+            nodes are marked; its first T0 / T1 pattern follows node 0's
+            transition; a node is new at p when some segment holding it was
+            first detected at p.
 """
 from __future__ import annotations
 
@@ -116,9 +122,30 @@ def suffix_paths(fanouts, po):
     return [paths(g) for g in range(len(po))]
 
 
-def grade(types, fanins, po, num_pi, vectors_a, vectors_b, robust=True):
+def segment_walks(fanouts, K):
+    """Every walk of K nodes along fanouts from each node in id order, fanouts
+    in list order, depth first; a walk ending early at a node without fanouts
+    is padded with None."""
+    out = []
+
+    def walk(path):
+        g = path[-1]
+        if len(path) == K or not fanouts[g]:
+            out.append(tuple(path) + (None,) * (K - len(path)))
+            return
+        for f in fanouts[g]:
+            walk(path + [f])
+
+    for g in range(len(fanouts)):
+        walk([g])
+    return out
+
+
+def grade(types, fanins, po, num_pi, vectors_a, vectors_b, robust=True, segs=1):
     """vectors_a/b: list of per-pattern PI value lists (pair p = (a[p], b[p])).
-    Returns dict(first_t0, first_t1, gate_counts, total, per_pattern)."""
+    segs: segment length K.  Returns dict(first_t0, first_t1, gate_counts,
+    total, per_pattern) and, for K >= 2, seg_t0 / seg_t1 per segment of
+    segment_walks."""
     n = len(types)
     fanouts = [[] for _ in range(n)]
     for g in range(n):
@@ -140,12 +167,33 @@ def grade(types, fanins, po, num_pi, vectors_a, vectors_b, robust=True):
                     first0[g] = p
                 if a == 0 and b == 1 and first1[g] is None:
                     first1[g] = p
+    walks = segment_walks(fanouts, segs) if segs > 1 else None
+    if walks is not None:
+        sfirst0 = [None] * len(walks)
+        sfirst1 = [None] * len(walks)
+        for p in range(P):
+            m = m_all[p]
+            for s, wk in enumerate(walks):
+                if all(m[v] for v in wk if v is not None):
+                    a, b = fr_all[p][wk[0]]
+                    if b == 0 and sfirst0[s] is None:
+                        sfirst0[s] = p
+                    if b == 1 and sfirst1[s] is None:
+                        sfirst1[s] = p
     sp = suffix_paths(fanouts, po)
     gate_counts = [0] * n
     per_pattern = []
     for p in range(P):
         m = m_all[p]
-        new = [first0[g] == p or first1[g] == p for g in range(n)]
+        if walks is None:
+            new = [first0[g] == p or first1[g] == p for g in range(n)]
+        else:
+            new = [False] * n
+            for s, wk in enumerate(walks):
+                if sfirst0[s] == p or sfirst1[s] == p:
+                    for v in wk:
+                        if v is not None:
+                            new[v] = True
         tot = 0
         for g in range(n):
             if not m[g]:
@@ -156,10 +204,15 @@ def grade(types, fanins, po, num_pi, vectors_a, vectors_b, robust=True):
                 tot += cnt
         per_pattern.append(tot)
     none = 0xFFFFFFFF
-    return dict(first_t0=[none if x is None else x for x in first0],
-                first_t1=[none if x is None else x for x in first1],
-                gate_counts=gate_counts, total=sum(per_pattern), per_pattern=per_pattern,
-                frames=fr_all, marks=m_all)
+    out = dict(first_t0=[none if x is None else x for x in first0],
+               first_t1=[none if x is None else x for x in first1],
+               gate_counts=gate_counts, total=sum(per_pattern), per_pattern=per_pattern,
+               frames=fr_all, marks=m_all)
+    if walks is not None:
+        out["segments"] = [[none if v is None else v for v in wk] for wk in walks]
+        out["seg_t0"] = [none if x is None else x for x in sfirst0]
+        out["seg_t1"] = [none if x is None else x for x in sfirst1]
+    return out
 
 
 def unpack_sequence(bits, num_pi, P):

@@ -13,6 +13,9 @@
  *   merge  mergekernel.cuh:116-198 (kernSegmentReduce2, K=1 segments)
  *   cover  coverkernel.cuh:171-230 (kernCover), 77-105 (kernSumSingle),
  *          233-305 (gpuCountPaths level loop)
+ *   K >= 2 mergekernel.cuh:91-110 (has_marked_segment), coverkernel.cuh:106-154
+ *          (findActiveThreads<K>); the segment list itself is this project's
+ *          enumeration (orc_segments), the reference's lives in util/
  *   loop   main.cu:171-231 (chunk loop, totals += coverage)
  *
  * Deliberate differences (resource-limit artefacts of the reference):
@@ -197,16 +200,16 @@ int orc_merge_chunk(const orc_netlist *nl, const uint8_t *codes, const uint8_t *
   return 0;
 }
 
-/* kernCover<1> + kernSumSingle over one chunk, exact uint64. */
-int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uint64_t start,
-                    const uint32_t *first_t0, const uint32_t *first_t1, uint64_t *gate_counts,
-                    uint64_t *chunk_total, uint32_t *flags) {
+/* kernCover<K> + kernSumSingle over one chunk, exact uint64; newm[g*W + p]
+ * says node g counts as new at pattern start+p (coverkernel.cuh:214-220:
+ * some segment holding g has start+p as its first T0 or T1 pattern). */
+int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8_t *newm, uint64_t W,
+                        uint64_t *gate_counts, uint64_t *chunk_total, uint32_t *flags) {
   uint64_t *c = (uint64_t *)malloc(sizeof(uint64_t) * nl->n);
   uint64_t *h = (uint64_t *)malloc(sizeof(uint64_t) * nl->n);
   if (!c || !h) { free(c); free(h); return -5; }
   uint64_t total = 0;
   for (uint64_t p = 0; p < W; ++p) {
-    const uint32_t pid = (uint32_t)(start + p);
     for (uint32_t gi = nl->n; gi-- > 0;) {
       const uint8_t cache = marks[(uint64_t)gi * W + p];
       uint64_t cc = 0, hh = 0;
@@ -220,7 +223,7 @@ int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uin
       if (rc > 0xFFFF || hc > 0xFFFF) *flags |= ORC_FLAG_EXCEEDS_U16;
       cc += rc;
       if (__builtin_add_overflow(hh, hc, &hh)) *flags |= ORC_FLAG_U64_OVERFLOW;
-      const int is_new = (first_t0[gi] == pid) || (first_t1[gi] == pid);
+      const int is_new = newm[(uint64_t)gi * W + p] != 0;
       /* coverkernel.cuh:214-220 */
       const uint64_t tmp = (cache > 0 && is_new) ? hh : 0;
       if (__builtin_add_overflow((cache > 0) ? cc : 0, tmp, &cc)) *flags |= ORC_FLAG_U64_OVERFLOW;
@@ -242,6 +245,139 @@ int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uin
   free(c);
   free(h);
   return 0;
+}
+
+/* K = 1 (findActiveThreads<1>, coverkernel.cuh:156-169): node g is new at p
+ * when first_T0[g] or first_T1[g] is p. */
+int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uint64_t start,
+                    const uint32_t *first_t0, const uint32_t *first_t1, uint64_t *gate_counts,
+                    uint64_t *chunk_total, uint32_t *flags) {
+  uint8_t *newm = (uint8_t *)calloc((size_t)nl->n * W, 1);
+  if (!newm) return -5;
+  for (uint32_t g = 0; g < nl->n; ++g) {
+    const uint32_t x[2] = {first_t0[g], first_t1[g]};
+    for (int k = 0; k < 2; ++k)
+      if (x[k] != 0xFFFFFFFFu && x[k] >= start && x[k] - start < W) newm[(uint64_t)g * W + (x[k] - start)] = 1;
+  }
+  const int rc = orc_cover_new_chunk(nl, marks, newm, W, gate_counts, chunk_total, flags);
+  free(newm);
+  return rc;
+}
+
+/* Segments of length K, 1 <= K <= 8: the reference's segment<K> list
+ * (mergekernel.cuh:205-221) comes from generateSegmentList in its absent
+ * util/ library, so this enumeration is the project's definition: for every
+ * node g0 in id order, every fanout walk g0 -> g1 -> ... of K nodes, fanouts
+ * taken in CSR order, depth first; a walk that reaches a node without
+ * fanouts before K nodes ends there and the rest of its entries are
+ * 0xFFFFFFFF (the reference skips negative ids, mergekernel.cuh:100-109).
+ * K = 1 gives one segment per node, in id order.  Writes min(count, cap)
+ * segments of K entries to out (may be NULL) and the total to *count. */
+static void seg_walk(const orc_netlist *nl, uint32_t K, uint32_t *path, uint32_t d, uint32_t *out, uint64_t cap,
+                     uint64_t *cnt) {
+  const uint32_t g = path[d - 1];
+  const uint32_t lo = nl->fout_off[g], hi = nl->fout_off[g + 1];
+  if (d == K || lo == hi) {
+    if (out && *cnt < cap)
+      for (uint32_t i = 0; i < K; ++i) out[*cnt * K + i] = i < d ? path[i] : 0xFFFFFFFFu;
+    ++*cnt;
+    return;
+  }
+  for (uint32_t k = lo; k < hi; ++k) {
+    path[d] = nl->fout[k];
+    seg_walk(nl, K, path, d + 1, out, cap, cnt);
+  }
+}
+
+int orc_segments(const orc_netlist *nl, uint32_t K, uint32_t *out, uint64_t cap, uint64_t *count) {
+  if (!nl || !count || K < 1 || K > 8) return -1;
+  uint32_t path[8];
+  uint64_t cnt = 0;
+  for (uint32_t g = 0; g < nl->n; ++g) {
+    path[0] = g;
+    seg_walk(nl, K, path, 1, out, cap, &cnt);
+  }
+  *count = cnt;
+  return 0;
+}
+
+/* kernSegmentReduce2<K> (mergekernel.cuh:116-198): a segment is marked at a
+ * pattern when every node of it is (has_marked_segment, 91-110); its earliest
+ * marked pattern with node 0 falling (T0) and rising (T1), min-combined with
+ * the previous chunks'. */
+int orc_merge_segs_chunk(const orc_netlist *nl, const uint8_t *codes, const uint8_t *marks, uint64_t W,
+                         uint64_t start, uint32_t merge_window, uint32_t K, const uint32_t *segs, uint64_t S,
+                         uint32_t *seg_t0, uint32_t *seg_t1) {
+  (void)nl;
+  const uint64_t lim = merge_window ? (W < merge_window ? W : merge_window) : W;
+  for (uint64_t s = 0; s < S; ++s) {
+    const uint32_t *sg = segs + s * K;
+    uint32_t bx = 0xFFFFFFFFu, by = 0xFFFFFFFFu;
+    for (uint64_t p = 0; p < lim && (bx == 0xFFFFFFFFu || by == 0xFFFFFFFFu); ++p) {
+      int all = 1;
+      for (uint32_t i = 0; i < K && all; ++i)
+        if (sg[i] != 0xFFFFFFFFu) all = marks[(uint64_t)sg[i] * W + p] != 0;
+      if (!all) continue;
+      const uint32_t pid = (uint32_t)(start + p);
+      if (codes[(uint64_t)sg[0] * W + p] & 1u) { if (by == 0xFFFFFFFFu) by = pid; }
+      else if (bx == 0xFFFFFFFFu) bx = pid;
+    }
+    if (bx < seg_t0[s]) seg_t0[s] = bx;
+    if (by < seg_t1[s]) seg_t1[s] = by;
+  }
+  return 0;
+}
+
+/* findActiveThreads<K> (coverkernel.cuh:127-154): node g is new at pattern p
+ * of the chunk when a segment holding g has p as its first T0 or T1. */
+int orc_new_from_segments(const orc_netlist *nl, uint64_t W, uint64_t start, uint32_t K, const uint32_t *segs,
+                          uint64_t S, const uint32_t *seg_t0, const uint32_t *seg_t1, uint8_t *newm) {
+  memset(newm, 0, (size_t)nl->n * W);
+  for (uint64_t s = 0; s < S; ++s) {
+    const uint32_t x[2] = {seg_t0[s], seg_t1[s]};
+    for (int k = 0; k < 2; ++k) {
+      if (x[k] == 0xFFFFFFFFu || x[k] < start || x[k] - start >= W) continue;
+      for (uint32_t i = 0; i < K; ++i)
+        if (segs[s * K + i] != 0xFFFFFFFFu) newm[(uint64_t)segs[s * K + i] * W + (x[k] - start)] = 1;
+    }
+  }
+  return 0;
+}
+
+/* The chunk loop with segments of length K (main.cu:185-223): sim, mark,
+ * segment merge, cover.  seg_t0/seg_t1 [S] and gate_counts [n] are
+ * accumulated (the caller initializes them). */
+int orc_grade_segs(const orc_netlist *nl, const uint64_t *a_bits, const uint64_t *b_bits, uint64_t num_pairs,
+                   uint64_t chunk, const orc_options *opt, uint32_t K, const uint32_t *segs, uint64_t S,
+                   uint32_t *seg_t0, uint32_t *seg_t1, uint64_t *gate_counts, uint64_t *chunk_totals,
+                   uint64_t *total, uint32_t *flags) {
+  if (!nl || !a_bits || !segs || !seg_t0 || !seg_t1 || !total || !flags || num_pairs == 0 || K < 1 || K > 8)
+    return -1;
+  if (chunk == 0 || chunk > num_pairs) chunk = num_pairs;
+  uint8_t *codes = (uint8_t *)malloc((size_t)nl->n * chunk);
+  uint8_t *marks = (uint8_t *)malloc((size_t)nl->n * chunk);
+  uint8_t *newm = (uint8_t *)malloc((size_t)nl->n * chunk);
+  int rc = 0;
+  if (!codes || !marks || !newm) rc = -5;
+  *total = 0;
+  *flags = 0;
+  uint64_t idx = 0;
+  for (uint64_t start = 0; !rc && start < num_pairs; start += chunk, ++idx) {
+    const uint64_t W = (num_pairs - start) < chunk ? (num_pairs - start) : chunk;
+    uint64_t ct = 0;
+    orc_sim_chunk(nl, a_bits, b_bits, num_pairs, start, W, codes);
+    orc_mark_chunk(nl, codes, W, opt ? opt->robust : 0, marks, NULL);
+    orc_merge_segs_chunk(nl, codes, marks, W, start, opt ? opt->merge_window : 0, K, segs, S, seg_t0, seg_t1);
+    orc_new_from_segments(nl, W, start, K, segs, S, seg_t0, seg_t1, newm);
+    rc = orc_cover_new_chunk(nl, marks, newm, W, gate_counts, &ct, flags);
+    if (rc) break;
+    if (chunk_totals) chunk_totals[idx] = ct;
+    if (__builtin_add_overflow(*total, ct, total)) *flags |= ORC_FLAG_U64_OVERFLOW;
+  }
+  free(codes);
+  free(marks);
+  free(newm);
+  return rc;
 }
 
 int orc_grade(const orc_netlist *nl, const uint64_t *a_bits, const uint64_t *b_bits, uint64_t num_pairs,

@@ -35,6 +35,18 @@ int orc_merge_chunk(const orc_netlist *nl, const uint8_t *codes, const uint8_t *
 int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uint64_t start,
                     const uint32_t *first_t0, const uint32_t *first_t1, uint64_t *gate_counts,
                     uint64_t *chunk_total, uint32_t *flags);
+int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8_t *newm, uint64_t W,
+                        uint64_t *gate_counts, uint64_t *chunk_total, uint32_t *flags);
+int orc_segments(const orc_netlist *nl, uint32_t K, uint32_t *out, uint64_t cap, uint64_t *count);
+int orc_merge_segs_chunk(const orc_netlist *nl, const uint8_t *codes, const uint8_t *marks, uint64_t W,
+                         uint64_t start, uint32_t merge_window, uint32_t K, const uint32_t *segs, uint64_t S,
+                         uint32_t *seg_t0, uint32_t *seg_t1);
+int orc_new_from_segments(const orc_netlist *nl, uint64_t W, uint64_t start, uint32_t K, const uint32_t *segs,
+                          uint64_t S, const uint32_t *seg_t0, const uint32_t *seg_t1, uint8_t *newm);
+int orc_grade_segs(const orc_netlist *nl, const uint64_t *a_bits, const uint64_t *b_bits, uint64_t num_pairs,
+                   uint64_t chunk, const orc_options *opt, uint32_t K, const uint32_t *segs, uint64_t S,
+                   uint32_t *seg_t0, uint32_t *seg_t1, uint64_t *gate_counts, uint64_t *chunk_totals,
+                   uint64_t *total, uint32_t *flags);
 int orc_grade(const orc_netlist *nl, const uint64_t *a_bits, const uint64_t *b_bits, uint64_t num_pairs,
               uint64_t chunk, const orc_options *opt, uint32_t *first_t0, uint32_t *first_t1,
               uint64_t *gate_counts, uint64_t *chunk_totals, uint64_t *total, uint32_t *flags);

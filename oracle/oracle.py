@@ -111,6 +111,42 @@ def grade(net: OracleNet, a_bits: np.ndarray, num_pairs: int, chunk: int, robust
     return dict(first_t0=f0, first_t1=f1, gate_counts=gc, chunk_totals=ct, total=tot.value, flags=flags.value)
 
 
+def segments(net: OracleNet, K: int) -> np.ndarray:
+    """Segments of length K (orc_segments): (S, K) uint32, 0xFFFFFFFF padded."""
+    cnt = C.c_uint64()
+    if lib().orc_segments(net.ref, C.c_uint32(K), None, C.c_uint64(0), C.byref(cnt)):
+        raise ValueError(f"bad segment length {K}")
+    out = np.zeros((max(cnt.value, 1), K), np.uint32)
+    lib().orc_segments(net.ref, C.c_uint32(K), _p(out), C.c_uint64(cnt.value), C.byref(cnt))
+    return out[:cnt.value]
+
+
+def grade_segs(net: OracleNet, a_bits: np.ndarray, num_pairs: int, chunk: int, robust: bool, K: int,
+               b_bits: np.ndarray | None = None, merge_window: int = 0) -> dict:
+    """The chunk loop with segments of length K (orc_grade_segs)."""
+    a_bits = np.ascontiguousarray(a_bits, dtype=np.uint64)
+    if b_bits is not None:
+        b_bits = np.ascontiguousarray(b_bits, dtype=np.uint64)
+    segs = np.ascontiguousarray(segments(net, K))
+    S = segs.shape[0]
+    chunk = num_pairs if chunk <= 0 or chunk > num_pairs else chunk
+    nchunks = (num_pairs + chunk - 1) // chunk
+    s0 = np.full(max(S, 1), 0xFFFFFFFF, np.uint32)
+    s1 = np.full(max(S, 1), 0xFFFFFFFF, np.uint32)
+    gc = np.zeros(net.n, np.uint64)
+    ct = np.zeros(nchunks, np.uint64)
+    tot = C.c_uint64()
+    flags = C.c_uint32()
+    opt = _Opt(int(robust), merge_window)
+    rc = lib().orc_grade_segs(net.ref, _p(a_bits), _p(b_bits), C.c_uint64(num_pairs), C.c_uint64(chunk),
+                              C.byref(opt), C.c_uint32(K), _p(segs), C.c_uint64(S), _p(s0), _p(s1), _p(gc), _p(ct),
+                              C.byref(tot), C.byref(flags))
+    if rc:
+        raise RuntimeError(f"orc_grade_segs failed ({rc})")
+    return dict(segments=segs, seg_t0=s0[:S], seg_t1=s1[:S], gate_counts=gc, chunk_totals=ct, total=tot.value,
+                flags=flags.value)
+
+
 def sim_chunk(net: OracleNet, a_bits, num_pairs: int, start: int, width: int, b_bits=None) -> np.ndarray:
     a_bits = np.ascontiguousarray(a_bits, dtype=np.uint64)
     if b_bits is not None:

@@ -1,7 +1,7 @@
 // fcount: command-line grader with the reference's flags and result line
 // (main.cu:32-237), driving the C ABI of libpdfgrade.
 //
-//   fcount [--robust] [--segs 1] [--num_patterns N] [--verbose [--sim]
+//   fcount [--robust] [--segs N] [--num_patterns N] [--verbose [--sim]
 //          [--mark] [--merge] [--cover]] [--fused] --bench CIRCUIT VECTORS...
 //
 // CIRCUIT: a .bench file when the name contains "bench", otherwise a
@@ -117,10 +117,11 @@ int main(int argc, char **argv) {
       case 's':
         segs = std::atoi(optarg);
         if (segs > 8) die("Segment lengths over 8 are not supported!");
+        if (segs < 1) die("Segment length must be at least 1");
         break;
       case 'h':
         std::printf("Usage: %s (options) /path/to/vectors1 /path/to/vectors2\n", argv[0]);
-        std::printf("\t--segs N : segment length (only 1 is implemented)\n");
+        std::printf("\t--segs N : Run fault grading over segments of length N, where 1 <= N <= 8\n");
         std::printf("\t--bench /path/to/ckt : the circuit to grade (.bench or .level)\n");
         std::printf("\t--robust : robust sensitization (default non-robust)\n");
         std::printf("\t--num_patterns N : pairs per chunk = N-1, rounded down to a multiple of 64\n");
@@ -131,7 +132,7 @@ int main(int argc, char **argv) {
   }
   for (int i = optind; i < argc; ++i) benchmarks.push_back(argv[i]);
   if (infile.empty()) die("--bench argument is required.");
-  if (segs != 1) die("only segment length 1 is implemented");
+  if (fused_flag && segs != 1) die("--fused grades segments of length 1 only");
 
   pdf_netlist *nl = nullptr;
   if (infile.find("bench") != std::string::npos) check(pdf_netlist_read_bench(infile.c_str(), &nl), "read_bench");
@@ -196,10 +197,23 @@ int main(int argc, char **argv) {
       std::vector<uint64_t> counts(info.n);
       check(pdf_get_first(g, f0.data(), f1.data()), "get_first");
       check(pdf_get_gate_counts(g, counts.data()), "get_gate_counts");
-      if (merge_flag) {
+      if (merge_flag && segs == 1) {
         std::ofstream f("gpumerge.log");
         for (uint32_t i = 0; i < info.n; ++i)
           f << "Segment " << i << "(" << i << "): " << (int32_t)f0[i] << "," << (int32_t)f1[i] << " \n";
+      } else if (merge_flag) {
+        // segment r (its node ids, -1 padded): first T0, first T1
+        uint64_t S = 0;
+        check(pdf_netlist_segments(nl, (uint32_t)segs, nullptr, 0, &S), "segments");
+        std::vector<uint32_t> ids(S * segs), s0(S), s1(S);
+        check(pdf_netlist_segments(nl, (uint32_t)segs, ids.data(), S, &S), "segments");
+        check(pdf_get_segment_first(g, s0.data(), s1.data(), S, &S), "get_segment_first");
+        std::ofstream f("gpumerge.log");
+        for (uint64_t r = 0; r < S; ++r) {
+          f << "Segment " << r << "(";
+          for (int j = 0; j < segs; ++j) f << (int32_t)ids[r * segs + j] << (j + 1 < segs ? "," : "");
+          f << "): " << (int32_t)s0[r] << "," << (int32_t)s1[r] << " \n";
+        }
       }
       if (cover_flag) {
         std::ofstream f("gpucover.log");

@@ -549,19 +549,28 @@ struct CoverLds {
 };
 
 // The DP of one pair (relative index rel in the chunk) by one wave.
-template <bool ALL_NEW>
+// SEGNEW (segments of length K >= 2): a node is new at p when its bit in
+// `newp` ([n][pl.pitch] words, built by k_seg_events) is set, instead of
+// when its own first T0 / T1 is p.
+template <bool ALL_NEW, bool SEGNEW = false>
 __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint32_t start,
                           const uint32_t *__restrict__ list, const uint32_t *__restrict__ list_cnt,
                           uint64_t list_stride, const uint32_t *__restrict__ first_t0,
                           const uint32_t *__restrict__ first_t1, uint64_t *c_s, uint64_t *h_s,
                           uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
                           uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
-                          uint32_t *__restrict__ pair_gain, uint32_t rel) {
+                          uint32_t *__restrict__ pair_gain, uint32_t rel,
+                          const uint64_t *__restrict__ newp = nullptr) {
   const uint32_t lane = threadIdx.x;
   const uint64_t below = (1ULL << lane) - 1;
   const uint32_t w = rel >> 6, j = rel & 63;
   const uint32_t p = start + rel;
   const uint32_t len = list_cnt[w];
+  auto is_new = [&](uint32_t g) -> bool {
+    if (ALL_NEW) return true;
+    if (SEGNEW) return (newp[(uint64_t)g * pl.pitch + w] >> j) & 1;
+    return first_t0[g] == p || first_t1[g] == p;
+  };
   const uint32_t *L = list + (uint64_t)w * list_stride;
   // 1. the nodes marked at p, in list order
   for (uint32_t t = lane; t < kHashSlots; t += 64) S.h_key[t] = kEmpty;
@@ -593,7 +602,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
         S.s_nfo[idx] = r.nfo;
         S.s_fo_off[idx] = r.fout_off;
         S.s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
-                                 ((ALL_NEW || first_t0[g] == p || first_t1[g] == p) ? 4 : 0));
+                                 (is_new(g) ? 4 : 0));
         uint32_t h = hash_slot(g);
         while (atomicCAS(&S.h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
         S.h_val[h] = (uint16_t)idx;
@@ -697,7 +706,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
           }
         }
         uint64_t c = C, h = H;
-        if (ALL_NEW || first_t0[g] == p || first_t1[g] == p) { ovf |= add_ovf(c, H); h = 0; }
+        if (is_new(g)) { ovf |= add_ovf(c, H); h = 0; }
         if (r.type == PDF_INPT) h = 0;
         c_s[g] = c;
         h_s[g] = h;
@@ -729,7 +738,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
   __syncthreads();
 }
 
-template <bool ALL_NEW>
+template <bool ALL_NEW, bool SEGNEW = false>
 __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t start, const uint32_t *__restrict__ list,
                                               const uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                               const uint32_t *__restrict__ act_list,
@@ -738,14 +747,100 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               const uint32_t *__restrict__ first_t1, uint64_t *cs, uint64_t *hs,
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
                                               uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
-                                              uint32_t width, uint32_t *__restrict__ pair_gain) {
+                                              uint32_t width, uint32_t *__restrict__ pair_gain,
+                                              const uint64_t *__restrict__ newp) {
   __shared__ CoverLds S;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
   uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
   const uint32_t nact = ALL_NEW ? width : *act_count;
   for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x)
-    cover_one<ALL_NEW>(S, net, pl, start, list, list_cnt, list_stride, first_t0, first_t1, c_s, h_s, gate_counts,
-                       chunk_total, grand_total, err, pair_gain, ALL_NEW ? k : act_list[k]);
+    cover_one<ALL_NEW, SEGNEW>(S, net, pl, start, list, list_cnt, list_stride, first_t0, first_t1, c_s, h_s,
+                               gate_counts, chunk_total, grand_total, err, pair_gain, ALL_NEW ? k : act_list[k], newp);
+}
+
+// gpuMergeSegments<K> for K >= 2 (kernSegmentReduce2 with has_marked_segment,
+// mergekernel.cuh:91-198): segment s ([K] node ids, PDF_NONE entries
+// skipped) is detected at a pair when every node of it is marked there; its
+// earliest detection with node 0 falling (T0) and rising (T1) is min-combined
+// into sf0 / sf1 (chunks run in order, one writer per segment).  One wave per
+// segment, lanes over the chunk's words 64 at a time; a block of 64 words is
+// skipped when the AND of the nodes' mark-present words is zero, and the scan
+// stops once both polarities are found.
+template <bool SEQ>
+__global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__restrict__ seg, uint32_t K,
+                                                   uint64_t S, uint32_t start, uint32_t *__restrict__ sf0,
+                                                   uint32_t *__restrict__ sf1) {
+  const uint64_t s = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= S) return;  // whole waves
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t mine = lane < K ? seg[s * K + lane] : PDF_NONE;
+  uint32_t gid[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) gid[i] = __builtin_amdgcn_readfirstlane(__shfl(mine, i));
+  uint32_t best0 = PDF_NONE, best1 = PDF_NONE;
+  for (uint32_t wb = 0; wb < pl.words && (best0 == PDF_NONE || best1 == PDF_NONE); wb += 64) {
+    uint64_t nz = ~0ULL;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (gid[i] != PDF_NONE) nz &= pl.NZ[(uint64_t)gid[i] * pl.ntiles + (wb >> 6)];
+    if (!nz) continue;
+    const uint32_t w = wb + lane;
+    uint64_t m = 0;
+    if (w < pl.words && ((nz >> lane) & 1)) {
+      m = ~0ULL;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (gid[i] != PDF_NONE) m &= pl.M[(uint64_t)gid[i] * pl.pitch + w];
+    }
+    uint64_t a = 0, b = 0;
+    if (m) frames<SEQ>(pl, gid[0], w, a, b);
+    const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
+    const uint32_t base = start + 64u * w;
+    const uint32_t c0 = t0 ? base + (uint32_t)__builtin_ctzll(t0) : PDF_NONE;
+    const uint32_t c1 = t1 ? base + (uint32_t)__builtin_ctzll(t1) : PDF_NONE;
+    const uint64_t b0 = __ballot(t0 != 0), b1 = __ballot(t1 != 0);
+    // the lowest lane holds the lowest word, hence the earliest pair
+    const uint32_t e0 = __shfl(c0, b0 ? (int)__builtin_ctzll(b0) : 0);
+    const uint32_t e1 = __shfl(c1, b1 ? (int)__builtin_ctzll(b1) : 0);
+    if (best0 == PDF_NONE && b0) best0 = e0;
+    if (best1 == PDF_NONE && b1) best1 = e1;
+  }
+  if (lane == 0) {
+    if (best0 < sf0[s]) sf0[s] = best0;
+    if (best1 < sf1[s]) sf1[s] = best1;
+  }
+}
+
+// findActiveThreads<K> (coverkernel.cuh:127-154) for K >= 2: every segment
+// whose first T0 or T1 pair lies in this chunk sets that pair's bit for each
+// of its nodes in the new-node plane, and the pair joins the chunk's list of
+// distinct first-detection pairs (as k_events does for K = 1).
+__global__ void k_seg_events(const uint32_t *__restrict__ seg, uint32_t K, uint64_t S,
+                             const uint32_t *__restrict__ sf0, const uint32_t *__restrict__ sf1, uint32_t start,
+                             uint32_t width, uint64_t *__restrict__ newp, uint64_t pitch,
+                             uint64_t *__restrict__ actmap, uint32_t *__restrict__ act_list,
+                             uint32_t *__restrict__ act_count) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint32_t x[2] = {sf0[s], sf1[s]};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (x[k] == PDF_NONE || x[k] < start || x[k] - start >= width) continue;
+    const uint32_t rel = x[k] - start;
+    const uint64_t bit = 1ULL << (rel & 63);
+    for (uint32_t i = 0; i < K; ++i) {
+      const uint32_t g = seg[s * K + i];
+      if (g != PDF_NONE) atomicOr((unsigned long long *)&newp[(uint64_t)g * pitch + (rel >> 6)], (unsigned long long)bit);
+    }
+    const uint64_t old = atomicOr((unsigned long long *)&actmap[rel >> 6], (unsigned long long)bit);
+    if (!(old & bit)) act_list[atomicAdd(act_count, 1u)] = rel;
+  }
+}
+
+// n words of p to v (grid-stride).
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
 }
 
 // ------------------------------------------------------------ pair features
@@ -1154,6 +1249,14 @@ struct pdf_grader {
   uint64_t *d_V = nullptr, *d_VB = nullptr, *d_M = nullptr, *d_NZ = nullptr, *d_ACC = nullptr;
   // merge / cover state
   uint32_t *d_first0 = nullptr, *d_first1 = nullptr;
+  // segments of length seg_k >= 2 (stage API): node ids [seg_count][seg_k]
+  // padded with PDF_NONE, each segment's first T0 / T1 pair, and the chunk's
+  // plane of nodes new at a pair ([n][pitch] bits, allocated on first use)
+  uint32_t seg_k = 0, merged_segs = 1;
+  uint64_t seg_count = 0;
+  uint32_t *d_seg = nullptr, *d_sfirst0 = nullptr, *d_sfirst1 = nullptr;
+  uint64_t *d_NEW = nullptr;
+  bool seg_stale = true;  // reset the segment firsts before the next merge
   uint64_t *d_gate = nullptr;
   uint64_t *d_grand = nullptr;  // exact total over all chunks
   uint32_t *d_err = nullptr;
@@ -1406,8 +1509,90 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_t
   k_cover<false><<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt,
                                                        g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                        g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
-                                                       g->d_grand, g->d_err, (uint32_t)width, pair_total);
+                                                       g->d_grand, g->d_err, (uint32_t)width, pair_total,
+                                                       nullptr);
   tm_end(g);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+// Segments of length K >= 2: the chunk's new-node plane from the segment
+// firsts, then the cover DP reading it.
+int launch_cover_segs(pdf_grader *g, uint64_t start, uint64_t width) {
+  const Planes pl = planes(g, width);
+  const uint64_t nw = (uint64_t)g->nl.n * g->pitch * 2;  // u32 words of the plane
+  tm_begin(g, kFill);
+  k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nw, 256), 65536), 256, 0, g->stream>>>(
+      reinterpret_cast<uint32_t *>(g->d_NEW), nw, 0u);
+  tm_end(g);
+  tm_begin(g, kEvents);
+  k_seg_events<<<grid_for(g->seg_count, 256), 256, 0, g->stream>>>(
+      g->d_seg, g->seg_k, g->seg_count, g->d_sfirst0, g->d_sfirst1, (uint32_t)start, (uint32_t)width, g->d_NEW,
+      g->pitch, g->d_actmap, g->d_act_list, g->d_act_count);
+  tm_end(g);
+  tm_begin(g, kCover);
+  k_cover<false, true><<<g->cover_waves, 64, 0, g->stream>>>(
+      g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt, g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
+      g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total, g->d_grand, g->d_err, (uint32_t)width, nullptr,
+      g->d_NEW);
+  tm_end(g);
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+constexpr uint64_t kMaxSegments = 1ULL << 27;  // segment list entries kept in HBM: 4 GiB at K = 8
+
+void free_segments(pdf_grader *g) {
+  dfree(g->d_seg);
+  dfree(g->d_sfirst0);
+  dfree(g->d_sfirst1);
+  g->seg_k = 0;
+  g->seg_count = 0;
+}
+
+// The segment list of length K (built once, kept until K changes), its
+// first-pair arrays and the new-node plane.
+int ensure_segments(pdf_grader *g, uint32_t K) {
+  if (g->seg_k != K) {
+    free_segments(g);
+    const uint64_t S = count_segments(g->nl, K);
+    if (S > kMaxSegments) {
+      set_error("pdf_merge: " + std::to_string(S) + " segments of length " + std::to_string(K) +
+                " exceed the supported " + std::to_string(kMaxSegments));
+      return PDF_ERR_UNSUPPORTED;
+    }
+    std::vector<uint32_t> h(S * K);
+    list_segments(g->nl, K, h.data(), S);
+    int rc = dalloc(&g->d_seg, S * K);
+    if (!rc) rc = dalloc(&g->d_sfirst0, S);
+    if (!rc) rc = dalloc(&g->d_sfirst1, S);
+    if (!rc && !hip_ok(hipMemcpy(g->d_seg, h.data(), S * K * sizeof(uint32_t), hipMemcpyHostToDevice), "hipMemcpy"))
+      rc = PDF_ERR_HIP;
+    if (rc) { free_segments(g); return rc; }
+    g->seg_k = K;
+    g->seg_count = S;
+    g->seg_stale = true;
+  }
+  if (!g->d_NEW) return dalloc(&g->d_NEW, (uint64_t)g->nl.n * g->pitch);
+  return PDF_OK;
+}
+
+template <bool SEQ>
+void launch_merge_seg_t(pdf_grader *g, uint64_t start, uint64_t width) {
+  const Planes pl = planes(g, width);
+  k_merge_seg<SEQ><<<grid_for(g->seg_count, 4), 256, 0, g->stream>>>(pl, g->d_seg, g->seg_k, g->seg_count,
+                                                                   (uint32_t)start, g->d_sfirst0, g->d_sfirst1);
+}
+
+int launch_merge_seg(pdf_grader *g, uint64_t start, uint64_t width) {
+  if (g->seg_stale) {
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(grid_for(g->seg_count, 256), 65536);
+    k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst0, g->seg_count, PDF_NONE);
+    k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst1, g->seg_count, PDF_NONE);
+    g->seg_stale = false;
+  }
+  if (g->seq) launch_merge_seg_t<true>(g, start, width);
+  else launch_merge_seg_t<false>(g, start, width);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
@@ -1520,6 +1705,7 @@ int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs) {
 }
 
 void free_chunk_buffers(pdf_grader *g) {
+  dfree(g->d_NEW);
   dfree(g->d_V);
   dfree(g->d_VB);
   dfree(g->d_M);
@@ -1544,6 +1730,7 @@ int queue_reset(pdf_grader *g) {
   HIPCHK(hipGetLastError());
   g->stage = 0;
   g->stage_chunk = -1;
+  g->seg_stale = true;
   return PDF_OK;
 }
 
@@ -1660,6 +1847,7 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_va);
   dfree(g->d_vb);
   free_rank(g);
+  free_segments(g);
   dfree(g->d_first0);
   dfree(g->d_first1);
   dfree(g->d_gate);
@@ -1845,31 +2033,36 @@ int pdf_mark(pdf_grader *g, uint64_t chunk, uint64_t start, int robust, float *m
 
 int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, float *ms) {
   if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
-  if (segs != 1) { set_error("pdf_merge: only segment length 1 is supported"); return PDF_ERR_UNSUPPORTED; }
+  if (segs < 1 || segs > 8) { set_error("pdf_merge: segment length must be 1..8"); return PDF_ERR_ARG; }
   uint64_t width;
   int rc = check_chunk(g, chunk, start, &width);
   if (rc) return rc;
   if (g->stage_chunk != (int64_t)chunk || g->stage < 2) { set_error("pdf_merge: mark this chunk first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
+  if (segs >= 2 && (rc = ensure_segments(g, (uint32_t)segs))) return rc;
   EventTimer tm(g, ms);
   if ((rc = reset_chunk(g))) return rc;
+  // per-node merge and the per-word lists of marked nodes (the cover's input)
   if ((rc = launch_reverse<false, true>(g, start, width))) return rc;
+  if (segs >= 2 && (rc = launch_merge_seg(g, start, width))) return rc;
   if ((rc = tm.finish())) return rc;
   g->stage = 3;
+  g->merged_segs = (uint32_t)segs;
   return PDF_OK;
 }
 
 int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t *coverage, float *ms) {
   if (!g || !coverage) { set_error("null argument"); return PDF_ERR_ARG; }
-  if (segs != 1) { set_error("pdf_cover: only segment length 1 is supported"); return PDF_ERR_UNSUPPORTED; }
+  if (segs < 1 || segs > 8) { set_error("pdf_cover: segment length must be 1..8"); return PDF_ERR_ARG; }
   uint64_t width;
   int rc = check_chunk(g, chunk, start, &width);
   if (rc) return rc;
   // exactly once per merged chunk: a second call would count its pairs again
   if (g->stage_chunk != (int64_t)chunk || g->stage != 3) { set_error("pdf_cover: merge this chunk first"); return PDF_ERR_STATE; }
+  if ((uint32_t)segs != g->merged_segs) { set_error("pdf_cover: segment length differs from the merge's"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   EventTimer tm(g, ms);
-  if ((rc = launch_cover(g, start, width))) return rc;
+  if ((rc = segs >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width))) return rc;
   if ((rc = tm.finish())) return rc;
   uint64_t tot = 0;
   uint32_t err = 0;
@@ -2046,6 +2239,7 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   HIPCHK(hipSetDevice(g->device));
   int rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  g->seg_stale = true;  // graph replays reset the node state without queue_reset
   if (g->tune_use_graph && !g->timing && !g->lanes.empty()) {
     if ((rc = queue_grade_lane_graphs(g, robust))) return rc;
   } else if (g->tune_use_graph && !g->timing) {
@@ -2193,6 +2387,23 @@ int pdf_get_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1) {
   return PDF_OK;
 }
 
+int pdf_get_segment_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1, uint64_t capacity,
+                          uint64_t *count) {
+  if (!g || !count) { set_error("null argument"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  *count = g->seg_count;
+  const uint64_t k = std::min(capacity, g->seg_count);
+  if (!k) return PDF_OK;
+  if (g->seg_stale) {  // reset since the last merge: nothing detected yet
+    if (first_t0) std::fill(first_t0, first_t0 + k, PDF_NONE);
+    if (first_t1) std::fill(first_t1, first_t1 + k, PDF_NONE);
+    return PDF_OK;
+  }
+  if (first_t0) HIPCHK(hipMemcpy(first_t0, g->d_sfirst0, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
+  if (first_t1) HIPCHK(hipMemcpy(first_t1, g->d_sfirst1, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
 int pdf_get_gate_counts(const pdf_grader *g, uint64_t *counts) {
   if (!g || !counts) { set_error("null argument"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
@@ -2291,7 +2502,7 @@ int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev) {
     k_cover<true><<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt,
                                                       g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
-                                                      g->d_grand, g->d_err, (uint32_t)width, gain_dev);
+                                                      g->d_grand, g->d_err, (uint32_t)width, gain_dev, nullptr);
     HIPCHK(hipGetLastError());
   }
   uint32_t err = 0;

@@ -223,6 +223,58 @@ static int generate_once(const pdf_gen_params &p, Rng &rng, Netlist &nl, std::st
   return finalize_netlist(nl, err);
 }
 
+// Walks counted level by level: cnt_k(g) = 1 when k = 1 or g has no fanouts,
+// else the sum over its fanouts of cnt_{k-1}.
+uint64_t count_segments(const Netlist &nl, uint32_t K) {
+  std::vector<uint64_t> cnt(nl.n, 1), nxt(nl.n);
+  for (uint32_t k = 1; k < K; ++k) {
+    for (uint32_t g = 0; g < nl.n; ++g) {
+      const uint32_t lo = nl.fout_off[g], hi = nl.fout_off[g + 1];
+      uint64_t s = lo == hi ? 1 : 0;
+      for (uint32_t e = lo; e < hi; ++e)
+        if (__builtin_add_overflow(s, cnt[nl.fout[e]], &s)) s = UINT64_MAX;
+      nxt[g] = s;
+    }
+    cnt.swap(nxt);
+  }
+  uint64_t total = 0;
+  for (uint64_t c : cnt)
+    if (__builtin_add_overflow(total, c, &total)) return UINT64_MAX;
+  return total;
+}
+
+void list_segments(const Netlist &nl, uint32_t K, uint32_t *out, uint64_t cap) {
+  // explicit stack: path[d] and the next fanout position to try below it
+  uint32_t path[8], next[8];
+  uint64_t row = 0;
+  for (uint32_t g0 = 0; g0 < nl.n && row < cap; ++g0) {
+    uint32_t d = 1;
+    path[0] = g0;
+    next[0] = nl.fout_off[g0];
+    bool emit = true;  // emit when a walk has just been extended
+    while (d > 0 && row < cap) {
+      const uint32_t g = path[d - 1];
+      const uint32_t lo = nl.fout_off[g], hi = nl.fout_off[g + 1];
+      if (emit && (d == K || lo == hi)) {
+        for (uint32_t i = 0; i < K; ++i) out[row * K + i] = i < d ? path[i] : PDF_NONE;
+        ++row;
+        --d;  // back to the parent
+        emit = false;
+        continue;
+      }
+      if (emit) next[d - 1] = lo;
+      if (next[d - 1] < hi) {
+        path[d] = nl.fout[next[d - 1]++];
+        ++d;
+        emit = true;
+      } else {
+        --d;
+        emit = false;
+      }
+    }
+  }
+}
+
 }  // namespace pdf
 
 using namespace pdf;
@@ -294,6 +346,14 @@ int pdf_netlist_get_info(const pdf_netlist *h, pdf_netlist_info *info) {
   info->levels = nl.L;
   info->edges = nl.fin.size();
   info->structural_paths = nl.structural_paths;
+  return PDF_OK;
+}
+
+int pdf_netlist_segments(const pdf_netlist *h, uint32_t segs, uint32_t *out, uint64_t capacity, uint64_t *count) {
+  if (!h || !count) { set_error("pdf_netlist_segments: null argument"); return PDF_ERR_ARG; }
+  if (segs < 1 || segs > 8) { set_error("pdf_netlist_segments: segment length must be 1..8"); return PDF_ERR_ARG; }
+  *count = count_segments(h->nl, segs);
+  if (out && capacity) list_segments(h->nl, segs, out, capacity);
   return PDF_OK;
 }
 

@@ -25,6 +25,13 @@ struct Netlist {
 // PDF_OK or PDF_ERR_NETLIST with a message in `err`.
 int finalize_netlist(Netlist &nl, std::string &err);
 
+// Segments of length K (1..8): for every node g0 in id order, every fanout
+// walk of K nodes, fanouts in CSR order, depth first; a walk reaching a node
+// without fanouts first ends there, the rest of its K entries PDF_NONE.
+// count_segments saturates at UINT64_MAX; list_segments writes at most `cap`.
+uint64_t count_segments(const Netlist &nl, uint32_t K);
+void list_segments(const Netlist &nl, uint32_t K, uint32_t *out, uint64_t cap);
+
 // Thread-local last-error text.
 void set_error(const std::string &msg);
 

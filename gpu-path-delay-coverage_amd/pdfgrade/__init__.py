@@ -87,6 +87,7 @@ SIGNATURES = {
     "pdf_netlist_free": ([_P], None),
     "pdf_netlist_get_info": ([_P, C.POINTER(_Info)], C.c_int),
     "pdf_netlist_export": ([_P] + [C.c_void_p] * 8, C.c_int),
+    "pdf_netlist_segments": ([_P, C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
     "pdf_netlist_generate": ([C.POINTER(_GenParams), C.POINTER(_P)], C.c_int),
     "pdf_gen_vectors": ([_U64, C.c_uint32, _U64, C.c_void_p], C.c_int),
     "pdf_vectors_count": ([C.c_char_p, C.c_uint32, _U64P], C.c_int),
@@ -111,6 +112,7 @@ SIGNATURES = {
     "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
     "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_get_segment_first": ([_P, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
     "pdf_get_gate_counts": ([_P, C.c_void_p], C.c_int),
     "pdf_get_total": ([_P, _U64P], C.c_int),
     "pdf_pair_features": ([_P, C.c_uint32, C.c_int, C.c_void_p, _U64], C.c_int),
@@ -250,6 +252,15 @@ class Netlist:
         _check(lib().pdf_netlist_get_info(self._h, C.byref(i)))
         return dict(n=i.n, num_pi=i.num_pi, num_po=i.num_po, levels=i.levels, edges=i.edges,
                     structural_paths=i.structural_paths)
+
+    def segments(self, segs: int) -> np.ndarray:
+        """Segments of length segs (pdf_netlist_segments): (S, segs) uint32,
+        NONE padded."""
+        cnt = C.c_uint64()
+        _check(lib().pdf_netlist_segments(self._h, segs, None, 0, C.byref(cnt)))
+        out = np.zeros((max(cnt.value, 1), segs), np.uint32)
+        _check(lib().pdf_netlist_segments(self._h, segs, _ptr(out), cnt.value, C.byref(cnt)))
+        return out[:cnt.value]
 
     def arrays(self) -> dict:
         """CSR arrays as numpy: type, po, level, fin_off, fin, fout_off, fout, level_off."""
@@ -443,6 +454,15 @@ class Grader:
         f1 = np.zeros(self.n, np.uint32)
         _check(lib().pdf_get_first(self._h, _ptr(f0), _ptr(f1)))
         return f0, f1
+
+    def segment_first(self):
+        """First T0 / T1 pair of every segment of the last segs >= 2 merge."""
+        cnt = C.c_uint64()
+        _check(lib().pdf_get_segment_first(self._h, None, None, 0, C.byref(cnt)))
+        f0 = np.zeros(max(cnt.value, 1), np.uint32)
+        f1 = np.zeros(max(cnt.value, 1), np.uint32)
+        _check(lib().pdf_get_segment_first(self._h, _ptr(f0), _ptr(f1), cnt.value, C.byref(cnt)))
+        return f0[:cnt.value], f1[:cnt.value]
 
     def gate_counts(self) -> np.ndarray:
         c = np.zeros(self.n, np.uint64)

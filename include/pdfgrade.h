@@ -120,6 +120,18 @@ int pdf_netlist_export(const pdf_netlist *nl, uint8_t *type, uint8_t *po,
                        uint32_t *fout_off, uint32_t *fout,
                        uint32_t *level_off);
 
+/* Segments of length segs (1..8), the list gpuMergeSegments<K> builds on its
+ * first call (mergekernel.cuh:205-221, generateSegmentList in the absent
+ * util/ library, so the enumeration is this library's): for every node g0 in
+ * id order, every fanout walk g0 -> g1 -> ... of segs nodes, fanouts in id
+ * order, depth first; a walk that reaches a node without fanouts first ends
+ * there and its remaining entries are PDF_NONE.  segs = 1 is one segment per
+ * node.  *count receives the number of segments (saturated at 2^64-1);
+ * min(count, capacity) segments of segs entries each are written to out
+ * (may be NULL). */
+int pdf_netlist_segments(const pdf_netlist *nl, uint32_t segs, uint32_t *out, uint64_t capacity,
+                         uint64_t *count);
+
 /* Seeded synthetic netlists (SplitMix64-seeded xoshiro256**).  The
  * reference ships no generator; this one is defined by this library. */
 typedef struct pdf_gen_params {
@@ -199,11 +211,18 @@ int pdf_sim(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, float *ms);
  * robust_flag (main.cu:7; markkernel.cuh:313) */
 int pdf_mark(pdf_grader *g, uint64_t chunk, uint64_t start_pattern,
              int robust, float *ms);
-/* gpuMergeSegments<K> (mergekernel.cuh:200-267); only segs == 1 */
+/* gpuMergeSegments<K> (mergekernel.cuh:200-267), segs = K in 1..8.  For
+ * segs >= 2 the first call builds the segment list (pdf_netlist_segments)
+ * and keeps it, with each segment's first T0 / T1 pair, until the segment
+ * length changes; a segment is detected at a pair when all its nodes are
+ * marked, its polarity is node 0's.  Use one segment length for all chunks
+ * of a vector set. */
 int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
               float *ms);
 /* gpuCountPaths<K> (coverkernel.cuh:233-333): *coverage receives (overwrites)
- * this chunk's exact total; per-gate counts accumulate inside the grader. */
+ * this chunk's exact total; per-gate counts accumulate inside the grader.
+ * segs must equal the merge's; a node counts as new at a pair when a
+ * segment holding it was first detected there (coverkernel.cuh:127-154). */
 int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
               uint64_t *coverage, float *ms);
 
@@ -243,6 +262,12 @@ int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane,
 int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane);
 /* first_T0 / first_T1: [n], PDF_NONE when never detected. */
 int pdf_get_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1);
+/* Each segment's first T0 / T1 pair (PDF_NONE when never detected), in the
+ * order of pdf_netlist_segments, for the segment length of the last
+ * pdf_merge with segs >= 2: min(count, capacity) entries each; *count gets
+ * the number of segments (0 when no segment list is kept). */
+int pdf_get_segment_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1, uint64_t capacity,
+                          uint64_t *count);
 /* Per-gate exact counts sum_p c(g,p) over all chunks graded so far: [n]. */
 int pdf_get_gate_counts(const pdf_grader *g, uint64_t *counts);
 /* Exact total over all chunks graded so far. */

@@ -241,8 +241,8 @@ def test_api_errors():
         assert e.value.code == pg.ERR_ARG
         g.mark(0, 0, True)
         with pytest.raises(pg.PdfError) as e:
-            g.merge(0, 0, segs=2)
-        assert e.value.code == pg.ERR_UNSUPPORTED
+            g.merge(0, 0, segs=9)  # segment lengths are 1..8 (main.cu:97-103)
+        assert e.value.code == pg.ERR_ARG
         with pytest.raises(pg.PdfError) as e:
             g.sim(4, 1024)
         assert e.value.code == pg.ERR_ARG

@@ -1,0 +1,171 @@
+"""Segments of length K >= 2 on the GPU (pdf_merge / pdf_cover with segs = K,
+the reference's gpuMergeSegments<K> / gpuCountPaths<K>), bit-exact against the
+oracle's orc_grade_segs: totals, per-chunk totals, per-gate counts, each
+segment's first T0 / T1 pair and the per-node firsts."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import pdfgrade as pg
+from pdfgrade.configs import GATES_50K
+from oracle import oracle as orc
+
+from helpers import biased_vectors, edge_netlist, pack, small_config, unpack
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_stages(nl, bits, P, chunk, robust, K, b_bits=None):
+    with pg.Grader(nl) as g:
+        if b_bits is None:
+            g.set_vectors(bits, P)
+        else:
+            g.set_pairs(bits, b_bits, P)
+        g.set_chunk(chunk)
+        total, per_chunk = g.grade_stages(robust, segs=K)
+        s0, s1 = g.segment_first()
+        f0, f1 = g.first()
+        return dict(total=total, per_chunk=per_chunk, seg_t0=s0, seg_t1=s1, first_t0=f0, first_t1=f1,
+                    gate_counts=g.gate_counts(), grader_total=g.total())
+
+
+def check(nl, bits, P, chunk, robust, K, b_bits=None, ctx=""):
+    net = orc.OracleNet(nl.arrays())
+    ref = orc.grade_segs(net, bits, P, chunk, robust, K, b_bits=b_bits)
+    node = orc.grade(net, bits, P, chunk, robust, b_bits=b_bits)
+    got = gpu_stages(nl, bits, P, chunk, robust, K, b_bits=b_bits)
+    assert got["total"] == ref["total"] == got["grader_total"], ctx
+    assert np.array_equal(np.array(got["per_chunk"], np.uint64), ref["chunk_totals"]), ctx
+    assert np.array_equal(got["gate_counts"], ref["gate_counts"]), ctx
+    assert np.array_equal(got["seg_t0"], ref["seg_t0"]), ctx
+    assert np.array_equal(got["seg_t1"], ref["seg_t1"]), ctx
+    assert np.array_equal(got["first_t0"], node["first_t0"]), ctx
+    assert np.array_equal(got["first_t1"], node["first_t1"]), ctx
+    return ref["total"]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_circuits_segments(seed):
+    say("segments random", seed)
+    nl = pg.Netlist.generate(small_config(seed, num_gates=40 if seed % 2 else 24, levels=6,
+                                          wide=seed % 2 == 0, alpha=2.2 if seed % 3 == 0 else 0.0))
+    P = 320 + 37 * (seed % 3)
+    bits = biased_vectors(seed, nl.info()["num_pi"], P, p_hold=0.5)
+    for K in (2, 3, 4):
+        for robust in (True, False):
+            check(nl, bits, P, 128 if seed % 2 else 0, robust, K, ctx=f"seed {seed} K {K} robust {robust}")
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_edge_netlist_segments(robust):
+    say("segments edge", robust)
+    nl = edge_netlist()
+    P = 300
+    bits = biased_vectors(9, 8, P, p_hold=0.5)
+    totals = [check(nl, bits, P, 128, robust, K, ctx=f"K {K}") for K in (2, 3, 8)]
+    assert max(totals) > 0
+
+
+def test_explicit_pairs_segments():
+    say("segments explicit pairs")
+    nl = pg.Netlist.generate(small_config(7, num_gates=40, levels=6))
+    P = 256
+    a = biased_vectors(3, nl.info()["num_pi"], P, p_hold=0.4)
+    b = pack(np.roll(unpack(a, P), -1, axis=0))
+    for K in (2, 5):
+        check(nl, a, P, 64, True, K, b_bits=b, ctx=f"K {K}")
+        # explicit (V[p], V[p+1]) pairs equal sequence mode
+        seq = gpu_stages(nl, a, P, 64, True, K)
+        exp = gpu_stages(nl, a, P, 64, True, K, b_bits=b)
+        assert seq["total"] == exp["total"] and np.array_equal(seq["seg_t0"], exp["seg_t0"])
+
+
+def test_medium_circuit_segments():
+    """3,000 gates, 2,048 pairs in 512-pair chunks, K = 2 and 5."""
+    say("segments medium")
+    cfg = pg.GenConfig(seed=31, num_pi=48, num_po=32, num_gates=3000, levels=20, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=16)
+    nl = pg.Netlist.generate(cfg)
+    P = 2048
+    bits = biased_vectors(31, 48, P, p_hold=0.7)
+    for K, robust in ((2, True), (5, False)):
+        check(nl, bits, P, 512, robust, K, ctx=f"K {K}")
+
+
+def test_50k_gates_segments_sample_and_chunking():
+    """The 50k-gate netlist with K = 2: its first 2,048 pairs (as a vector set
+    of their own) against the oracle, and all 65,536 pairs graded in 8,192-
+    and 32,768-pair chunks with equal results."""
+    say("segments 50k")
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    I = nl.info()["num_pi"]
+    bits = pg.gen_vectors(GATES_50K.vector_seed, I, 2048)
+    check(nl, bits, 2048, 1024, False, 2, ctx="50k sample")
+    P = GATES_50K.pairs
+    full = pg.gen_vectors(GATES_50K.vector_seed, I, P)
+    a = gpu_stages(nl, full, P, 8192, False, 2)
+    b = gpu_stages(nl, full, P, 32768, False, 2)
+    assert a["total"] == b["total"] and sum(a["per_chunk"]) == a["total"]
+    for k in ("seg_t0", "seg_t1", "gate_counts"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_segment_api_errors():
+    say("segments api errors")
+    nl = pg.Netlist.generate(small_config(2))
+    P = 128
+    bits = biased_vectors(2, nl.info()["num_pi"], P)
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(0)
+        g.sim(0, 0)
+        g.mark(0, 0, True)
+        for bad in (0, 9):
+            with pytest.raises(pg.PdfError):
+                g.merge(0, 0, bad)
+        g.merge(0, 0, 3)
+        with pytest.raises(pg.PdfError):
+            g.cover(0, 0, 2)  # merged with K = 3
+        g.cover(0, 0, 3)
+        assert g.segment_first()[0].shape[0] == nl.segments(3).shape[0]
+        # a reset forgets the segment firsts
+        g.reset()
+        s0, s1 = g.segment_first()
+        assert np.all(s0 == pg.NONE) and np.all(s1 == pg.NONE)
+
+
+def test_fcount_cli_segments(tmp_path):
+    """fcount --segs K on c17: the result line's total equals the oracle's, and
+    the merge log lists every segment with its node ids and first pairs."""
+    import subprocess
+    from helpers import C17_BENCH
+    bench = tmp_path / "c17.bench"
+    bench.write_text(C17_BENCH)
+    nl = pg.Netlist.read_bench(str(bench))
+    V = 256
+    bits = pg.gen_vectors(17, 5, V)
+    vec = tmp_path / "c17.vec"
+    pg.write_vectors(str(vec), bits, V)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpu-path-delay-coverage_amd",
+                       "fcount")
+    net = orc.OracleNet(nl.arrays())
+    for K in (2, 3):
+        say("fcount segs", K)
+        r = subprocess.run([exe, "--bench", str(bench), "--segs", str(K), "--robust", "--verbose", "--merge", str(vec)],
+                           capture_output=True, text=True, timeout=120, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        line = [l for l in r.stdout.splitlines() if l.startswith(str(vec))][-1]
+        assert line.split(":", 1)[1].split(";")[0] == f"{V},{nl.info()['n']},{K}"
+        want = orc.grade_segs(net, bits, V, V, True, K)
+        assert int(line.rsplit(";", 1)[1]) == want["total"]
+        log = (tmp_path / "gpumerge.log").read_text().splitlines()
+        assert len(log) == want["segments"].shape[0]
+        first = log[0].split(": ")[1].split(",")
+        assert int(first[0]) == np.int32(want["seg_t0"][0].view(np.int32))

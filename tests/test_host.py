@@ -10,7 +10,7 @@ import pytest
 import pdfgrade as pg
 from pdfgrade.configs import GATES_50K, TINY
 
-from helpers import C17_BENCH
+from helpers import C17_BENCH, small_config
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -224,3 +224,17 @@ def test_vector_file_errors(tmp_path):
     rc = pg.lib().pdf_vectors_read(str(tmp_path / "many.vec").encode(), 4, buf[:, :1].copy().ctypes.data, 64,
                                    C.byref(got))
     assert rc == pg.ERR_ARG
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_library_segments_equal_oracle(seed):
+    """pdf_netlist_segments (the grader's segment list) equals the oracle's
+    enumeration entry for entry, K = 1..8."""
+    from oracle import oracle as orc
+    nl = pg.Netlist.generate(small_config(seed, num_gates=50, levels=7, wide=seed % 2 == 0,
+                                          alpha=2.2 if seed % 2 else 0.0))
+    net = orc.OracleNet(nl.arrays())
+    for K in range(1, 9):
+        assert np.array_equal(nl.segments(K), orc.segments(net, K)), K
+    with pytest.raises(pg.PdfError):
+        nl.segments(9)

@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=4096,
                     help="pairs graded by the serial oracle beside the CPU baseline (0 = skip every CPU leg)")
     ap.add_argument("--robust", type=int, default=1)
+    ap.add_argument("--segs", type=int, default=1,
+                    help="single workloads: segment length K (1..8); K >= 2 times pdf_grade_segs")
     ap.add_argument("--graph", type=int, default=None, help="batch workloads: force graph replay on (1) or off (0)")
     ap.add_argument("--learned", type=int, default=1, help="50k workload: run the learned-ranking experiment")
     ap.add_argument("--autotune", type=int, default=None,
@@ -121,6 +123,19 @@ def cpu_baseline(nl, bits, wl, sample, robust, gpu):
                             and np.array_equal(r["gate_counts"], gpu["gate_counts"])) if whole else None,
             "serial_oracle": {"value": serial_oracle_rate(net, bits, wl.pairs, sample, robust), "unit": "pairs/s",
                               "cores": 1, "sample": f"first {sample} pairs, byte-per-pattern oracle.c, one chunk"}}
+
+
+def segs_cpu_baseline(nl, bits, wl, sample, robust, K):
+    """Serial C oracle with segments of length K (one core) on the first
+    `sample` pairs, graded as a vector set of their own."""
+    from oracle import oracle as orc
+
+    net = orc.OracleNet(nl.arrays())
+    t0 = time.perf_counter()
+    orc.grade_segs(net, bits, sample, sample, robust, K)
+    return {"value": sample / (time.perf_counter() - t0), "unit": "pairs/s", "cores": 1, "kind": "port",
+            "sample": f"first {sample} of {wl.pairs} pairs of the {wl.name} netlist, one chunk, serial C oracle "
+                      f"with segments of length {K}"}
 
 
 def bench_batch(args, wl):
@@ -241,12 +256,16 @@ def main():
     g.set_chunk(wl.chunk)
     tune = g.autotune(robust, reps=5) if args.autotune else {"lanes": wl.lanes, "chunk_pairs": wl.chunk,
                                                               "use_graph": 1, "source": "recorded"}
+    K = args.segs
+    grade = (lambda: g.grade(robust)) if K == 1 else (lambda: g.grade_segs(robust, K))
+    if K > 1:  # segment merges min-combine chunk by chunk: one lane
+        tune = {"lanes": 1, "chunk_pairs": wl.chunk, "use_graph": 1, "source": "one lane for segments"}
     total = None
     for _ in range(args.warmup):
-        total = g.grade(robust)
+        total = grade()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        t = g.grade(robust)
+        t = grade()
         assert total is None or t == total
         total = t
     dt = time.perf_counter() - t0
@@ -256,15 +275,19 @@ def main():
     # live per-launch timing of the dominant kernel family: the same chunk
     # shape with the launches serialized (lanes overlap kernels, which would
     # blur one kernel's duration)
-    t_total, kt = g.grade_timed(robust)
+    t_total, kt = g.grade_timed(robust) if K == 1 else g.grade_segs(robust, K, timed=True)
     assert t_total == total
-    fam = max(("fwd", "reverse", "cover", "events"), key=lambda k: kt[f"{k}_ms"])
+    fam = max(("fwd", "reverse", "cover", "events", "merge"), key=lambda k: kt[f"{k}_ms"])
     # algorithmic bytes per pair, from the per-stage figures of the
     # measurement plan: sim (2n + I)/8 and forward mark 3n/8 for the fused
-    # forward kernel; removal 2n/8 and merge 2n/8 for the fused reverse kernel
+    # forward kernel; removal 2n/8 and merge 2n/8 for the fused reverse kernel;
+    # for segments of length K, each of the S segments reads K mark bits and
+    # node 0's final-value bit per pair: S (K + 1)/8
     fwd_bytes = (5 * n + I) / 8.0 * wl.pairs
     rev_bytes = 4 * n / 8.0 * wl.pairs
-    fam_bytes = {"fwd": fwd_bytes, "reverse": rev_bytes}.get(fam, 0.0)
+    num_segs = nl.segments(K).shape[0] if K > 1 else n
+    seg_bytes = num_segs * (K + 1) / 8.0 * wl.pairs
+    fam_bytes = {"fwd": fwd_bytes, "reverse": rev_bytes, "merge": seg_bytes}.get(fam, 0.0)
     launches = max(1, kt[f"{fam}_launches"])
     avg_ms = kt[f"{fam}_ms"] / launches
     achieved = (fam_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
@@ -274,17 +297,20 @@ def main():
     e2e_gbs = e2e_bytes / (ms_per_step * 1e-3) / 1e9
 
     cpu = None
-    if args.cpu_sample > 0:
+    if args.cpu_sample > 0 and K == 1:
         gpu = {"total": total, "first": g.first(), "gate_counts": g.gate_counts()}
         cpu = cpu_baseline(nl, bits, wl, min(args.cpu_sample, wl.pairs), robust, gpu)
+    elif args.cpu_sample > 0:
+        cpu = segs_cpu_baseline(nl, bits, wl, min(args.cpu_sample, wl.pairs), robust, K)
 
     learned_part = None
-    if args.learned and wl.name == "single-50k-gates":
+    if args.learned and wl.name == "single-50k-gates" and K == 1:
         learned_part = [learned_vs_plain(robust), learned_vs_plain(not robust)]
 
     out = {
         "metric": ("vector-pairs graded/s at 50k gates; time-to-95%-coverage learned vs plain"
-                   if wl.name == "single-50k-gates" else f"vector-pairs graded/s ({wl.name})"),
+                   if wl.name == "single-50k-gates" and K == 1 else
+                   f"vector-pairs graded/s ({wl.name}{f', segments of length {K}' if K > 1 else ''})"),
         "value": pairs_per_s,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -298,13 +324,15 @@ def main():
         "data": "synthetic (seeded netlist and vectors)",
         "config": {"workload": wl.name, "gates": info["n"] - I, "levels": info["levels"], "pis": I,
                    "pos": info["num_po"], "pairs": wl.pairs, "chunk": wl.chunk,
-                   "mode": "robust" if robust else "non-robust", "total_paths": total, "tuning": tune},
+                   "mode": "robust" if robust else "non-robust", "total_paths": total, "tuning": tune,
+                   "segs": K, "segments": num_segs},
         "gate_evals_per_s": pairs_per_s * (n - I),
         "roofline": {"bound": "hbm", "kernel": f"k_{fam}", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name, fam),
                      "avg_launch_us": avg_ms * 1e3, "launches_per_step": launches},
         "end_to_end": {"algorithmic_GBps": e2e_gbs, "frac": e2e_gbs / HBM_PEAK_GBS},
-        "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "wall_ms")},
+        "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "merge_ms",
+                                         "wall_ms")},
         "cpu_baseline": cpu,
         "learned_vs_plain": learned_part,
     }

@@ -763,9 +763,10 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
 // skipped) is detected at a pair when every node of it is marked there; its
 // earliest detection with node 0 falling (T0) and rising (T1) is min-combined
 // into sf0 / sf1 (chunks run in order, one writer per segment).  One wave per
-// segment, lanes over the chunk's words 64 at a time; a block of 64 words is
-// skipped when the AND of the nodes' mark-present words is zero, and the scan
-// stops once both polarities are found.
+// segment: lane b first ANDs the nodes' mark-present words of 64-word block
+// b (one round of independent loads), then the wave visits only the blocks
+// where every node has marks, in order, lanes over the block's 64 words,
+// until both polarities are found.
 template <bool SEQ>
 __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__restrict__ seg, uint32_t K,
                                                    uint64_t S, uint32_t start, uint32_t *__restrict__ sf0,
@@ -778,32 +779,43 @@ __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__
 #pragma unroll
   for (int i = 0; i < 8; ++i) gid[i] = __builtin_amdgcn_readfirstlane(__shfl(mine, i));
   uint32_t best0 = PDF_NONE, best1 = PDF_NONE;
-  for (uint32_t wb = 0; wb < pl.words && (best0 == PDF_NONE || best1 == PDF_NONE); wb += 64) {
-    uint64_t nz = ~0ULL;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-      if (gid[i] != PDF_NONE) nz &= pl.NZ[(uint64_t)gid[i] * pl.ntiles + (wb >> 6)];
-    if (!nz) continue;
-    const uint32_t w = wb + lane;
-    uint64_t m = 0;
-    if (w < pl.words && ((nz >> lane) & 1)) {
-      m = ~0ULL;
+  const uint32_t nblk = (pl.words + 63) >> 6;
+  for (uint32_t g0 = 0; g0 < nblk && (best0 == PDF_NONE || best1 == PDF_NONE); g0 += 64) {
+    const uint32_t blk = g0 + lane;
+    uint64_t nzl = 0;
+    if (blk < nblk) {
+      nzl = ~0ULL;
 #pragma unroll
       for (int i = 0; i < 8; ++i)
-        if (gid[i] != PDF_NONE) m &= pl.M[(uint64_t)gid[i] * pl.pitch + w];
+        if (gid[i] != PDF_NONE) nzl &= pl.NZ[(uint64_t)gid[i] * pl.ntiles + blk];
     }
-    uint64_t a = 0, b = 0;
-    if (m) frames<SEQ>(pl, gid[0], w, a, b);
-    const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
-    const uint32_t base = start + 64u * w;
-    const uint32_t c0 = t0 ? base + (uint32_t)__builtin_ctzll(t0) : PDF_NONE;
-    const uint32_t c1 = t1 ? base + (uint32_t)__builtin_ctzll(t1) : PDF_NONE;
-    const uint64_t b0 = __ballot(t0 != 0), b1 = __ballot(t1 != 0);
-    // the lowest lane holds the lowest word, hence the earliest pair
-    const uint32_t e0 = __shfl(c0, b0 ? (int)__builtin_ctzll(b0) : 0);
-    const uint32_t e1 = __shfl(c1, b1 ? (int)__builtin_ctzll(b1) : 0);
-    if (best0 == PDF_NONE && b0) best0 = e0;
-    if (best1 == PDF_NONE && b1) best1 = e1;
+    uint64_t todo = __ballot(nzl != 0);
+    while (todo && (best0 == PDF_NONE || best1 == PDF_NONE)) {
+      const int l = (int)__builtin_ctzll(todo);
+      todo &= todo - 1;
+      const uint64_t nz = ((uint64_t)(uint32_t)__shfl((uint32_t)(nzl >> 32), l) << 32) |
+                          (uint32_t)__shfl((uint32_t)nzl, l);
+      const uint32_t w = (g0 + (uint32_t)l) * 64 + lane;
+      uint64_t m = 0;
+      if (w < pl.words && ((nz >> lane) & 1)) {
+        m = ~0ULL;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (gid[i] != PDF_NONE) m &= pl.M[(uint64_t)gid[i] * pl.pitch + w];
+      }
+      uint64_t a = 0, b = 0;
+      if (m) frames<SEQ>(pl, gid[0], w, a, b);
+      const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
+      const uint32_t base = start + 64u * w;
+      const uint32_t c0 = t0 ? base + (uint32_t)__builtin_ctzll(t0) : PDF_NONE;
+      const uint32_t c1 = t1 ? base + (uint32_t)__builtin_ctzll(t1) : PDF_NONE;
+      const uint64_t b0 = __ballot(t0 != 0), b1 = __ballot(t1 != 0);
+      // the lowest lane holds the lowest word, hence the earliest pair
+      const uint32_t e0 = __shfl(c0, b0 ? (int)__builtin_ctzll(b0) : 0);
+      const uint32_t e1 = __shfl(c1, b1 ? (int)__builtin_ctzll(b1) : 0);
+      if (best0 == PDF_NONE && b0) best0 = e0;
+      if (best1 == PDF_NONE && b1) best1 = e1;
+    }
   }
   if (lane == 0) {
     if (best0 < sf0[s]) sf0[s] = best0;
@@ -834,6 +846,25 @@ __global__ void k_seg_events(const uint32_t *__restrict__ seg, uint32_t K, uint6
     }
     const uint64_t old = atomicOr((unsigned long long *)&actmap[rel >> 6], (unsigned long long)bit);
     if (!(old & bit)) act_list[atomicAdd(act_count, 1u)] = rel;
+  }
+}
+
+// The new-node plane back to zero after the cover: the words k_seg_events
+// set for this chunk (the segment firsts have not changed since).
+__global__ void k_seg_clear(const uint32_t *__restrict__ seg, uint32_t K, uint64_t S,
+                            const uint32_t *__restrict__ sf0, const uint32_t *__restrict__ sf1, uint32_t start,
+                            uint32_t width, uint64_t *__restrict__ newp, uint64_t pitch) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint32_t x[2] = {sf0[s], sf1[s]};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (x[k] == PDF_NONE || x[k] < start || x[k] - start >= width) continue;
+    const uint32_t rel = x[k] - start;
+    for (uint32_t i = 0; i < K; ++i) {
+      const uint32_t g = seg[s * K + i];
+      if (g != PDF_NONE) newp[(uint64_t)g * pitch + (rel >> 6)] = 0;
+    }
   }
 }
 
@@ -1257,6 +1288,7 @@ struct pdf_grader {
   uint32_t *d_seg = nullptr, *d_sfirst0 = nullptr, *d_sfirst1 = nullptr;
   uint64_t *d_NEW = nullptr;
   bool seg_stale = true;  // reset the segment firsts before the next merge
+  uint32_t grade_segs = 1;  // segment length of the fused grade being queued
   uint64_t *d_gate = nullptr;
   uint64_t *d_grand = nullptr;  // exact total over all chunks
   uint32_t *d_err = nullptr;
@@ -1304,7 +1336,7 @@ struct pdf_grader {
   size_t rk_pairs = 0, rk_vsize = 0, rk_temp_bytes = 0;
   // captured fused grade (pdf_grade), rebuilt when its shape changes
   hipGraphExec_t graph = nullptr;
-  int graph_robust = -1;
+  int graph_robust = -1;  // robust | segs << 1 of the captured grade
   // call-order tracking for the stage API
   int64_t stage_chunk = -1;
   int stage = 0;  // 1 = simulated, 2 = marked, 3 = merged, 4 = covered
@@ -1346,7 +1378,7 @@ void dfree(T *&p) {
 
 uint32_t grid_for(uint64_t items, uint32_t block) { return (uint32_t)((items + block - 1) / block); }
 
-enum Family { kFwd = 0, kRev = 1, kEvents = 2, kCover = 3, kFill = 4 };
+enum Family { kFwd = 0, kRev = 1, kEvents = 2, kCover = 3, kFill = 4, kSegMerge = 5 };
 
 // With g->timing set, each run of consecutive launches of one kernel family
 // (e.g. the forward pass's level launches) is bracketed by a pair of pooled
@@ -1517,14 +1549,10 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_t
 }
 
 // Segments of length K >= 2: the chunk's new-node plane from the segment
-// firsts, then the cover DP reading it.
+// firsts, the cover DP reading it, and the plane's words back to zero (it is
+// all zero between chunks).
 int launch_cover_segs(pdf_grader *g, uint64_t start, uint64_t width) {
   const Planes pl = planes(g, width);
-  const uint64_t nw = (uint64_t)g->nl.n * g->pitch * 2;  // u32 words of the plane
-  tm_begin(g, kFill);
-  k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nw, 256), 65536), 256, 0, g->stream>>>(
-      reinterpret_cast<uint32_t *>(g->d_NEW), nw, 0u);
-  tm_end(g);
   tm_begin(g, kEvents);
   k_seg_events<<<grid_for(g->seg_count, 256), 256, 0, g->stream>>>(
       g->d_seg, g->seg_k, g->seg_count, g->d_sfirst0, g->d_sfirst1, (uint32_t)start, (uint32_t)width, g->d_NEW,
@@ -1536,8 +1564,22 @@ int launch_cover_segs(pdf_grader *g, uint64_t start, uint64_t width) {
       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total, g->d_grand, g->d_err, (uint32_t)width, nullptr,
       g->d_NEW);
   tm_end(g);
+  tm_begin(g, kEvents);
+  k_seg_clear<<<grid_for(g->seg_count, 256), 256, 0, g->stream>>>(g->d_seg, g->seg_k, g->seg_count, g->d_sfirst0,
+                                                                  g->d_sfirst1, (uint32_t)start, (uint32_t)width,
+                                                                  g->d_NEW, g->pitch);
+  tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
+}
+
+void drop_graph(pdf_grader *g);
+
+// After a failed launch the new-node plane may not be all zero: free it (and
+// any captured grade holding its address); the next use allocates it zeroed.
+void drop_new_plane(pdf_grader *g) {
+  drop_graph(g);
+  dfree(g->d_NEW);
 }
 
 constexpr uint64_t kMaxSegments = 1ULL << 27;  // segment list entries kept in HBM: 4 GiB at K = 8
@@ -1554,6 +1596,7 @@ void free_segments(pdf_grader *g) {
 // first-pair arrays and the new-node plane.
 int ensure_segments(pdf_grader *g, uint32_t K) {
   if (g->seg_k != K) {
+    drop_graph(g);  // a captured grade holds the old list's addresses
     free_segments(g);
     const uint64_t S = count_segments(g->nl, K);
     if (S > kMaxSegments) {
@@ -1573,7 +1616,14 @@ int ensure_segments(pdf_grader *g, uint32_t K) {
     g->seg_count = S;
     g->seg_stale = true;
   }
-  if (!g->d_NEW) return dalloc(&g->d_NEW, (uint64_t)g->nl.n * g->pitch);
+  if (!g->d_NEW) {
+    // zero once here; every cover leaves it zero again
+    const uint64_t words = (uint64_t)g->nl.n * g->pitch;
+    int rc = dalloc(&g->d_NEW, words);
+    if (!rc && !hip_ok(hipMemset(g->d_NEW, 0, words * sizeof(uint64_t)), "hipMemset")) rc = PDF_ERR_HIP;
+    if (rc) dfree(g->d_NEW);
+    return rc;
+  }
   return PDF_OK;
 }
 
@@ -1584,15 +1634,25 @@ void launch_merge_seg_t(pdf_grader *g, uint64_t start, uint64_t width) {
                                                                    (uint32_t)start, g->d_sfirst0, g->d_sfirst1);
 }
 
+// Segment firsts to PDF_NONE (queued; a kernel, so it may be graph-captured).
+int queue_seg_reset(pdf_grader *g) {
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(grid_for(g->seg_count, 256), 65536);
+  tm_begin(g, kFill);
+  k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst0, g->seg_count, PDF_NONE);
+  k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst1, g->seg_count, PDF_NONE);
+  tm_end(g);
+  HIPCHK(hipGetLastError());
+  g->seg_stale = false;
+  return PDF_OK;
+}
+
 int launch_merge_seg(pdf_grader *g, uint64_t start, uint64_t width) {
-  if (g->seg_stale) {
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>(grid_for(g->seg_count, 256), 65536);
-    k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst0, g->seg_count, PDF_NONE);
-    k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst1, g->seg_count, PDF_NONE);
-    g->seg_stale = false;
-  }
+  int rc;
+  if (g->seg_stale && (rc = queue_seg_reset(g))) return rc;
+  tm_begin(g, kSegMerge);
   if (g->seq) launch_merge_seg_t<true>(g, start, width);
   else launch_merge_seg_t<false>(g, start, width);
+  tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
 }
@@ -2062,7 +2122,10 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
   if ((uint32_t)segs != g->merged_segs) { set_error("pdf_cover: segment length differs from the merge's"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
   EventTimer tm(g, ms);
-  if ((rc = segs >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width))) return rc;
+  if ((rc = segs >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width))) {
+    if (segs >= 2) drop_new_plane(g);
+    return rc;
+  }
   if ((rc = tm.finish())) return rc;
   uint64_t tot = 0;
   uint32_t err = 0;
@@ -2217,10 +2280,15 @@ static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
   return rc;
 }
 
+// One lane; with segments of length K >= 2 (g->grade_segs) each chunk's
+// reverse pass is followed by the segment merge and the cover reads the
+// new-node plane built from the segment firsts.
 static int queue_grade(pdf_grader *g, int robust) {
-  if (!g->lanes.empty() && !g->timing) return queue_grade_lanes(g, robust);
+  const uint32_t K = g->grade_segs;
+  if (!g->lanes.empty() && !g->timing && K == 1) return queue_grade_lanes(g, robust);
   int rc = queue_reset(g);
   if (rc) return rc;
+  if (K >= 2 && (rc = queue_seg_reset(g))) return rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   for (uint64_t c = 0; c < nchunks; ++c) {
     const uint64_t start = c * g->chunk_pairs;
@@ -2228,7 +2296,12 @@ static int queue_grade(pdf_grader *g, int robust) {
     if ((rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true))) return rc;
     if (!g->seq && (rc = reset_chunk(g))) return rc;
     if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
-    if ((rc = launch_cover(g, start, width))) return rc;
+    if (K >= 2) {
+      if ((rc = launch_merge_seg(g, start, width))) return rc;
+      if ((rc = launch_cover_segs(g, start, width))) return rc;
+    } else if ((rc = launch_cover(g, start, width))) {
+      return rc;
+    }
   }
   return PDF_OK;
 }
@@ -2240,12 +2313,13 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   int rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   g->seg_stale = true;  // graph replays reset the node state without queue_reset
-  if (g->tune_use_graph && !g->timing && !g->lanes.empty()) {
+  const int key = (robust != 0) | (int)(g->grade_segs << 1);
+  if (g->tune_use_graph && !g->timing && !g->lanes.empty() && g->grade_segs == 1) {
     if ((rc = queue_grade_lane_graphs(g, robust))) return rc;
   } else if (g->tune_use_graph && !g->timing) {
-    // the launch sequence depends only on the chunk shape and the robust
-    // flag, so it is captured once and replayed
-    if (!g->graph || g->graph_robust != (robust != 0)) {
+    // the launch sequence depends only on the chunk shape, the robust flag
+    // and the segment length, so it is captured once and replayed
+    if (!g->graph || g->graph_robust != key) {
       drop_graph(g);
       hipGraph_t graph = nullptr;
       HIPCHK(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
@@ -2259,7 +2333,7 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
       const hipError_t ie = hipGraphInstantiate(&g->graph, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ie != hipSuccess) { g->graph = nullptr; HIPCHK(ie); }
-      g->graph_robust = robust != 0;
+      g->graph_robust = key;
     }
     HIPCHK(hipGraphLaunch(g->graph, g->stream));
   } else {
@@ -2274,6 +2348,7 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
+  if (g->grade_segs >= 2) g->seg_stale = false;  // the grade reset and filled them
   if (err) { set_error("pdf_grade: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   *total = tot;
   return PDF_OK;
@@ -2306,9 +2381,7 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
   return PDF_OK;
 }
 
-int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times) {
-  if (!g || !total || !times) { set_error("null argument"); return PDF_ERR_ARG; }
-  HIPCHK(hipSetDevice(g->device));
+static int grade_timed_impl(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times) {
   g->ev_family.clear();
   g->ev_count.clear();
   g->tm_open = false;
@@ -2319,9 +2392,10 @@ int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(g->stream));
   std::memset(times, 0, sizeof(*times));
-  double *ms[5] = {&times->fwd_ms, &times->reverse_ms, &times->events_ms, &times->cover_ms, &times->fill_ms};
-  uint32_t *cnt[5] = {&times->fwd_launches, &times->reverse_launches, &times->events_launches,
-                      &times->cover_launches, &times->fill_launches};
+  double *ms[6] = {&times->fwd_ms, &times->reverse_ms, &times->events_ms, &times->cover_ms, &times->fill_ms,
+                   &times->merge_ms};
+  uint32_t *cnt[6] = {&times->fwd_launches, &times->reverse_launches, &times->events_launches,
+                      &times->cover_launches, &times->fill_launches, &times->merge_launches};
   for (size_t i = 0; i < g->ev_family.size(); ++i) {
     float t = 0;
     HIPCHK(hipEventElapsedTime(&t, g->ev_pool[2 * i], g->ev_pool[2 * i + 1]));
@@ -2334,6 +2408,27 @@ int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times
     times->wall_ms = t;
   }
   return PDF_OK;
+}
+
+int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times) {
+  if (!g || !total || !times) { set_error("null argument"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  return grade_timed_impl(g, robust, total, times);
+}
+
+int pdf_grade_segs(pdf_grader *g, int robust, int segs, uint64_t *total, pdf_kernel_times *times) {
+  if (!g || !total) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (segs < 1 || segs > 8) { set_error("pdf_grade_segs: segment length must be 1..8"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_grade_segs: load vectors first"); return PDF_ERR_STATE; }
+  HIPCHK(hipSetDevice(g->device));
+  int rc;
+  if (segs >= 2 && (rc = ensure_segments(g, (uint32_t)segs))) return rc;
+  g->grade_segs = (uint32_t)segs;
+  rc = times ? grade_timed_impl(g, robust, total, times) : grade_impl(g, robust, total);
+  g->grade_segs = 1;
+  if (rc && rc != PDF_ERR_OVERFLOW && segs >= 2) drop_new_plane(g);
+  if (!rc) g->merged_segs = (uint32_t)segs;
+  return rc;
 }
 
 int pdf_get_sim_planes(const pdf_grader *gc, uint64_t *a_plane, uint64_t *b_plane) {

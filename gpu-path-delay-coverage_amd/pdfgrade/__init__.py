@@ -69,7 +69,8 @@ class KernelTimes(C.Structure):
     _fields_ = [("fwd_ms", C.c_double), ("reverse_ms", C.c_double), ("events_ms", C.c_double),
                 ("cover_ms", C.c_double), ("fill_ms", C.c_double), ("fwd_launches", C.c_uint32),
                 ("reverse_launches", C.c_uint32), ("events_launches", C.c_uint32),
-                ("cover_launches", C.c_uint32), ("fill_launches", C.c_uint32), ("wall_ms", C.c_double)]
+                ("cover_launches", C.c_uint32), ("fill_launches", C.c_uint32), ("wall_ms", C.c_double),
+                ("merge_ms", C.c_double), ("merge_launches", C.c_uint32)]
 
 
 # exported symbols of include/pdfgrade.h with their argument types
@@ -108,6 +109,7 @@ SIGNATURES = {
     "pdf_cover": ([_P, _U64, _U64, C.c_int, _U64P, _FP], C.c_int),
     "pdf_grade": ([_P, C.c_int, _U64P], C.c_int),
     "pdf_grade_timed": ([_P, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
+    "pdf_grade_segs": ([_P, C.c_int, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
     "pdf_grader_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
     "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
@@ -419,6 +421,14 @@ class Grader:
         t = KernelTimes()
         _check(lib().pdf_grade_timed(self._h, int(robust), C.byref(tot), C.byref(t)))
         return tot.value, {k: getattr(t, k) for k, _ in KernelTimes._fields_}
+
+    def grade_segs(self, robust: bool, segs: int, timed: bool = False):
+        """pdf_grade_segs: the fused grade with segments of length segs;
+        returns the total, or (total, times dict) when timed."""
+        tot = C.c_uint64()
+        t = KernelTimes() if timed else None
+        _check(lib().pdf_grade_segs(self._h, int(robust), segs, C.byref(tot), C.byref(t) if timed else None))
+        return (tot.value, {k: getattr(t, k) for k, _ in KernelTimes._fields_}) if timed else tot.value
 
     def grade_stages(self, robust: bool, segs: int = 1):
         """The reference's chunk loop (main.cu:171-231) through the four stage

@@ -250,8 +250,19 @@ typedef struct pdf_kernel_times {
   double fwd_ms, reverse_ms, events_ms, cover_ms, fill_ms; /* summed durations */
   uint32_t fwd_launches, reverse_launches, events_launches, cover_launches, fill_launches;
   double wall_ms; /* first to last event */
+  double merge_ms;         /* segment merge (segs >= 2) */
+  uint32_t merge_launches;
 } pdf_kernel_times;
 int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times *times);
+
+/* pdf_grade with segments of length segs (1..8): per chunk the fused forward
+ * and reverse passes, then for segs >= 2 the segment merge and the cover
+ * over the nodes of newly detected segments (main.cu:185-223 with
+ * gpuMergeSegments<K> / gpuCountPaths<K>); one lane, graph replay as
+ * PDF_TUNE_USE_GRAPH says.  Results equal the four stage calls per chunk
+ * with the same segs.  times (may be NULL) receives per-family kernel times
+ * as pdf_grade_timed does (slower, serialized launches). */
+int pdf_grade_segs(pdf_grader *g, int robust, int segs, uint64_t *total, pdf_kernel_times *times);
 
 /* Parity getters (the reference's --verbose dumps, simkernel.cuh:250-295;
  * markkernel.cuh:357-395; mergekernel.cuh:25-47; coverkernel.cuh:44-66).

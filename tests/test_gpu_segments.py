@@ -169,3 +169,69 @@ def test_fcount_cli_segments(tmp_path):
         assert len(log) == want["segments"].shape[0]
         first = log[0].split(": ")[1].split(",")
         assert int(first[0]) == np.int32(want["seg_t0"][0].view(np.int32))
+
+
+def _fused(g, robust, K):
+    f0, f1 = g.first()
+    s0, s1 = g.segment_first()
+    return dict(total=g.total(), gate_counts=g.gate_counts(), seg_t0=s0, seg_t1=s1, first_t0=f0, first_t1=f1)
+
+
+def test_fused_grade_segs():
+    """pdf_grade_segs equals the oracle (and the stage calls) for K = 2..4 with
+    graph replay on and off, the timed variant, lanes configured, repeated
+    replays, and pdf_grade (K = 1) afterwards."""
+    say("fused segments")
+    cfg = pg.GenConfig(seed=33, num_pi=40, num_po=24, num_gates=1500, levels=16, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=16)
+    nl = pg.Netlist.generate(cfg)
+    P = 1536
+    bits = biased_vectors(33, 40, P, p_hold=0.7)
+    net = orc.OracleNet(nl.arrays())
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_tuning(pg.TUNE_LANES, 2)
+        g.set_chunk(512)
+        for K in (2, 3, 4):
+            for robust in (True, False):
+                ref = orc.grade_segs(net, bits, P, 512, robust, K)
+                for graph in (1, 0):
+                    g.set_tuning(pg.TUNE_USE_GRAPH, graph)
+                    for rep in range(2):
+                        tot = g.grade_segs(robust, K)
+                        got = _fused(g, robust, K)
+                        ctx = f"K {K} robust {robust} graph {graph} rep {rep}"
+                        assert tot == got["total"] == ref["total"], ctx
+                        assert np.array_equal(got["gate_counts"], ref["gate_counts"]), ctx
+                        assert np.array_equal(got["seg_t0"], ref["seg_t0"]), ctx
+                        assert np.array_equal(got["seg_t1"], ref["seg_t1"]), ctx
+                tot, t = g.grade_segs(robust, K, timed=True)
+                assert tot == ref["total"] and t["merge_launches"] == 3 and t["merge_ms"] > 0
+        g.set_tuning(pg.TUNE_USE_GRAPH, 1)
+        node = orc.grade(net, bits, P, 512, True)
+        assert g.grade(True) == node["total"]
+        assert np.array_equal(g.gate_counts(), node["gate_counts"])
+        assert g.grade_segs(True, 1) == node["total"]
+        with pytest.raises(pg.PdfError):
+            g.grade_segs(True, 9)
+
+
+def test_50k_gates_fused_segments_equal_stages():
+    """All 65,536 pairs of the 50k-gate workload, K = 2 and 3, non-robust:
+    the fused grade (32,768-pair chunks) equals the stage calls (8,192-pair
+    chunks)."""
+    say("fused segments 50k")
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = GATES_50K.pairs
+    bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
+    for K in (2, 3):
+        st = gpu_stages(nl, bits, P, 8192, False, K)
+        with pg.Grader(nl) as g:
+            g.set_vectors(bits, P)
+            g.set_chunk(32768)
+            for rep in range(2):
+                tot = g.grade_segs(False, K)
+                got = _fused(g, False, K)
+                assert tot == st["total"], (K, rep)
+                for k in ("seg_t0", "seg_t1", "gate_counts"):
+                    assert np.array_equal(got[k], st[k]), (K, rep, k)

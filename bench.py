@@ -75,6 +75,20 @@ def host_threads() -> int:
     return max(1, min(cores, env) if env > 0 else cores)
 
 
+def host_cpu() -> dict:
+    """The host's processor count and model (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0)), "model": model}
+
+
 def serial_oracle_rate(net, bits, pairs, sample, robust):
     """Serial byte-per-pattern C oracle (one core) on the first `sample` pairs."""
     import numpy as np
@@ -117,7 +131,7 @@ def cpu_baseline(nl, bits, wl, sample, robust, gpu):
     return {"value": rate[threads], "unit": "pairs/s", "cores": threads, "kind": "port",
             "sample": (f"{what} pairs of the {wl.name} netlist, chunk {chunk}, bit-sliced C grader "
                        f"(oracle/bitsliced.c), {threads} OpenMP threads"),
-            "one_core": rate[1], "total_paths": r["total"],
+            "one_core": rate[1], "total_paths": r["total"], "host": host_cpu(),
             "matches_gpu": (r["total"] == gpu["total"] and np.array_equal(r["first_t0"], gpu["first"][0])
                             and np.array_equal(r["first_t1"], gpu["first"][1])
                             and np.array_equal(r["gate_counts"], gpu["gate_counts"])) if whole else None,
@@ -135,7 +149,7 @@ def segs_cpu_baseline(nl, bits, wl, sample, robust, K):
     orc.grade_segs(net, bits, sample, sample, robust, K)
     return {"value": sample / (time.perf_counter() - t0), "unit": "pairs/s", "cores": 1, "kind": "port",
             "sample": f"first {sample} of {wl.pairs} pairs of the {wl.name} netlist, one chunk, serial C oracle "
-                      f"with segments of length {K}"}
+                      f"with segments of length {K}", "host": host_cpu()}
 
 
 def bench_batch(args, wl):

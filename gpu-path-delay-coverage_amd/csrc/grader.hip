@@ -762,40 +762,56 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
 // mergekernel.cuh:91-198): segment s ([K] node ids, PDF_NONE entries
 // skipped) is detected at a pair when every node of it is marked there; its
 // earliest detection with node 0 falling (T0) and rising (T1) is min-combined
-// into sf0 / sf1 (chunks run in order, one writer per segment).  One wave per
-// segment: lane b first ANDs the nodes' mark-present words of 64-word block
-// b (one round of independent loads), then the wave visits only the blocks
-// where every node has marks, in order, lanes over the block's 64 words,
-// until both polarities are found.
+// into sf0 / sf1 (chunks run in order, one writer per segment).
+// A wave holds 64 >> lps_log segments, 1 << lps_log lanes each (at least the
+// chunk's number of 64-word blocks, or 64).  Phase 1: lane `sub` of a
+// segment ANDs the nodes' mark-present words of block `sub` (one round of
+// independent loads).  Phase 2: the whole wave visits each (segment, block)
+// whose nodes all have marks, in lane order (so a segment's blocks in
+// ascending order), lanes over the block's 64 words, and drops a segment's
+// remaining blocks once both polarities are found.
 template <bool SEQ>
 __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__restrict__ seg, uint32_t K,
-                                                   uint64_t S, uint32_t start, uint32_t *__restrict__ sf0,
-                                                   uint32_t *__restrict__ sf1) {
-  const uint64_t s = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (s >= S) return;  // whole waves
+                                                   uint64_t S, uint32_t lps_log, uint32_t start,
+                                                   uint32_t *__restrict__ sf0, uint32_t *__restrict__ sf1) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t mine = lane < K ? seg[s * K + lane] : PDF_NONE;
-  uint32_t gid[8];
+  const uint32_t lps = 1u << lps_log;
+  const uint64_t s_base = ((uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) << (6 - lps_log);
+  if (s_base >= S) return;  // whole waves
+  const uint32_t slot = lane >> lps_log, sub = lane & (lps - 1);
+  const uint64_t s = s_base + slot;
+  const bool valid = s < S;
+  uint32_t ids[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) gid[i] = __builtin_amdgcn_readfirstlane(__shfl(mine, i));
-  uint32_t best0 = PDF_NONE, best1 = PDF_NONE;
+  for (int i = 0; i < 8; ++i) ids[i] = (valid && (uint32_t)i < K) ? seg[s * K + i] : PDF_NONE;
+  const uint64_t slot_mask = lps == 64 ? ~0ULL : ((1ULL << lps) - 1);
   const uint32_t nblk = (pl.words + 63) >> 6;
-  for (uint32_t g0 = 0; g0 < nblk && (best0 == PDF_NONE || best1 == PDF_NONE); g0 += 64) {
-    const uint32_t blk = g0 + lane;
+  uint32_t best0 = PDF_NONE, best1 = PDF_NONE;  // this lane's segment
+  for (uint32_t g0 = 0; g0 < nblk; g0 += lps) {
+    const uint32_t blk = g0 + sub;
     uint64_t nzl = 0;
-    if (blk < nblk) {
+    if (valid && blk < nblk && (best0 == PDF_NONE || best1 == PDF_NONE)) {
       nzl = ~0ULL;
 #pragma unroll
       for (int i = 0; i < 8; ++i)
-        if (gid[i] != PDF_NONE) nzl &= pl.NZ[(uint64_t)gid[i] * pl.ntiles + blk];
+        if (ids[i] != PDF_NONE) nzl &= pl.NZ[(uint64_t)ids[i] * pl.ntiles + blk];
     }
     uint64_t todo = __ballot(nzl != 0);
-    while (todo && (best0 == PDF_NONE || best1 == PDF_NONE)) {
+    while (todo) {
       const int l = (int)__builtin_ctzll(todo);
+      const uint32_t tslot = (uint32_t)l >> lps_log;
+      const uint32_t f0 = __shfl(best0, l), f1 = __shfl(best1, l);
+      if (f0 != PDF_NONE && f1 != PDF_NONE) {  // this segment is done
+        todo &= ~(slot_mask << (tslot << lps_log));
+        continue;
+      }
       todo &= todo - 1;
       const uint64_t nz = ((uint64_t)(uint32_t)__shfl((uint32_t)(nzl >> 32), l) << 32) |
                           (uint32_t)__shfl((uint32_t)nzl, l);
-      const uint32_t w = (g0 + (uint32_t)l) * 64 + lane;
+      uint32_t gid[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) gid[i] = __builtin_amdgcn_readfirstlane(__shfl(ids[i], l));
+      const uint32_t w = (g0 + ((uint32_t)l & (lps - 1))) * 64 + lane;
       uint64_t m = 0;
       if (w < pl.words && ((nz >> lane) & 1)) {
         m = ~0ULL;
@@ -813,11 +829,13 @@ __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__
       // the lowest lane holds the lowest word, hence the earliest pair
       const uint32_t e0 = __shfl(c0, b0 ? (int)__builtin_ctzll(b0) : 0);
       const uint32_t e1 = __shfl(c1, b1 ? (int)__builtin_ctzll(b1) : 0);
-      if (best0 == PDF_NONE && b0) best0 = e0;
-      if (best1 == PDF_NONE && b1) best1 = e1;
+      if (slot == tslot) {
+        if (best0 == PDF_NONE && b0) best0 = e0;
+        if (best1 == PDF_NONE && b1) best1 = e1;
+      }
     }
   }
-  if (lane == 0) {
+  if (valid && sub == 0) {
     if (best0 < sf0[s]) sf0[s] = best0;
     if (best1 < sf1[s]) sf1[s] = best1;
   }
@@ -1630,8 +1648,13 @@ int ensure_segments(pdf_grader *g, uint32_t K) {
 template <bool SEQ>
 void launch_merge_seg_t(pdf_grader *g, uint64_t start, uint64_t width) {
   const Planes pl = planes(g, width);
-  k_merge_seg<SEQ><<<grid_for(g->seg_count, 4), 256, 0, g->stream>>>(pl, g->d_seg, g->seg_k, g->seg_count,
-                                                                   (uint32_t)start, g->d_sfirst0, g->d_sfirst1);
+  // lanes per segment: the chunk's 64-word blocks rounded up to a power of two, at most 64
+  const uint32_t nblk = (pl.words + 63) / 64;
+  uint32_t lps_log = 0;
+  while (lps_log < 6 && (1u << lps_log) < nblk) ++lps_log;
+  const uint64_t waves = (g->seg_count + (64u >> lps_log) - 1) >> (6 - lps_log);
+  k_merge_seg<SEQ><<<grid_for(waves, 4), 256, 0, g->stream>>>(pl, g->d_seg, g->seg_k, g->seg_count, lps_log,
+                                                            (uint32_t)start, g->d_sfirst0, g->d_sfirst1);
 }
 
 // Segment firsts to PDF_NONE (queued; a kernel, so it may be graph-captured).

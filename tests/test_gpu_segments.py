@@ -14,6 +14,8 @@ from oracle import oracle as orc
 
 from helpers import biased_vectors, edge_netlist, pack, small_config, unpack
 
+NONE = 0xFFFFFFFF
+
 pytestmark = pytest.mark.gpu
 
 
@@ -235,3 +237,22 @@ def test_50k_gates_fused_segments_equal_stages():
                 assert tot == st["total"], (K, rep)
                 for k in ("seg_t0", "seg_t1", "gate_counts"):
                     assert np.array_equal(got[k], st[k]), (K, rep, k)
+
+
+def test_segments_one_long_chunk():
+    """300,000 pairs in one chunk (74 blocks of 64 words: the merge kernel's
+    segment spans two groups of 64 blocks), vectors constant for the first
+    270,000 pairs so that every detection falls in the second group."""
+    say("segments long chunk")
+    cfg = pg.GenConfig(seed=41, num_pi=24, num_po=16, num_gates=300, levels=10, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=16)
+    nl = pg.Netlist.generate(cfg)
+    P = 300_000
+    vals = unpack(biased_vectors(41, 24, P, p_hold=0.7), P)
+    vals[:270_000] = 0
+    bits = pack(vals)
+    for K, robust in ((2, True), (3, False)):
+        check(nl, bits, P, 0, robust, K, ctx=f"K {K}")
+        seg = orc.grade_segs(orc.OracleNet(nl.arrays()), bits, P, P, robust, K)
+        hit = seg["seg_t0"][seg["seg_t0"] != NONE]
+        assert len(hit) > 0 and hit.min() >= 270_000

@@ -272,8 +272,6 @@ def main():
                                                               "use_graph": 1, "source": "recorded"}
     K = args.segs
     grade = (lambda: g.grade(robust)) if K == 1 else (lambda: g.grade_segs(robust, K))
-    if K > 1:  # segment merges min-combine chunk by chunk: one lane
-        tune = {"lanes": 1, "chunk_pairs": wl.chunk, "use_graph": 1, "source": "one lane for segments"}
     total = None
     for _ in range(args.warmup):
         total = grade()

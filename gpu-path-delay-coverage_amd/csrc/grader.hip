@@ -762,7 +762,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
 // mergekernel.cuh:91-198): segment s ([K] node ids, PDF_NONE entries
 // skipped) is detected at a pair when every node of it is marked there; its
 // earliest detection with node 0 falling (T0) and rising (T1) is min-combined
-// into sf0 / sf1 (chunks run in order, one writer per segment).
+// into sf0 / sf1.
 // A wave holds 64 >> lps_log segments, 1 << lps_log lanes each (at least the
 // chunk's number of 64-word blocks, or 64).  Phase 1: lane `sub` of a
 // segment ANDs the nodes' mark-present words of block `sub` (one round of
@@ -835,9 +835,9 @@ __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__
       }
     }
   }
-  if (valid && sub == 0) {
-    if (best0 < sf0[s]) sf0[s] = best0;
-    if (best1 < sf1[s]) sf1[s] = best1;
+  if (valid && sub == 0) {  // atomic: chunks on different lanes merge concurrently
+    if (best0 != PDF_NONE) atomicMin(&sf0[s], best0);
+    if (best1 != PDF_NONE) atomicMin(&sf1[s], best1);
   }
 }
 
@@ -1337,6 +1337,7 @@ struct pdf_grader {
     uint64_t *actmap = nullptr, *chunk_total = nullptr;
     uint32_t *list_cnt = nullptr, *act_count = nullptr;
     uint64_t *cs = nullptr, *hs = nullptr;
+    uint64_t *NEW = nullptr;    // new-node plane (segments of length K >= 2)
     hipEvent_t done = nullptr;  // merges of this lane's latest chunk and all before it
   };
   std::vector<Lane> lanes;      // lanes 1..; empty for one lane
@@ -1598,6 +1599,7 @@ void drop_graph(pdf_grader *g);
 void drop_new_plane(pdf_grader *g) {
   drop_graph(g);
   dfree(g->d_NEW);
+  for (auto &l : g->lanes) dfree(l.NEW);
 }
 
 constexpr uint64_t kMaxSegments = 1ULL << 27;  // segment list entries kept in HBM: 4 GiB at K = 8
@@ -1634,15 +1636,19 @@ int ensure_segments(pdf_grader *g, uint32_t K) {
     g->seg_count = S;
     g->seg_stale = true;
   }
-  if (!g->d_NEW) {
-    // zero once here; every cover leaves it zero again
-    const uint64_t words = (uint64_t)g->nl.n * g->pitch;
-    int rc = dalloc(&g->d_NEW, words);
-    if (!rc && !hip_ok(hipMemset(g->d_NEW, 0, words * sizeof(uint64_t)), "hipMemset")) rc = PDF_ERR_HIP;
-    if (rc) dfree(g->d_NEW);
+  // one new-node plane per lane, zeroed once here; every cover leaves it zero again
+  const uint64_t words = (uint64_t)g->nl.n * g->pitch;
+  auto alloc_new = [&](uint64_t *&p) -> int {
+    if (p) return PDF_OK;
+    int rc = dalloc(&p, words);
+    if (!rc && !hip_ok(hipMemset(p, 0, words * sizeof(uint64_t)), "hipMemset")) rc = PDF_ERR_HIP;
+    if (rc) dfree(p);
     return rc;
-  }
-  return PDF_OK;
+  };
+  int rc = alloc_new(g->d_NEW);
+  for (auto &l : g->lanes)
+    if (!rc) rc = alloc_new(l.NEW);
+  return rc;
 }
 
 template <bool SEQ>
@@ -1718,6 +1724,7 @@ void lane_save(const pdf_grader *g, pdf_grader::Lane &l) {
   l.act_count = g->d_act_count;
   l.cs = g->d_cs;
   l.hs = g->d_hs;
+  l.NEW = g->d_NEW;
 }
 
 void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
@@ -1736,6 +1743,7 @@ void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
   g->d_act_count = l.act_count;
   g->d_cs = l.cs;
   g->d_hs = l.hs;
+  g->d_NEW = l.NEW;
 }
 
 void free_chunk_buffers(pdf_grader *g);
@@ -2172,11 +2180,13 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
 // Results equal the one-lane order.
 static int queue_grade_lanes(pdf_grader *g, int robust) {
   const uint32_t L = 1 + (uint32_t)g->lanes.size();
+  const uint32_t K = g->grade_segs;
   std::vector<pdf_grader::Lane> ln(L);
   lane_save(g, ln[0]);
   ln[0].done = g->lane0_done;
   for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
   int rc = queue_reset(g);
+  if (!rc && K >= 2) rc = queue_seg_reset(g);
   if (rc) return rc;
   HIPCHK(hipEventRecord(g->lane_start, ln[0].stream));
   for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamWaitEvent(ln[i].stream, g->lane_start, 0));
@@ -2204,13 +2214,18 @@ static int queue_grade_lanes(pdf_grader *g, int robust) {
         chunk(c, start, width);
         rc = launch_reverse<true, true>(g, start, width, k, k);
       }
+    if (K >= 2)
+      for (uint64_t c = c0; c < c1 && !rc; ++c) {
+        chunk(c, start, width);
+        rc = launch_merge_seg(g, start, width);
+      }
     for (uint64_t c = c0; c < c1 && !rc; ++c) {
       chunk(c, start, width);
       const pdf_grader::Lane &l = ln[c % L];
       if (c > 0 && !hip_ok(hipStreamWaitEvent(l.stream, ln[(c - 1) % L].done, 0), "hipStreamWaitEvent"))
         rc = PDF_ERR_HIP;
       if (!rc && !hip_ok(hipEventRecord(l.done, l.stream), "hipEventRecord")) rc = PDF_ERR_HIP;
-      if (!rc) rc = launch_cover(g, start, width);
+      if (!rc) rc = K >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width);
     }
   }
   lane_load(g, ln[0]);
@@ -2270,11 +2285,13 @@ static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
     if (rc) return rc;
   }
   const uint32_t L = 1 + (uint32_t)g->lanes.size();
+  const uint32_t K = g->grade_segs;
   std::vector<pdf_grader::Lane> ln(L);
   lane_save(g, ln[0]);
   ln[0].done = g->lane0_done;
   for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
   int rc = queue_reset(g);
+  if (!rc && K >= 2) rc = queue_seg_reset(g);
   if (rc) return rc;
   HIPCHK(hipEventRecord(g->lane_start, ln[0].stream));
   for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamWaitEvent(ln[i].stream, g->lane_start, 0));
@@ -2283,6 +2300,13 @@ static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
     const uint64_t c1 = std::min<uint64_t>(nchunks, c0 + L);
     for (uint64_t c = c0; c < c1 && !rc; ++c)
       if (!hip_ok(hipGraphLaunch(g->chunk_graphs[c], ln[c % L].stream), "hipGraphLaunch")) rc = PDF_ERR_HIP;
+    // segment merges behind each chunk's graph, on its lane (atomic, so the
+    // lanes' merges may overlap)
+    for (uint64_t c = c0; c < c1 && !rc && K >= 2; ++c) {
+      lane_load(g, ln[c % L]);
+      const uint64_t start = c * g->chunk_pairs;
+      rc = launch_merge_seg(g, start, std::min(g->chunk_pairs, g->num_pairs - start));
+    }
     for (uint64_t c = c0; c < c1 && !rc; ++c) {
       const pdf_grader::Lane &l = ln[c % L];
       lane_load(g, l);
@@ -2291,7 +2315,7 @@ static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
       if (c > 0 && !hip_ok(hipStreamWaitEvent(l.stream, ln[(c - 1) % L].done, 0), "hipStreamWaitEvent"))
         rc = PDF_ERR_HIP;
       if (!rc && !hip_ok(hipEventRecord(l.done, l.stream), "hipEventRecord")) rc = PDF_ERR_HIP;
-      if (!rc) rc = launch_cover(g, start, width);
+      if (!rc) rc = K >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width);
     }
   }
   lane_load(g, ln[0]);
@@ -2303,12 +2327,12 @@ static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
   return rc;
 }
 
-// One lane; with segments of length K >= 2 (g->grade_segs) each chunk's
-// reverse pass is followed by the segment merge and the cover reads the
-// new-node plane built from the segment firsts.
+// With segments of length K >= 2 (g->grade_segs) each chunk's reverse pass
+// is followed by the segment merge and the cover reads the new-node plane
+// built from the segment firsts.
 static int queue_grade(pdf_grader *g, int robust) {
   const uint32_t K = g->grade_segs;
-  if (!g->lanes.empty() && !g->timing && K == 1) return queue_grade_lanes(g, robust);
+  if (!g->lanes.empty() && !g->timing) return queue_grade_lanes(g, robust);
   int rc = queue_reset(g);
   if (rc) return rc;
   if (K >= 2 && (rc = queue_seg_reset(g))) return rc;
@@ -2337,7 +2361,7 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
   g->seg_stale = true;  // graph replays reset the node state without queue_reset
   const int key = (robust != 0) | (int)(g->grade_segs << 1);
-  if (g->tune_use_graph && !g->timing && !g->lanes.empty() && g->grade_segs == 1) {
+  if (g->tune_use_graph && !g->timing && !g->lanes.empty()) {
     if ((rc = queue_grade_lane_graphs(g, robust))) return rc;
   } else if (g->tune_use_graph && !g->timing) {
     // the launch sequence depends only on the chunk shape, the robust flag

@@ -258,8 +258,8 @@ int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times
 /* pdf_grade with segments of length segs (1..8): per chunk the fused forward
  * and reverse passes, then for segs >= 2 the segment merge and the cover
  * over the nodes of newly detected segments (main.cu:185-223 with
- * gpuMergeSegments<K> / gpuCountPaths<K>); one lane, graph replay as
- * PDF_TUNE_USE_GRAPH says.  Results equal the four stage calls per chunk
+ * gpuMergeSegments<K> / gpuCountPaths<K>); lanes and graph replay as
+ * PDF_TUNE_LANES / PDF_TUNE_USE_GRAPH say.  Results equal the four stage calls per chunk
  * with the same segs.  times (may be NULL) receives per-family kernel times
  * as pdf_grade_timed does (slower, serialized launches). */
 int pdf_grade_segs(pdf_grader *g, int robust, int segs, uint64_t *total, pdf_kernel_times *times);

@@ -220,23 +220,27 @@ def test_fused_grade_segs():
 
 def test_50k_gates_fused_segments_equal_stages():
     """All 65,536 pairs of the 50k-gate workload, K = 2 and 3, non-robust:
-    the fused grade (32,768-pair chunks) equals the stage calls (8,192-pair
-    chunks)."""
+    the fused grade (32,768-pair chunks on 1 and 2 lanes, graph replay on and
+    off) equals the stage calls (8,192-pair chunks)."""
     say("fused segments 50k")
     nl = pg.Netlist.generate(GATES_50K.netlist)
     P = GATES_50K.pairs
     bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
     for K in (2, 3):
         st = gpu_stages(nl, bits, P, 8192, False, K)
-        with pg.Grader(nl) as g:
-            g.set_vectors(bits, P)
-            g.set_chunk(32768)
-            for rep in range(2):
-                tot = g.grade_segs(False, K)
-                got = _fused(g, False, K)
-                assert tot == st["total"], (K, rep)
-                for k in ("seg_t0", "seg_t1", "gate_counts"):
-                    assert np.array_equal(got[k], st[k]), (K, rep, k)
+        for lanes in (1, 2):
+            with pg.Grader(nl) as g:
+                g.set_vectors(bits, P)
+                g.set_tuning(pg.TUNE_LANES, lanes)
+                g.set_chunk(32768)
+                for graph in (1, 0, 1):
+                    g.set_tuning(pg.TUNE_USE_GRAPH, graph)
+                    tot = g.grade_segs(False, K)
+                    got = _fused(g, False, K)
+                    ctx = (K, lanes, graph)
+                    assert tot == st["total"], ctx
+                    for k in ("seg_t0", "seg_t1", "gate_counts"):
+                        assert np.array_equal(got[k], st[k]), (ctx, k)
 
 
 def test_segments_one_long_chunk():

@@ -208,9 +208,28 @@ def bench_batch(args, wl):
         "per_circuit_plain_ms": plain_ms,
         "speedup_vs_per_circuit": plain_ms / (dt * 1e3),
         "end_to_end": {"algorithmic_GBps": algo_bytes / dt / 1e9, "frac": algo_bytes / dt / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": batch_cpu_baseline(nets, vecs, pairs, robust, tot) if args.cpu_sample > 0 else None,
     }
     print(json.dumps(out))
     b.close()
+
+
+def batch_cpu_baseline(nets, vecs, pairs, robust, gpu_totals, stride=8):
+    """The bit-sliced C grader on every `stride`-th circuit of the batch (each
+    whole, 16,384-pair chunks) on every host thread: pairs/s over that sample,
+    and whether its totals equal the batched GPU grade's."""
+    from oracle import oracle as orc
+
+    threads = host_threads()
+    idx = list(range(0, len(nets), stride))
+    onets = {i: orc.OracleNet(nets[i].arrays()) for i in idx}
+    t0 = time.perf_counter()
+    totals = [orc.grade_bitsliced(onets[i], vecs[i], pairs[i], 16384, robust, threads=threads)["total"] for i in idx]
+    dt = time.perf_counter() - t0
+    return {"value": sum(pairs[i] for i in idx) / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"circuits {idx[0]}, {idx[1]}, ... (every {stride}th, {len(idx)} of {len(nets)}), all their "
+                      f"pairs, bit-sliced C grader (oracle/bitsliced.c), {threads} OpenMP threads",
+            "matches_gpu": all(int(gpu_totals[i]) == t for i, t in zip(idx, totals)), "host": host_cpu()}
 
 
 def learned_vs_plain(robust: bool):

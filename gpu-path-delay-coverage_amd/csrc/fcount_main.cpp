@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -57,32 +58,75 @@ const char *code_name(int c) {
   return n[c & 3];
 }
 
-// --verbose dumps in the layout of the reference's debug files
-// (simkernel.cuh:250-295 gpusim-p2.log, markkernel.cuh:357-395 gpumark.log,
-// mergekernel.cuh:25-47 gpumerge.log); cover writes per-gate totals.
+// --verbose logs in the reference's formats (column width OUTJUST = 4,
+// main.cu:26): a "Gate:" header row on the first chunk, then one row per
+// pair, appended chunk by chunk.
+//   gpusim-p2.log    (--sim)  node codes S0/S1/T0/T1, simkernel.cuh:250-295
+//   gpumark-full.log (--mark) marks before the reverse removal, and
+//   gpumark.log      (--mark) after it, Y/N, markkernel.cuh:323-349, 357-395
+//   gpucover.log              c of every PI, coverkernel.cuh:44-66, 309-312
+//                             (exact; the reference prints the low 16 bits)
+// gpumerge.log (--merge) is written after the chunk loop (the reference
+// rewrites it at every merge, so its final content is the same).
+constexpr int kOutJust = 4;
+
+std::ofstream open_log(const char *name, bool first_chunk) {
+  return std::ofstream(name, first_chunk ? std::ios::out : std::ios::app);
+}
+
+void gate_header(std::ofstream &f, const char *label, uint32_t count) {
+  f << label << "\t";
+  for (uint32_t i = 0; i < count; ++i) f << std::setw(kOutJust) << i << " ";
+  f << "\n";
+}
+
+void dump_marks(const char *name, const std::vector<uint64_t> &m, uint32_t n, uint64_t width, uint64_t start,
+                bool first_chunk) {
+  const uint64_t words = (width + 63) / 64;
+  std::ofstream f = open_log(name, first_chunk);
+  if (first_chunk) gate_header(f, "Gate:     ", n);
+  for (uint64_t p = 0; p < width; ++p) {
+    f << "Vector " << start + p << ":\t";
+    for (uint32_t i = 0; i < n; ++i)
+      f << std::setw(kOutJust) << (((m[i * words + p / 64] >> (p % 64)) & 1) ? "Y" : "N") << " ";
+    f << "\n";
+  }
+}
+
 void dump_planes(pdf_grader *g, uint32_t n, uint64_t width, uint64_t start, bool first_chunk) {
   const uint64_t words = (width + 63) / 64;
-  std::vector<uint64_t> a(n * words), b(n * words), m(n * words);
   if (sim_flag) {
+    std::vector<uint64_t> a(n * words), b(n * words);
     check(pdf_get_sim_planes(g, a.data(), b.data()), "pdf_get_sim_planes");
-    std::ofstream f("gpusim-p2.log", first_chunk ? std::ios::out : std::ios::app);
+    std::ofstream f = open_log("gpusim-p2.log", first_chunk);
+    if (first_chunk) gate_header(f, "Gate:     ", n);
     for (uint64_t p = 0; p < width; ++p) {
       f << "Vector " << start + p << ":\t";
       for (uint32_t i = 0; i < n; ++i) {
         const int av = (a[i * words + p / 64] >> (p % 64)) & 1, bv = (b[i * words + p / 64] >> (p % 64)) & 1;
-        f << code_name(bv | ((av ^ bv) << 1)) << " ";
+        f << std::setw(kOutJust + 1) << (std::string(code_name(bv | ((av ^ bv) << 1))) + " ");
       }
       f << "\n";
     }
   }
   if (mark_flag) {
+    std::vector<uint64_t> m(n * words);
+    check(pdf_get_forward_mark_plane(g, m.data()), "pdf_get_forward_mark_plane");
+    dump_marks("gpumark-full.log", m, n, width, start, first_chunk);
     check(pdf_get_mark_plane(g, m.data()), "pdf_get_mark_plane");
-    std::ofstream f("gpumark.log", first_chunk ? std::ios::out : std::ios::app);
-    for (uint64_t p = 0; p < width; ++p) {
-      f << "Vector " << start + p << ":\t";
-      for (uint32_t i = 0; i < n; ++i) f << (((m[i * words + p / 64] >> (p % 64)) & 1) ? "Y " : "N ");
-      f << "\n";
-    }
+    dump_marks("gpumark.log", m, n, width, start, first_chunk);
+  }
+}
+
+void dump_cover(pdf_grader *g, uint32_t num_pi, uint64_t width, uint64_t start, bool first_chunk) {
+  std::vector<uint64_t> c((uint64_t)num_pi * width);
+  check(pdf_get_pi_cover(g, c.data()), "pdf_get_pi_cover");
+  std::ofstream f = open_log("gpucover.log", first_chunk);
+  if (first_chunk) gate_header(f, "Gate:   ", num_pi);
+  for (uint64_t p = 0; p < width; ++p) {
+    f << "Vector " << start + p << ":\t";
+    for (uint32_t i = 0; i < num_pi; ++i) f << std::setw(kOutJust) << c[(uint64_t)i * width + p] << " ";
+    f << "\n";
   }
 }
 
@@ -142,6 +186,8 @@ int main(int argc, char **argv) {
   std::clog << "Circuit size is: " << info.n << " Levels: " << info.levels << std::endl;
   pdf_grader *g = nullptr;
   check(pdf_grader_create(0, nl, &g), "pdf_grader_create");
+  if (verbose_flag && !fused_flag)
+    check(pdf_grader_set_debug(g, (mark_flag ? PDF_DEBUG_FORWARD_MARKS : 0u) | PDF_DEBUG_PI_COVER), "set_debug");
 
   for (const std::string &vf : benchmarks) {
     uint64_t V = 0;
@@ -186,6 +232,7 @@ int main(int argc, char **argv) {
         if (verbose_flag) dump_planes(g, info.n, width, start, c == 0);
         check(pdf_merge(g, c, start, segs, &merge), "pdf_merge");
         check(pdf_cover(g, c, start, segs, &coverage, &cover), "pdf_cover");
+        if (verbose_flag) dump_cover(g, info.num_pi, width, start, c == 0);
         gpu += sim + mark + merge + cover;
         totals += coverage;
         std::cerr << "GPU Coverage: " << coverage << ", total: " << totals << std::endl;
@@ -215,8 +262,8 @@ int main(int argc, char **argv) {
           f << "): " << (int32_t)s0[r] << "," << (int32_t)s1[r] << " \n";
         }
       }
-      if (cover_flag) {
-        std::ofstream f("gpucover.log");
+      if (cover_flag) {  // per-node totals over all pairs (no reference counterpart)
+        std::ofstream f("gpucover-gates.log");
         for (uint32_t i = 0; i < info.n; ++i) f << "Gate " << i << ": " << counts[i] << "\n";
       }
     }

@@ -560,7 +560,8 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
                           uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
                           uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
                           uint32_t *__restrict__ pair_gain, uint32_t rel,
-                          const uint64_t *__restrict__ newp = nullptr) {
+                          const uint64_t *__restrict__ newp = nullptr, uint64_t *__restrict__ pi_cover = nullptr,
+                          uint64_t pi_ld = 0) {
   const uint32_t lane = threadIdx.x;
   const uint64_t below = (1ULL << lane) - 1;
   const uint32_t w = rel >> 6, j = rel & 63;
@@ -683,7 +684,10 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
         const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[S.s_node[idx]], (unsigned long long)c);
         ovf |= old + c < old;
       }
-      if (S.s_flags[idx] & 2) ovf |= add_ovf(tot, c);
+      if (S.s_flags[idx] & 2) {
+        ovf |= add_ovf(tot, c);
+        if (pi_cover) pi_cover[(uint64_t)S.s_node[idx] * pi_ld + rel] = c;
+      }
     }
   } else {
     // too many marked nodes for LDS: c/h in global scratch, the whole list
@@ -715,7 +719,10 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
             const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[g], (unsigned long long)c);
             ovf |= old + c < old;
           }
-          if (r.type == PDF_INPT) ovf |= add_ovf(tot, c);
+          if (r.type == PDF_INPT) {
+            ovf |= add_ovf(tot, c);
+            if (pi_cover) pi_cover[(uint64_t)g * pi_ld + rel] = c;
+          }
         }
       }
       __threadfence_block();
@@ -748,14 +755,16 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               uint64_t *__restrict__ gate_counts, uint64_t *__restrict__ chunk_total,
                                               uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
                                               uint32_t width, uint32_t *__restrict__ pair_gain,
-                                              const uint64_t *__restrict__ newp) {
+                                              const uint64_t *__restrict__ newp, uint64_t *__restrict__ pi_cover,
+                                              uint64_t pi_ld) {
   __shared__ CoverLds S;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
   uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
   const uint32_t nact = ALL_NEW ? width : *act_count;
   for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x)
     cover_one<ALL_NEW, SEGNEW>(S, net, pl, start, list, list_cnt, list_stride, first_t0, first_t1, c_s, h_s,
-                               gate_counts, chunk_total, grand_total, err, pair_gain, ALL_NEW ? k : act_list[k], newp);
+                               gate_counts, chunk_total, grand_total, err, pair_gain, ALL_NEW ? k : act_list[k], newp,
+                               pi_cover, pi_ld);
 }
 
 // gpuMergeSegments<K> for K >= 2 (kernSegmentReduce2 with has_marked_segment,
@@ -1307,6 +1316,13 @@ struct pdf_grader {
   uint64_t *d_NEW = nullptr;
   bool seg_stale = true;  // reset the segment firsts before the next merge
   uint32_t grade_segs = 1;  // segment length of the fused grade being queued
+  // stage-API debug copies (pdf_grader_set_debug): the forward marks before
+  // removal (mark plane + present bits) and each PI's c per pair of the
+  // chunk ([num_pi][chunk_pairs])
+  uint32_t debug = 0;
+  uint64_t *d_dbg_M = nullptr, *d_dbg_NZ = nullptr, *d_dbg_pic = nullptr;
+  int64_t dbg_fwd_chunk = -1, dbg_pic_chunk = -1;
+  uint64_t *pic_target = nullptr;  // d_dbg_pic while pdf_cover runs with the debug flag
   uint64_t *d_gate = nullptr;
   uint64_t *d_grand = nullptr;  // exact total over all chunks
   uint32_t *d_err = nullptr;
@@ -1561,7 +1577,7 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_t
                                                        g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                        g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
                                                        g->d_grand, g->d_err, (uint32_t)width, pair_total,
-                                                       nullptr);
+                                                       nullptr, g->pic_target, g->chunk_pairs);
   tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
@@ -1581,7 +1597,7 @@ int launch_cover_segs(pdf_grader *g, uint64_t start, uint64_t width) {
   k_cover<false, true><<<g->cover_waves, 64, 0, g->stream>>>(
       g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt, g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total, g->d_grand, g->d_err, (uint32_t)width, nullptr,
-      g->d_NEW);
+      g->d_NEW, g->pic_target, g->chunk_pairs);
   tm_end(g);
   tm_begin(g, kEvents);
   k_seg_clear<<<grid_for(g->seg_count, 256), 256, 0, g->stream>>>(g->d_seg, g->seg_k, g->seg_count, g->d_sfirst0,
@@ -1766,6 +1782,10 @@ void free_chunk(pdf_grader *g) {
     lane_load(g, l0);
   }
   free_chunk_buffers(g);
+  dfree(g->d_dbg_M);
+  dfree(g->d_dbg_NZ);
+  dfree(g->d_dbg_pic);
+  g->dbg_fwd_chunk = g->dbg_pic_chunk = -1;
   g->chunk_pairs = 0;
 }
 
@@ -2113,8 +2133,18 @@ int pdf_mark(pdf_grader *g, uint64_t chunk, uint64_t start, int robust, float *m
   if (rc) return rc;
   if (g->stage_chunk != (int64_t)chunk || g->stage < 1) { set_error("pdf_mark: simulate this chunk first"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
+  if ((g->debug & PDF_DEBUG_FORWARD_MARKS) && !g->d_dbg_M) {
+    if ((rc = dalloc(&g->d_dbg_M, (uint64_t)g->nl.n * g->pitch))) return rc;
+    if ((rc = dalloc(&g->d_dbg_NZ, (uint64_t)g->nl.n * g->ntiles))) return rc;
+  }
   EventTimer tm(g, ms);
   if ((rc = launch_fwd<false, true>(g, start, width, robust))) return rc;
+  if (g->debug & PDF_DEBUG_FORWARD_MARKS) {  // markkernel.cuh:323-324 (gpumark-full.log)
+    HIPCHK(hipMemcpyAsync(g->d_dbg_M, g->d_M, (uint64_t)g->nl.n * g->pitch * 8, hipMemcpyDeviceToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(g->d_dbg_NZ, g->d_NZ, (uint64_t)g->nl.n * g->ntiles * 8, hipMemcpyDeviceToDevice,
+                          g->stream));
+    g->dbg_fwd_chunk = (int64_t)chunk;
+  }
   if ((rc = launch_reverse<true, false>(g, start, width))) return rc;
   if ((rc = tm.finish())) return rc;
   g->stage = 2;
@@ -2152,11 +2182,23 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
   if (g->stage_chunk != (int64_t)chunk || g->stage != 3) { set_error("pdf_cover: merge this chunk first"); return PDF_ERR_STATE; }
   if ((uint32_t)segs != g->merged_segs) { set_error("pdf_cover: segment length differs from the merge's"); return PDF_ERR_STATE; }
   HIPCHK(hipSetDevice(g->device));
+  if (segs >= 2 && !g->d_NEW && (rc = ensure_segments(g, (uint32_t)segs))) return rc;
+  const bool pic = g->debug & PDF_DEBUG_PI_COVER;
+  if (pic && !g->d_dbg_pic && (rc = dalloc(&g->d_dbg_pic, (uint64_t)g->nl.num_pi * g->chunk_pairs))) return rc;
   EventTimer tm(g, ms);
-  if ((rc = segs >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width))) {
+  if (pic) {  // coverkernel.cuh:309-312 (gpucover.log): c of every PI per pair, zero where not covered
+    const uint64_t nw = (uint64_t)g->nl.num_pi * g->chunk_pairs * 2;
+    k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nw, 256), 65536), 256, 0, g->stream>>>(
+        reinterpret_cast<uint32_t *>(g->d_dbg_pic), nw, 0u);
+    g->pic_target = g->d_dbg_pic;
+  }
+  rc = segs >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width);
+  g->pic_target = nullptr;
+  if (rc) {
     if (segs >= 2) drop_new_plane(g);
     return rc;
   }
+  if (pic) g->dbg_pic_chunk = (int64_t)chunk;
   if ((rc = tm.finish())) return rc;
   uint64_t tot = 0;
   uint32_t err = 0;
@@ -2521,6 +2563,49 @@ int pdf_get_mark_plane(const pdf_grader *gc, uint64_t *mark_plane) {
   return PDF_OK;
 }
 
+int pdf_grader_set_debug(pdf_grader *g, uint32_t flags) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  if (flags & ~(uint32_t)(PDF_DEBUG_FORWARD_MARKS | PDF_DEBUG_PI_COVER)) {
+    set_error("pdf_grader_set_debug: unknown flag");
+    return PDF_ERR_ARG;
+  }
+  g->debug = flags;
+  return PDF_OK;
+}
+
+int pdf_get_forward_mark_plane(const pdf_grader *gc, uint64_t *mark_plane) {
+  pdf_grader *g = const_cast<pdf_grader *>(gc);
+  if (!g || !mark_plane) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (g->dbg_fwd_chunk < 0 || g->dbg_fwd_chunk != g->stage_chunk) {
+    set_error("pdf_get_forward_mark_plane: mark this chunk with PDF_DEBUG_FORWARD_MARKS set first");
+    return PDF_ERR_STATE;
+  }
+  HIPCHK(hipSetDevice(g->device));
+  const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
+  const uint64_t words = (width + 63) / 64, n = g->nl.n;
+  std::vector<uint64_t> nz(n * g->ntiles);
+  int rc = copy_rows(g, mark_plane, g->d_dbg_M, words);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(nz.data(), g->d_dbg_NZ, nz.size() * 8, hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n; ++i)
+    for (uint64_t k = 0; k < words; ++k)
+      if (!((nz[i * g->ntiles + k / 64] >> (k % 64)) & 1)) mark_plane[i * words + k] = 0;
+  return PDF_OK;
+}
+
+int pdf_get_pi_cover(const pdf_grader *g, uint64_t *pi_cover) {
+  if (!g || !pi_cover) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (g->dbg_pic_chunk < 0 || g->dbg_pic_chunk != g->stage_chunk || g->stage != 4) {
+    set_error("pdf_get_pi_cover: cover this chunk with PDF_DEBUG_PI_COVER set first");
+    return PDF_ERR_STATE;
+  }
+  HIPCHK(hipSetDevice(g->device));
+  const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
+  HIPCHK(hipMemcpy2D(pi_cover, width * 8, g->d_dbg_pic, g->chunk_pairs * 8, width * 8, g->nl.num_pi,
+                     hipMemcpyDeviceToHost));
+  return PDF_OK;
+}
+
 int pdf_get_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1) {
   if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
@@ -2644,7 +2729,8 @@ int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev) {
     k_cover<true><<<g->cover_waves, 64, 0, g->stream>>>(g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt,
                                                       g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
-                                                      g->d_grand, g->d_err, (uint32_t)width, gain_dev, nullptr);
+                                                      g->d_grand, g->d_err, (uint32_t)width, gain_dev, nullptr,
+                                                      nullptr, 0);
     HIPCHK(hipGetLastError());
   }
   uint32_t err = 0;

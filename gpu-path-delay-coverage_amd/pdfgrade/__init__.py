@@ -30,6 +30,7 @@ TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF",
 S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
 TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES = 1, 2, 3, 4, 5
+DEBUG_FORWARD_MARKS, DEBUG_PI_COVER = 1, 2
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
@@ -115,6 +116,9 @@ SIGNATURES = {
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
     "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_segment_first": ([_P, C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
+    "pdf_grader_set_debug": ([_P, C.c_uint32], C.c_int),
+    "pdf_get_forward_mark_plane": ([_P, C.c_void_p], C.c_int),
+    "pdf_get_pi_cover": ([_P, C.c_void_p], C.c_int),
     "pdf_get_gate_counts": ([_P, C.c_void_p], C.c_int),
     "pdf_get_total": ([_P, _U64P], C.c_int),
     "pdf_pair_features": ([_P, C.c_uint32, C.c_int, C.c_void_p, _U64], C.c_int),
@@ -464,6 +468,22 @@ class Grader:
         f1 = np.zeros(self.n, np.uint32)
         _check(lib().pdf_get_first(self._h, _ptr(f0), _ptr(f1)))
         return f0, f1
+
+    def set_debug(self, flags: int):
+        """DEBUG_FORWARD_MARKS | DEBUG_PI_COVER (pdf_grader_set_debug)."""
+        _check(lib().pdf_grader_set_debug(self._h, flags))
+
+    def forward_mark_plane(self, width: int) -> np.ndarray:
+        """Forward marks (before removal) of the chunk last marked: (n, words)."""
+        out = np.zeros((self.n, (width + 63) // 64), np.uint64)
+        _check(lib().pdf_get_forward_mark_plane(self._h, _ptr(out)))
+        return out
+
+    def pi_cover(self, num_pi: int, width: int) -> np.ndarray:
+        """c of every PI per pair of the chunk last covered: (num_pi, width)."""
+        out = np.zeros((num_pi, width), np.uint64)
+        _check(lib().pdf_get_pi_cover(self._h, _ptr(out)))
+        return out
 
     def segment_first(self):
         """First T0 / T1 pair of every segment of the last segs >= 2 merge."""

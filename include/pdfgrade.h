@@ -273,6 +273,20 @@ int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane,
 int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane);
 /* first_T0 / first_T1: [n], PDF_NONE when never detected. */
 int pdf_get_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1);
+/* Stage-API debug copies for the reference's --verbose logs (no effect on
+ * results or on pdf_grade): PDF_DEBUG_FORWARD_MARKS makes pdf_mark keep the
+ * marks before the reverse removal (gpumark-full.log, markkernel.cuh:323-324);
+ * PDF_DEBUG_PI_COVER makes pdf_cover keep c(g, p) of every primary input g
+ * for every pair p of the chunk, zero where the DP did not reach
+ * (gpucover.log, coverkernel.cuh:44-66, 309-312). */
+#define PDF_DEBUG_FORWARD_MARKS 1u
+#define PDF_DEBUG_PI_COVER 2u
+int pdf_grader_set_debug(pdf_grader *g, uint32_t flags);
+/* Forward marks of the chunk last marked, [n][words] as pdf_get_mark_plane. */
+int pdf_get_forward_mark_plane(const pdf_grader *g, uint64_t *mark_plane);
+/* c of every primary input for every pair of the chunk last covered:
+ * [num_pi][width] (exact; the reference prints the low 16 bits). */
+int pdf_get_pi_cover(const pdf_grader *g, uint64_t *pi_cover);
 /* Each segment's first T0 / T1 pair (PDF_NONE when never detected), in the
  * order of pdf_netlist_segments, for the segment length of the last
  * pdf_merge with segs >= 2: min(count, capacity) entries each; *count gets

@@ -183,6 +183,7 @@ def grade(types, fanins, po, num_pi, vectors_a, vectors_b, robust=True, segs=1):
     sp = suffix_paths(fanouts, po)
     gate_counts = [0] * n
     per_pattern = []
+    node_counts = []  # per pattern: c(g, p) of every node
     for p in range(P):
         m = m_all[p]
         if walks is None:
@@ -195,19 +196,22 @@ def grade(types, fanins, po, num_pi, vectors_a, vectors_b, robust=True, segs=1):
                         if v is not None:
                             new[v] = True
         tot = 0
+        counts = [0] * n
         for g in range(n):
             if not m[g]:
                 continue
             cnt = sum(1 for path in sp[g] if all(m[v] for v in path) and any(new[v] for v in path))
+            counts[g] = cnt
             gate_counts[g] += cnt
             if types[g] == INPT:
                 tot += cnt
         per_pattern.append(tot)
+        node_counts.append(counts)
     none = 0xFFFFFFFF
     out = dict(first_t0=[none if x is None else x for x in first0],
                first_t1=[none if x is None else x for x in first1],
                gate_counts=gate_counts, total=sum(per_pattern), per_pattern=per_pattern,
-               frames=fr_all, marks=m_all)
+               node_counts=node_counts, frames=fr_all, marks=m_all)
     if walks is not None:
         out["segments"] = [[none if v is None else v for v in wk] for wk in walks]
         out["seg_t0"] = [none if x is None else x for x in sfirst0]

@@ -202,9 +202,11 @@ int orc_merge_chunk(const orc_netlist *nl, const uint8_t *codes, const uint8_t *
 
 /* kernCover<K> + kernSumSingle over one chunk, exact uint64; newm[g*W + p]
  * says node g counts as new at pattern start+p (coverkernel.cuh:214-220:
- * some segment holding g has start+p as its first T0 or T1 pattern). */
+ * some segment holding g has start+p as its first T0 or T1 pattern).
+ * pi_c (may be NULL) receives c(g, p) of every primary input g < num_pi,
+ * [num_pi][W], the values debugCover prints (coverkernel.cuh:44-66). */
 int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8_t *newm, uint64_t W,
-                        uint64_t *gate_counts, uint64_t *chunk_total, uint32_t *flags) {
+                        uint64_t *gate_counts, uint64_t *chunk_total, uint32_t *flags, uint64_t *pi_c) {
   uint64_t *c = (uint64_t *)malloc(sizeof(uint64_t) * nl->n);
   uint64_t *h = (uint64_t *)malloc(sizeof(uint64_t) * nl->n);
   if (!c || !h) { free(c); free(h); return -5; }
@@ -239,6 +241,8 @@ int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8
     for (uint32_t gi = 0; gi < nl->level_off[1]; ++gi)
       if (nl->type[gi] == INPT && __builtin_add_overflow(total, c[gi], &total))
         *flags |= ORC_FLAG_U64_OVERFLOW;
+    if (pi_c)
+      for (uint32_t gi = 0; gi < nl->num_pi; ++gi) pi_c[(uint64_t)gi * W + p] = c[gi];
   }
   if (total > 0xFFFFFFFFull) *flags |= ORC_FLAG_EXCEEDS_U32;
   *chunk_total = total;
@@ -259,7 +263,7 @@ int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uin
     for (int k = 0; k < 2; ++k)
       if (x[k] != 0xFFFFFFFFu && x[k] >= start && x[k] - start < W) newm[(uint64_t)g * W + (x[k] - start)] = 1;
   }
-  const int rc = orc_cover_new_chunk(nl, marks, newm, W, gate_counts, chunk_total, flags);
+  const int rc = orc_cover_new_chunk(nl, marks, newm, W, gate_counts, chunk_total, flags, NULL);
   free(newm);
   return rc;
 }
@@ -369,7 +373,7 @@ int orc_grade_segs(const orc_netlist *nl, const uint64_t *a_bits, const uint64_t
     orc_mark_chunk(nl, codes, W, opt ? opt->robust : 0, marks, NULL);
     orc_merge_segs_chunk(nl, codes, marks, W, start, opt ? opt->merge_window : 0, K, segs, S, seg_t0, seg_t1);
     orc_new_from_segments(nl, W, start, K, segs, S, seg_t0, seg_t1, newm);
-    rc = orc_cover_new_chunk(nl, marks, newm, W, gate_counts, &ct, flags);
+    rc = orc_cover_new_chunk(nl, marks, newm, W, gate_counts, &ct, flags, NULL);
     if (rc) break;
     if (chunk_totals) chunk_totals[idx] = ct;
     if (__builtin_add_overflow(*total, ct, total)) *flags |= ORC_FLAG_U64_OVERFLOW;

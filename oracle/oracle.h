@@ -36,7 +36,7 @@ int orc_cover_chunk(const orc_netlist *nl, const uint8_t *marks, uint64_t W, uin
                     const uint32_t *first_t0, const uint32_t *first_t1, uint64_t *gate_counts,
                     uint64_t *chunk_total, uint32_t *flags);
 int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8_t *newm, uint64_t W,
-                        uint64_t *gate_counts, uint64_t *chunk_total, uint32_t *flags);
+                        uint64_t *gate_counts, uint64_t *chunk_total, uint32_t *flags, uint64_t *pi_c);
 int orc_segments(const orc_netlist *nl, uint32_t K, uint32_t *out, uint64_t cap, uint64_t *count);
 int orc_merge_segs_chunk(const orc_netlist *nl, const uint8_t *codes, const uint8_t *marks, uint64_t W,
                          uint64_t start, uint32_t merge_window, uint32_t K, const uint32_t *segs, uint64_t S,

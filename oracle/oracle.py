@@ -201,3 +201,27 @@ def grade_bitsliced(net: OracleNet, a_bits: np.ndarray, num_pairs: int, chunk: i
     if rc:
         raise RuntimeError(f"bsl_grade failed ({rc})")
     return dict(first_t0=f0, first_t1=f1, gate_counts=gc, total=tot.value, flags=flags.value)
+
+
+def cover_new_chunk(net: OracleNet, marks, newm, gate_counts, num_pi: int):
+    """orc_cover_new_chunk: (chunk total, flags, c of every PI [num_pi][W])."""
+    marks = np.ascontiguousarray(marks, dtype=np.uint8)
+    newm = np.ascontiguousarray(newm, dtype=np.uint8)
+    W = marks.shape[1]
+    pic = np.zeros((num_pi, W), np.uint64)
+    tot = C.c_uint64()
+    flags = C.c_uint32()
+    lib().orc_cover_new_chunk(net.ref, _p(marks), _p(newm), C.c_uint64(W), _p(gate_counts), C.byref(tot),
+                              C.byref(flags), _p(pic))
+    return tot.value, flags.value, pic
+
+
+def new_from_firsts(n: int, W: int, start: int, first_t0, first_t1) -> np.ndarray:
+    """K = 1 new-node matrix (node g new at pattern start+p when its first T0
+    or T1 is start+p), [n][W] bytes."""
+    newm = np.zeros((n, W), np.uint8)
+    for arr in (first_t0, first_t1):
+        arr = np.asarray(arr, dtype=np.int64)
+        sel = (arr != 0xFFFFFFFF) & (arr >= start) & (arr < start + W)
+        newm[np.nonzero(sel)[0], arr[sel] - start] = 1
+    return newm

@@ -255,3 +255,33 @@ def test_planted_mistakes_are_caught():
     assert dp("none") == (ref["total"], ref["gate_counts"])
     for v in ("new_ignores_T1", "no_history", "keep_history_when_new"):
         assert dp(v) != (ref["total"], ref["gate_counts"]), v
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_pi_cover_matches_brute_force(seed):
+    """c(g, p) of every PI per pattern (the values of the reference's
+    gpucover.log) from the oracle's cover, chunk by chunk, equals the
+    brute-force count of marked PI->PO paths through a new node."""
+    from helpers import brute_grade
+    nl = pg.Netlist.generate(small_config(seed, wide=seed % 2 == 0))
+    arrs = nl.arrays()
+    I = arrs["num_pi"]
+    P, W = 128, 64
+    bits = biased_vectors(seed, I, P, p_hold=0.5)
+    for robust in (True, False):
+        ref = brute_grade(nl, bits, P, robust)
+        net = orc.OracleNet(arrs)
+        f0 = np.full(net.n, 0xFFFFFFFF, np.uint32)
+        f1 = f0.copy()
+        gc = np.zeros(net.n, np.uint64)
+        for start in range(0, P, W):
+            codes = orc.sim_chunk(net, bits, P, start, W)
+            marks, _ = orc.mark_chunk(net, codes, robust)
+            orc.merge_chunk(net, codes, marks, start, f0, f1)
+            newm = orc.new_from_firsts(net.n, W, start, f0, f1)
+            tot, _, pic = orc.cover_new_chunk(net, marks, newm, gc, I)
+            for p in range(W):
+                want = [ref["node_counts"][start + p][g] for g in range(I)]
+                assert pic[:, p].tolist() == want, (robust, start + p)
+                assert int(pic[:, p].sum()) == ref["per_pattern"][start + p]
+        assert gc.tolist() == ref["gate_counts"]

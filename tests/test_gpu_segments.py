@@ -260,3 +260,27 @@ def test_segments_one_long_chunk():
         seg = orc.grade_segs(orc.OracleNet(nl.arrays()), bits, P, P, robust, K)
         hit = seg["seg_t0"][seg["seg_t0"] != NONE]
         assert len(hit) > 0 and hit.min() >= 270_000
+
+
+def test_fused_grade_segs_explicit_pairs_on_lanes():
+    """Explicit (A, B) pairs, segments of length 3, 2 lanes of 256-pair
+    chunks, graph replay on and off: equal to the oracle."""
+    say("fused segments explicit pairs")
+    cfg = pg.GenConfig(seed=34, num_pi=32, num_po=20, num_gates=800, levels=12, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=16)
+    nl = pg.Netlist.generate(cfg)
+    P = 1000
+    a = biased_vectors(34, 32, P, p_hold=0.6)
+    b = biased_vectors(35, 32, P, p_hold=0.6)
+    for robust in (True, False):
+        ref = orc.grade_segs(orc.OracleNet(nl.arrays()), a, P, 256, robust, 3, b_bits=b)
+        with pg.Grader(nl) as g:
+            g.set_pairs(a, b, P)
+            g.set_tuning(pg.TUNE_LANES, 2)
+            g.set_chunk(256)
+            for graph in (1, 0):
+                g.set_tuning(pg.TUNE_USE_GRAPH, graph)
+                assert g.grade_segs(robust, 3) == ref["total"]
+                got = _fused(g, robust, 3)
+                for k in ("seg_t0", "seg_t1", "gate_counts"):
+                    assert np.array_equal(got[k], ref[k]), (robust, graph, k)

@@ -1157,6 +1157,10 @@ __global__ void k_iota(uint32_t *p, uint64_t n) {
 }
 
 __global__ void k_copy_u64(const uint64_t *src, uint64_t *dst) { *dst = *src; }
+__global__ void k_copy_result(const uint64_t *total, const uint32_t *err, uint64_t *total_out, uint32_t *err_out) {
+  *total_out = *total;
+  if (err_out) *err_out = *err;
+}
 
 // ------------------------------------------------------------ batched grading
 // Many circuits, one chunk each, graded by level launches over the union of
@@ -1314,8 +1318,8 @@ struct pdf_grader {
   uint64_t seg_count = 0;
   uint32_t *d_seg = nullptr, *d_sfirst0 = nullptr, *d_sfirst1 = nullptr;
   uint64_t *d_NEW = nullptr;
-  bool seg_stale = true;  // reset the segment firsts before the next merge
   uint32_t grade_segs = 1;  // segment length of the fused grade being queued
+  bool one_lane = false;    // queue the fused grade on one stream (pdf_grade_async)
   // stage-API debug copies (pdf_grader_set_debug): the forward marks before
   // removal (mark plane + present bits) and each PI's c per pair of the
   // chunk ([num_pi][chunk_pairs])
@@ -1609,6 +1613,7 @@ int launch_cover_segs(pdf_grader *g, uint64_t start, uint64_t width) {
 }
 
 void drop_graph(pdf_grader *g);
+int queue_seg_reset(pdf_grader *g);
 
 // After a failed launch the new-node plane may not be all zero: free it (and
 // any captured grade holding its address); the next use allocates it zeroed.
@@ -1650,7 +1655,8 @@ int ensure_segments(pdf_grader *g, uint32_t K) {
     if (rc) { free_segments(g); return rc; }
     g->seg_k = K;
     g->seg_count = S;
-    g->seg_stale = true;
+    if ((rc = queue_seg_reset(g))) return rc;
+    HIPCHK(hipStreamSynchronize(g->stream));
   }
   // one new-node plane per lane, zeroed once here; every cover leaves it zero again
   const uint64_t words = (uint64_t)g->nl.n * g->pitch;
@@ -1687,13 +1693,10 @@ int queue_seg_reset(pdf_grader *g) {
   k_fill_u32<<<blocks, 256, 0, g->stream>>>(g->d_sfirst1, g->seg_count, PDF_NONE);
   tm_end(g);
   HIPCHK(hipGetLastError());
-  g->seg_stale = false;
   return PDF_OK;
 }
 
 int launch_merge_seg(pdf_grader *g, uint64_t start, uint64_t width) {
-  int rc;
-  if (g->seg_stale && (rc = queue_seg_reset(g))) return rc;
   tm_begin(g, kSegMerge);
   if (g->seq) launch_merge_seg_t<true>(g, start, width);
   else launch_merge_seg_t<false>(g, start, width);
@@ -1841,12 +1844,14 @@ int queue_reset(pdf_grader *g) {
   HIPCHK(hipGetLastError());
   g->stage = 0;
   g->stage_chunk = -1;
-  g->seg_stale = true;
   return PDF_OK;
 }
 
+// A new vector set or an explicit reset: node state, and the segment firsts
+// if a segment list exists (grades with segments also reset them first).
 int reset_state(pdf_grader *g) {
   int rc = queue_reset(g);
+  if (!rc && g->seg_count) rc = queue_seg_reset(g);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(g->stream));
   return PDF_OK;
@@ -2374,7 +2379,7 @@ static int queue_grade_lane_graphs(pdf_grader *g, int robust) {
 // built from the segment firsts.
 static int queue_grade(pdf_grader *g, int robust) {
   const uint32_t K = g->grade_segs;
-  if (!g->lanes.empty() && !g->timing) return queue_grade_lanes(g, robust);
+  if (!g->lanes.empty() && !g->timing && !g->one_lane) return queue_grade_lanes(g, robust);
   int rc = queue_reset(g);
   if (rc) return rc;
   if (K >= 2 && (rc = queue_seg_reset(g))) return rc;
@@ -2401,7 +2406,6 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   HIPCHK(hipSetDevice(g->device));
   int rc;
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
-  g->seg_stale = true;  // graph replays reset the node state without queue_reset
   const int key = (robust != 0) | (int)(g->grade_segs << 1);
   if (g->tune_use_graph && !g->timing && !g->lanes.empty()) {
     if ((rc = queue_grade_lane_graphs(g, robust))) return rc;
@@ -2437,7 +2441,6 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
-  if (g->grade_segs >= 2) g->seg_stale = false;  // the grade reset and filled them
   if (err) { set_error("pdf_grade: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   *total = tot;
   return PDF_OK;
@@ -2503,6 +2506,40 @@ int pdf_grade_timed(pdf_grader *g, int robust, uint64_t *total, pdf_kernel_times
   if (!g || !total || !times) { set_error("null argument"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
   return grade_timed_impl(g, robust, total, times);
+}
+
+int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t *total_dev, uint32_t *err_dev) {
+  if (!g || !stream || !total_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (segs < 1 || segs > 8) { set_error("pdf_grade_async: segment length must be 1..8"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error("pdf_grade_async: load vectors first"); return PDF_ERR_STATE; }
+  if (segs >= 2 && (g->seg_k != (uint32_t)segs || !g->d_NEW)) {
+    set_error("pdf_grade_async: no segment list of this length yet (one pdf_grade_segs call sets it up)");
+    return PDF_ERR_STATE;
+  }
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != g->device) HIPCHK(hipSetDevice(g->device));
+  // everything below only queues kernels on the caller's stream: no
+  // allocation, no synchronous copy, no wait, no memset node
+  const hipStream_t own = g->stream;
+  g->stream = static_cast<hipStream_t>(stream);
+  g->grade_segs = (uint32_t)segs;
+  g->one_lane = true;
+  int rc = queue_grade(g, robust);
+  if (!rc) {
+    k_copy_result<<<1, 1, 0, g->stream>>>(g->d_grand, g->d_err, total_dev, err_dev);
+    if (!hip_ok(hipGetLastError(), "k_copy_result")) rc = PDF_ERR_HIP;
+  }
+  g->stream = own;
+  g->grade_segs = 1;
+  g->one_lane = false;
+  if (rc) return rc;
+  const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  g->stage = 4;
+  g->stage_chunk = (int64_t)nchunks - 1;
+  g->stage_robust = robust;
+  g->merged_segs = (uint32_t)segs;
+  return PDF_OK;
 }
 
 int pdf_grade_segs(pdf_grader *g, int robust, int segs, uint64_t *total, pdf_kernel_times *times) {
@@ -2621,11 +2658,6 @@ int pdf_get_segment_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *fir
   *count = g->seg_count;
   const uint64_t k = std::min(capacity, g->seg_count);
   if (!k) return PDF_OK;
-  if (g->seg_stale) {  // reset since the last merge: nothing detected yet
-    if (first_t0) std::fill(first_t0, first_t0 + k, PDF_NONE);
-    if (first_t1) std::fill(first_t1, first_t1 + k, PDF_NONE);
-    return PDF_OK;
-  }
   if (first_t0) HIPCHK(hipMemcpy(first_t0, g->d_sfirst0, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
   if (first_t1) HIPCHK(hipMemcpy(first_t1, g->d_sfirst1, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
   return PDF_OK;

@@ -111,6 +111,7 @@ SIGNATURES = {
     "pdf_grade": ([_P, C.c_int, _U64P], C.c_int),
     "pdf_grade_timed": ([_P, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
     "pdf_grade_segs": ([_P, C.c_int, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
+    "pdf_grade_async": ([_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grader_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
     "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
@@ -433,6 +434,13 @@ class Grader:
         t = KernelTimes() if timed else None
         _check(lib().pdf_grade_segs(self._h, int(robust), segs, C.byref(tot), C.byref(t) if timed else None))
         return (tot.value, {k: getattr(t, k) for k, _ in KernelTimes._fields_}) if timed else tot.value
+
+    def grade_async(self, robust: bool, segs: int, stream: int, total_dev: int, err_dev: int = 0):
+        """pdf_grade_async: queue the fused grade on a HIP stream handle
+        (e.g. torch.cuda.Stream().cuda_stream); the total lands in the device
+        u64 at total_dev once the stream has run it."""
+        _check(lib().pdf_grade_async(self._h, int(robust), segs, C.c_void_p(stream), C.c_void_p(total_dev),
+                                     C.c_void_p(err_dev) if err_dev else None))
 
     def grade_stages(self, robust: bool, segs: int = 1):
         """The reference's chunk loop (main.cu:171-231) through the four stage

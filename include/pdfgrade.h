@@ -231,6 +231,20 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
  * total.  Equivalent to calling the four stages per chunk. */
 int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
 
+/* The fused grade (pdf_grade_segs with segs = 1..8) queued on the caller's
+ * HIP stream (a hipStream_t passed as void *) without waiting for it, so
+ * that it can be captured into the caller's HIP graph: it allocates
+ * nothing, copies nothing synchronously and records no memset.  On
+ * completion *total_dev (device memory) holds the exact total and, if
+ * err_dev is not NULL, *err_dev is non-zero when a uint64 count overflowed.
+ * One lane.  The buffers it uses must exist: pdf_grader_set_chunk makes
+ * those of segs = 1, one pdf_grade_segs call those of a segs >= 2.  Calls
+ * that overlap on one grader race; the getters see the results once the
+ * caller's stream has finished (main.cu:171-231 with the stream made
+ * explicit). */
+int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t *total_dev,
+                    uint32_t *err_dev);
+
 /* Tuning knobs (no reference counterpart; the reference fixes its shapes
  * with compile-time constants such as SIM_BLOCK, MARK_BLOCK, PARALLEL_SEGS,
  * simkernel.cuh:29-30, markkernel.cuh:7-8, mergekernel.cuh:49-52).
@@ -288,9 +302,11 @@ int pdf_get_forward_mark_plane(const pdf_grader *g, uint64_t *mark_plane);
  * [num_pi][width] (exact; the reference prints the low 16 bits). */
 int pdf_get_pi_cover(const pdf_grader *g, uint64_t *pi_cover);
 /* Each segment's first T0 / T1 pair (PDF_NONE when never detected), in the
- * order of pdf_netlist_segments, for the segment length of the last
- * pdf_merge with segs >= 2: min(count, capacity) entries each; *count gets
- * the number of segments (0 when no segment list is kept). */
+ * order of pdf_netlist_segments, for the kept segment list (the length of
+ * the last merge or grade with segs >= 2): min(count, capacity) entries
+ * each; *count gets the number of segments (0 when no list is kept).
+ * Grades with segs >= 2 and pdf_grader_reset start them over; grades with
+ * segs = 1 leave them alone. */
 int pdf_get_segment_first(const pdf_grader *g, uint32_t *first_t0, uint32_t *first_t1, uint64_t capacity,
                           uint64_t *count);
 /* Per-gate exact counts sum_p c(g,p) over all chunks graded so far: [n]. */

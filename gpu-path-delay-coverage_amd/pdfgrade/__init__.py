@@ -15,6 +15,7 @@ Loading fails loudly when the library has not been built.
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import os
 from dataclasses import dataclass
 from typing import Optional
@@ -201,9 +202,10 @@ class Netlist:
     def __init__(self, handle):
         self._h = handle
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the module's globals are gone; the process frees everything
         h = getattr(self, "_h", None)
-        if h:
+        if h and not _finalizing():
             lib().pdf_netlist_free(h)
             self._h = None
 
@@ -345,8 +347,9 @@ class Grader:
             lib().pdf_grader_destroy(self._h)
         self._h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if not _finalizing():
+            self.close()
 
     def __enter__(self):
         return self
@@ -587,8 +590,9 @@ class Batch:
             lib().pdf_batch_destroy(self._h)
             self._h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if not _finalizing():
+            self.close()
 
     def grade(self, robust: bool):
         """(per-circuit totals as uint64 array, GPU milliseconds)."""
@@ -628,8 +632,9 @@ class Mlp:
             lib().pdf_mlp_destroy(self._h)
             self._h = None
 
-    def __del__(self):
-        self.close()
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if not _finalizing():
+            self.close()
 
     def params(self) -> np.ndarray:
         out = np.zeros(MLP_PARAMS, np.float32)

@@ -256,7 +256,8 @@ def learned_vs_plain(robust: bool):
         "time_to_95pct_paths_learned_over_plain_same": s95["learned_over_plain_same_fraction"],
         "pairs_to_95pct_paths": {"learned": s95["pairs_learned_mean"], "plain": s95["pairs_plain_mean"]},
         "pairs_to_95pct_segments": {"learned": g95["pairs_learned_mean"], "plain": g95["pairs_plain_mean"]},
-        "overhead_ms": {k: sum(p[k] for p in per) / len(per) for k in ("features_ms", "score_ms", "rank_ms")},
+        "overhead_ms": {k: sum(p[k] for p in per) / len(per)
+                        for k in ("features_score_ms", "rank_ms", "features_ms", "score_ms")},
         "plain_full_ms": sum(c["plain_ms"] for c in res["circuits"]) / len(res["circuits"]),
         "heldout_circuits": len(res["circuits"]), "candidates": res["config"]["candidates"],
         "records": res["records"],

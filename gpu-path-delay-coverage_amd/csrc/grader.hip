@@ -35,6 +35,7 @@
 #include <string>
 #include <vector>
 
+#include "mlp_internal.hpp"
 #include "pdf_internal.hpp"
 
 using pdf::Netlist;
@@ -2679,27 +2680,24 @@ int pdf_get_total(const pdf_grader *g, uint64_t *total) {
 
 // ------------------------------------------------------- learned proposal
 
-int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *feat_dev, uint64_t ld) {
-  if (!g || !feat_dev) { set_error("null argument"); return PDF_ERR_ARG; }
-  if (!g->chunk_pairs) { set_error("pdf_pair_features: load pairs first"); return PDF_ERR_STATE; }
-  if (ld % 64 || ld < g->num_pairs || (uintptr_t)feat_dev % 16) {
-    set_error("pdf_pair_features: ld must be a multiple of 64 and >= the pair count, feat_dev 16-byte aligned");
-    return PDF_ERR_ARG;
-  }
-  HIPCHK(hipSetDevice(g->device));
-  const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
+extern "C++" {  // helpers of the feature entry points (C++ linkage)
+
+// The feature pass: every chunk's levels 0..last simulated and forward-marked,
+// then the level and PI features, a group of `lanes` chunks at a time with
+// their level launches interleaved across the lanes' streams.  After each
+// group is queued, on_group(first_pair, end_pair, lanes, lanes_used) runs
+// (the fused scoring pass chains the scorer there).  Queues only; the lanes'
+// streams are left running.
+template <typename OnGroup>
+static int queue_features(pdf_grader *g, uint32_t last, int robust, uint16_t *feat_dev, uint64_t ld,
+                          std::vector<pdf_grader::Lane> &ln, OnGroup &&on_group) {
   const uint64_t nchunks = (g->num_pairs + g->chunk_pairs - 1) / g->chunk_pairs;
+  const uint32_t L = (uint32_t)ln.size();
   int rc = PDF_OK;
   if (last < 127) {  // level features past the simulated depth are zero
     for (uint64_t f0 : {(uint64_t)last + 1, (uint64_t)128 + last + 1})
       HIPCHK(hipMemset2DAsync(feat_dev + f0 * ld, ld * 2, 0, g->num_pairs * 2, 127 - last, g->stream));
   }
-  // chunks are independent here: a group of `lanes` chunks at a time, their
-  // level launches interleaved across the lanes' streams
-  const uint32_t L = 1 + (uint32_t)g->lanes.size();
-  std::vector<pdf_grader::Lane> ln(L);
-  lane_save(g, ln[0]);
-  for (uint32_t i = 1; i < L; ++i) ln[i] = g->lanes[i - 1];
   if (L > 1) {
     HIPCHK(hipEventRecord(g->lane_start, ln[0].stream));
     for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamWaitEvent(ln[i].stream, g->lane_start, 0));
@@ -2734,12 +2732,82 @@ int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *f
       }
       if (!hip_ok(hipGetLastError(), "k_features")) rc = PDF_ERR_HIP;
     }
+    if (!rc)
+      rc = on_group(c0 * g->chunk_pairs, std::min(c1 * g->chunk_pairs, g->num_pairs), (uint32_t)(c1 - c0));
   }
   lane_load(g, ln[0]);
-  for (uint32_t i = 1; i < L; ++i) HIPCHK(hipStreamSynchronize(ln[i].stream));
+  return rc;
+}
+
+static int check_features_args(pdf_grader *g, uint16_t *feat_dev, uint64_t ld, const char *who) {
+  if (!g || !feat_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->chunk_pairs) { set_error(std::string(who) + ": load pairs first"); return PDF_ERR_STATE; }
+  if (ld % 64 || ld < g->num_pairs || (uintptr_t)feat_dev % 16) {
+    set_error(std::string(who) +
+              ": ld must be a multiple of 64 and >= the pair count, feat_dev 16-byte aligned");
+    return PDF_ERR_ARG;
+  }
+  return PDF_OK;
+}
+
+static std::vector<pdf_grader::Lane> all_lanes(pdf_grader *g) {
+  std::vector<pdf_grader::Lane> ln(1 + g->lanes.size());
+  lane_save(g, ln[0]);
+  for (size_t i = 1; i < ln.size(); ++i) ln[i] = g->lanes[i - 1];
+  return ln;
+}
+
+}  // extern "C++"
+
+int pdf_pair_features(pdf_grader *g, uint32_t max_level, int robust, uint16_t *feat_dev, uint64_t ld) {
+  int rc = check_features_args(g, feat_dev, ld, "pdf_pair_features");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(g->device));
+  const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
+  std::vector<pdf_grader::Lane> ln = all_lanes(g);
+  rc = queue_features(g, last, robust, feat_dev, ld, ln, [](uint64_t, uint64_t, uint32_t) { return PDF_OK; });
+  for (size_t i = 1; i < ln.size(); ++i) HIPCHK(hipStreamSynchronize(ln[i].stream));
   HIPCHK(hipStreamSynchronize(g->stream));
   if (rc) return rc;
   g->stage = 0;  // the planes hold a partial simulation
+  g->stage_chunk = -1;
+  return PDF_OK;
+}
+
+int pdf_score_pairs(pdf_grader *g, pdf_mlp *m, uint32_t max_level, int robust, uint16_t *feat_dev, uint64_t ld,
+                    float *score_dev) {
+  int rc = check_features_args(g, feat_dev, ld, "pdf_score_pairs");
+  if (rc) return rc;
+  if (!m || !score_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (pdf::mlp_device(m) != g->device) { set_error("pdf_score_pairs: grader and scorer on different devices"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
+  std::vector<pdf_grader::Lane> ln = all_lanes(g);
+  if ((rc = pdf::mlp_prepare(m, feat_dev, ld, std::min<uint64_t>(g->num_pairs, g->chunk_pairs * ln.size())))) return rc;
+  // one event per lane, re-recorded for every group (a wait takes the
+  // record current when it is queued)
+  std::vector<hipEvent_t> ev(ln.size(), nullptr);
+  for (auto &e : ev)
+    if (!hip_ok(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate")) rc = PDF_ERR_HIP;
+  const hipStream_t ms = pdf::mlp_stream(m);
+  if (!rc)
+    rc = queue_features(g, last, robust, feat_dev, ld, ln, [&](uint64_t p0, uint64_t p1, uint32_t used) -> int {
+      // the scorer takes the group once every lane that worked on it is done
+      for (uint32_t i = 0; i < used; ++i) {
+        if (!hip_ok(hipEventRecord(ev[i], ln[i].stream), "hipEventRecord") ||
+            !hip_ok(hipStreamWaitEvent(ms, ev[i], 0), "hipStreamWaitEvent"))
+          return PDF_ERR_HIP;
+      }
+      return pdf::mlp_queue_forward(m, feat_dev, ld, p0, p1, score_dev);
+    });
+  for (size_t i = 1; i < ln.size(); ++i) (void)hipStreamSynchronize(ln[i].stream);
+  (void)hipStreamSynchronize(g->stream);
+  const hipError_t se = hipStreamSynchronize(ms);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (!rc && !hip_ok(se, "hipStreamSynchronize")) rc = PDF_ERR_HIP;
+  if (rc) return rc;
+  g->stage = 0;
   g->stage_chunk = -1;
   return PDF_OK;
 }

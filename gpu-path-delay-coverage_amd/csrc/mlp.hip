@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "mlp_internal.hpp"
 #include "pdf_internal.hpp"
 
 namespace {
@@ -522,6 +523,29 @@ int check_rows(const pdf_mlp *m, const void *X, uint64_t ld, uint64_t rows) {
 }
 
 }  // namespace
+
+namespace pdf {
+
+hipStream_t mlp_stream(const pdf_mlp *m) { return m->stream; }
+int mlp_device(const pdf_mlp *m) { return m->device; }
+
+int mlp_prepare(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t rows) {
+  int rc = check_rows(m, feat_dev, ld, rows);
+  if (rc) return rc;
+  MLPCHK(hipSetDevice(m->device));
+  return ensure_rows(m, std::min<uint64_t>(rows, kSlab));
+}
+
+int mlp_queue_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t r0, uint64_t r1, float *score_dev) {
+  for (uint64_t a = r0; a < r1; a += kSlab) {
+    const uint32_t r = (uint32_t)std::min<uint64_t>(kSlab, r1 - a);
+    const int rc = forward(m, feat_dev + a, ld, r, score_dev + a, nullptr);
+    if (rc) return rc;
+  }
+  return PDF_OK;
+}
+
+}  // namespace pdf
 
 extern "C" {
 

@@ -124,6 +124,7 @@ SIGNATURES = {
     "pdf_get_gate_counts": ([_P, C.c_void_p], C.c_int),
     "pdf_get_total": ([_P, _U64P], C.c_int),
     "pdf_pair_features": ([_P, C.c_uint32, C.c_int, C.c_void_p, _U64], C.c_int),
+    "pdf_score_pairs": ([_P, _P, C.c_uint32, C.c_int, C.c_void_p, _U64, C.c_void_p], C.c_int),
     "pdf_pair_gains": ([_P, C.c_int, C.c_void_p], C.c_int),
     "pdf_grader_rank_pairs": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grade_curve": ([_P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
@@ -524,6 +525,17 @@ class Grader:
         if cols < self._num_pairs:
             raise ValueError("feature tensor has fewer columns than pairs")
         _check(lib().pdf_pair_features(self._h, max_level, int(robust), C.c_void_p(ptr), ld))
+
+    def score_pairs(self, mlp: "Mlp", max_level: int, robust: bool, feat, scores):
+        """pair_features and mlp.forward over every loaded pair, pipelined
+        chunk group by chunk group (pdf_score_pairs): same features in
+        `feat`, same scores in `scores` (float32 device tensor, num_pairs)."""
+        ptr, ld, cols = _feat_arg(feat)
+        if cols < self._num_pairs:
+            raise ValueError("feature tensor has fewer columns than pairs")
+        _need_device(scores, self._num_pairs, 4)
+        _check(lib().pdf_score_pairs(self._h, mlp._h, max_level, int(robust), C.c_void_p(ptr), ld,
+                                     C.c_void_p(scores.data_ptr())))
 
     def pair_gains(self, robust: bool, gains):
         """Each pair's marked PI->PO path count graded alone (int32 device

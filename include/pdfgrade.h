@@ -373,6 +373,15 @@ int pdf_mlp_train_step(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, const 
                        float lr, float *loss);
 int pdf_mlp_sync(pdf_mlp *m);
 
+/* pdf_pair_features followed by pdf_mlp_forward on every pair, pipelined (no
+ * reference counterpart): each group of `lanes` chunks is scored on the
+ * scorer's stream as soon as its features are written, while the next
+ * group's features are computed.  feat_dev receives the features as
+ * pdf_pair_features writes them and score_dev [num_pairs] the scores; both
+ * equal the two separate calls.  Grader and scorer on the same device. */
+int pdf_score_pairs(pdf_grader *g, pdf_mlp *m, uint32_t max_level, int robust, uint16_t *feat_dev, uint64_t ld,
+                    float *score_dev);
+
 /* Autotuner (no reference counterpart): times the grade (reps runs each,
  * median) over chunk lanes 1..3 (chunk = pairs / lanes), graph replay on and
  * off and words per lane, checks that every configuration gives the same

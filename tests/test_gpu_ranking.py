@@ -302,3 +302,30 @@ def test_mlp_training_reduces_loss():
     _say("loss", first, last)
     assert last < 0.2 * first
     m.close()
+
+
+@pytest.mark.parametrize("lanes,chunk,P", [(2, 1024, 5000), (1, 2048, 4096), (3, 512, 3000)])
+def test_score_pairs_equals_features_then_forward(lanes, chunk, P):
+    """The pipelined feature + scoring pass writes the same features and the
+    same scores (bit for bit) as pair_features followed by Mlp.forward."""
+    _say("score pairs", lanes, chunk, P)
+    nl = pg.Netlist.generate(small_config(61, num_pi=40, num_gates=900, levels=14))
+    a, b = _explicit(40, P, 610)
+    m = pg.Mlp(seed=61)
+    ld = (P + 63) // 64 * 64
+    with pg.Grader(nl) as g:
+        g.set_pairs(a, b, P)
+        g.set_tuning(pg.TUNE_LANES, lanes)
+        g.set_chunk(chunk)
+        f1 = torch.full((pg.FEATURES, ld), 0x7F7F, dtype=torch.int16, device=DEV)
+        s1 = torch.zeros(P, dtype=torch.float32, device=DEV)
+        g.pair_features(4, False, f1)
+        m.forward(f1[:, :P], s1)
+        f2 = torch.full((pg.FEATURES, ld), 0x7F7F, dtype=torch.int16, device=DEV)
+        s2 = torch.zeros(P, dtype=torch.float32, device=DEV)
+        g.score_pairs(m, 4, False, f2, s2)
+        torch.cuda.synchronize()
+        assert torch.equal(f1[:, :P], f2[:, :P])
+        assert torch.equal(s1, s2)
+        assert float(s1.abs().sum()) > 0
+    m.close()

@@ -399,14 +399,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
 // before its level runs); the reader clears ACC again, so it is all zero
 // between passes.  DO_MERGE: earliest marked T0/T1 pair per node
 // (mergekernel.cuh:141-197) and the per-word list of marked nodes.
-template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WIDE>
-__device__ __forceinline__ void rev_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
-                                         const LevelMap &mp, uint32_t start, uint32_t *__restrict__ first_t0,
+// One (gate, word) item of the reverse pass.
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+__device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
+                                         uint32_t g, uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
                                          uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
-                                         uint32_t *__restrict__ list_cnt, uint64_t list_stride, uint32_t blk) {
-  uint32_t g, w;
-  if (!map_item(mp, blk, g, w)) return;
-  if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
+                                         uint32_t *__restrict__ list_cnt, uint64_t list_stride) {
   // first memory trip: this node's mark-present word and record
   const uint64_t nzw = pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)];
   const GateRec r = recs[g];
@@ -455,6 +453,17 @@ __device__ __forceinline__ void rev_body(const DevNet &net, const GateRec *__res
     const uint32_t pos = atomicAdd(&list_cnt[w], 1u);
     list[(uint64_t)w * list_stride + pos] = g;
   }
+}
+
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WIDE>
+__device__ __forceinline__ void rev_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
+                                         const LevelMap &mp, uint32_t start, uint32_t *__restrict__ first_t0,
+                                         uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                         uint32_t *__restrict__ list_cnt, uint64_t list_stride, uint32_t blk) {
+  uint32_t g, w;
+  if (!map_item(mp, blk, g, w)) return;
+  if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
+  rev_item<SEQ, DO_REMOVE, DO_MERGE>(net, recs, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride);
 }
 
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WIDE>
@@ -1200,16 +1209,18 @@ __device__ __forceinline__ uint32_t find_seg(const BatchSeg *__restrict__ segs, 
   return lo;
 }
 
-template <bool ROBUST, bool WIDE>
+// NW = 2 (whole-wave tiles only): segments of 128-word tiles, each lane
+// computing words w and w + 64.
+template <bool ROBUST, bool WIDE, int NW>
 __global__ __launch_bounds__(kBlock) void k_fwd_batch(const BatchCircuit *__restrict__ circ,
                                                       const BatchSeg *__restrict__ segs, uint32_t nseg) {
   const BatchSeg sg = segs[find_seg(segs, nseg, blockIdx.x)];
   const BatchCircuit &c = circ[sg.circuit];
   const DevNet net = c.net;
   const Planes pl = c.pl;
-  fwd_body<true, true, true, ROBUST, WIDE, 1>(net, c.recs, pl, c.va, nullptr, c.vstride, 0, sg.mp,
-                                              sg.clear ? c.cstate : nullptr, sg.clear ? c.n_cstate : 0,
-                                              blockIdx.x - sg.block0, sg.nblocks);
+  fwd_body<true, true, true, ROBUST, WIDE, NW>(net, c.recs, pl, c.va, nullptr, c.vstride, 0, sg.mp,
+                                               sg.clear ? c.cstate : nullptr, sg.clear ? c.n_cstate : 0,
+                                               blockIdx.x - sg.block0, sg.nblocks);
 }
 
 template <bool WIDE>
@@ -1221,6 +1232,72 @@ __global__ __launch_bounds__(kBlock) void k_rev_batch(const BatchCircuit *__rest
   const Planes pl = c.pl;
   rev_body<true, true, true, WIDE>(net, c.recs, pl, sg.mp, 0, c.first0, c.first1, c.list, c.list_cnt,
                                    c.list_stride, blockIdx.x - sg.block0);
+}
+
+// Compacted reverse pass (PDF_TUNE_REV_COMPACT): most (gate, 64-word tile)
+// items of a level hold no mark at all (their mark-present word is zero), yet
+// a plain level launch spends a wave on each.  A scan launch, thread per
+// item, appends the items with a non-zero mark-present word to a work list;
+// a work launch, a fixed grid of waves striding over the list, runs the same
+// per-(gate, word) reverse logic on them, lane = word of the tile.
+struct BatchScanSeg {
+  uint32_t circuit, lo, count, tiles;  // items = count gates x tiles 64-word tiles
+  uint32_t block0, pad;
+};
+struct RevItem {
+  uint32_t circuit, gate, tile;
+};
+
+__device__ __forceinline__ uint32_t find_scan_seg(const BatchScanSeg *__restrict__ segs, uint32_t nseg, uint32_t b) {
+  uint32_t lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (segs[mid].block0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_rev_scan_batch(const BatchCircuit *__restrict__ circ,
+                                                        const BatchScanSeg *__restrict__ segs, uint32_t nseg,
+                                                        RevItem *__restrict__ work, uint32_t *__restrict__ work_cnt) {
+  const BatchScanSeg sg = segs[find_scan_seg(segs, nseg, blockIdx.x)];
+  const uint64_t item = (uint64_t)(blockIdx.x - sg.block0) * 256 + threadIdx.x;
+  bool marked = false;
+  uint32_t g = 0, t = 0;
+  if (item < (uint64_t)sg.count * sg.tiles) {
+    g = sg.lo + (uint32_t)(item / sg.tiles);
+    t = (uint32_t)(item % sg.tiles);
+    const Planes &pl = circ[sg.circuit].pl;
+    marked = pl.NZ[(uint64_t)g * pl.ntiles + t] != 0;
+  }
+  const uint64_t bal = __ballot(marked);
+  if (!bal) return;  // whole wave
+  const uint32_t lane = threadIdx.x & 63;
+  const int leader = (int)__builtin_ctzll(bal);
+  uint32_t base = 0;
+  if ((int)lane == leader) base = atomicAdd(work_cnt, (uint32_t)__builtin_popcountll(bal));
+  base = __shfl(base, leader);
+  if (marked) work[base + (uint32_t)__builtin_popcountll(bal & ((1ULL << lane) - 1))] = RevItem{sg.circuit, g, t};
+}
+
+__global__ __launch_bounds__(256) void k_rev_work_batch(const BatchCircuit *__restrict__ circ,
+                                                        const RevItem *__restrict__ work,
+                                                        const uint32_t *__restrict__ work_cnt) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t nwaves = gridDim.x * 4;
+  const uint32_t cnt = *work_cnt;
+  for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < cnt; i += nwaves) {
+    const RevItem it = work[i];
+    const uint32_t ci = __builtin_amdgcn_readfirstlane(it.circuit);
+    const uint32_t g = __builtin_amdgcn_readfirstlane(it.gate);
+    const uint32_t t = __builtin_amdgcn_readfirstlane(it.tile);
+    const BatchCircuit &c = circ[ci];
+    const uint32_t w = t * 64 + lane;
+    if (w < c.pl.words)
+      rev_item<true, true, true>(c.net, c.recs, c.pl, g, w, 0, c.first0, c.first1, c.list, c.list_cnt,
+                                 c.list_stride);
+  }
 }
 
 // Thread t = node t - node_base[c] of circuit c (node_base: prefix sums).
@@ -3045,8 +3122,16 @@ struct pdf_batch {
   uint32_t *d_errs = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_robust = -1;
+  // compacted reverse pass: scan segments per level, the work list and its
+  // per-level counters
+  BatchScanSeg *d_scan = nullptr;
+  std::vector<uint32_t> scan_off, scan_cnt, scan_grid, work_grid;  // per level
+  RevItem *d_work = nullptr;
+  uint32_t *d_work_cnt = nullptr;
   // tuning knobs (pdf_batch_set_tuning / pdf_batch_autotune)
-  uint32_t tune_cover_waves = 512, tune_order = 1, tune_use_graph = 1;
+  uint32_t tune_cover_waves = 512, tune_order = 1, tune_use_graph = 1, tune_rev_compact = 1, tune_wpl = 2;
+  // words per lane of the forward launches: 2 only with whole-wave tiles
+  uint32_t fwd_nw() const { return wide && tune_wpl == 2 ? 2 : 1; }
 };
 
 namespace {
@@ -3078,6 +3163,11 @@ int batch_build_segments(pdf_batch *b) {
         const uint32_t words = (uint32_t)((g->num_pairs + 63) / 64);
         BatchSeg sg{};
         sg.mp = level_map(words, lo, count, pass == 0, &sg.nblocks);
+        if (pass == 0 && b->fwd_nw() == 2) {  // 128-word tiles, one node per wave
+          sg.mp.T = (words + 127) / 128;
+          sg.mp.main_blocks = sg.mp.T * ((count + (kBlock >> 6) - 1) / (kBlock >> 6));
+          sg.nblocks = sg.mp.main_blocks + grid_for(count, kBlock);
+        }
         sg.circuit = c;
         sg.clear = pass == 0 && k == g->nl.L;
         lv.push_back(sg);
@@ -3100,6 +3190,44 @@ int batch_build_segments(pdf_batch *b) {
   if (!rc && !hip_ok(hipMemcpy(b->d_segs, all.data(), all.size() * sizeof(BatchSeg), hipMemcpyHostToDevice),
                      "hipMemcpy"))
     rc = PDF_ERR_HIP;
+  if (rc) return rc;
+  // the compacted reverse pass: per level, one scan segment per circuit with
+  // that level (items = its gates x its 64-word tiles)
+  std::vector<BatchScanSeg> scan;
+  b->scan_off.assign(b->maxL + 1, 0);
+  b->scan_cnt.assign(b->maxL + 1, 0);
+  b->scan_grid.assign(b->maxL + 1, 0);
+  b->work_grid.assign(b->maxL + 1, 0);
+  uint64_t cap = 1;
+  for (uint32_t k = 0; k <= b->maxL; ++k) {
+    b->scan_off[k] = (uint32_t)scan.size();
+    uint64_t blk = 0, items = 0;
+    for (uint32_t c = 0; c < b->C; ++c) {
+      const pdf_grader *g = b->members[c];
+      if (k > g->nl.L) continue;
+      const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
+      const uint32_t tiles = (uint32_t)(((g->num_pairs + 63) / 64 + 63) / 64);
+      BatchScanSeg sg{c, lo, count, tiles, (uint32_t)blk, 0};
+      scan.push_back(sg);
+      items += (uint64_t)count * tiles;
+      blk += ((uint64_t)count * tiles + 255) / 256;
+    }
+    b->scan_cnt[k] = (uint32_t)scan.size() - b->scan_off[k];
+    b->scan_grid[k] = (uint32_t)blk;
+    // up to one item per wave, at most 2,048 blocks (8,192 waves: one round)
+    b->work_grid[k] = (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(1, (items + 3) / 4));
+    cap = std::max(cap, items);
+  }
+  dfree(b->d_scan);
+  dfree(b->d_work);
+  dfree(b->d_work_cnt);
+  rc = dalloc(&b->d_scan, std::max<size_t>(1, scan.size()));
+  if (!rc) rc = dalloc(&b->d_work, cap);
+  if (!rc) rc = dalloc(&b->d_work_cnt, b->maxL + 1);
+  if (!rc && !scan.empty() &&
+      !hip_ok(hipMemcpy(b->d_scan, scan.data(), scan.size() * sizeof(BatchScanSeg), hipMemcpyHostToDevice),
+              "hipMemcpy"))
+    rc = PDF_ERR_HIP;
   return rc;
 }
 
@@ -3116,17 +3244,31 @@ template <bool R>
 void batch_queue_t(pdf_batch *b) {
   hipStream_t st = b->stream;
   k_reset_batch<<<grid_for(b->total_nodes, 256), 256, 0, st>>>(b->d_circ, b->d_node_base, b->C, b->total_nodes);
+  const uint32_t nw = b->fwd_nw();
   for (uint32_t k = 0; k <= b->maxL; ++k) {
-    if (b->wide)
-      k_fwd_batch<R, true><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->fwd_off[k], b->fwd_cnt[k]);
+    const BatchSeg *sg = b->d_segs + b->fwd_off[k];
+    if (nw == 2)
+      k_fwd_batch<R, true, 2><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, sg, b->fwd_cnt[k]);
+    else if (b->wide)
+      k_fwd_batch<R, true, 1><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, sg, b->fwd_cnt[k]);
     else
-      k_fwd_batch<R, false><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->fwd_off[k], b->fwd_cnt[k]);
+      k_fwd_batch<R, false, 1><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, sg, b->fwd_cnt[k]);
   }
-  for (uint32_t k = b->maxL + 1; k-- > 0;) {
-    if (b->wide)
-      k_rev_batch<true><<<b->rev_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->rev_off[k], b->rev_cnt[k]);
-    else
-      k_rev_batch<false><<<b->rev_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->rev_off[k], b->rev_cnt[k]);
+  if (b->tune_rev_compact) {
+    k_fill_u32<<<1, 256, 0, st>>>(b->d_work_cnt, b->maxL + 1, 0u);
+    for (uint32_t k = b->maxL + 1; k-- > 0;) {
+      if (!b->scan_grid[k]) continue;
+      k_rev_scan_batch<<<b->scan_grid[k], 256, 0, st>>>(b->d_circ, b->d_scan + b->scan_off[k], b->scan_cnt[k],
+                                                       b->d_work, b->d_work_cnt + k);
+      k_rev_work_batch<<<b->work_grid[k], 256, 0, st>>>(b->d_circ, b->d_work, b->d_work_cnt + k);
+    }
+  } else {
+    for (uint32_t k = b->maxL + 1; k-- > 0;) {
+      if (b->wide)
+        k_rev_batch<true><<<b->rev_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->rev_off[k], b->rev_cnt[k]);
+      else
+        k_rev_batch<false><<<b->rev_grid[k], kBlock, 0, st>>>(b->d_circ, b->d_segs + b->rev_off[k], b->rev_cnt[k]);
+    }
   }
   k_events_batch<<<grid_for(b->total_nodes, 256), 256, 0, st>>>(b->d_circ, b->d_node_base, b->C, b->total_nodes);
   k_cover_batch<<<b->cover_waves, 64, 0, st>>>(b->d_circ, b->C, b->d_cs, b->d_hs, b->max_n);
@@ -3276,6 +3418,9 @@ void pdf_batch_destroy(pdf_batch *b) {
   dfree(b->d_circ);
   dfree(b->d_node_base);
   dfree(b->d_segs);
+  dfree(b->d_scan);
+  dfree(b->d_work);
+  dfree(b->d_work_cnt);
   dfree(b->d_cs);
   dfree(b->d_hs);
   dfree(b->d_totals);
@@ -3329,6 +3474,16 @@ int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value) {
       b->tune_order = (uint32_t)value;
       batch_drop_graph(b);
       return batch_build_segments(b);
+    case PDF_TUNE_REV_COMPACT:
+      if (value != 0 && value != 1) break;
+      b->tune_rev_compact = (uint32_t)value;
+      batch_drop_graph(b);
+      return PDF_OK;
+    case PDF_TUNE_WORDS_PER_LANE:
+      if (value != 1 && value != 2) break;
+      b->tune_wpl = (uint32_t)value;
+      batch_drop_graph(b);
+      return batch_build_segments(b);
     default:
       set_error("pdf_batch_set_tuning: unknown key");
       return PDF_ERR_ARG;
@@ -3339,38 +3494,67 @@ int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value) {
 
 int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best) {
   if (!b || reps == 0) { set_error("pdf_batch_autotune: bad argument"); return PDF_ERR_ARG; }
-  const uint32_t waves[] = {256, 512, 1024, 2048};
+  // one knob at a time over its values, the others held at their best so far;
+  // two sweeps, each configuration timed once
+  struct Knob {
+    int key;
+    std::vector<uint32_t> values;
+  };
+  const Knob knobs[] = {{PDF_TUNE_SEGMENT_ORDER, {0, 1}},
+                        {PDF_TUNE_COVER_WAVES, {256, 512, 1024, 2048}},
+                        {PDF_TUNE_USE_GRAPH, {0, 1}},
+                        {PDF_TUNE_REV_COMPACT, {0, 1}},
+                        {PDF_TUNE_WORDS_PER_LANE, b->wide ? std::vector<uint32_t>{1, 2} : std::vector<uint32_t>{1}}};
+  constexpr int kKnobs = 5;
+  uint32_t cur[kKnobs] = {b->tune_order, b->tune_cover_waves, b->tune_use_graph, b->tune_rev_compact,
+                          b->wide ? b->tune_wpl : 1u};
+  std::vector<std::pair<std::vector<uint32_t>, float>> seen;
   std::vector<uint64_t> ref(b->C), tot(b->C);
   bool have_ref = false;
-  pdf_batch_tuning bt{};
-  bt.ms = 1e30f;
   int rc;
-  for (uint32_t order = 0; order < 2; ++order)
-    for (uint32_t wv : waves)
-      for (uint32_t graph = 0; graph < 2; ++graph) {
-        if ((rc = pdf_batch_set_tuning(b, PDF_TUNE_SEGMENT_ORDER, order)) ||
-            (rc = pdf_batch_set_tuning(b, PDF_TUNE_COVER_WAVES, wv)) ||
-            (rc = pdf_batch_set_tuning(b, PDF_TUNE_USE_GRAPH, graph)))
-          return rc;
-        std::vector<float> t(reps + 1);
-        for (uint32_t r = 0; r <= reps; ++r)  // the first run warms up (and captures)
-          if ((rc = pdf_batch_grade(b, robust, tot.data(), &t[r]))) return rc;
-        if (!have_ref) {
-          ref = tot;
-          have_ref = true;
-        } else if (tot != ref) {
-          set_error("pdf_batch_autotune: totals differ between configurations");
-          return PDF_ERR_STATE;
+  auto apply = [&](const uint32_t *v) {
+    int r = PDF_OK;
+    for (int i = 0; i < kKnobs && !r; ++i) r = pdf_batch_set_tuning(b, knobs[i].key, v[i]);
+    return r;
+  };
+  auto timed = [&](const uint32_t *v, float *med) {
+    const std::vector<uint32_t> key(v, v + kKnobs);
+    for (const auto &e : seen)
+      if (e.first == key) { *med = e.second; return PDF_OK; }
+    int r = apply(v);
+    if (r) return r;
+    std::vector<float> t(reps + 1);
+    for (uint32_t i = 0; i <= reps; ++i)  // the first run warms up (and captures)
+      if ((r = pdf_batch_grade(b, robust, tot.data(), &t[i]))) return r;
+    if (!have_ref) {
+      ref = tot;
+      have_ref = true;
+    } else if (tot != ref) {
+      set_error("pdf_batch_autotune: totals differ between configurations");
+      return (int)PDF_ERR_STATE;
+    }
+    std::sort(t.begin() + 1, t.end());
+    *med = t[1 + reps / 2];
+    seen.emplace_back(key, *med);
+    return PDF_OK;
+  };
+  float best_ms;
+  if ((rc = timed(cur, &best_ms))) return rc;
+  for (int sweep = 0; sweep < 2; ++sweep)
+    for (int i = 0; i < kKnobs; ++i)
+      for (uint32_t v : knobs[i].values) {
+        uint32_t trial[kKnobs];
+        std::copy(cur, cur + kKnobs, trial);
+        trial[i] = v;
+        float ms;
+        if ((rc = timed(trial, &ms))) return rc;
+        if (ms < best_ms) {
+          best_ms = ms;
+          std::copy(trial, trial + kKnobs, cur);
         }
-        std::sort(t.begin() + 1, t.end());
-        const float med = t[1 + reps / 2];
-        if (med < bt.ms) bt = pdf_batch_tuning{order, wv, graph, med};
       }
-  if ((rc = pdf_batch_set_tuning(b, PDF_TUNE_SEGMENT_ORDER, bt.segment_order)) ||
-      (rc = pdf_batch_set_tuning(b, PDF_TUNE_COVER_WAVES, bt.cover_waves)) ||
-      (rc = pdf_batch_set_tuning(b, PDF_TUNE_USE_GRAPH, bt.use_graph)))
-    return rc;
-  if (best) *best = bt;
+  if ((rc = apply(cur))) return rc;
+  if (best) *best = pdf_batch_tuning{cur[0], cur[1], cur[2], cur[3], cur[4], best_ms};
   return PDF_OK;
 }
 

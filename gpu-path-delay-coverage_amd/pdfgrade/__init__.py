@@ -30,7 +30,7 @@ UNKN, INPT, AND, NAND, OR, NOR, XOR, XNOR, BUFF, NOT, FROM, DFF = range(12)
 TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF", "NOT", "FROM", "DFF"]
 S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
-TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES = 1, 2, 3, 4, 5
+TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES, TUNE_REV_COMPACT = 1, 2, 3, 4, 5, 6
 DEBUG_FORWARD_MARKS, DEBUG_PI_COVER = 1, 2
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
@@ -574,7 +574,7 @@ MLP_PARAMS = 197377
 
 class _BatchTuning(C.Structure):
     _fields_ = [("segment_order", C.c_uint32), ("cover_waves", C.c_uint32), ("use_graph", C.c_uint32),
-                ("ms", C.c_float)]
+                ("rev_compact", C.c_uint32), ("words_per_lane", C.c_uint32), ("ms", C.c_float)]
 
 
 class Batch:

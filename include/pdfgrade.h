@@ -250,10 +250,11 @@ int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t 
  * simkernel.cuh:29-30, markkernel.cuh:7-8, mergekernel.cuh:49-52).
  * Results never depend on them. */
 #define PDF_TUNE_WORDS_PER_LANE 1 /* 1 or 2 words per lane in the fused forward pass */
-#define PDF_TUNE_USE_GRAPH 2
-#define PDF_TUNE_COVER_WAVES 3   /* batch: waves of the cover launch */
-#define PDF_TUNE_LANES 5         /* grader: chunks graded concurrently on 1..4 streams */
-#define PDF_TUNE_SEGMENT_ORDER 4 /* batch: 0 circuit order, 1 largest segments first */      /* 1: pdf_grade replays a captured HIP graph */
+#define PDF_TUNE_USE_GRAPH 2      /* 1: pdf_grade / pdf_batch_grade replay a captured HIP graph */
+#define PDF_TUNE_COVER_WAVES 3    /* batch: waves of the cover launch */
+#define PDF_TUNE_SEGMENT_ORDER 4  /* batch: 0 circuit order, 1 largest segments first */
+#define PDF_TUNE_LANES 5          /* grader: chunks graded concurrently on 1..4 streams */
+#define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same
@@ -402,7 +403,7 @@ int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf_grader_tun
 /* ------------------------------------------------------------------------ */
 typedef struct pdf_batch pdf_batch;
 typedef struct {
-  uint32_t segment_order, cover_waves, use_graph;
+  uint32_t segment_order, cover_waves, use_graph, rev_compact, words_per_lane;
   float ms; /* median grade time of this configuration */
 } pdf_batch_tuning;
 
@@ -417,10 +418,12 @@ int pdf_batch_grade(pdf_batch *b, int robust, uint64_t *totals, float *ms);
 /* Circuit c's grader, for the getters (pdf_get_first, pdf_get_gate_counts,
  * pdf_get_total) after pdf_batch_grade; owned by the batch. */
 int pdf_batch_member(pdf_batch *b, uint32_t circuit, pdf_grader **out);
-/* PDF_TUNE_USE_GRAPH, PDF_TUNE_COVER_WAVES, PDF_TUNE_SEGMENT_ORDER. */
+/* PDF_TUNE_USE_GRAPH, PDF_TUNE_COVER_WAVES, PDF_TUNE_SEGMENT_ORDER, PDF_TUNE_REV_COMPACT and
+ * PDF_TUNE_WORDS_PER_LANE (2 takes effect only when every circuit has at least 4,096 pairs). */
 int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value);
-/* Time every configuration of the knobs (reps runs each, median), check that
- * all give the same totals, keep the fastest and report it in *best. */
+/* Tune the knobs one at a time (each value timed over reps runs, median; two
+ * sweeps over the knobs), check that every configuration gives the same
+ * totals, keep the fastest and report it in *best. */
 int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best);
 
 #ifdef __cplusplus

@@ -35,17 +35,19 @@ def test_small_batch_matches_oracle(robust):
     nets = [pg.Netlist.generate(c) for c, _ in specs]
     vecs = [pg.gen_vectors(400 + i, c.num_pi, p) for i, (c, p) in enumerate(specs)]
     b = pg.Batch(nets, vecs, [p for _, p in specs])
-    tot, ms = b.grade(robust)
-    for i, (nl, v, (c, p)) in enumerate(zip(nets, vecs, specs)):
-        want = orc.grade(orc.OracleNet(nl.arrays()), v, p, p, robust)
-        assert tot[i] == want["total"], i
-        m = b.member(i)
-        f0, f1 = m.first()
-        np.testing.assert_array_equal(f0, want["first_t0"])
-        np.testing.assert_array_equal(f1, want["first_t1"])
-        np.testing.assert_array_equal(m.gate_counts(), want["gate_counts"])
-        assert m.total() == want["total"]
-    assert any(t > 0 for t in tot)
+    wants = [orc.grade(orc.OracleNet(nl.arrays()), v, p, p, robust) for nl, v, (_, p) in zip(nets, vecs, specs)]
+    for compact in (1, 0):  # reverse pass over the marked-item list, then over every item
+        b.set_tuning(pg.TUNE_REV_COMPACT, compact)
+        tot, ms = b.grade(robust)
+        for i, want in enumerate(wants):
+            assert tot[i] == want["total"], (compact, i)
+            m = b.member(i)
+            f0, f1 = m.first()
+            np.testing.assert_array_equal(f0, want["first_t0"])
+            np.testing.assert_array_equal(f1, want["first_t1"])
+            np.testing.assert_array_equal(m.gate_counts(), want["gate_counts"])
+            assert m.total() == want["total"]
+        assert any(t > 0 for t in tot)
     b.close()
 
 
@@ -76,24 +78,60 @@ def test_wide_batch_matches_plain_grading_and_tuning():
     for order in (0, 1):
         for waves in (64, 1024):
             for graph in (0, 1):
-                b.set_tuning(pg.TUNE_SEGMENT_ORDER, order)
-                b.set_tuning(pg.TUNE_COVER_WAVES, waves)
-                b.set_tuning(pg.TUNE_USE_GRAPH, graph)
-                for rep in range(2):
-                    tot, _ = b.grade(True)
-                    np.testing.assert_array_equal(tot, np.array([w[0] for w in want], np.uint64),
-                                                  err_msg=f"order {order} waves {waves} graph {graph} rep {rep}")
-    for i, w in enumerate(want):
-        m = b.member(i)
-        f0, f1 = m.first()
-        np.testing.assert_array_equal(f0, w[1])
-        np.testing.assert_array_equal(f1, w[2])
-        np.testing.assert_array_equal(m.gate_counts(), w[3])
+                for compact, wpl in ((0, 1), (1, 1), (0, 2), (1, 2)):
+                    b.set_tuning(pg.TUNE_SEGMENT_ORDER, order)
+                    b.set_tuning(pg.TUNE_COVER_WAVES, waves)
+                    b.set_tuning(pg.TUNE_USE_GRAPH, graph)
+                    b.set_tuning(pg.TUNE_REV_COMPACT, compact)
+                    b.set_tuning(pg.TUNE_WORDS_PER_LANE, wpl)
+                    ctx = f"order {order} waves {waves} graph {graph} compact {compact} wpl {wpl}"
+                    for rep in range(2):
+                        tot, _ = b.grade(True)
+                        np.testing.assert_array_equal(tot, np.array([w[0] for w in want], np.uint64),
+                                                      err_msg=f"{ctx} rep {rep}")
+                    for i, w in enumerate(want):
+                        m = b.member(i)
+                        f0, f1 = m.first()
+                        np.testing.assert_array_equal(f0, w[1], err_msg=ctx)
+                        np.testing.assert_array_equal(f1, w[2], err_msg=ctx)
+                        np.testing.assert_array_equal(m.gate_counts(), w[3], err_msg=ctx)
     best = b.autotune(True, reps=2)
     _say("autotune", best)
-    assert best["cover_waves"] in (256, 512, 1024, 2048) and best["ms"] > 0
+    assert best["cover_waves"] in (256, 512, 1024, 2048) and best["rev_compact"] in (0, 1) and best["ms"] > 0
+    assert best["words_per_lane"] in (1, 2)
     tot, _ = b.grade(True)
     np.testing.assert_array_equal(tot, np.array([w[0] for w in want], np.uint64))
+    b.close()
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_wide_batch_ragged_pairs_two_words_per_lane(robust):
+    """Pair counts that are not multiples of 64 and word counts that are not
+    multiples of 128 (the second half of a 128-word tile partly or wholly
+    past the end): every reverse and forward mode equals pdf_grade."""
+    _say("ragged wide batch", robust)
+    specs = [(_wide_cfg(3301, 3000, 92), 4305), (_wide_cfg(3302, 6000, 100), 8191),
+             (_wide_cfg(3303, 2500, 90), 4096 + 64 * 70 + 5), (_wide_cfg(3304, 9000, 105), 12345)]
+    nets = [pg.Netlist.generate(c) for c, _ in specs]
+    vecs = [pg.gen_vectors(13300 + i, c.num_pi, p) for i, (c, p) in enumerate(specs)]
+    want = [_plain(nl, v, p, robust) for nl, v, (_, p) in zip(nets, vecs, specs)]
+    assert any(w[0] > 0 for w in want)
+    b = pg.Batch(nets, vecs, [p for _, p in specs])
+    for wpl in (2, 1):
+        for compact in (1, 0):
+            b.set_tuning(pg.TUNE_WORDS_PER_LANE, wpl)
+            b.set_tuning(pg.TUNE_REV_COMPACT, compact)
+            tot, _ = b.grade(robust)
+            ctx = f"wpl {wpl} compact {compact}"
+            np.testing.assert_array_equal(tot, np.array([w[0] for w in want], np.uint64), err_msg=ctx)
+            for i, w in enumerate(want):
+                m = b.member(i)
+                f0, f1 = m.first()
+                np.testing.assert_array_equal(f0, w[1], err_msg=ctx)
+                np.testing.assert_array_equal(f1, w[2], err_msg=ctx)
+                np.testing.assert_array_equal(m.gate_counts(), w[3], err_msg=ctx)
+    with pytest.raises(pg.PdfError):
+        b.set_tuning(pg.TUNE_WORDS_PER_LANE, 3)
     b.close()
 
 

@@ -107,6 +107,16 @@ __device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t b, uint32_
 
 __device__ __forceinline__ uint64_t shift_next(uint64_t cur, uint64_t nxt) { return (cur >> 1) | (nxt << 63); }
 
+// Holds x in a register at this point of the program (an empty asm statement
+// with x as input; no instruction is emitted): the loads x depends on complete
+// here, so the compiler cannot sink them past a later early exit, and loads
+// issued together before it share one round trip.
+__device__ __forceinline__ void pin(uint64_t x) { asm volatile("" : : "v"(x)); }
+__device__ __forceinline__ void pin(uint32_t x) { asm volatile("" : : "v"(x)); }
+// Like pin, and every later use of x reads it after this point, so work on x
+// cannot be moved up between loads issued before it.
+__device__ __forceinline__ void hold(uint64_t &x) { asm volatile("" : "+v"(x)); }
+
 // Frames (a, b) of node f at word w.
 template <bool SEQ>
 __device__ __forceinline__ void frames(const Planes &pl, uint32_t f, uint32_t w, uint64_t &a, uint64_t &b) {
@@ -242,6 +252,11 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
   }
   if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
   const GateRec r = recs[g];
+  // the whole record in the first round trip (the compiler would otherwise
+  // fetch the fanin ids only after testing the type)
+  pin((uint32_t)r.type);
+  pin((uint32_t)r.nfi);
+  pin(r.fin[0]);
   const uint8_t ty = r.type;
   bool live[NW];
 #pragma unroll
@@ -283,18 +298,31 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
       nz0[u] = DO_MARK && live[u] ? pl.NZ[(uint64_t)r.fin[0] * pl.ntiles + ((w + 64 * u) >> 6)] : 0;
     uint64_t fa[NW][kRegFanin], fb[NW][kRegFanin];  // constant indices only
     if (DO_SIM) {
+      // the fanins' raw words, all issued before any is used (one round trip):
+      // frame 1, and frame 2's source (the next word in sequence mode, the
+      // second plane otherwise)
 #pragma unroll
       for (int j = 0; j < kRegFanin; ++j) {
+#pragma unroll
+        for (int u = 0; u < NW; ++u) fa[u][j] = fb[u][j] = 0;
         if (j == 0 || (multi && (uint32_t)j < nfi)) {
 #pragma unroll
           for (int u = 0; u < NW; ++u) {
-            fa[u][j] = fb[u][j] = 0;
             if (!live[u]) continue;
-            if (kBoth) frames<SEQ>(pl, r.fin[j], w + 64 * u, fa[u][j], fb[u][j]);
-            else fa[u][j] = pl.V[(uint64_t)r.fin[j] * pl.pitch + w + 64 * u];
+            const uint64_t at = (uint64_t)r.fin[j] * pl.pitch + w + 64 * u;
+            fa[u][j] = pl.V[at];
+            if (kBoth) fb[u][j] = SEQ ? pl.V[at + 1] : pl.VB[at];
           }
         }
       }
+#pragma unroll
+      for (int j = 0; j < kRegFanin; ++j)
+#pragma unroll
+        for (int u = 0; u < NW; ++u) {
+          hold(fa[u][j]);
+          if (kBoth) hold(fb[u][j]);
+          if (kBoth && SEQ) fb[u][j] = shift_next(fa[u][j], fb[u][j]);
+        }
 #pragma unroll
       for (int u = 0; u < NW; ++u) {
         a[u] = fa[u][0];
@@ -408,20 +436,52 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
   // first memory trip: this node's mark-present word and record
   const uint64_t nzw = pl.NZ[(uint64_t)g * pl.ntiles + (w >> 6)];
   const GateRec r = recs[g];
+  if (DO_REMOVE) {
+    pin(r.fin[0]);
+    pin((uint32_t)r.nfi);
+    pin((uint32_t)r.po);
+  }
   if (!((nzw >> (w & 63)) & 1)) return;
   const uint64_t row = (uint64_t)g * pl.pitch + w;
   // second trip: mark, accumulator, the fanins' mark-present words and, for the
   // merge, both frames
   uint64_t m = pl.M[row];
   const uint64_t acc = DO_REMOVE ? pl.ACC[row] : 0;
-  uint64_t nzf[kRegFanin];
+  // with them, the first fanins' mark words, read whatever their mark-present
+  // bits say (a word whose bit is clear is ignored below): all of these loads
+  // go out together instead of one round trip per fanin
+  // (the merge's frames as raw words: frame 1 and frame 2's source)
+  uint64_t a = 0, b = 0;
+  if (DO_MERGE) {
+    a = pl.V[row];
+    b = SEQ ? pl.V[row + 1] : pl.VB[row];
+  }
+  uint64_t mf[kRegFanin], nzf[kRegFanin];
   if (DO_REMOVE) {
 #pragma unroll
-    for (int j = 0; j < kRegFanin; ++j)
-      nzf[j] = (uint32_t)j < r.nfi ? pl.NZ[(uint64_t)r.fin[j] * pl.ntiles + (w >> 6)] : 0;
+    for (int j = 0; j < kRegFanin; ++j) {
+      nzf[j] = mf[j] = 0;
+      if ((uint32_t)j < r.nfi) {
+        nzf[j] = pl.NZ[(uint64_t)r.fin[j] * pl.ntiles + (w >> 6)];
+        mf[j] = pl.M[(uint64_t)r.fin[j] * pl.pitch + w];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j) {
+      hold(nzf[j]);
+      hold(mf[j]);
+    }
+    if (DO_MERGE) {
+      hold(a);
+      hold(b);
+    }
   }
-  uint64_t a = 0, b = 0;
-  if (DO_MERGE) frames<SEQ>(pl, g, w, a, b);
+  if (DO_MERGE && SEQ) b = shift_next(a, b);
+  if (DO_REMOVE) {
+    __builtin_amdgcn_sched_barrier(0);  // every load above is issued before any use
+    pin(m);
+    pin(acc);
+  }
   if (!m) return;
   if (DO_REMOVE) {
     if (acc) pl.ACC[row] = 0;
@@ -430,18 +490,19 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
       pl.M[row] = m;
     }
     if (m) {
-      // third trip: marks of the fanins present in this word, then the pushes
-      auto push = [&](uint32_t f, uint64_t nz) {
-        if (!((nz >> (w & 63)) & 1)) return;
-        const uint64_t mf = pl.M[(uint64_t)f * pl.pitch + w] & m;
-        if (mf) atomicOr((unsigned long long *)&pl.ACC[(uint64_t)f * pl.pitch + w], (unsigned long long)mf);
-      };
+      // the pushes: this node's final mark into ACC of each fanin marked at
+      // the same pairs (all-ones mask for a fanin whose bit is set)
+#pragma unroll
+      for (int j = 0; j < kRegFanin; ++j) mf[j] &= (uint64_t)0 - ((nzf[j] >> (w & 63)) & 1);
 #pragma unroll
       for (int j = 0; j < kRegFanin; ++j)
-        if ((uint32_t)j < r.nfi) push(r.fin[j], nzf[j]);
+        if (mf[j] & m)
+          atomicOr((unsigned long long *)&pl.ACC[(uint64_t)r.fin[j] * pl.pitch + w], (unsigned long long)(mf[j] & m));
       for (uint32_t j = kRegFanin; j < r.nfi; ++j) {
         const uint32_t f = net.fin[r.fin_off + j];
-        push(f, pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)]);
+        if (!((pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)] >> (w & 63)) & 1)) continue;
+        const uint64_t x = pl.M[(uint64_t)f * pl.pitch + w] & m;
+        if (x) atomicOr((unsigned long long *)&pl.ACC[(uint64_t)f * pl.pitch + w], (unsigned long long)x);
       }
     }
   }

@@ -20,6 +20,11 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950",
              # packed FP32 instructions stay off in every HIP build here
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # an atomic whose address is the same for the whole wave is
+             # combined across the wave with DPP row operations (a fixed few
+             # steps) instead of a loop over the active lanes; integer results
+             # are identical (first-detection minima, counts)
+             "-mllvm", "-amdgpu-atomic-optimizer-strategy=DPP",
              "-Wall", "-Wno-unused-result", f"-I{INCLUDE}"]
 LIB_SOURCES = [os.path.join(CSRC, f) for f in ("netlist.cpp", "grader.hip", "mlp.hip")]
 

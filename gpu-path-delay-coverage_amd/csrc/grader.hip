@@ -450,11 +450,15 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
   // with them, the first fanins' mark words, read whatever their mark-present
   // bits say (a word whose bit is clear is ignored below): all of these loads
   // go out together instead of one round trip per fanin
-  // (the merge's frames as raw words: frame 1 and frame 2's source)
+  // (the merge's frames as raw words: frame 1 and frame 2's source, and the
+  // node's first-detection pairs so far)
   uint64_t a = 0, b = 0;
+  uint32_t cur0 = 0, cur1 = 0;
   if (DO_MERGE) {
     a = pl.V[row];
     b = SEQ ? pl.V[row + 1] : pl.VB[row];
+    cur0 = first_t0[g];
+    cur1 = first_t1[g];
   }
   uint64_t mf[kRegFanin], nzf[kRegFanin];
   if (DO_REMOVE) {
@@ -509,8 +513,13 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
   if (DO_MERGE && m) {
     const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
     const uint32_t base = start + 64u * w;
-    if (t0) atomicMin(&first_t0[g], base + (uint32_t)__builtin_ctzll(t0));
-    if (t1) atomicMin(&first_t1[g], base + (uint32_t)__builtin_ctzll(t1));
+    // the first-detection pairs only decrease within a grade (reset by an
+    // earlier kernel), so a value read above is at least the current one: a
+    // candidate that is not below it cannot lower the minimum and is skipped
+    const uint32_t c0 = t0 ? base + (uint32_t)__builtin_ctzll(t0) : PDF_NONE;
+    const uint32_t c1 = t1 ? base + (uint32_t)__builtin_ctzll(t1) : PDF_NONE;
+    if (c0 < cur0) atomicMin(&first_t0[g], c0);
+    if (c1 < cur1) atomicMin(&first_t1[g], c1);
     const uint32_t pos = atomicAdd(&list_cnt[w], 1u);
     list[(uint64_t)w * list_stride + pos] = g;
   }

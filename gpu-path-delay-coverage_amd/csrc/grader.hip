@@ -192,13 +192,23 @@ __device__ __forceinline__ void fold(uint8_t ty, uint64_t &v, uint64_t x) {
 __device__ __forceinline__ uint64_t sim_one_frame(const DevNet &net, const Planes &pl, const GateRec &r,
                                                   uint32_t w) {
   const uint8_t ty = r.type;
-  uint64_t v = pl.V[(uint64_t)r.fin[0] * pl.pitch + w];
+  const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+  // the register fanins' words, all issued before any is folded
+  uint64_t x[kRegFanin];
+#pragma unroll
+  for (int j = 0; j < kRegFanin; ++j) {
+    x[j] = 0;
+    if (j == 0 || (multi && (uint32_t)j < r.nfi)) x[j] = pl.V[(uint64_t)r.fin[j] * pl.pitch + w];
+  }
+#pragma unroll
+  for (int j = 0; j < kRegFanin; ++j) hold(x[j]);
+  uint64_t v = x[0];
   if (ty == PDF_NOT) v = ~v;
   if (r.nfi > 1) {
-    if (ty >= PDF_AND && ty <= PDF_NOR) {
+    if (multi) {
 #pragma unroll
       for (int j = 1; j < kRegFanin; ++j)
-        if ((uint32_t)j < r.nfi) fold(ty, v, pl.V[(uint64_t)r.fin[j] * pl.pitch + w]);
+        if ((uint32_t)j < r.nfi) fold(ty, v, x[j]);
       for (uint32_t j = kRegFanin; j < r.nfi; ++j)
         fold(ty, v, pl.V[(uint64_t)net.fin[r.fin_off + j] * pl.pitch + w]);
     } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
@@ -234,6 +244,9 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
     if (gi >= mp.count) return;
     const uint32_t g = mp.lo + gi;
     const GateRec r = recs[g];
+    pin((uint32_t)r.type);
+    pin((uint32_t)r.nfi);
+    pin(r.fin[0]);
     const uint64_t v =
         r.type == PDF_INPT ? va[(uint64_t)g * vstride + wbase + mp.words] : sim_one_frame(net, pl, r, mp.words);
     pl.V[(uint64_t)g * pl.pitch + mp.words] = v;

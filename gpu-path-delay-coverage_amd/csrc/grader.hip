@@ -456,15 +456,14 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
   }
   if (!((nzw >> (w & 63)) & 1)) return;
   const uint64_t row = (uint64_t)g * pl.pitch + w;
-  // second trip: mark, accumulator, the fanins' mark-present words and, for the
-  // merge, both frames
+  // second trip, all loads issued before any is used: this node's mark and
+  // accumulator; for the merge, its frames as raw words (frame 1 and frame
+  // 2's source) and its first-detection pairs so far; for the removal, the
+  // register fanins' mark-present words and mark words (a mark word whose
+  // bit is clear is read anyway and ignored below, instead of costing one
+  // more round trip per fanin)
   uint64_t m = pl.M[row];
   const uint64_t acc = DO_REMOVE ? pl.ACC[row] : 0;
-  // with them, the first fanins' mark words, read whatever their mark-present
-  // bits say (a word whose bit is clear is ignored below): all of these loads
-  // go out together instead of one round trip per fanin
-  // (the merge's frames as raw words: frame 1 and frame 2's source, and the
-  // node's first-detection pairs so far)
   uint64_t a = 0, b = 0;
   uint32_t cur0 = 0, cur1 = 0;
   if (DO_MERGE) {

@@ -890,12 +890,21 @@ __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__
   uint32_t best0 = PDF_NONE, best1 = PDF_NONE;  // this lane's segment
   for (uint32_t g0 = 0; g0 < nblk; g0 += lps) {
     const uint32_t blk = g0 + sub;
+    const bool want = valid && blk < nblk && (best0 == PDF_NONE || best1 == PDF_NONE);
+    // the nodes' mark-present words, all issued before they are combined
+    uint64_t z[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      z[i] = ~0ULL;
+      if (want && ids[i] != PDF_NONE) z[i] = pl.NZ[(uint64_t)ids[i] * pl.ntiles + blk];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) hold(z[i]);
     uint64_t nzl = 0;
-    if (valid && blk < nblk && (best0 == PDF_NONE || best1 == PDF_NONE)) {
+    if (want) {
       nzl = ~0ULL;
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (ids[i] != PDF_NONE) nzl &= pl.NZ[(uint64_t)ids[i] * pl.ntiles + blk];
+      for (int i = 0; i < 8; ++i) nzl &= z[i];
     }
     uint64_t todo = __ballot(nzl != 0);
     while (todo) {
@@ -913,15 +922,31 @@ __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__
 #pragma unroll
       for (int i = 0; i < 8; ++i) gid[i] = __builtin_amdgcn_readfirstlane(__shfl(ids[i], l));
       const uint32_t w = (g0 + ((uint32_t)l & (lps - 1))) * 64 + lane;
+      // the nodes' mark words and node 0's frames (raw: frame 1 and frame
+      // 2's source), all issued before any is used: one round trip a visit
+      const bool on = w < pl.words && ((nz >> lane) & 1);
+      uint64_t x[8], a = 0, b = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        x[i] = ~0ULL;
+        if (on && gid[i] != PDF_NONE) x[i] = pl.M[(uint64_t)gid[i] * pl.pitch + w];
+      }
+      if (on && gid[0] != PDF_NONE) {
+        const uint64_t at = (uint64_t)gid[0] * pl.pitch + w;
+        a = pl.V[at];
+        b = SEQ ? pl.V[at + 1] : pl.VB[at];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) hold(x[i]);
+      hold(a);
+      hold(b);
+      if (SEQ) b = shift_next(a, b);
       uint64_t m = 0;
-      if (w < pl.words && ((nz >> lane) & 1)) {
+      if (on) {
         m = ~0ULL;
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-          if (gid[i] != PDF_NONE) m &= pl.M[(uint64_t)gid[i] * pl.pitch + w];
+        for (int i = 0; i < 8; ++i) m &= x[i];
       }
-      uint64_t a = 0, b = 0;
-      if (m) frames<SEQ>(pl, gid[0], w, a, b);
       const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
       const uint32_t base = start + 64u * w;
       const uint32_t c0 = t0 ? base + (uint32_t)__builtin_ctzll(t0) : PDF_NONE;

@@ -1075,11 +1075,13 @@ __device__ __forceinline__ uint32_t vc_get(const uint64_t (&c)[kCountBits], uint
 // Level features: block = kFeatWords words x kFeatSlices slices of one
 // feature f, blockIdx.y mapped to the simulated levels' transition features
 // and then their forward-mark features.  Each slice counts its share of the
-// level's gates, consecutive lanes reading consecutive words of each row; the
-// slices' bit-sliced counters are added in LDS, and each slice converts and
-// writes 8 of its word's 64 pairs (one 16-byte store).  Levels past the
-// simulated depth are zero-filled by the host.
-constexpr uint32_t kFeatWords = 32, kFeatSlices = 8;
+// level's gates (a level of ~800 gates is ~26 per slice: the slices are what
+// gives the launch enough waves), consecutive lanes reading consecutive words
+// of each row; the slices' bit-sliced counters are added in a tree in LDS,
+// and each slice converts and writes 2 of its word's 64 pairs.  Levels past
+// the simulated depth are zero-filled by the host.
+constexpr uint32_t kFeatWords = 8, kFeatSlices = 32;
+constexpr uint32_t kFeatPairsPerSlice = 64 / kFeatSlices;
 
 template <bool SEQ>
 __global__ __launch_bounds__(256) void k_features(const Planes pl, const uint32_t *__restrict__ level_off,
@@ -1137,18 +1139,23 @@ __global__ __launch_bounds__(256) void k_features(const Planes pl, const uint32_
       }
     }
   }
-  {  // add the slices' counters
-    __shared__ uint64_t red[kFeatSlices][kCountBits][kFeatWords];
+  {  // add the slices' counters: a tree, the upper half into the lower
+    __shared__ uint64_t red[kFeatSlices / 2][kCountBits][kFeatWords];
+    for (uint32_t half = kFeatSlices / 2; half >= 1; half >>= 1) {
+      if (sl >= half && sl < 2 * half) {
 #pragma unroll
-    for (int k = 0; k < kCountBits; ++k) red[sl][k][wl] = c[k];
-    __syncthreads();
-    if (sl == 0) {
-      for (uint32_t o = 1; o < kFeatSlices; ++o) {
+        for (int k = 0; k < kCountBits; ++k) red[sl - half][k][wl] = c[k];
+      }
+      __syncthreads();
+      if (sl < half) {
         uint64_t y[kCountBits];
 #pragma unroll
-        for (int k = 0; k < kCountBits; ++k) y[k] = red[o][k][wl];
+        for (int k = 0; k < kCountBits; ++k) y[k] = red[sl][k][wl];
         vc_add_sliced(c, y);
       }
+      __syncthreads();
+    }
+    if (sl == 0) {
 #pragma unroll
       for (int k = 0; k < kCountBits; ++k) red[0][k][wl] = c[k];
     }
@@ -1159,18 +1166,15 @@ __global__ __launch_bounds__(256) void k_features(const Planes pl, const uint32_
   if (!live) return;
   const float scale = denom > 0 ? 1.0f / denom : 0.0f;
   const uint64_t base = first_pair + 64ull * w;
-  const uint32_t j0 = 8 * sl;
+  static_assert(kFeatPairsPerSlice == 2, "one 4-byte store per slice");
+  const uint32_t j0 = kFeatPairsPerSlice * sl;
   uint16_t *dst = feat + (uint64_t)f * ld + base;
-  uint32_t v[4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    v[e] = (uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e) * scale) |
-           ((uint32_t)to_bf16((float)vc_get(c, j0 + 2 * e + 1) * scale) << 16);
-  if (base + j0 + 8 <= num_pairs) {
-    *reinterpret_cast<uint4 *>(dst + j0) = make_uint4(v[0], v[1], v[2], v[3]);
-  } else {
-    for (uint32_t e = 0; e < 8 && base + j0 + e < num_pairs; ++e)
-      dst[j0 + e] = (uint16_t)(v[e >> 1] >> (16 * (e & 1)));
+  const uint32_t v = (uint32_t)to_bf16((float)vc_get(c, j0) * scale) |
+                     ((uint32_t)to_bf16((float)vc_get(c, j0 + 1) * scale) << 16);
+  if (base + j0 + 2 <= num_pairs) {
+    *reinterpret_cast<uint32_t *>(dst + j0) = v;
+  } else if (base + j0 < num_pairs) {
+    dst[j0] = (uint16_t)v;
   }
 }
 

@@ -2970,6 +2970,10 @@ int pdf_score_pairs(pdf_grader *g, pdf_mlp *m, uint32_t max_level, int robust, u
   if (pdf::mlp_device(m) != g->device) { set_error("pdf_score_pairs: grader and scorer on different devices"); return PDF_ERR_ARG; }
   HIPCHK(hipSetDevice(g->device));
   const uint32_t last = std::min(std::min(max_level, g->nl.L), 127u);
+  // 64-feature stages that hold only levels past `last`: stage 1 (transition
+  // features of levels 64-127) and stage 3 (forward-mark features of levels
+  // 64-127) when last < 64; zero in every row, so the scorer skips them
+  const uint32_t zero_stages = last < 64 ? (1u << 1) | (1u << 3) : 0u;
   std::vector<pdf_grader::Lane> ln = all_lanes(g);
   if ((rc = pdf::mlp_prepare(m, feat_dev, ld, std::min<uint64_t>(g->num_pairs, g->chunk_pairs * ln.size())))) return rc;
   // one event per lane, re-recorded for every group (a wait takes the
@@ -2986,7 +2990,7 @@ int pdf_score_pairs(pdf_grader *g, pdf_mlp *m, uint32_t max_level, int robust, u
             !hip_ok(hipStreamWaitEvent(ms, ev[i], 0), "hipStreamWaitEvent"))
           return PDF_ERR_HIP;
       }
-      return pdf::mlp_queue_forward(m, feat_dev, ld, p0, p1, score_dev);
+      return pdf::mlp_queue_forward(m, feat_dev, ld, p0, p1, score_dev, zero_stages);
     });
   for (size_t i = 1; i < ln.size(); ++i) (void)hipStreamSynchronize(ln[i].stream);
   (void)hipStreamSynchronize(g->stream);

@@ -118,6 +118,8 @@ struct Tile {
 // B(k, j) = TB ? B[j * ldb + k] : B[k * ldb + j].
 // Block = BM x BN tile of C, 4 waves as 2 x 2, each wave (BM/2) x (BN/2) in
 // 16 x 16 MFMA tiles.  Split z covers k in [z * kchunk, (z + 1) * kchunk).
+// Bit s of `skip` drops the k stage [s kBK, (s + 1) kBK): the caller knows
+// those rows of A are zero (their products would add nothing).
 // Host-checked: leading dimensions are multiples of 8, kchunk a multiple of
 // kBK, pointers 16-byte aligned.  Loads move 8 elements along an operand's
 // contiguous dimension and are taken whole when their first element is in
@@ -127,7 +129,7 @@ struct Tile {
 template <bool TA, bool TB, int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, uint32_t lda,
                                               const uint16_t *__restrict__ B, uint32_t ldb, uint32_t M, uint32_t N,
-                                              uint32_t K, uint32_t kchunk, EpiArgs ep) {
+                                              uint32_t K, uint32_t kchunk, uint32_t skip, EpiArgs ep) {
   static_assert(BM == 128 && BN == 128, "the epilogue stages 64-row halves of a 128 x 128 tile");
   // operand tiles during the k loop; the epilogue reuses the space for one
   // half of the C tile in FP32 (64 x (BN + 4) floats)
@@ -152,18 +154,24 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, ui
 
   Tile<TA, BM> ta;
   Tile<!TB, BN> tb;  // B's rows are the N index: k contiguous when TB
-  if (kbeg < kend) {
-    ta.load(A, lda, m0, M, kbeg, kend);
-    tb.load(B, ldb, n0, N, kbeg, kend);
+  auto next = [&](uint32_t k) {  // the first stage at or after k that is not skipped
+    while (k < kend && ((skip >> (k / kBK)) & 1)) k += kBK;
+    return k;
+  };
+  uint32_t k0 = next(kbeg);
+  if (k0 < kend) {
+    ta.load(A, lda, m0, M, k0, kend);
+    tb.load(B, ldb, n0, N, k0, kend);
   }
-  for (uint32_t k0 = kbeg; k0 < kend; k0 += kBK) {
+  for (; k0 < kend;) {
+    const uint32_t kn = next(k0 + kBK);
     __syncthreads();  // the previous stage's MFMA operands have been read
     ta.store(As);
     tb.store(Bs);
     __syncthreads();
-    if (k0 + kBK < kend) {  // prefetch the next stage while this one computes
-      ta.load(A, lda, m0, M, k0 + kBK, kend);
-      tb.load(B, ldb, n0, N, k0 + kBK, kend);
+    if (kn < kend) {  // prefetch the next stage while this one computes
+      ta.load(A, lda, m0, M, kn, kend);
+      tb.load(B, ldb, n0, N, kn, kend);
     }
 #pragma unroll
     for (int ks = 0; ks < kBK / 32; ++ks) {
@@ -183,6 +191,7 @@ __global__ __launch_bounds__(256) void k_gemm(const uint16_t *__restrict__ A, ui
         for (int ni = 0; ni < NI; ++ni)
           acc[mi][ni] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mi], b[ni], acc[mi][ni], 0, 0, 0);
     }
+    k0 = kn;
   }
 
   // Epilogue, one 64-row half at a time: the waves holding that half write
@@ -436,7 +445,7 @@ int ensure_part(pdf_mlp *m, size_t floats) {
 
 template <bool TA, bool TB, int EPI>
 int gemm(pdf_mlp *m, const uint16_t *A, uint32_t lda, const uint16_t *B, uint32_t ldb, uint32_t M, uint32_t N,
-         uint32_t K, uint32_t splits, const EpiArgs &ep) {
+         uint32_t K, uint32_t splits, const EpiArgs &ep, uint32_t skip = 0) {
   constexpr int BM = 128, BN = 128;
   if (lda % 8 || ldb % 8 || ((uintptr_t)A | (uintptr_t)B) % 16 || N % 8 || ep.ldc % 8) {
     pdf::set_error("mlp gemm: operand not 16-byte granular");
@@ -448,16 +457,24 @@ int gemm(pdf_mlp *m, const uint16_t *A, uint32_t lda, const uint16_t *B, uint32_
     pdf::set_error("mlp gemm: too many rows in one launch");
     return PDF_ERR_ARG;
   }
-  k_gemm<TA, TB, BM, BN, EPI><<<grid, 256, 0, m->stream>>>(A, lda, B, ldb, M, N, K, kchunk, ep);
+  if (K > 32 * kBK && skip) {
+    pdf::set_error("mlp gemm: stage mask covers 32 stages");
+    return PDF_ERR_ARG;
+  }
+  k_gemm<TA, TB, BM, BN, EPI><<<grid, 256, 0, m->stream>>>(A, lda, B, ldb, M, N, K, kchunk, skip, ep);
   MLPCHK(hipGetLastError());
   return PDF_OK;
 }
 
 // H1, H2 (and y) of `rows` rows of X (feature-major: X[f * ld + r]).
-int forward(pdf_mlp *m, const uint16_t *X, uint64_t ld, uint32_t rows, float *y, const float *target) {
+// zero_stages: bit s says features [64 s, 64 s + 64) are zero in every row.
+int forward(pdf_mlp *m, const uint16_t *X, uint64_t ld, uint32_t rows, float *y, const float *target,
+            uint32_t zero_stages = 0) {
+  static_assert(kBK == 64, "zero_stages counts 64-feature stages");
   int rc;
   EpiArgs e1{m->w + kB1, nullptr, m->h1, nullptr, kHid};
-  if ((rc = gemm<true, true, EPI_BIAS_RELU>(m, X, (uint32_t)ld, m->w1b, kIn, rows, kHid, kIn, 1, e1))) return rc;
+  if ((rc = gemm<true, true, EPI_BIAS_RELU>(m, X, (uint32_t)ld, m->w1b, kIn, rows, kHid, kIn, 1, e1, zero_stages)))
+    return rc;
   EpiArgs e2{m->w + kB2, nullptr, m->h2, nullptr, kHid};
   if ((rc = gemm<false, true, EPI_BIAS_RELU>(m, m->h1, kHid, m->w2b, kHid, rows, kHid, kHid, 1, e2))) return rc;
   k_head_fwd<<<blocks(rows, 4), 256, 0, m->stream>>>(m->h2, m->w, rows, y, target, m->dy, m->sq);
@@ -536,10 +553,11 @@ int mlp_prepare(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t rows
   return ensure_rows(m, std::min<uint64_t>(rows, kSlab));
 }
 
-int mlp_queue_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t r0, uint64_t r1, float *score_dev) {
+int mlp_queue_forward(pdf_mlp *m, const uint16_t *feat_dev, uint64_t ld, uint64_t r0, uint64_t r1, float *score_dev,
+                      uint32_t zero_stages) {
   for (uint64_t a = r0; a < r1; a += kSlab) {
     const uint32_t r = (uint32_t)std::min<uint64_t>(kSlab, r1 - a);
-    const int rc = forward(m, feat_dev + a, ld, r, score_dev + a, nullptr);
+    const int rc = forward(m, feat_dev + a, ld, r, score_dev + a, nullptr, zero_stages);
     if (rc) return rc;
   }
   return PDF_OK;

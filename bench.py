@@ -292,10 +292,13 @@ def main():
     g.set_vectors(bits, wl.pairs)
     # the workload's recorded tuning (what pdf_grader_autotune chose on an
     # MI355X); --autotune 1 runs the autotuner again, untimed
-    g.set_tuning(pg.TUNE_LANES, wl.lanes)
+    # (chunk first: the lanes' buffers are sized for the current chunk)
     g.set_chunk(wl.chunk)
+    g.set_tuning(pg.TUNE_LANES, wl.lanes)
+    g.set_tuning(pg.TUNE_WORDS_PER_LANE, wl.words_per_lane)
     tune = g.autotune(robust, reps=5) if args.autotune else {"lanes": wl.lanes, "chunk_pairs": wl.chunk,
-                                                              "use_graph": 1, "source": "recorded"}
+                                                              "use_graph": 1, "words_per_lane": wl.words_per_lane,
+                                                              "source": "recorded"}
     K = args.segs
     grade = (lambda: g.grade(robust)) if K == 1 else (lambda: g.grade_segs(robust, K))
     total = None

@@ -364,6 +364,28 @@ def test_50k_gates_full_size_against_bitsliced(robust):
         assert ref["total"] > 0
 
 
+def test_2m_gates_recorded_tuning():
+    """The 2,000,000-gate netlist on 262,144 pairs with its recorded tuning
+    (2 lanes x 131,072-pair chunks x 2 words per lane) against 1 lane x
+    32,768-pair chunks x 1 word: same total, first detections and counts."""
+    from pdfgrade.configs import GATES_2M
+    say("2m gates recorded tuning")
+    assert (GATES_2M.lanes, GATES_2M.chunk, GATES_2M.words_per_lane) == (2, 131072, 2)
+    nl = pg.Netlist.generate(GATES_2M.netlist)
+    P = 262144
+    bits = pg.gen_vectors(GATES_2M.vector_seed, nl.info()["num_pi"], P)
+    ref = gpu_grade(nl, bits, P, 32768, True)
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(GATES_2M.chunk)
+        g.set_tuning(pg.TUNE_LANES, GATES_2M.lanes)
+        g.set_tuning(pg.TUNE_WORDS_PER_LANE, GATES_2M.words_per_lane)
+        for rep in range(2):
+            total = g.grade(True)
+            f0, f1 = g.first()
+            assert_same(dict(total=total, first_t0=f0, first_t1=f1, gate_counts=g.gate_counts()), ref, rep)
+
+
 def test_2m_gates_prefix_against_bitsliced():
     """The 2,000,000-gate / 200-level netlist on 4,096 pairs (as a vector set
     of its own), bit-exact against the bit-sliced CPU grader, both modes."""

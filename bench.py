@@ -248,10 +248,12 @@ def learned_vs_plain(robust: bool):
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "learned.json")
         cmd = [sys.executable, "-m", "pdfgrade.learned", "--out", out] + ([] if robust else ["--non-robust"])
-        r = subprocess.run(cmd, cwd=os.path.join(ROOT, "gpu-path-delay-coverage_amd"), stdout=subprocess.DEVNULL,
-                           stderr=subprocess.PIPE, text=True)
+        # the child's progress lines go to this process's stderr (stdout holds
+        # only the JSON line)
+        print(f"learned experiment ({'robust' if robust else 'non-robust'}) ...", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, cwd=os.path.join(ROOT, "gpu-path-delay-coverage_amd"), stdout=sys.stderr)
         if r.returncode != 0:
-            raise RuntimeError(f"learned experiment failed ({r.returncode}): {r.stderr[-2000:]}")
+            raise RuntimeError(f"learned experiment failed ({r.returncode}); its output is on stderr above")
         with open(out) as f:
             res = json.load(f)
     s95, g95 = res["summary"]["paths_95"], res["summary"]["segments_95"]

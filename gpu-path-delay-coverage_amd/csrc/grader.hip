@@ -3154,12 +3154,35 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   HIPCHK(hipSetDevice(g->device));
   const uint64_t P = g->num_pairs;
   auto round64 = [](uint64_t x) { return (x + 63) / 64 * 64; };
+  // Candidates: 1..4 lanes, each with the even split of the pairs and the
+  // power-of-two chunks from 32,768 pairs below it (from 65,536 on, 2 words
+  // per lane applies), kept when the lanes' chunk buffers fit in 85% of the
+  // HBM that is free once this grader's own chunk buffers are released.
+  HIPCHK(hipStreamSynchronize(g->stream));
+  free_chunk(g);
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  const double budget = 0.85 * (double)free_b;
+  auto lane_bytes = [&](uint64_t chunk) {  // what alloc_chunk_buffers takes
+    const uint64_t words = (chunk + 63) / 64, pitch = (words + 1 + 15) / 16 * 16, ntiles = (words + 63) / 64;
+    return (double)g->nl.n * (8.0 * pitch * (g->seq ? 3 : 4) + 8.0 * ntiles + 4.0 * words);
+  };
   struct Cand { uint32_t lanes; uint64_t chunk; };
   std::vector<Cand> cands;
-  for (uint32_t lanes = 1; lanes <= 3; ++lanes) {
-    const uint64_t chunk = lanes == 1 ? P : round64((P + lanes - 1) / lanes);
-    if (lanes > 1 && chunk >= P) continue;
+  auto add = [&](uint32_t lanes, uint64_t chunk) {
+    if ((P + chunk - 1) / chunk < lanes) return;  // fewer chunks than lanes
+    if (lanes * lane_bytes(chunk) > budget) return;
     cands.push_back({lanes, chunk});
+  };
+  for (uint32_t lanes = 1; lanes <= 4; ++lanes) {
+    const uint64_t even = lanes == 1 ? P : round64((P + lanes - 1) / lanes);
+    if (lanes > 1 && even >= P) continue;
+    add(lanes, even);
+    for (uint64_t c = 32768; c < even; c *= 2) add(lanes, c);
+  }
+  if (cands.empty()) {
+    set_error("pdf_grader_autotune: no chunk configuration fits in free HBM");
+    return PDF_ERR_STATE;
   }
   uint64_t ref = 0;
   bool have_ref = false;

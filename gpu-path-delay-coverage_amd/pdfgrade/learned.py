@@ -209,9 +209,11 @@ def ranking_quality(scores, gains) -> dict:
 
 
 def evaluate_circuit(mlp: Mlp, nl: Netlist, a, b, cfg: LearnedConfig, device: str = "cuda:0",
-                     with_gain_order: bool = True) -> dict:
+                     with_gain_order: bool = True, plain_tuning: dict | None = None) -> dict:
     """Grade the candidates in plain order, in learned order and (reference)
-    in exact solo-gain order; return coverage and timing figures."""
+    in exact solo-gain order; return coverage and timing figures.  The plain
+    grade runs at plain_tuning (a pdf_grader_autotune result), or at what the
+    autotuner picks for this circuit when none is given."""
     torch = _torch()
     P = cfg.candidates
     feat = torch.zeros((FEATURES, _round64(P)), dtype=torch.int16, device=device)
@@ -219,9 +221,15 @@ def evaluate_circuit(mlp: Mlp, nl: Netlist, a, b, cfg: LearnedConfig, device: st
     g = Grader(nl)
     g.set_pairs(a, b, P)
     # plain in-order grading of every candidate at its fastest configuration:
-    # the grader's autotuner (lanes x one chunk per lane x graph replay x
-    # words per lane), then a warm-up and the timed grade
-    plain_tuning = g.autotune(cfg.robust, reps=3)
+    # the grader's autotuner (lanes x chunk x graph replay x words per lane),
+    # then a warm-up and the timed grade
+    if plain_tuning is None:
+        plain_tuning = g.autotune(cfg.robust, reps=3)
+    else:
+        g.set_chunk(plain_tuning["chunk_pairs"])
+        g.set_tuning(TUNE_LANES, plain_tuning["lanes"])
+        g.set_tuning(TUNE_USE_GRAPH, plain_tuning["use_graph"])
+        g.set_tuning(TUNE_WORDS_PER_LANE, plain_tuning["words_per_lane"])
     g.grade(cfg.robust)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -319,10 +327,14 @@ def run(cfg: LearnedConfig, device: str = "cuda:0", log=print) -> dict:
     del feat
     res = {"config": asdict(cfg), "records": rows, "log_s": t_log, "train_s": t_train, "losses": losses,
            "circuits": []}
+    # the plain baseline's configuration: autotuned on the first held-out
+    # circuit and kept for the others (same gate, level and pair counts)
+    plain_tuning = None
     for j, s in enumerate(cfg.heldout_seeds):
         nl = Netlist.generate(circuit(s, cfg.num_gates, cfg.levels, cfg.num_pi, cfg.num_po))
         a, b = candidate_bits(cfg.num_pi, cfg.candidates, cfg.candidate_seed + j, cfg.q_lo, cfg.q_hi)
-        r = evaluate_circuit(mlp, nl, a, b, cfg, device)
+        r = evaluate_circuit(mlp, nl, a, b, cfg, device, plain_tuning=plain_tuning)
+        plain_tuning = r["plain_tuning"]
         r["seed"] = s
         res["circuits"].append(r)
         log(json.dumps({k: r[k] for k in ("seed", "full_paths", "full_segments", "plain_ms", "ranking_quality")})

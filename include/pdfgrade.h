@@ -384,9 +384,11 @@ int pdf_score_pairs(pdf_grader *g, pdf_mlp *m, uint32_t max_level, int robust, u
                     float *score_dev);
 
 /* Autotuner (no reference counterpart): times the grade (reps runs each,
- * median) over chunk lanes 1..3 (chunk = pairs / lanes), graph replay on and
- * off and words per lane, checks that every configuration gives the same
- * total, and leaves the grader in the fastest one. */
+ * median) over chunk lanes 1..4, each with chunk = pairs / lanes and the
+ * power-of-two chunks from 32,768 pairs below it (only configurations whose
+ * chunk buffers fit in 85% of the free HBM), graph replay on and off and
+ * words per lane; checks that every configuration gives the same total and
+ * leaves the grader in the fastest one. */
 typedef struct {
   uint32_t lanes;
   uint64_t chunk_pairs;

@@ -291,6 +291,36 @@ def test_graph_replays_match_eager():
             assert g.grade(True) == eager[0]
 
 
+def test_grader_autotune_candidates():
+    """pdf_grader_autotune on 262,144 pairs of a 1,500-gate netlist: the
+    chosen configuration is one of its candidates (1-4 lanes; chunk = pairs /
+    lanes, or a power of two from 32,768 pairs below it), and the grader left
+    in it grades bit-exact against the bit-sliced CPU grader, twice."""
+    say("grader autotune")
+    cfg = pg.GenConfig(seed=77, num_pi=64, num_po=64, num_gates=1500, levels=20, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 262144
+    bits = biased_vectors(77, nl.info()["num_pi"], P, p_hold=0.6)
+    ref = bitsliced_grade(nl, bits, P, 65536, True)
+    assert ref["total"] > 0
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(8192)
+        best = g.autotune(True, reps=2)
+        say("autotune", best)
+        lanes, chunk = best["lanes"], best["chunk_pairs"]
+        even = P if lanes == 1 else (-(-P // lanes) + 63) // 64 * 64
+        assert 1 <= lanes <= 4
+        assert chunk == even or (32768 <= chunk < even and chunk & (chunk - 1) == 0), best
+        assert best["use_graph"] in (0, 1) and best["words_per_lane"] in (1, 2) and best["ms"] > 0
+        assert g.num_chunks() == -(-P // chunk)
+        for rep in range(2):
+            got = dict(total=g.grade(True), gate_counts=g.gate_counts())
+            got["first_t0"], got["first_t1"] = g.first()
+            assert_same(got, ref, f"rep {rep}")
+
+
 def test_2m_gates_properties():
     """The 2,000,000-gate / 200-level netlist on 65,536 of its pairs: chunk
     invariance across 32,768- and 16,384-pair blocks, and the planes of an

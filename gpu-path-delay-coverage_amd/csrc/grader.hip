@@ -440,6 +440,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
 // before its level runs); the reader clears ACC again, so it is all zero
 // between passes.  DO_MERGE: earliest marked T0/T1 pair per node
 // (mergekernel.cuh:141-197) and the per-word list of marked nodes.
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+__device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
+                                           uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
+                                           uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                           uint32_t *__restrict__ list_cnt, uint64_t list_stride);
+
 // One (gate, word) item of the reverse pass.
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
 __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
@@ -455,6 +461,16 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
     pin((uint32_t)r.po);
   }
   if (!((nzw >> (w & 63)) & 1)) return;
+  rev_marked<SEQ, DO_REMOVE, DO_MERGE>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride);
+}
+
+// The rest of rev_item for a (gate, word) whose mark-present bit is set; r is
+// the gate's record, already loaded.
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+__device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
+                                           uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
+                                           uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                           uint32_t *__restrict__ list_cnt, uint64_t list_stride) {
   const uint64_t row = (uint64_t)g * pl.pitch + w;
   // second trip, all loads issued before any is used: this node's mark and
   // accumulator; for the merge, its frames as raw words (frame 1 and frame
@@ -556,6 +572,47 @@ __global__ __launch_bounds__(kBlock) void k_reverse(DevNet net, const GateRec *_
                                                     uint64_t list_stride) {
   rev_body<SEQ, DO_REMOVE, DO_MERGE, WIDE>(net, recs, pl, mp, start, first_t0, first_t1, list, list_cnt, list_stride,
                                            blockIdx.x);
+}
+
+// Reverse pass with RT consecutive 64-word tiles per wave (64-word tiles
+// only): one round trip loads the gate's record and its RT mark-present
+// words, and only the tiles holding a mark run rev_marked, one after another.
+// Where few (gate, tile) items hold a mark (long chunks of large netlists),
+// RT times fewer waves are dispatched.  Block b takes tile group b % TG and
+// gate group b / TG, one gate per wave.
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, int RT>
+__global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                          LevelMap mp, uint32_t start,
+                                                          uint32_t *__restrict__ first_t0,
+                                                          uint32_t *__restrict__ first_t1,
+                                                          uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                          uint64_t list_stride) {
+  const uint32_t TG = (mp.T + RT - 1) / RT;
+  const uint32_t grp = blockIdx.x / TG, tg = blockIdx.x - grp * TG;
+  const uint32_t gi = grp * (kBlock / 64) + (threadIdx.x >> 6);
+  if (gi >= mp.count) return;
+  const uint32_t g = __builtin_amdgcn_readfirstlane(mp.lo + gi);
+  const uint32_t lane = threadIdx.x & 63;
+  uint64_t nz[RT];
+#pragma unroll
+  for (int k = 0; k < RT; ++k) {
+    const uint32_t t = tg * RT + k;
+    nz[k] = t < mp.T ? pl.NZ[(uint64_t)g * pl.ntiles + t] : 0;
+  }
+  const GateRec r = recs[g];
+  if (DO_REMOVE) {
+    pin(r.fin[0]);
+    pin((uint32_t)r.nfi);
+    pin((uint32_t)r.po);
+  }
+#pragma unroll
+  for (int k = 0; k < RT; ++k) hold(nz[k]);
+#pragma unroll
+  for (int k = 0; k < RT; ++k) {
+    const uint32_t w = (tg * RT + k) * 64 + lane;
+    if (((nz[k] >> lane) & 1) && w < mp.words)
+      rev_marked<SEQ, DO_REMOVE, DO_MERGE>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride);
+  }
 }
 
 // Cross-chunk state to its initial value (one launch per grade).
@@ -1535,6 +1592,7 @@ struct pdf_grader {
   bool timing = false, tm_open = false;
   // tuning knobs (pdf_grader_set_tuning)
   uint32_t tune_words_per_lane = 1;
+  uint32_t tune_rev_tiles = 0;  // PDF_TUNE_REV_TILES (0: by tile count)
   uint32_t tune_use_graph = 1;
   // chunk lanes (PDF_TUNE_LANES): the fused grade runs consecutive chunks on
   // `lanes` streams, each lane with its own chunk buffers; lane 0 is the set
@@ -1723,6 +1781,20 @@ int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool c
   return PDF_OK;
 }
 
+// 64-word tiles per wave of the reverse pass for a level of `count` gates
+// and T tiles: the knob's value, or (0) the largest of 8, 4, 2 that leaves
+// the launch at least 16,384 waves (64 per CU), else 1.  Measured on an
+// MI355X: 2M gates x 32 tiles (320k waves) 8 tiles per wave, reverse pass
+// 53.6 -> 33.4 ms a grade; 50k gates x 128 tiles (106k waves) 4 or 8,
+// 8.3 -> 7.9 ms; 50k gates x 8, 16 or 32 tiles (6.6k-27k waves) 1: there
+// most waves hold a mark and the extra tiles only lengthen each wave.
+uint32_t rev_tiles(uint32_t want, uint32_t T, uint32_t count) {
+  if (want) return want;
+  uint32_t rt = 8;
+  while (rt > 1 && (uint64_t)count * T < 16384ull * rt) rt >>= 1;
+  return rt;
+}
+
 template <bool SEQ, bool R, bool M>
 void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_lo = 0,
                       uint32_t k_hi = 0xFFFFFFFFu) {
@@ -1733,7 +1805,21 @@ void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_
     const LevelMap mp =
         level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], false, &grid);
     tm_begin(g, kRev);
-    if (mp.twlog == 6)
+    const uint32_t rt = mp.twlog == 6 ? rev_tiles(g->tune_rev_tiles, mp.T, mp.count) : 1;
+    const uint32_t rgrid = ((mp.count + kBlock / 64 - 1) / (kBlock / 64)) * ((mp.T + rt - 1) / rt);
+    if (rt == 8)
+      k_reverse_tiles<SEQ, R, M, 8><<<rgrid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, mp, (uint32_t)start,
+                                                                      g->d_first0, g->d_first1, g->d_list,
+                                                                      g->d_list_cnt, g->nl.n);
+    else if (rt == 4)
+      k_reverse_tiles<SEQ, R, M, 4><<<rgrid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, mp, (uint32_t)start,
+                                                                      g->d_first0, g->d_first1, g->d_list,
+                                                                      g->d_list_cnt, g->nl.n);
+    else if (rt == 2)
+      k_reverse_tiles<SEQ, R, M, 2><<<rgrid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, mp, (uint32_t)start,
+                                                                      g->d_first0, g->d_first1, g->d_list,
+                                                                      g->d_list_cnt, g->nl.n);
+    else if (mp.twlog == 6)
       k_reverse<SEQ, R, M, true><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, mp, (uint32_t)start, g->d_first0,
                                                                  g->d_first1, g->d_list, g->d_list_cnt, g->nl.n);
     else
@@ -2651,6 +2737,13 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
       break;
     case PDF_TUNE_USE_GRAPH:
       g->tune_use_graph = value != 0;
+      break;
+    case PDF_TUNE_REV_TILES:
+      if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) {
+        set_error("reverse tiles per wave must be 0 (automatic), 1, 2, 4 or 8");
+        return PDF_ERR_ARG;
+      }
+      g->tune_rev_tiles = (uint32_t)value;
       break;
     default:
       set_error("unknown tuning key");

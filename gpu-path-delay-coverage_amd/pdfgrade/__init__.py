@@ -31,6 +31,7 @@ TYPE_NAMES = ["UNKN", "INPT", "AND", "NAND", "OR", "NOR", "XOR", "XNOR", "BUFF",
 S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
 TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES, TUNE_REV_COMPACT = 1, 2, 3, 4, 5, 6
+TUNE_REV_TILES = 7
 DEBUG_FORWARD_MARKS, DEBUG_PI_COVER = 1, 2
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
@@ -387,8 +388,8 @@ class Grader:
         _check(lib().pdf_grader_reset(self._h))
 
     def set_tuning(self, key: int, value: int):
-        """TUNE_WORDS_PER_LANE (1/2), TUNE_USE_GRAPH (0/1) or TUNE_LANES (1..4);
-        results never change."""
+        """TUNE_WORDS_PER_LANE (1/2), TUNE_USE_GRAPH (0/1), TUNE_LANES (1..4) or
+        TUNE_REV_TILES (0 automatic, 1/2/4/8); results never change."""
         _check(lib().pdf_grader_set_tuning(self._h, key, value))
 
     def autotune(self, robust: bool, reps: int = 5) -> dict:

@@ -255,6 +255,7 @@ int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t 
 #define PDF_TUNE_SEGMENT_ORDER 4  /* batch: 0 circuit order, 1 largest segments first */
 #define PDF_TUNE_LANES 5          /* grader: chunks graded concurrently on 1..4 streams */
 #define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items */
+#define PDF_TUNE_REV_TILES 7      /* grader: 64-word tiles per reverse-pass wave, 1/2/4/8 (0: by waves per launch) */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same

@@ -42,7 +42,7 @@ def test_binding_constants_equal_header():
             assert getattr(pg, name) == val, name
             checked.append(name)
     # error codes, gate types, tuning keys, debug flags, sizes
-    for must in ("OK", "ERR_ARG", "ERR_OVERFLOW", "INPT", "DFF", "NONE", "TUNE_LANES", "TUNE_REV_COMPACT",
+    for must in ("OK", "ERR_ARG", "ERR_OVERFLOW", "INPT", "DFF", "NONE", "TUNE_LANES", "TUNE_REV_COMPACT", "TUNE_REV_TILES",
                  "TUNE_WORDS_PER_LANE", "DEBUG_PI_COVER", "FEATURES", "MLP_PARAMS"):
         assert must in checked, must
 

@@ -264,6 +264,38 @@ def test_two_words_per_lane_path(robust):
     assert_same(gpu_grade(nl, bits, P, 8192, robust), ref, "nw1 chunks")
 
 
+@pytest.mark.parametrize("robust", [True, False])
+def test_reverse_tiles_per_wave(robust):
+    """The reverse pass with 1, 2, 4 and 8 64-word tiles per wave, and the
+    automatic choice, on a 1,500-gate netlist: a 65,536-pair chunk (16
+    tiles) and a ragged 40,859-pair last chunk (639 words: 9 full tiles and
+    a partial one, so the last tile group is short), fused and stage by
+    stage, against the oracle."""
+    say("reverse tiles", robust)
+    cfg = pg.GenConfig(seed=77, num_pi=64, num_po=64, num_gates=1500, levels=20, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 65536 + 40896 - 37
+    bits = biased_vectors(77, nl.info()["num_pi"], P, p_hold=0.6)
+    ref = oracle_grade(nl, bits, P, 65536, robust)
+    assert ref["total"] > 0
+    for rt in (0, 1, 2, 4, 8):
+        with pg.Grader(nl) as g:
+            g.set_vectors(bits, P)
+            g.set_chunk(65536)
+            g.set_tuning(pg.TUNE_REV_TILES, rt)
+            for stages in (False, True):
+                if stages:
+                    total, per_chunk = g.grade_stages(robust)
+                else:
+                    total, per_chunk = g.grade(robust), None
+                f0, f1 = g.first()
+                got = dict(total=total, per_chunk=per_chunk, first_t0=f0, first_t1=f1, gate_counts=g.gate_counts())
+                assert_same(got, ref, f"rt {rt} stages {stages}")
+        with pg.Grader(nl) as g, pytest.raises(pg.PdfError):
+            g.set_tuning(pg.TUNE_REV_TILES, 3)
+
+
 def test_graph_replays_match_eager():
     """pdf_grade replays a captured HIP graph; every replay, with eager stage
     calls and synchronizations in between, equals the eager launch sequence."""
@@ -396,15 +428,22 @@ def test_50k_gates_full_size_against_bitsliced(robust):
 
 def test_2m_gates_recorded_tuning():
     """The 2,000,000-gate netlist on 262,144 pairs with its recorded tuning
-    (2 lanes x 131,072-pair chunks x 2 words per lane) against 1 lane x
-    32,768-pair chunks x 1 word: same total, first detections and counts."""
+    (2 lanes x 131,072-pair chunks x 2 words per lane, reverse tiles per wave
+    chosen per level: 8 on the large levels) against 1 lane x 32,768-pair
+    chunks x 1 word with one tile per reverse wave: same total, first
+    detections and counts."""
     from pdfgrade.configs import GATES_2M
     say("2m gates recorded tuning")
     assert (GATES_2M.lanes, GATES_2M.chunk, GATES_2M.words_per_lane) == (2, 131072, 2)
     nl = pg.Netlist.generate(GATES_2M.netlist)
     P = 262144
     bits = pg.gen_vectors(GATES_2M.vector_seed, nl.info()["num_pi"], P)
-    ref = gpu_grade(nl, bits, P, 32768, True)
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(32768)
+        g.set_tuning(pg.TUNE_REV_TILES, 1)
+        ref = dict(total=g.grade(True), gate_counts=g.gate_counts())
+        ref["first_t0"], ref["first_t1"] = g.first()
     with pg.Grader(nl) as g:
         g.set_vectors(bits, P)
         g.set_chunk(GATES_2M.chunk)

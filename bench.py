@@ -298,8 +298,10 @@ def main():
     g.set_chunk(wl.chunk)
     g.set_tuning(pg.TUNE_LANES, wl.lanes)
     g.set_tuning(pg.TUNE_WORDS_PER_LANE, wl.words_per_lane)
+    g.set_tuning(pg.TUNE_USE_GRAPH, wl.use_graph)
     tune = g.autotune(robust, reps=5) if args.autotune else {"lanes": wl.lanes, "chunk_pairs": wl.chunk,
-                                                              "use_graph": 1, "words_per_lane": wl.words_per_lane,
+                                                              "use_graph": wl.use_graph,
+                                                              "words_per_lane": wl.words_per_lane,
                                                               "source": "recorded"}
     K = args.segs
     grade = (lambda: g.grade(robust)) if K == 1 else (lambda: g.grade_segs(robust, K))

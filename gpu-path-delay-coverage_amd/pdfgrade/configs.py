@@ -25,6 +25,7 @@ class Workload:
     robust: bool = True
     lanes: int = 1  # chunks graded concurrently (pdf_grader_autotune's choice, see bench.py)
     words_per_lane: int = 1  # PDF_TUNE_WORDS_PER_LANE (2 needs chunks of a multiple of 65,536 pairs)
+    use_graph: int = 1  # PDF_TUNE_USE_GRAPH: replay a captured graph (1) or launch directly (0)
 
 
 TINY = Workload(
@@ -37,7 +38,10 @@ GATES_50K = Workload(
     name="single-50k-gates",
     netlist=GenConfig(seed=2, num_pi=256, num_po=128, num_gates=50_000, levels=60, fanin_p=WIDE_FANIN,
                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64),
-    vector_seed=102, pairs=65_536, chunk=32_768, lanes=2)
+    # graph replay: direct launches on the two lanes' streams are faster on
+    # an idle host (0.721 against 0.753 ms a grade on an MI355X) but take
+    # 0.79-0.94 ms when host threads compete for the CPU
+    vector_seed=102, pairs=65_536, chunk=32_768, lanes=2, use_graph=1)
 
 GATES_2M = Workload(
     name="single-2m-gates",

@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2821,6 +2822,189 @@ int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t 
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
   g->merged_segs = (uint32_t)segs;
+  return PDF_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------- grading from a static scratch
+// pdf_grade_noalloc carves every chunk buffer (value, mark, present and
+// accumulator planes, node lists, counters, the cover's c/h fallback) out
+// of one file-scope device array instead of the grader's own allocations.
+// Each device holds its own copy of the array for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress gives on
+// that device (the array's name is a host-side stand-in).  All calls on one
+// device share the array, so calls that overlap on one device race.
+namespace {
+
+static __device__ __align__(256) uint64_t scratch[PDF_NOALLOC_SCRATCH_BYTES / 8];
+
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mu;
+uint64_t *scratch_dev[kScratchDevices] = {};
+
+// The scratch array's address on the current device `dev` (looked up once
+// per device, with that device current).
+int scratch_address(int dev, uint64_t **out) {
+  if (dev < 0 || dev >= kScratchDevices) {
+    set_error("pdf_grade_noalloc: device index " + std::to_string(dev) + " beyond the supported " +
+              std::to_string(kScratchDevices));
+    return PDF_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      set_error(std::string("pdf_grade_noalloc: hipGetSymbolAddress of the scratch array failed: ") +
+                hipGetErrorString(e));
+      return PDF_ERR_HIP;
+    }
+    scratch_dev[dev] = static_cast<uint64_t *>(p);
+  }
+  *out = scratch_dev[dev];
+  return PDF_OK;
+}
+
+// Where one chunk's buffers sit in the scratch array (byte offsets, each
+// 256-byte aligned) for `chunk` pairs per chunk and `waves` cover waves.
+struct ScratchPlan {
+  uint64_t chunk = 0, words = 0, pitch = 0, ntiles = 0;
+  uint32_t waves = 0, n_cstate = 0;
+  uint64_t V = 0, VB = 0, M = 0, NZ = 0, ACC = 0, list = 0, cstate = 0, act = 0, cs = 0, hs = 0, bytes = 0;
+};
+
+ScratchPlan scratch_layout(uint64_t n, bool seq, uint64_t chunk, uint32_t waves) {
+  ScratchPlan p;
+  p.chunk = chunk;
+  p.waves = waves;
+  p.words = (chunk + 63) / 64;
+  p.pitch = (p.words + 1 + 15) / 16 * 16;  // as pdf_grader_set_chunk lays out the planes
+  p.ntiles = (p.words + 63) / 64;
+  p.n_cstate = (uint32_t)(((3 * p.words + 2) & ~1ULL) + 2);
+  uint64_t at = 0;
+  auto take = [&](uint64_t bytes) {
+    const uint64_t o = at;
+    at += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  p.V = take(n * p.pitch * 8);
+  p.VB = seq ? 0 : take(n * p.pitch * 8);
+  p.M = take(n * p.pitch * 8);
+  p.NZ = take(n * p.ntiles * 8);
+  p.ACC = take(n * p.pitch * 8);
+  p.list = take(n * p.words * 4);
+  p.cstate = take((uint64_t)p.n_cstate * 4);
+  p.act = take(chunk * 4);
+  p.cs = take(n * waves * 8);
+  p.hs = take(n * waves * 8);
+  p.bytes = at;
+  return p;
+}
+
+// The largest chunk (a multiple of 64 pairs, or all of them) whose buffers
+// fit in the scratch array, with as many cover waves (64 down to 1) as fit.
+bool plan_scratch(uint64_t n, bool seq, uint64_t pairs, ScratchPlan *out) {
+  for (uint32_t waves : {64u, 16u, 4u, 1u}) {
+    if (scratch_layout(n, seq, 64, waves).bytes > PDF_NOALLOC_SCRATCH_BYTES) continue;
+    uint64_t lo = 1, hi = (pairs + 63) / 64;  // words per chunk
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi + 1) / 2;
+      if (scratch_layout(n, seq, mid * 64, waves).bytes <= PDF_NOALLOC_SCRATCH_BYTES) lo = mid;
+      else hi = mid - 1;
+    }
+    *out = scratch_layout(n, seq, std::min(lo * 64, pairs), waves);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_dev, uint32_t *err_dev) {
+  if (!g || !stream || !total_dev) { set_error("null argument"); return PDF_ERR_ARG; }
+  if (!g->num_pairs) { set_error("pdf_grade_noalloc: load vectors first"); return PDF_ERR_STATE; }
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != g->device) HIPCHK(hipSetDevice(g->device));
+  uint64_t *base = nullptr;
+  int rc = scratch_address(g->device, &base);
+  if (rc) return rc;
+  ScratchPlan p;
+  if (!plan_scratch(g->nl.n, g->seq, g->num_pairs, &p)) {
+    set_error("pdf_grade_noalloc: " + std::to_string(g->nl.n) + " nodes need " +
+              std::to_string(scratch_layout(g->nl.n, g->seq, 64, 1).bytes) +
+              " bytes of chunk buffers even at 64 pairs per chunk; the static scratch holds " +
+              std::to_string((unsigned long long)PDF_NOALLOC_SCRATCH_BYTES));
+    return PDF_ERR_UNSUPPORTED;
+  }
+  // the grader's chunk fields point into the scratch array for this call only
+  struct Saved {
+    pdf_grader::Lane lane;
+    uint64_t chunk_pairs, pitch, ntiles;
+    uint32_t n_cstate, cover_waves, grade_segs;
+    bool one_lane;
+  } sv;
+  lane_save(g, sv.lane);
+  sv.chunk_pairs = g->chunk_pairs;
+  sv.pitch = g->pitch;
+  sv.ntiles = g->ntiles;
+  sv.n_cstate = g->n_cstate;
+  sv.cover_waves = g->cover_waves;
+  sv.grade_segs = g->grade_segs;
+  sv.one_lane = g->one_lane;
+  char *b = reinterpret_cast<char *>(base);
+  g->stream = static_cast<hipStream_t>(stream);
+  g->d_V = reinterpret_cast<uint64_t *>(b + p.V);
+  g->d_VB = g->seq ? nullptr : reinterpret_cast<uint64_t *>(b + p.VB);
+  g->d_M = reinterpret_cast<uint64_t *>(b + p.M);
+  g->d_NZ = reinterpret_cast<uint64_t *>(b + p.NZ);
+  g->d_ACC = reinterpret_cast<uint64_t *>(b + p.ACC);
+  g->d_list = reinterpret_cast<uint32_t *>(b + p.list);
+  g->d_cstate = reinterpret_cast<uint32_t *>(b + p.cstate);
+  g->d_actmap = reinterpret_cast<uint64_t *>(g->d_cstate);
+  g->d_list_cnt = g->d_cstate + 2 * p.words;
+  g->d_act_count = g->d_cstate + 3 * p.words;
+  g->d_chunk_total = reinterpret_cast<uint64_t *>(g->d_cstate + (((3 * p.words + 2) & ~1ULL)));
+  g->d_act_list = reinterpret_cast<uint32_t *>(b + p.act);
+  g->d_cs = reinterpret_cast<uint64_t *>(b + p.cs);
+  g->d_hs = reinterpret_cast<uint64_t *>(b + p.hs);
+  g->d_NEW = nullptr;
+  g->chunk_pairs = p.chunk;
+  g->pitch = p.pitch;
+  g->ntiles = p.ntiles;
+  g->n_cstate = p.n_cstate;
+  g->cover_waves = p.waves;
+  g->grade_segs = 1;
+  g->one_lane = true;
+  // the removal accumulator must start at zero and the array keeps whatever
+  // the last call on this device left there; the mark-present words and the
+  // per-chunk counters are rewritten by the passes themselves
+  const uint64_t acc_words = (uint64_t)g->nl.n * p.pitch * 2;
+  k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(acc_words, 256), 65536), 256, 0, g->stream>>>(
+      reinterpret_cast<uint32_t *>(g->d_ACC), acc_words, 0u);
+  if (!hip_ok(hipGetLastError(), "k_fill_u32")) rc = PDF_ERR_HIP;
+  if (!rc) rc = queue_grade(g, robust);
+  if (!rc) {
+    k_copy_result<<<1, 1, 0, g->stream>>>(g->d_grand, g->d_err, total_dev, err_dev);
+    if (!hip_ok(hipGetLastError(), "k_copy_result")) rc = PDF_ERR_HIP;
+  }
+  lane_load(g, sv.lane);
+  g->chunk_pairs = sv.chunk_pairs;
+  g->pitch = sv.pitch;
+  g->ntiles = sv.ntiles;
+  g->n_cstate = sv.n_cstate;
+  g->cover_waves = sv.cover_waves;
+  g->grade_segs = sv.grade_segs;
+  g->one_lane = sv.one_lane;
+  if (rc) return rc;
+  // first / per-gate counts / total now belong to this grade; the grader's
+  // own planes do not hold its chunks
+  g->stage = 0;
+  g->stage_chunk = -1;
   return PDF_OK;
 }
 

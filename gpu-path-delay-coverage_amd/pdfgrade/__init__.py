@@ -114,6 +114,7 @@ SIGNATURES = {
     "pdf_grade_timed": ([_P, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
     "pdf_grade_segs": ([_P, C.c_int, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
     "pdf_grade_async": ([_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_grade_noalloc": ([_P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grader_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
     "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
@@ -446,6 +447,13 @@ class Grader:
         u64 at total_dev once the stream has run it."""
         _check(lib().pdf_grade_async(self._h, int(robust), segs, C.c_void_p(stream), C.c_void_p(total_dev),
                                      C.c_void_p(err_dev) if err_dev else None))
+
+    def grade_noalloc(self, robust: bool, stream: int, total_dev: int, err_dev: int = 0):
+        """pdf_grade_noalloc: pdf_grade_async (segments of length 1) with every
+        chunk buffer taken from the library's static per-GPU scratch array;
+        nothing is allocated."""
+        _check(lib().pdf_grade_noalloc(self._h, int(robust), C.c_void_p(stream), C.c_void_p(total_dev),
+                                       C.c_void_p(err_dev) if err_dev else None))
 
     def grade_stages(self, robust: bool, segs: int = 1):
         """The reference's chunk loop (main.cu:171-231) through the four stage

@@ -245,6 +245,24 @@ int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
 int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t *total_dev,
                     uint32_t *err_dev);
 
+/* pdf_grade_async (segs = 1) with no chunk buffers of the grader's own.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every chunk buffer (planes, node lists,
+ * counters, cover scratch) is carved out of one static device array of
+ * PDF_NOALLOC_SCRATCH_BYTES per GPU, held by the library until the process
+ * ends; the call allocates nothing, copies nothing synchronously, records
+ * no memset and only queues kernels on `stream`.  The chunk width is the
+ * largest multiple of 64 pairs whose buffers fit (results do not depend on
+ * it).  A netlist whose buffers do not fit even at 64 pairs per chunk (about
+ * 650,000 nodes) is refused with PDF_ERR_UNSUPPORTED.  Results land where
+ * pdf_grade_async puts them (*total_dev, *err_dev, and the grader's first /
+ * per-gate / total getters); the plane getters do not see this grade.  All
+ * calls on one GPU share its array, which keeps the last call's contents:
+ * calls that overlap on one GPU race. */
+#define PDF_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_dev, uint32_t *err_dev);
+
 /* Tuning knobs (no reference counterpart; the reference fixes its shapes
  * with compile-time constants such as SIM_BLOCK, MARK_BLOCK, PARALLEL_SEGS,
  * simkernel.cuh:29-30, markkernel.cuh:7-8, mergekernel.cuh:49-52).

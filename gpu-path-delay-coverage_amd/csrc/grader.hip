@@ -118,6 +118,15 @@ __device__ __forceinline__ void pin(uint32_t x) { asm volatile("" : : "v"(x)); }
 // cannot be moved up between loads issued before it.
 __device__ __forceinline__ void hold(uint64_t &x) { asm volatile("" : "+v"(x)); }
 
+// OR x into a removal accumulator word: a device-scope atomic, or for the
+// level walk (whose accumulator words each belong to one workgroup) a
+// workgroup-scope one.
+template <bool WG>
+__device__ __forceinline__ void push_acc(uint64_t *p, uint64_t x) {
+  if (WG) __hip_atomic_fetch_or(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  else atomicOr((unsigned long long *)p, (unsigned long long)x);
+}
+
 // Frames (a, b) of node f at word w.
 template <bool SEQ>
 __device__ __forceinline__ void frames(const Planes &pl, uint32_t f, uint32_t w, uint64_t &a, uint64_t &b) {
@@ -144,18 +153,46 @@ __device__ __forceinline__ uint64_t sens(uint64_t aon, uint64_t bon, uint64_t ao
 }
 
 // OR_i AND_{j != i} S(fanin_i, fanin_j) over fanin positions, for
-// nfi <= kRegFanin fanins held in registers.
+// nfi <= kRegFanin fanins held in registers.  Every S above factors as
+// S(on, off) = (X(on) & U(off)) | (Y(on) & W(off)) with X and Y disjoint
+// (robust: X = on transitions, U = the off-path value the gate needs, Y = 0;
+// non-robust: X and Y the two transition directions, U and W what each asks
+// of the off-path input).  So per bit AND_{j != i} S(i, j) is
+// X_i & AND_{j != i} U_j  |  Y_i & AND_{j != i} W_j   (for nfi >= 2),
+// and the exclusive ANDs come from prefix / suffix ANDs: O(nfi) work, not
+// O(nfi^2).  nfi = 1 gives the empty AND (all ones), nfi = 0 nothing.
 template <bool ROBUST, bool ANDLIKE>
 __device__ __forceinline__ uint64_t side_ok(const uint64_t (&fa)[kRegFanin], const uint64_t (&fb)[kRegFanin],
                                             uint32_t nfi) {
-  uint64_t r = 0;
+  static_assert(kRegFanin == 4, "the exclusive ANDs below are written out for four fanins");
+  if (nfi < 2) return nfi ? ~0ULL : 0ULL;
+  uint64_t X[4], Y[4], U[4], W[4];
 #pragma unroll
-  for (int i = 0; i < kRegFanin; ++i) {
-    uint64_t acc = ~0ULL;
-#pragma unroll
-    for (int j = 0; j < kRegFanin; ++j)
-      if (j != i && (uint32_t)j < nfi) acc &= sens<ROBUST, ANDLIKE>(fa[i], fb[i], fa[j], fb[j]);
-    if ((uint32_t)i < nfi) r |= acc;
+  for (int j = 0; j < 4; ++j) {
+    const uint64_t a = fa[j], b = fb[j];
+    if (ROBUST) {
+      X[j] = a ^ b;
+      Y[j] = 0;
+      U[j] = ANDLIKE ? (a & b) : ~(a | b);
+      W[j] = ~0ULL;
+    } else if (ANDLIKE) {
+      X[j] = a & ~b;  // on T0 needs off in {S1, T1}
+      U[j] = b;
+      Y[j] = ~a & b;  // on T1 needs off = S1
+      W[j] = a & b;
+    } else {
+      X[j] = ~a & b;  // on T1 needs off in {S0, T0}
+      U[j] = ~b;
+      Y[j] = a & ~b;  // on T0 needs off = S0
+      W[j] = ~a & ~b;
+    }
+    if ((uint32_t)j >= nfi) { X[j] = Y[j] = 0; U[j] = W[j] = ~0ULL; }
+  }
+  const uint64_t u01 = U[0] & U[1], u23 = U[2] & U[3];
+  uint64_t r = (X[0] & U[1] & u23) | (X[1] & U[0] & u23) | (X[2] & u01 & U[3]) | (X[3] & u01 & U[2]);
+  if (!ROBUST) {
+    const uint64_t w01 = W[0] & W[1], w23 = W[2] & W[3];
+    r |= (Y[0] & W[1] & w23) | (Y[1] & W[0] & w23) | (Y[2] & w01 & W[3]) | (Y[3] & w01 & W[2]);
   }
   return r;
 }
@@ -187,6 +224,16 @@ __device__ __forceinline__ void fold(uint8_t ty, uint64_t &v, uint64_t x) {
     case PDF_OR: v |= x; break;
     default: v = ~(v | x); break;
   }
+}
+
+// The same fold without a branch on the type (for lanes of different gate
+// types): om = all ones for OR/NOR, inv = all ones for NAND/NOR.
+__device__ __forceinline__ void fold_masks(uint8_t ty, uint64_t &om, uint64_t &inv) {
+  om = (ty == PDF_OR || ty == PDF_NOR) ? ~0ULL : 0ULL;
+  inv = (ty == PDF_NAND || ty == PDF_NOR) ? ~0ULL : 0ULL;
+}
+__device__ __forceinline__ void fold_m(uint64_t &v, uint64_t x, uint64_t om, uint64_t inv) {
+  v = ((v & x) | (om & (v | x))) ^ inv;
 }
 
 // One frame of one node at word w (used for the extra sequence column).
@@ -441,11 +488,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd(DevNet net, const GateRec *__res
 // before its level runs); the reader clears ACC again, so it is all zero
 // between passes.  DO_MERGE: earliest marked T0/T1 pair per node
 // (mergekernel.cuh:141-197) and the per-word list of marked nodes.
-template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WALK = false>
 __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
                                            uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
                                            uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
-                                           uint32_t *__restrict__ list_cnt, uint64_t list_stride);
+                                           uint32_t *__restrict__ list_cnt, uint64_t list_stride,
+                                           const uint64_t *fa_src = nullptr, const uint64_t *fb_src = nullptr);
 
 // One (gate, word) item of the reverse pass.
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
@@ -466,12 +514,18 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
 }
 
 // The rest of rev_item for a (gate, word) whose mark-present bit is set; r is
-// the gate's record, already loaded.
-template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+// the gate's record, already loaded.  WALK (the level walk): every push into
+// a word's accumulator comes from the workgroup that owns the word, earlier
+// in the same launch, so the pushes are workgroup-scope atomics and the
+// barrier between levels orders them before the read; the node's frame-1
+// word and frame 2's source word are read from fa_src and fb_src (the walk's
+// tile-major value plane) instead of V / VB.
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WALK>
 __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
                                            uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
                                            uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
-                                           uint32_t *__restrict__ list_cnt, uint64_t list_stride) {
+                                           uint32_t *__restrict__ list_cnt, uint64_t list_stride,
+                                           const uint64_t *fa_src, const uint64_t *fb_src) {
   const uint64_t row = (uint64_t)g * pl.pitch + w;
   // second trip, all loads issued before any is used: this node's mark and
   // accumulator; for the merge, its frames as raw words (frame 1 and frame
@@ -484,8 +538,8 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
   uint64_t a = 0, b = 0;
   uint32_t cur0 = 0, cur1 = 0;
   if (DO_MERGE) {
-    a = pl.V[row];
-    b = SEQ ? pl.V[row + 1] : pl.VB[row];
+    a = WALK ? *fa_src : pl.V[row];
+    b = WALK ? *fb_src : SEQ ? pl.V[row + 1] : pl.VB[row];
     cur0 = first_t0[g];
     cur1 = first_t1[g];
   }
@@ -529,13 +583,12 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
       for (int j = 0; j < kRegFanin; ++j) mf[j] &= (uint64_t)0 - ((nzf[j] >> (w & 63)) & 1);
 #pragma unroll
       for (int j = 0; j < kRegFanin; ++j)
-        if (mf[j] & m)
-          atomicOr((unsigned long long *)&pl.ACC[(uint64_t)r.fin[j] * pl.pitch + w], (unsigned long long)(mf[j] & m));
+        if (mf[j] & m) push_acc<WALK>(&pl.ACC[(uint64_t)r.fin[j] * pl.pitch + w], mf[j] & m);
       for (uint32_t j = kRegFanin; j < r.nfi; ++j) {
         const uint32_t f = net.fin[r.fin_off + j];
         if (!((pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)] >> (w & 63)) & 1)) continue;
         const uint64_t x = pl.M[(uint64_t)f * pl.pitch + w] & m;
-        if (x) atomicOr((unsigned long long *)&pl.ACC[(uint64_t)f * pl.pitch + w], (unsigned long long)x);
+        if (x) push_acc<WALK>(&pl.ACC[(uint64_t)f * pl.pitch + w], x);
       }
     }
   }
@@ -613,6 +666,342 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
     const uint32_t w = (tg * RT + k) * 64 + lane;
     if (((nz[k] >> lane) & 1) && w < mp.words)
       rev_marked<SEQ, DO_REMOVE, DO_MERGE>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride);
+  }
+}
+
+// ------------------------------------------------------------- level walk
+// The forward and reverse passes in one launch each instead of one launch per
+// level (the reference's host level loops, simkernel.cuh:213-235 and
+// markkernel.cuh:308-342).  Pairs are independent in both passes, so a
+// workgroup that owns a tile of TW consecutive words of the chunk can walk
+// every level on its own, with a workgroup barrier between levels and no
+// wait on any other workgroup.  A lane takes one gate of the level and all
+// TW words of the tile.  In sequence mode frame 2 of word u is frame 1 of
+// words u and u + 1 shifted by one bit, so each node carries the word after
+// the tile too (the halo, also word 0 of the next tile): NF = TW + 1 frame
+// words a node; explicit pairs carry both frames, NF = 2 TW.
+//
+// Where the frames live.  A tile's frames of node g are one NV-word record of
+// the tile-major plane VT[tile][n][NV]: a level's lanes take consecutive
+// gates, so their stores are contiguous.  The workgroup also keeps the
+// records of the most recent `cap` node ids in LDS (slot g % cap): fanins
+// come 70% from the previous level in the synthetic netlists, and ids are
+// sorted by level, so most fanin records are read from LDS and only older
+// ones from VT.  While a level [lo, hi) runs, a record is taken from LDS iff
+// its id is >= hi - cap (its slot is not rewritten by this level), and the
+// level writes the slots of ids >= hi - cap only (one writer per slot).
+// A record's last word holds the tile's mark-present bits of the node (bit u:
+// a mark at word u), so a gate reads fanin 0's with its frames.  Marks also
+// go to the node-major M / NZ planes of the level kernels for the reverse
+// pass and the cover; an NZ word spans 64 / TW tiles, so the forward walk
+// sets its bits with atomicOr on a zeroed plane and never reads it.
+// Workgroup b takes tile (b % 8) * per_xcd + b / 8: blocks are dealt to the
+// 8 XCDs round-robin, so neighbouring tiles share an XCD and its L2.
+constexpr int kWalkBlock = 1024;
+constexpr uint32_t kWalkLdsBytes = 128 * 1024;
+// the LDS window's words, typed as LDS (address space 3) so every access is
+// an LDS instruction with no generic-address check
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+// records move as 16-byte words (ds_read_b128 / global_load_dwordx4)
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u64x2 lds_u64x2;
+
+template <bool SEQ, int TW>
+struct WalkShape {
+  static constexpr int NF = SEQ ? TW + 1 : 2 * TW;           // frame words of a node record
+  static constexpr int NV = (NF + 2) & ~1;                   // + the tile's mark-present bits (bit u: word u),
+                                                             // padded to whole 16-byte words
+  static constexpr uint32_t cap = kWalkLdsBytes / (8 * NV);  // node records in LDS
+};
+
+template <int TW>
+__device__ __forceinline__ bool walk_tile(const Planes &pl, uint32_t tiles, uint32_t &tile, uint32_t &w0,
+                                          uint32_t &nw) {
+  const uint32_t per_xcd = (tiles + 7) / 8;
+  tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (tile >= tiles) return false;
+  w0 = tile * TW;
+  nw = min((uint32_t)TW, pl.words - w0);
+  return true;
+}
+
+// Frame 2 of word u from a node record: the next bit of frame 1 in sequence
+// mode, the second frame's word for explicit pairs.
+template <bool SEQ, int TW, int NV>
+__device__ __forceinline__ uint64_t frame2(const uint64_t (&rec)[NV], int u) {
+  if constexpr (SEQ) return shift_next(rec[u], rec[u + 1]);
+  else return rec[TW + u];
+}
+
+// The first N words of node f's NV-word record of the tile: from its LDS
+// slot when f is in the window, else from the tile-major plane.
+template <int N, int NV, uint32_t CAP>
+__device__ __forceinline__ void walk_fetch(uint64_t (&dst)[NV], uint32_t f, uint32_t win_lo, const lds_u64 *lds,
+                                           const uint64_t *__restrict__ vt) {
+  static_assert(NV % 2 == 0 && N <= NV, "records are whole 16-byte words");
+  constexpr int Q = (N + 1) / 2;
+  if (f >= win_lo) {
+    const lds_u64x2 *s = reinterpret_cast<const lds_u64x2 *>(lds + (f % CAP) * NV);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const u64x2 x = s[q];
+      dst[2 * q] = x.x;
+      dst[2 * q + 1] = x.y;
+    }
+  } else {
+    const u64x2 *s = reinterpret_cast<const u64x2 *>(vt + (uint64_t)f * NV);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const u64x2 x = s[q];
+      dst[2 * q] = x.x;
+      dst[2 * q + 1] = x.y;
+    }
+  }
+}
+
+// Sim + forward mark of gate g (record r) over words w0 .. w0+nw-1 of the
+// tile; the node's record (frames of every held word) goes to VT and, when
+// `keep`, to its LDS slot.
+template <bool SEQ, bool ROBUST, int TW>
+__device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &r, const Planes &pl,
+                                              const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
+                                              uint64_t vstride, uint64_t vbase, uint32_t g, uint32_t w0,
+                                              uint32_t nw, uint64_t *__restrict__ vt, lds_u64 *lds,
+                                              uint32_t win_lo, bool keep) {
+  using S = WalkShape<SEQ, TW>;
+  constexpr int NV = S::NV;
+  constexpr int NU = SEQ ? TW + 1 : TW;  // frame-1 words held: the tile's and the halo
+  const uint8_t ty = r.type;
+  uint64_t a[NU], b[TW], m[TW];
+#pragma unroll
+  for (int u = 0; u < TW; ++u) { b[u] = 0; m[u] = 0; }
+  if (ty == PDF_INPT) {
+    const uint64_t *src = va + (uint64_t)g * vstride + vbase;
+    const uint32_t nu = SEQ ? nw + 1 : nw;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) a[u] = (uint32_t)u < nu ? src[u] : 0;
+    if (!SEQ) {
+      const uint64_t *srcb = vb + (uint64_t)g * vstride + vbase;
+#pragma unroll
+      for (int u = 0; u < TW; ++u) b[u] = (uint32_t)u < nw ? srcb[u] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+      if ((uint32_t)u >= nw) continue;
+      const uint64_t bu = SEQ ? shift_next(a[u], a[SEQ ? u + 1 : u]) : b[u];
+      // pairs past the chunk's last one never mark (every other mark needs
+      // its fanin 0 marked)
+      m[u] = (a[u] ^ bu) & (w0 + u + 1 == pl.words ? pl.last_mask : ~0ULL);
+    }
+  } else {
+    const uint32_t nfi = r.nfi;
+    const uint32_t *more = net.fin + r.fin_off;
+    const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+    uint64_t x[kRegFanin][NV];
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j) {
+#pragma unroll
+      for (int u = 0; u < NV; ++u) x[j][u] = 0;
+      if (j == 0 || (multi && (uint32_t)j < nfi)) {
+        // fanin 0's record with its mark-present bits, the others' frames
+        if (j == 0) walk_fetch<NV, NV, S::cap>(x[j], r.fin[j], win_lo, lds, vt);
+        else walk_fetch<S::NF, NV, S::cap>(x[j], r.fin[j], win_lo, lds, vt);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j)
+#pragma unroll
+      for (int u = 0; u < NV; ++u) hold(x[j][u]);
+    const uint64_t nz0 = x[0][S::NF];  // fanin 0's mark-present bits
+    // the chained 2-input fold (simkernel.cuh:131-148) of every held word of
+    // every frame
+    uint64_t v[S::NF];
+#pragma unroll
+    for (int u = 0; u < S::NF; ++u) v[u] = ty == PDF_NOT ? ~x[0][u] : x[0][u];
+    if (nfi > 1) {
+      if (multi) {
+        uint64_t om, inv;
+        fold_masks(ty, om, inv);
+#pragma unroll
+        for (int j = 1; j < kRegFanin; ++j) {
+          if ((uint32_t)j >= nfi) continue;
+#pragma unroll
+          for (int u = 0; u < S::NF; ++u) fold_m(v[u], x[j][u], om, inv);
+        }
+        for (uint32_t j = kRegFanin; j < nfi; ++j) {
+          uint64_t y[NV];
+          walk_fetch<S::NF, NV, S::cap>(y, more[j], win_lo, lds, vt);
+#pragma unroll
+          for (int u = 0; u < S::NF; ++u) fold_m(v[u], y[u], om, inv);
+        }
+      } else if (ty != PDF_BUFF && ty != PDF_DFF && ty != PDF_FROM) {
+        // XOR/XNOR/NOT/others with >= 2 fanins give S0 (simkernel.cuh:89-92)
+#pragma unroll
+        for (int u = 0; u < S::NF; ++u) v[u] = 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) a[u] = v[u];
+#pragma unroll
+    for (int u = 0; u < TW; ++u) b[u] = frame2<SEQ, TW>(v, u);
+    // forward mark (markkernel.cuh:228-267): fanin 0's mark words where its
+    // mark-present bit is set and this gate has a transition, one round trip
+    const bool passes = ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT ||
+                        (ty >= PDF_AND && ty <= PDF_XNOR && (nfi == 1 || multi));
+    // (unknown types and XOR/XNOR with >= 2 fanins never mark)
+    uint64_t m0[TW];
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+      m0[u] = 0;
+      const uint64_t tr = passes && (uint32_t)u < nw ? a[u] ^ b[u] : 0;
+      if (tr && ((nz0 >> u) & 1)) m0[u] = pl.M[(uint64_t)r.fin[0] * pl.pitch + w0 + u];
+    }
+#pragma unroll
+    for (int u = 0; u < TW; ++u) {
+      if (!passes || (uint32_t)u >= nw) continue;
+      const uint64_t on = m0[u] & (a[u] ^ b[u]);
+      if (!on) continue;
+      if (nfi == 1 || !multi) {
+        m[u] = on;  // the empty AND over off-path inputs
+      } else {
+        const bool andlike = ty == PDF_AND || ty == PDF_NAND;
+        uint64_t rr;
+        if (nfi <= kRegFanin) {
+          uint64_t fa[kRegFanin], fb[kRegFanin];
+#pragma unroll
+          for (int j = 0; j < kRegFanin; ++j) {
+            fa[j] = x[j][u];
+            fb[j] = frame2<SEQ, TW>(x[j], u);
+          }
+          rr = andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+        } else {
+          // more fanins than registers: each off-path pair re-read from LDS / VT
+          rr = 0;
+          for (uint32_t i = 0; i < nfi; ++i) {
+            uint64_t si[NV];
+            walk_fetch<S::NF, NV, S::cap>(si, more[i], win_lo, lds, vt);
+            const uint64_t ai = si[u], bi = frame2<SEQ, TW>(si, u);
+            uint64_t acc = ~0ULL;
+            for (uint32_t j = 0; j < nfi && acc; ++j) {
+              if (j == i) continue;
+              uint64_t sj[NV];
+              walk_fetch<S::NF, NV, S::cap>(sj, more[j], win_lo, lds, vt);
+              const uint64_t aj = sj[u], bj = frame2<SEQ, TW>(sj, u);
+              acc &= andlike ? sens<ROBUST, true>(ai, bi, aj, bj) : sens<ROBUST, false>(ai, bi, aj, bj);
+            }
+            rr |= acc;
+          }
+        }
+        // markkernel.cuh:253-261: the on-path mark is always fanin 0's
+        m[u] = rr & on;
+      }
+    }
+  }
+  // the node's record: frame 1 of the held words (and frame 2 for explicit
+  // pairs), then its mark-present bits
+  uint64_t rec[NV];
+#pragma unroll
+  for (int u = S::NF; u < NV; ++u) rec[u] = 0;
+  if constexpr (SEQ) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) rec[u] = a[u];
+  } else {
+#pragma unroll
+    for (int u = 0; u < TW; ++u) { rec[u] = a[u]; rec[TW + u] = b[u]; }
+  }
+  const uint64_t row = (uint64_t)g * pl.pitch + w0;
+  uint64_t bits = 0;
+#pragma unroll
+  for (int u = 0; u < TW; ++u)
+    if (m[u]) {
+      pl.M[row + u] = m[u];
+      bits |= 1ULL << u;
+    }
+  rec[S::NF] = bits;
+  u64x2 *dst = reinterpret_cast<u64x2 *>(vt + (uint64_t)g * NV);
+#pragma unroll
+  for (int q = 0; q < NV / 2; ++q) dst[q] = u64x2{rec[2 * q], rec[2 * q + 1]};
+  if (keep) {
+    lds_u64x2 *slot = reinterpret_cast<lds_u64x2 *>(lds + (g % S::cap) * NV);
+#pragma unroll
+    for (int q = 0; q < NV / 2; ++q) slot[q] = u64x2{rec[2 * q], rec[2 * q + 1]};
+  }
+  if (bits)
+    atomicOr((unsigned long long *)&pl.NZ[(uint64_t)g * pl.ntiles + (w0 >> 6)],
+             (unsigned long long)(bits << (w0 & 63)));
+}
+
+// Forward walk: levels 0..levels-1 of one tile, sim + forward mark.  VT is
+// the tile-major value plane ([tiles][n][NV] words).  The NZ plane must be
+// all zero (a fill kernel runs first); every block also clears its share of
+// the per-chunk counters (clear32) for the merge and cover.
+template <bool SEQ, bool ROBUST, int TW>
+__global__ __launch_bounds__(kWalkBlock) void k_walk_fwd(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                          const uint64_t *__restrict__ va,
+                                                          const uint64_t *__restrict__ vb, uint64_t vstride,
+                                                          uint64_t wbase, uint64_t *__restrict__ VT,
+                                                          const uint32_t *__restrict__ level_off, uint32_t levels,
+                                                          uint32_t tiles, uint32_t *__restrict__ clear32,
+                                                          uint32_t clear32_n) {
+  using S = WalkShape<SEQ, TW>;
+  __shared__ uint64_t lds_words[S::cap * S::NV];
+  lds_u64 *lds = (lds_u64 *)lds_words;
+  for (uint32_t i = blockIdx.x * kWalkBlock + threadIdx.x; i < clear32_n; i += gridDim.x * kWalkBlock)
+    clear32[i] = 0;
+  uint32_t tile, w0, nw;
+  if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
+  uint64_t *vt = VT + (uint64_t)tile * net.n * S::NV;
+  // each lane's first gate of the next level: its netlist record is loaded
+  // one level ahead (records do not change), off the level's critical path
+  GateRec rn{};
+  if (level_off[0] + threadIdx.x < level_off[1]) rn = recs[level_off[0] + threadIdx.x];
+  for (uint32_t k = 0; k < levels; ++k) {
+    const uint32_t lo = level_off[k], hi = level_off[k + 1];
+    const uint32_t win_lo = hi > S::cap ? hi - S::cap : 0;
+    const GateRec r0 = rn;
+    if (k + 1 < levels && hi + threadIdx.x < level_off[k + 2]) rn = recs[hi + threadIdx.x];
+    for (uint32_t g = lo + threadIdx.x; g < hi; g += kWalkBlock) {
+      const GateRec r = g == lo + threadIdx.x ? r0 : recs[g];
+      walk_fwd_gate<SEQ, ROBUST, TW>(net, r, pl, va, vb, vstride, wbase + w0, g, w0, nw, vt, lds, win_lo,
+                                     g >= win_lo);
+    }
+    __syncthreads();  // this level's records are visible to the next level's lanes
+  }
+}
+
+// Reverse walk: levels levels-1..0 of one tile, removal + merge + the
+// per-word lists of marked nodes (rev_marked on every marked (gate, word) of
+// the tile, frames from VT).  A node's fanouts are on higher levels, so they
+// have pushed into its accumulator before its level runs; its fanins are on
+// lower levels, processed after the barrier.
+template <bool SEQ, int TW>
+__global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                          const uint64_t *__restrict__ VT,
+                                                          const uint32_t *__restrict__ level_off, uint32_t levels,
+                                                          uint32_t tiles, uint32_t start,
+                                                          uint32_t *__restrict__ first_t0,
+                                                          uint32_t *__restrict__ first_t1,
+                                                          uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                          uint64_t list_stride) {
+  using S = WalkShape<SEQ, TW>;
+  uint32_t tile, w0, nw;
+  if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
+  const uint64_t *vt = VT + (uint64_t)tile * net.n * S::NV;
+  for (uint32_t k = levels; k-- > 0;) {
+    const uint32_t lo = level_off[k], hi = level_off[k + 1];
+    for (uint32_t g = lo + threadIdx.x; g < hi; g += kWalkBlock) {
+      uint64_t nzw = vt[(uint64_t)g * S::NV + S::NF];  // the node's mark-present bits of the tile
+      if (!nzw) continue;
+      const GateRec r = recs[g];
+      while (nzw) {
+        const uint32_t w = w0 + (uint32_t)__builtin_ctzll(nzw);
+        nzw &= nzw - 1;
+        const uint64_t *fa = vt + (uint64_t)g * S::NV + (w - w0);
+        rev_marked<SEQ, true, true, true>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride,
+                                          fa, SEQ ? fa + 1 : fa + TW);
+      }
+    }
+    __syncthreads();  // pushes into lower levels' accumulators are visible
   }
 }
 
@@ -1556,6 +1945,11 @@ struct pdf_grader {
   // chunk planes
   uint64_t chunk_pairs = 0, pitch = 0, ntiles = 0;
   uint64_t *d_V = nullptr, *d_VB = nullptr, *d_M = nullptr, *d_NZ = nullptr, *d_ACC = nullptr;
+  // the level walk's tile-major value plane ([tiles][n][words a node]), when
+  // PDF_TUNE_WALK is set; walk_planes: the last chunk graded left its value
+  // planes there (its tile width), not in V / VB
+  uint64_t *d_VT = nullptr;
+  uint32_t walk_planes = 0;
   // merge / cover state
   uint32_t *d_first0 = nullptr, *d_first1 = nullptr;
   // segments of length seg_k >= 2 (stage API): node ids [seg_count][seg_k]
@@ -1595,12 +1989,13 @@ struct pdf_grader {
   uint32_t tune_words_per_lane = 1;
   uint32_t tune_rev_tiles = 0;  // PDF_TUNE_REV_TILES (0: by tile count)
   uint32_t tune_use_graph = 1;
+  uint32_t tune_walk = 0;  // PDF_TUNE_WALK: 0 level launches, else words per tile of the level walk
   // chunk lanes (PDF_TUNE_LANES): the fused grade runs consecutive chunks on
   // `lanes` streams, each lane with its own chunk buffers; lane 0 is the set
   // of fields above, lanes 1.. are kept here
   struct Lane {
     hipStream_t stream = nullptr;
-    uint64_t *V = nullptr, *VB = nullptr, *M = nullptr, *NZ = nullptr, *ACC = nullptr;
+    uint64_t *V = nullptr, *VB = nullptr, *M = nullptr, *NZ = nullptr, *ACC = nullptr, *VT = nullptr;
     uint32_t *list = nullptr, *cstate = nullptr, *act_list = nullptr;
     uint64_t *actmap = nullptr, *chunk_total = nullptr;
     uint32_t *list_cnt = nullptr, *act_count = nullptr;
@@ -1846,6 +2241,68 @@ int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_lo 
   return PDF_OK;
 }
 
+template <bool SEQ, bool R, int TW>
+void launch_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tiles) {
+  const DevNet net = g->net();
+  const uint32_t grid = 8 * ((tiles + 7) / 8);  // walk_tile skips the blocks past the last tile
+  tm_begin(g, kFwd);
+  k_walk_fwd<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride,
+                                                              start / 64, g->d_VT, g->d_level_off, g->nl.L + 1,
+                                                              tiles, g->d_cstate, g->n_cstate);
+  tm_end(g);
+  tm_begin(g, kRev);
+  k_walk_rev<SEQ, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_VT, g->d_level_off, g->nl.L + 1,
+                                                          tiles, (uint32_t)start, g->d_first0, g->d_first1,
+                                                          g->d_list, g->d_list_cnt, g->nl.n);
+  tm_end(g);
+}
+
+template <bool SEQ, bool R>
+void launch_walk_tw(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tw) {
+  const uint32_t tiles = (pl.words + tw - 1) / tw;
+  switch (tw) {
+    case 1: launch_walk_t<SEQ, R, 1>(g, start, pl, tiles); break;
+    case 2: launch_walk_t<SEQ, R, 2>(g, start, pl, tiles); break;
+    case 4: launch_walk_t<SEQ, R, 4>(g, start, pl, tiles); break;
+    default: launch_walk_t<SEQ, R, 8>(g, start, pl, tiles); break;
+  }
+}
+
+// The forward and reverse passes of one chunk as two level walks (words per
+// tile = g->tune_walk), after zeroing the mark-present plane the forward
+// walk ORs into.  The forward walk also zeroes the per-chunk counters.
+int launch_walk(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
+  const Planes pl = planes(g, width);
+  const uint64_t nz_words = (uint64_t)g->nl.n * g->ntiles * 2;
+  tm_begin(g, kFill);
+  k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nz_words, 256), 65536), 256, 0, g->stream>>>(
+      reinterpret_cast<uint32_t *>(g->d_NZ), nz_words, 0u);
+  tm_end(g);
+  const uint32_t tw = g->tune_walk;
+  g->walk_planes = tw;
+  if (g->seq) {
+    if (robust) launch_walk_tw<true, true>(g, start, pl, tw);
+    else launch_walk_tw<true, false>(g, start, pl, tw);
+  } else {
+    if (robust) launch_walk_tw<false, true>(g, start, pl, tw);
+    else launch_walk_tw<false, false>(g, start, pl, tw);
+  }
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
+// Sim, mark, removal and merge of one chunk of a fused grade: the level walk
+// when PDF_TUNE_WALK is set, else the per-level launches.
+int launch_passes(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
+  // (segment merges for K >= 2 read node-major value planes)
+  if (g->tune_walk && g->grade_segs == 1 && g->d_VT) return launch_walk(g, start, width, robust);
+  g->walk_planes = 0;
+  int rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true);
+  if (!rc && !g->seq) rc = reset_chunk(g);
+  if (!rc) rc = launch_reverse<true, true>(g, start, width);
+  return rc;
+}
+
 int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_total = nullptr) {
   const Planes pl = planes(g, width);
   tm_begin(g, kEvents);
@@ -2010,6 +2467,7 @@ void lane_save(const pdf_grader *g, pdf_grader::Lane &l) {
   l.M = g->d_M;
   l.NZ = g->d_NZ;
   l.ACC = g->d_ACC;
+  l.VT = g->d_VT;
   l.list = g->d_list;
   l.cstate = g->d_cstate;
   l.act_list = g->d_act_list;
@@ -2029,6 +2487,7 @@ void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
   g->d_M = l.M;
   g->d_NZ = l.NZ;
   g->d_ACC = l.ACC;
+  g->d_VT = l.VT;
   g->d_list = l.list;
   g->d_cstate = l.cstate;
   g->d_act_list = l.act_list;
@@ -2078,6 +2537,10 @@ int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs) {
   if (!rc) rc = dalloc(&g->d_ACC, n * g->pitch);
   // the removal accumulator is zero between passes (each reader clears it)
   if (!rc && !hip_ok(hipMemset(g->d_ACC, 0, n * g->pitch * 8), "hipMemset")) rc = PDF_ERR_HIP;
+  if (!rc && g->tune_walk) {
+    const uint64_t tw = g->tune_walk, nv = ((g->seq ? tw + 1 : 2 * tw) + 2) & ~1ULL;  // + the mark-present word, even
+    rc = dalloc(&g->d_VT, n * ((words + tw - 1) / tw) * nv);
+  }
   if (!rc) rc = dalloc(&g->d_list, n * words);
   const uint32_t ct_off = (uint32_t)((3 * words + 2) & ~1ULL);
   g->n_cstate = ct_off + 2;
@@ -2101,6 +2564,8 @@ void free_chunk_buffers(pdf_grader *g) {
   dfree(g->d_M);
   dfree(g->d_NZ);
   dfree(g->d_ACC);
+  dfree(g->d_VT);
+  g->walk_planes = 0;
   dfree(g->d_list);
   dfree(g->d_cstate);
   g->d_actmap = g->d_chunk_total = nullptr;
@@ -2400,6 +2865,7 @@ int pdf_sim(pdf_grader *g, uint64_t chunk, uint64_t start, float *ms) {
   if (rc) return rc;
   HIPCHK(hipSetDevice(g->device));
   EventTimer tm(g, ms);
+  g->walk_planes = 0;
   if ((rc = launch_fwd<true, false>(g, start, width, 0))) return rc;
   if ((rc = tm.finish())) return rc;
   g->stage_chunk = (int64_t)chunk;
@@ -2522,21 +2988,29 @@ static int queue_grade_lanes(pdf_grader *g, int robust) {
       width = std::min(g->chunk_pairs, g->num_pairs - start);
     };
     uint64_t start, width;
-    if (!g->seq)
+    if (g->tune_walk && K == 1 && g->d_VT) {
+      // one forward and one reverse walk per chunk, each on its lane
       for (uint64_t c = c0; c < c1 && !rc; ++c) {
         chunk(c, start, width);
-        rc = reset_chunk(g);
+        rc = launch_walk(g, start, width, robust);
       }
-    for (uint32_t k = 0; k <= g->nl.L && !rc; ++k)
-      for (uint64_t c = c0; c < c1 && !rc; ++c) {
-        chunk(c, start, width);
-        rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true, 0xFFFFFFFFu, k, k);
-      }
-    for (uint32_t k = g->nl.L + 1; k-- > 0 && !rc;)
-      for (uint64_t c = c0; c < c1 && !rc; ++c) {
-        chunk(c, start, width);
-        rc = launch_reverse<true, true>(g, start, width, k, k);
-      }
+    } else {
+      if (!g->seq)
+        for (uint64_t c = c0; c < c1 && !rc; ++c) {
+          chunk(c, start, width);
+          rc = reset_chunk(g);
+        }
+      for (uint32_t k = 0; k <= g->nl.L && !rc; ++k)
+        for (uint64_t c = c0; c < c1 && !rc; ++c) {
+          chunk(c, start, width);
+          rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true, 0xFFFFFFFFu, k, k);
+        }
+      for (uint32_t k = g->nl.L + 1; k-- > 0 && !rc;)
+        for (uint64_t c = c0; c < c1 && !rc; ++c) {
+          chunk(c, start, width);
+          rc = launch_reverse<true, true>(g, start, width, k, k);
+        }
+    }
     if (K >= 2)
       for (uint64_t c = c0; c < c1 && !rc; ++c) {
         chunk(c, start, width);
@@ -2580,9 +3054,7 @@ static int build_chunk_graphs(pdf_grader *g, int robust) {
       rc = PDF_ERR_HIP;
       break;
     }
-    if (!g->seq) rc = reset_chunk(g);
-    if (!rc) rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true);
-    if (!rc) rc = launch_reverse<true, true>(g, start, width);
+    rc = launch_passes(g, start, width, robust);
     const hipError_t e = hipStreamEndCapture(g->stream, &graph);
     if (!rc && !hip_ok(e, "hipStreamEndCapture")) rc = PDF_ERR_HIP;
     hipGraphExec_t exec = nullptr;
@@ -2663,9 +3135,7 @@ static int queue_grade(pdf_grader *g, int robust) {
   for (uint64_t c = 0; c < nchunks; ++c) {
     const uint64_t start = c * g->chunk_pairs;
     const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - start);
-    if ((rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true))) return rc;
-    if (!g->seq && (rc = reset_chunk(g))) return rc;
-    if ((rc = launch_reverse<true, true>(g, start, width))) return rc;
+    if ((rc = launch_passes(g, start, width, robust))) return rc;
     if (K >= 2) {
       if ((rc = launch_merge_seg(g, start, width))) return rc;
       if ((rc = launch_cover_segs(g, start, width))) return rc;
@@ -2739,6 +3209,15 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
     case PDF_TUNE_USE_GRAPH:
       g->tune_use_graph = value != 0;
       break;
+    case PDF_TUNE_WALK:
+      if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) {
+        set_error("words per walk tile must be 0 (per-level launches), 1, 2, 4 or 8");
+        return PDF_ERR_ARG;
+      }
+      if (g->tune_walk == (uint32_t)value) break;
+      g->tune_walk = (uint32_t)value;
+      // the walk's tile-major plane is one of the chunk buffers
+      return g->chunk_pairs ? pdf_grader_set_chunk(g, g->chunk_pairs) : PDF_OK;
     case PDF_TUNE_REV_TILES:
       if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) {
         set_error("reverse tiles per wave must be 0 (automatic), 1, 2, 4 or 8");
@@ -2978,6 +3457,7 @@ int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_d
   g->ntiles = p.ntiles;
   g->n_cstate = p.n_cstate;
   g->cover_waves = p.waves;
+  g->d_VT = nullptr;  // no tile-major plane in the scratch: per-level launches
   g->grade_segs = 1;
   g->one_lane = true;
   // the removal accumulator must start at zero and the array keeps whatever
@@ -3030,6 +3510,25 @@ int pdf_get_sim_planes(const pdf_grader *gc, uint64_t *a_plane, uint64_t *b_plan
   HIPCHK(hipSetDevice(g->device));
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
   const uint64_t words = (width + 63) / 64, n = g->nl.n;
+  const uint64_t last = width % 64 ? (1ULL << (width % 64)) - 1 : ~0ULL;
+  if (g->walk_planes) {
+    // the level walk's tile-major records: [tiles][n][NV], TW + 1 frame
+    // words (frame 1 of the tile's words and the halo) or 2 TW (both
+    // frames), then the tile's mark-present bits, padded to an even count
+    const uint64_t tw = g->walk_planes, nv = ((g->seq ? tw + 1 : 2 * tw) + 2) & ~1ULL, tiles = (words + tw - 1) / tw;
+    std::vector<uint64_t> vt(tiles * n * nv);
+    HIPCHK(hipMemcpy(vt.data(), g->d_VT, vt.size() * 8, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i)
+      for (uint64_t k = 0; k < words; ++k) {
+        const uint64_t *rec = vt.data() + ((k / tw) * n + i) * nv;
+        const uint64_t u = k % tw, a = rec[u];
+        uint64_t b = g->seq ? ((a >> 1) | (rec[u + 1] << 63)) : rec[tw + u];
+        if (k + 1 == words) b = (b & last) | (a & ~last);  // lanes past the last pair read as stable
+        if (a_plane) a_plane[i * words + k] = a;
+        if (b_plane) b_plane[i * words + k] = b;
+      }
+    return PDF_OK;
+  }
   std::vector<uint64_t> v(n * (words + 1)), vb;
   int rc = copy_rows(g, v.data(), g->d_V, words + 1);
   if (rc) return rc;
@@ -3037,7 +3536,6 @@ int pdf_get_sim_planes(const pdf_grader *gc, uint64_t *a_plane, uint64_t *b_plan
     vb.resize(n * words);
     if ((rc = copy_rows(g, vb.data(), g->d_VB, words))) return rc;
   }
-  const uint64_t last = width % 64 ? (1ULL << (width % 64)) - 1 : ~0ULL;
   for (uint64_t i = 0; i < n; ++i)
     for (uint64_t k = 0; k < words; ++k) {
       const uint64_t a = v[i * (words + 1) + k];

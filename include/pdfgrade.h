@@ -274,6 +274,9 @@ int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_d
 #define PDF_TUNE_LANES 5          /* grader: chunks graded concurrently on 1..4 streams */
 #define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items */
 #define PDF_TUNE_REV_TILES 7      /* grader: 64-word tiles per reverse-pass wave, 1/2/4/8 (0: by waves per launch) */
+#define PDF_TUNE_WALK 8           /* grader: 0 per-level launches (the reference's host level loops); 1/2/4/8 fused
+                                   * grades run each chunk's forward and reverse passes as one launch each, a
+                                   * workgroup per tile of that many 64-pair words walking every level */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same

@@ -1,0 +1,132 @@
+"""The level walk (PDF_TUNE_WALK: one forward and one reverse launch per chunk,
+a workgroup per tile of 1/2/4/8 words walking every level) against the CPU
+oracle on small circuits and the bit-sliced CPU grader at full size; integer
+work, so every output must be equal."""
+import sys
+
+import numpy as np
+import pytest
+
+import pdfgrade as pg
+from pdfgrade.configs import GATES_50K, TINY
+
+from helpers import biased_vectors, bitsliced_grade, edge_netlist, oracle_grade, pack, small_config, unpack
+
+pytestmark = pytest.mark.gpu
+
+WALKS = (1, 2, 4, 8)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def walk_grade(nl, bits, P, chunk, robust, tw, b_bits=None, lanes=1, graph=1, reps=1):
+    with pg.Grader(nl) as g:
+        if b_bits is None:
+            g.set_vectors(bits, P)
+        else:
+            g.set_pairs(bits, b_bits, P)
+        g.set_chunk(chunk)
+        g.set_tuning(pg.TUNE_WALK, tw)
+        g.set_tuning(pg.TUNE_LANES, lanes)
+        g.set_tuning(pg.TUNE_USE_GRAPH, graph)
+        out = []
+        for _ in range(reps):
+            total = g.grade(robust)
+            f0, f1 = g.first()
+            out.append(dict(total=total, first_t0=f0, first_t1=f1, gate_counts=g.gate_counts()))
+        return out
+
+
+def assert_same(got, ref, ctx=""):
+    assert got["total"] == ref["total"], (ctx, got["total"], ref["total"])
+    assert np.array_equal(got["first_t0"], ref["first_t0"]), ctx
+    assert np.array_equal(got["first_t1"], ref["first_t1"]), ctx
+    assert np.array_equal(got["gate_counts"], ref["gate_counts"]), ctx
+
+
+@pytest.mark.parametrize("tw", WALKS)
+def test_walk_tiny_workload(tw):
+    nl = pg.Netlist.generate(TINY.netlist)
+    bits = pg.gen_vectors(TINY.vector_seed, nl.info()["num_pi"], TINY.pairs)
+    for robust in (True, False):
+        for chunk in (64, 320, 1024):
+            say("walk tiny", tw, robust, chunk)
+            ref = oracle_grade(nl, bits, TINY.pairs, chunk, robust)
+            for got in walk_grade(nl, bits, TINY.pairs, chunk, robust, tw, reps=2):
+                assert_same(got, ref, (tw, robust, chunk))
+
+
+@pytest.mark.parametrize("tw", WALKS)
+def test_walk_random_circuits(tw):
+    for seed in range(10):
+        cfg = small_config(100 + seed, num_pi=6 + seed % 5, num_gates=40 + 9 * seed, levels=4 + seed % 7,
+                           wide=seed % 2 == 0, alpha=2.2 if seed % 3 == 0 else 0.0)
+        nl = pg.Netlist.generate(cfg)
+        P = 64 * (3 + seed % 5) + (seed * 13) % 64  # ragged: not a multiple of 64 for most seeds
+        bits = biased_vectors(seed, nl.info()["num_pi"], P, p_hold=0.3 + 0.1 * (seed % 5))
+        for robust in (True, False):
+            say("walk random", tw, seed, robust)
+            chunk = 64 * (1 + seed % 3)
+            ref = oracle_grade(nl, bits, P, chunk, robust)
+            assert_same(walk_grade(nl, bits, P, chunk, robust, tw)[0], ref, (tw, seed, robust))
+
+
+@pytest.mark.parametrize("tw", WALKS)
+def test_walk_edge_netlist_and_explicit_pairs(tw):
+    nl = edge_netlist()
+    P = 704  # 11 words: a partial last tile for every tw > 1
+    bits = biased_vectors(9, 8, P, p_hold=0.5)
+    I = 8
+    b = pack(np.where(np.random.default_rng(3).random((P, I)) < 0.3, 1 - unpack(bits, P), unpack(bits, P)))
+    for robust in (True, False):
+        say("walk edge", tw, robust)
+        ref = oracle_grade(nl, bits, P, 448, robust)
+        assert_same(walk_grade(nl, bits, P, 448, robust, tw)[0], ref, ("edge", tw, robust))
+        ref = oracle_grade(nl, bits, P, 448, robust, b_bits=b)
+        assert_same(walk_grade(nl, bits, P, 448, robust, tw, b_bits=b)[0], ref, ("explicit", tw, robust))
+
+
+@pytest.mark.parametrize("tw", (2, 4))
+def test_walk_lanes_and_graphs(tw):
+    """Several chunks on 2 lanes, with and without graph replay, repeated."""
+    nl = pg.Netlist.generate(small_config(7, num_pi=16, num_po=24, num_gates=900, levels=14, wide=True,
+                                          alpha=2.2))
+    P = 8192
+    bits = biased_vectors(7, nl.info()["num_pi"], P, p_hold=0.6)
+    for robust in (True, False):
+        ref = oracle_grade(nl, bits, P, 2048, robust)
+        for graph in (0, 1):
+            say("walk lanes", tw, robust, graph)
+            for got in walk_grade(nl, bits, P, 2048, robust, tw, lanes=2, graph=graph, reps=2):
+                assert_same(got, ref, (tw, robust, graph))
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_walk_50k_full_size_against_bitsliced(robust):
+    """All 65,536 pairs of the 50k-gate workload in one chunk, every tile
+    width, against the bit-sliced CPU grader."""
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = GATES_50K.pairs
+    bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
+    ref = bitsliced_grade(nl, bits, P, P, robust)
+    for tw in WALKS:
+        say("walk 50k vs bit-sliced", robust, tw)
+        for got in walk_grade(nl, bits, P, P, robust, tw, reps=2):
+            assert_same(got, ref, (robust, tw))
+
+
+def test_walk_50k_hold_biased_against_bitsliced():
+    """Hold-biased vectors (many more marks and paths than uniform ones) on
+    the 50k-gate netlist, both modes, against the bit-sliced CPU grader."""
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = 16384
+    bits = biased_vectors(41, nl.info()["num_pi"], P, p_hold=0.9)
+    for robust in (True, False):
+        ref = bitsliced_grade(nl, bits, P, P, robust)
+        assert ref["total"] > 0
+        for tw in (2, 4):
+            say("walk 50k hold-biased", robust, tw)
+            assert_same(walk_grade(nl, bits, P, P, robust, tw)[0], ref, (robust, tw))

@@ -1063,6 +1063,15 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
 __device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 2654435761u) >> 22; }  // 10 bits
 
+// How much of the cover DP stays in LDS: at most `nodes` marked nodes of a
+// pair (else the DP keeps c/h in global scratch) and `pool` fanout ids (the
+// rest are read from the netlist in the DP).  The defaults are the LDS
+// sizes; smaller values (a tuning knob for tests) send more pairs down the
+// global-memory paths.
+struct CoverCaps {
+  uint32_t nodes = kCoverCap, pool = kPool;
+};
+
 // Reverse path-count DP (coverkernel.cuh:194-228) for each distinct
 // first-detection pair p of the chunk, over the nodes marked at p.  One wave
 // per pair.  The word's node list (descending level order) is filtered to the
@@ -1100,7 +1109,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
                           uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
                           uint32_t *__restrict__ pair_gain, uint32_t rel,
                           const uint64_t *__restrict__ newp = nullptr, uint64_t *__restrict__ pi_cover = nullptr,
-                          uint64_t pi_ld = 0) {
+                          uint64_t pi_ld = 0, CoverCaps caps = CoverCaps{}) {
   const uint32_t lane = threadIdx.x;
   const uint64_t below = (1ULL << lane) - 1;
   const uint32_t w = rel >> 6, j = rel & 63;
@@ -1125,7 +1134,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
     K += (uint32_t)__builtin_popcountll(bal);
   }
   __syncthreads();
-  const bool in_lds = K <= kCoverCap;
+  const bool in_lds = K <= caps.nodes;
   uint64_t tot = 0;
   bool ovf = false;
   if (in_lds) {
@@ -1158,7 +1167,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
     __syncthreads();
     // 3. fanout ids into the pool, one pool slot per lane and eight loads in
     // flight (nodes past the pool's end read theirs in the DP)
-    const uint32_t F = carry < kPool ? carry : kPool;
+    const uint32_t F = carry < caps.pool ? carry : caps.pool;
     for (uint32_t base = 0; base < F; base += 8 * 64) {
       uint32_t ids[8];
 #pragma unroll
@@ -1193,7 +1202,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
       if (lane < run) {
         const uint8_t fl = S.s_flags[idx];
         const uint32_t off = S.s_pool_off[idx], nfo = S.s_nfo[idx];
-        const bool pooled = off + nfo <= kPool;
+        const bool pooled = off + nfo <= caps.pool;
         const uint32_t fo_off = S.s_fo_off[idx];
         uint64_t C = 0, H = fl & 1;
         for (uint32_t e = 0; e < nfo; ++e) {
@@ -1295,7 +1304,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
                                               uint64_t *__restrict__ grand_total, uint32_t *__restrict__ err,
                                               uint32_t width, uint32_t *__restrict__ pair_gain,
                                               const uint64_t *__restrict__ newp, uint64_t *__restrict__ pi_cover,
-                                              uint64_t pi_ld) {
+                                              uint64_t pi_ld, CoverCaps caps) {
   __shared__ CoverLds S;
   uint64_t *c_s = cs + (uint64_t)blockIdx.x * net.n;
   uint64_t *h_s = hs + (uint64_t)blockIdx.x * net.n;
@@ -1303,7 +1312,7 @@ __global__ __launch_bounds__(64) void k_cover(DevNet net, Planes pl, uint32_t st
   for (uint32_t k = blockIdx.x; k < nact; k += gridDim.x)
     cover_one<ALL_NEW, SEGNEW>(S, net, pl, start, list, list_cnt, list_stride, first_t0, first_t1, c_s, h_s,
                                gate_counts, chunk_total, grand_total, err, pair_gain, ALL_NEW ? k : act_list[k], newp,
-                               pi_cover, pi_ld);
+                               pi_cover, pi_ld, caps);
 }
 
 // gpuMergeSegments<K> for K >= 2 (kernSegmentReduce2 with has_marked_segment,
@@ -1979,6 +1988,7 @@ struct pdf_grader {
   uint32_t *d_list_cnt = nullptr, *d_act_count = nullptr;
   uint64_t *d_cs = nullptr, *d_hs = nullptr;
   uint32_t cover_waves = 0;
+  CoverCaps cover_caps;  // PDF_TUNE_COVER_NODES / PDF_TUNE_COVER_POOL
   uint32_t max_cover_waves = 512;  // 1 for batch members (the batch has its own scratch)
   // per-launch event timing (pdf_grade_timed)
   std::vector<hipEvent_t> ev_pool;
@@ -2314,7 +2324,7 @@ int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_t
                                                        g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                        g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
                                                        g->d_grand, g->d_err, (uint32_t)width, pair_total,
-                                                       nullptr, g->pic_target, g->chunk_pairs);
+                                                       nullptr, g->pic_target, g->chunk_pairs, g->cover_caps);
   tm_end(g);
   HIPCHK(hipGetLastError());
   return PDF_OK;
@@ -2334,7 +2344,7 @@ int launch_cover_segs(pdf_grader *g, uint64_t start, uint64_t width) {
   k_cover<false, true><<<g->cover_waves, 64, 0, g->stream>>>(
       g->net(), pl, (uint32_t)start, g->d_list, g->d_list_cnt, g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total, g->d_grand, g->d_err, (uint32_t)width, nullptr,
-      g->d_NEW, g->pic_target, g->chunk_pairs);
+      g->d_NEW, g->pic_target, g->chunk_pairs, g->cover_caps);
   tm_end(g);
   tm_begin(g, kEvents);
   k_seg_clear<<<grid_for(g->seg_count, 256), 256, 0, g->stream>>>(g->d_seg, g->seg_k, g->seg_count, g->d_sfirst0,
@@ -3209,6 +3219,14 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
     case PDF_TUNE_USE_GRAPH:
       g->tune_use_graph = value != 0;
       break;
+    case PDF_TUNE_COVER_NODES:
+      if (value < 1 || value > (int64_t)kCoverCap) { set_error("cover LDS nodes must be 1..512"); return PDF_ERR_ARG; }
+      g->cover_caps.nodes = (uint32_t)value;
+      break;
+    case PDF_TUNE_COVER_POOL:
+      if (value < 0 || value > (int64_t)kPool) { set_error("cover fanout pool must be 0..4096"); return PDF_ERR_ARG; }
+      g->cover_caps.pool = (uint32_t)value;
+      break;
     case PDF_TUNE_WALK:
       if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) {
         set_error("words per walk tile must be 0 (per-level launches), 1, 2, 4 or 8");
@@ -3797,7 +3815,7 @@ int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev) {
                                                       g->nl.n, g->d_act_list, g->d_act_count, g->d_first0,
                                                       g->d_first1, g->d_cs, g->d_hs, g->d_gate, g->d_chunk_total,
                                                       g->d_grand, g->d_err, (uint32_t)width, gain_dev, nullptr,
-                                                      nullptr, 0);
+                                                      nullptr, 0, g->cover_caps);
     HIPCHK(hipGetLastError());
   }
   uint32_t err = 0;

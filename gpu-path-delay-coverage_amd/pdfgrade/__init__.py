@@ -33,6 +33,7 @@ NONE = 0xFFFFFFFF
 TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES, TUNE_REV_COMPACT = 1, 2, 3, 4, 5, 6
 TUNE_REV_TILES = 7
 TUNE_WALK = 8
+TUNE_COVER_NODES, TUNE_COVER_POOL = 9, 10
 DEBUG_FORWARD_MARKS, DEBUG_PI_COVER = 1, 2
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (

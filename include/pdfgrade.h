@@ -277,6 +277,10 @@ int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_d
 #define PDF_TUNE_WALK 8           /* grader: 0 per-level launches (the reference's host level loops); 1/2/4/8 fused
                                    * grades run each chunk's forward and reverse passes as one launch each, a
                                    * workgroup per tile of that many 64-pair words walking every level */
+#define PDF_TUNE_COVER_NODES 9    /* grader: marked nodes of a pair whose cover DP stays in LDS, 1..512 (512);
+                                   * pairs with more keep c/h in global scratch */
+#define PDF_TUNE_COVER_POOL 10    /* grader: fanout ids of those nodes kept in LDS, 0..4096 (4096); the rest are
+                                   * read from the netlist in the DP (both knobs exist to exercise those paths) */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same

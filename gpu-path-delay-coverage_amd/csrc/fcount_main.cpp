@@ -136,6 +136,7 @@ int main(int argc, char **argv) {
   std::string infile;
   std::vector<std::string> benchmarks;
   long override_patterns = 0;
+  long merge_window = 0;
   int segs = 1;
   static struct option long_options[] = {{"verbose", no_argument, &verbose_flag, 1},
                                          {"cover", no_argument, &cover_flag, 1},
@@ -149,6 +150,8 @@ int main(int argc, char **argv) {
                                          {"bench", required_argument, 0, 'b'},
                                          {"num_patterns", required_argument, 0, 'o'},
                                          {"segs", required_argument, 0, 's'},
+                                         {"merge_window", required_argument, 0, 'w'},
+                                         {"ref_window", no_argument, 0, 'r'},
                                          {0, 0, 0, 0}};
   while (true) {
     int idx = 0;
@@ -158,6 +161,8 @@ int main(int argc, char **argv) {
       case 0: break;
       case 'b': infile = optarg; break;
       case 'o': override_patterns = std::atol(optarg); break;
+      case 'w': merge_window = std::atol(optarg); break;
+      case 'r': merge_window = 4224; break;
       case 's':
         segs = std::atoi(optarg);
         if (segs > 8) die("Segment lengths over 8 are not supported!");
@@ -170,6 +175,8 @@ int main(int argc, char **argv) {
         std::printf("\t--robust : robust sensitization (default non-robust)\n");
         std::printf("\t--num_patterns N : pairs per chunk = N-1, rounded down to a multiple of 64\n");
         std::printf("\t--fused : whole chunk loop in one call (pdf_grade)\n");
+        std::printf("\t--ref_window : merge only the first 4224 pairs of each chunk, as the reference does\n");
+        std::printf("\t--merge_window N : merge only the first N pairs of each chunk (0 = all)\n");
         return 0;
       default: return EXIT_FAILURE;
     }
@@ -186,6 +193,8 @@ int main(int argc, char **argv) {
   std::clog << "Circuit size is: " << info.n << " Levels: " << info.levels << std::endl;
   pdf_grader *g = nullptr;
   check(pdf_grader_create(0, nl, &g), "pdf_grader_create");
+  if (merge_window < 0 || merge_window > 0xFFFFFFFFL) die("--merge_window must be 0..4294967295");
+  check(pdf_grader_set_merge_window(g, (uint32_t)merge_window), "set_merge_window");
   if (verbose_flag && !fused_flag)
     check(pdf_grader_set_debug(g, (mark_flag ? PDF_DEBUG_FORWARD_MARKS : 0u) | PDF_DEBUG_PI_COVER), "set_debug");
 
@@ -267,6 +276,16 @@ int main(int argc, char **argv) {
         for (uint32_t i = 0; i < info.n; ++i) f << "Gate " << i << ": " << counts[i] << "\n";
       }
     }
+    // where the reference's own run would differ (its resource limits)
+    uint32_t ref = 0;
+    uint64_t unmerged = 0;
+    check(pdf_grader_ref_flags(g, &ref, &unmerged), "ref_flags");
+    if (ref & PDF_REF_EXCEEDS_U16) std::cerr << "note: a path count exceeds the reference's 16-bit fields\n";
+    if (ref & PDF_REF_EXCEEDS_U32) std::cerr << "note: a chunk total exceeds the reference's uint32 sums\n";
+    if (ref & PDF_REF_SEGMENT_STRIDE)
+      std::cerr << "note: the reference's merge would skip " << unmerged << " segments\n";
+    if (ref & PDF_REF_MERGE_WINDOW)
+      std::cerr << "note: chunks exceed the reference's 4224-pattern merge window (see --ref_window)\n";
     std::cerr << "   GPU: " << gpu << " ms" << std::endl;
     std::cout << vf << ":" << V << "," << info.n << "," << segs << ";" << gpu << ";" << sim << ";" << mark << ";"
               << merge << ";" << cover << ";" << totals << std::endl;

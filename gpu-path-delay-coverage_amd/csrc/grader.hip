@@ -81,7 +81,26 @@ struct Planes {
   uint32_t words;      // pair words in this chunk
   uint32_t ntiles;     // NZ words per node
   uint64_t last_mask;  // valid pairs of the chunk's last word
+  uint32_t merge_window = 0;  // 0: merge every pair of the chunk; else only its first merge_window pairs
 };
+
+// Pairs of word w that the merge may take as first detections: all of them,
+// or with a merge window only those whose index in the chunk is below it
+// (the reference examines the first 4,224 patterns of a chunk only,
+// mergekernel.cuh:125-130).
+__device__ __forceinline__ uint64_t merge_mask(const Planes &pl, uint32_t w) {
+  if (!pl.merge_window) return ~0ULL;
+  const uint64_t lo = 64ull * w;
+  if (lo >= pl.merge_window) return 0;
+  const uint64_t room = pl.merge_window - lo;
+  return room >= 64 ? ~0ULL : (1ULL << room) - 1;
+}
+
+// Bits of the per-grade error word: an exact uint64 count overflowed, and
+// the flags of values that leave the reference's ranges (results stay exact)
+constexpr uint32_t kErrOverflow = 1u;
+constexpr uint32_t kErrRefU16 = 2u;  // a c / h the reference packs into 16 bits (coverkernel.cuh:41-43)
+constexpr uint32_t kErrRefU32 = 4u;  // a chunk total past the reference's uint32 sums (coverkernel.cuh:79-101)
 
 // Thread -> (gate, word) for a level launch.  A block of kBlock threads covers
 // (kBlock >> twlog) gates x (1 << twlog) consecutive words; block b takes word
@@ -593,7 +612,8 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
     }
   }
   if (DO_MERGE && m) {
-    const uint64_t t0 = m & ~b, t1 = m & b;  // marked => transition; b = 1 means T1
+    const uint64_t mm = m & merge_mask(pl, w);
+    const uint64_t t0 = mm & ~b, t1 = mm & b;  // marked => transition; b = 1 means T1
     const uint32_t base = start + 64u * w;
     // the first-detection pairs only decrease within a grade (reset by an
     // earlier kernel), so a value read above is at least the current one: a
@@ -1136,7 +1156,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
   __syncthreads();
   const bool in_lds = K <= caps.nodes;
   uint64_t tot = 0;
-  bool ovf = false;
+  bool ovf = false, ref16 = false;
   if (in_lds) {
     // 2. records, hash, and the fanout-id pool offsets
     uint32_t carry = 0;
@@ -1218,6 +1238,9 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
         uint64_t c = C, h = H;
         if (fl & 4) { ovf |= add_ovf(c, H); h = 0; }
         if (fl & 2) h = 0;
+        // the reference's 16-bit fields: its fanout sums and final c / h
+        // (coverkernel.cuh:210-211, 224-225); H includes the PO's own path
+        ref16 |= C > 0xFFFF || H - (fl & 1) > 0xFFFF || c > 0xFFFF || h > 0xFFFF;
         S.s_c[idx] = c;
         S.s_h[idx] = h;
       }
@@ -1260,6 +1283,7 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
         uint64_t c = C, h = H;
         if (is_new(g)) { ovf |= add_ovf(c, H); h = 0; }
         if (r.type == PDF_INPT) h = 0;
+        ref16 |= C > 0xFFFF || H - (r.po == 1 ? 1 : 0) > 0xFFFF || c > 0xFFFF || h > 0xFFFF;
         c_s[g] = c;
         h_s[g] = h;
         if (c) {
@@ -1284,12 +1308,18 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
     ovf |= add_ovf(tot, o);
   }
   if (pair_gain && lane == 0) pair_gain[p] = tot > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)tot;
+  bool ref32 = false;
   if (!ALL_NEW && lane == 0 && tot) {
     const uint64_t o0 = atomicAdd((unsigned long long *)chunk_total, (unsigned long long)tot);
     const uint64_t o1 = atomicAdd((unsigned long long *)grand_total, (unsigned long long)tot);
     ovf |= (o0 + tot < o0) || (o1 + tot < o1);
+    // the chunk's running total passing 2^32 - 1: the reference's uint32
+    // partial sums would wrap
+    ref32 = o0 + tot > 0xFFFFFFFFull;
   }
-  if (__ballot(ovf)) { if (lane == 0) atomicOr(err, 1u); }
+  const uint32_t e = (__ballot(ovf) ? kErrOverflow : 0u) | (!ALL_NEW && __ballot(ref16) ? kErrRefU16 : 0u) |
+                     (__ballot(ref32) ? kErrRefU32 : 0u);
+  if (e && lane == 0) atomicOr(err, e);
   __syncthreads();
 }
 
@@ -1399,7 +1429,7 @@ __global__ __launch_bounds__(256) void k_merge_seg(Planes pl, const uint32_t *__
       if (SEQ) b = shift_next(a, b);
       uint64_t m = 0;
       if (on) {
-        m = ~0ULL;
+        m = merge_mask(pl, w);
 #pragma unroll
         for (int i = 0; i < 8; ++i) m &= x[i];
       }
@@ -1736,7 +1766,7 @@ __global__ void k_iota(uint32_t *p, uint64_t n) {
 __global__ void k_copy_u64(const uint64_t *src, uint64_t *dst) { *dst = *src; }
 __global__ void k_copy_result(const uint64_t *total, const uint32_t *err, uint64_t *total_out, uint32_t *err_out) {
   *total_out = *total;
-  if (err_out) *err_out = *err;
+  if (err_out) *err_out = *err & kErrOverflow;
 }
 
 // ------------------------------------------------------------ batched grading
@@ -2000,6 +2030,7 @@ struct pdf_grader {
   uint32_t tune_rev_tiles = 0;  // PDF_TUNE_REV_TILES (0: by tile count)
   uint32_t tune_use_graph = 1;
   uint32_t tune_walk = 0;  // PDF_TUNE_WALK: 0 level launches, else words per tile of the level walk
+  uint32_t merge_window = 0;  // pdf_grader_set_merge_window
   // chunk lanes (PDF_TUNE_LANES): the fused grade runs consecutive chunks on
   // `lanes` streams, each lane with its own chunk buffers; lane 0 is the set
   // of fields above, lanes 1.. are kept here
@@ -2114,7 +2145,7 @@ int check_chunk(pdf_grader *g, uint64_t chunk, uint64_t start, uint64_t *width) 
 Planes planes(pdf_grader *g, uint64_t width) {
   const uint32_t tail = (uint32_t)(width % 64);
   return Planes{g->d_V,  g->d_VB, g->d_M, g->d_NZ, g->d_ACC, g->pitch, (uint32_t)((width + 63) / 64),
-                (uint32_t)g->ntiles, tail ? (1ULL << tail) - 1 : ~0ULL};
+                (uint32_t)g->ntiles, tail ? (1ULL << tail) - 1 : ~0ULL, g->merge_window};
 }
 
 LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, bool extra_column, uint32_t *grid) {
@@ -2962,7 +2993,7 @@ int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, uint64_t 
   HIPCHK(hipMemcpy(&tot, g->d_chunk_total, sizeof(tot), hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost));
   g->stage = 4;
-  if (err) { set_error("pdf_cover: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  if (err & kErrOverflow) { set_error("pdf_cover: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   *coverage = tot;
   return PDF_OK;
 }
@@ -3197,12 +3228,55 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
-  if (err) { set_error("pdf_grade: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  g->merged_segs = g->grade_segs;
+  if (err & kErrOverflow) { set_error("pdf_grade: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   *total = tot;
   return PDF_OK;
 }
 
 int pdf_grade(pdf_grader *g, int robust, uint64_t *total) { return grade_impl(g, robust, total); }
+
+int pdf_grader_set_merge_window(pdf_grader *g, uint32_t pairs) {
+  if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  HIPCHK(hipStreamSynchronize(g->stream));
+  g->merge_window = pairs;
+  drop_graph(g);  // captured launches hold the old window
+  return PDF_OK;
+}
+
+uint64_t pdf_ref_unmerged_segments(uint64_t S) {
+  // gpuMergeSegments (mergekernel.cuh:229-250): ceil(S / 20) blocks of 20
+  // segments; launch i runs min(65535, blocks left) of them starting at
+  // segment 65535 i (count += BLOCK_STEP), so it reaches segments
+  // [65535 i, 65535 i + 20 b_i); the blocks left drop by 65535 per launch
+  const uint64_t step = 65535, per = 20;
+  uint64_t left = (S + per - 1) / per, covered = 0, reach = 0;
+  for (uint64_t i = 0; left > 0; ++i) {
+    const uint64_t b = std::min(left, step);
+    const uint64_t lo = std::min(S, i * step), hi = std::min(S, i * step + per * b);
+    if (hi > reach) {  // launches start in increasing order: count what extends past the reach so far
+      covered += hi - std::max(lo, reach);
+      reach = hi;
+    }
+    left = left >= step ? left - step : 0;
+  }
+  return S - covered;
+}
+
+int pdf_grader_ref_flags(const pdf_grader *g, uint32_t *flags, uint64_t *unmerged_segments) {
+  if (!g || !flags) { set_error("null argument"); return PDF_ERR_ARG; }
+  HIPCHK(hipSetDevice(g->device));
+  uint32_t err = 0;
+  HIPCHK(hipMemcpy(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost));
+  const uint64_t S = g->merged_segs >= 2 ? g->seg_count : g->nl.n;
+  const uint64_t un = pdf_ref_unmerged_segments(S);
+  *flags = ((err & kErrRefU16) ? PDF_REF_EXCEEDS_U16 : 0u) | ((err & kErrRefU32) ? PDF_REF_EXCEEDS_U32 : 0u) |
+           (un ? PDF_REF_SEGMENT_STRIDE : 0u) |
+           (g->merge_window == 0 && g->chunk_pairs > 4224 ? PDF_REF_MERGE_WINDOW : 0u);
+  if (unmerged_segments) *unmerged_segments = un;
+  return PDF_OK;
+}
 
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
   if (!g) { set_error("null grader"); return PDF_ERR_ARG; }
@@ -3823,7 +3897,7 @@ int pdf_pair_gains(pdf_grader *g, int robust, uint32_t *gain_dev) {
   HIPCHK(hipStreamSynchronize(g->stream));
   g->stage = 0;
   g->stage_chunk = -1;
-  if (err) { set_error("pdf_pair_gains: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  if (err & kErrOverflow) { set_error("pdf_pair_gains: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   return PDF_OK;
 }
 
@@ -3933,7 +4007,7 @@ int pdf_grade_curve(pdf_grader *g, int robust, uint64_t *cum_total, float *cum_m
   if (rc) return rc;
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
-  if (err) { set_error("pdf_grade_curve: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
+  if (err & kErrOverflow) { set_error("pdf_grade_curve: uint64 path count overflow"); return PDF_ERR_OVERFLOW; }
   return PDF_OK;
 }
 
@@ -4247,7 +4321,7 @@ int batch_results(pdf_batch *b, uint64_t *totals) {
   if (totals) HIPCHK(hipMemcpyAsync(totals, b->d_totals, b->C * 8, hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   for (uint32_t c = 0; c < b->C; ++c)
-    if (errs[c]) {
+    if (errs[c] & kErrOverflow) {
       set_error("pdf_batch_grade: uint64 path count overflow in circuit " + std::to_string(c));
       return PDF_ERR_OVERFLOW;
     }

@@ -34,6 +34,8 @@ TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_
 TUNE_REV_TILES = 7
 TUNE_WALK = 8
 TUNE_COVER_NODES, TUNE_COVER_POOL = 9, 10
+REF_EXCEEDS_U16, REF_EXCEEDS_U32, REF_SEGMENT_STRIDE, REF_MERGE_WINDOW = 1, 2, 4, 8
+REF_MERGE_PATTERNS = 4224  # the reference's merge window (mergekernel.cuh:125-130)
 DEBUG_FORWARD_MARKS, DEBUG_PI_COVER = 1, 2
 
 OK, ERR_ARG, ERR_NETLIST, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_UNSUPPORTED, ERR_STATE, ERR_OVERFLOW = (
@@ -118,6 +120,9 @@ SIGNATURES = {
     "pdf_grade_async": ([_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grade_noalloc": ([_P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grader_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
+    "pdf_grader_set_merge_window": ([_P, C.c_uint32], C.c_int),
+    "pdf_grader_ref_flags": ([_P, C.POINTER(C.c_uint32), _U64P], C.c_int),
+    "pdf_ref_unmerged_segments": ([_U64], _U64),
     "pdf_get_sim_planes": ([_P, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_get_mark_plane": ([_P, C.c_void_p], C.c_int),
     "pdf_get_first": ([_P, C.c_void_p, C.c_void_p], C.c_int),
@@ -456,6 +461,19 @@ class Grader:
         nothing is allocated."""
         _check(lib().pdf_grade_noalloc(self._h, int(robust), C.c_void_p(stream), C.c_void_p(total_dev),
                                        C.c_void_p(err_dev) if err_dev else None))
+
+    def set_merge_window(self, pairs: int):
+        """Take first detections only from the first `pairs` pairs of each
+        chunk (REF_MERGE_PATTERNS reproduces the reference; 0 = whole chunk)."""
+        _check(lib().pdf_grader_set_merge_window(self._h, pairs))
+
+    def ref_flags(self):
+        """(REF_* flags, segments the reference's merge would leave
+        unmerged) of the last grade."""
+        f = C.c_uint32()
+        un = C.c_uint64()
+        _check(lib().pdf_grader_ref_flags(self._h, C.byref(f), C.byref(un)))
+        return f.value, un.value
 
     def grade_stages(self, robust: bool, segs: int = 1):
         """The reference's chunk loop (main.cu:171-231) through the four stage

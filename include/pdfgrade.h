@@ -263,6 +263,33 @@ int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t 
 #define PDF_NOALLOC_SCRATCH_BYTES (256ull << 20)
 int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_dev, uint32_t *err_dev);
 
+/* The reference's resource limits (results of this library are exact; these
+ * say where the reference's own run would have printed something else).
+ * pdf_grader_set_merge_window: with pairs > 0 the merge takes first
+ * detections only from the first `pairs` pairs of each chunk, as the
+ * reference's kernSegmentReduce2 examines only its first 4,224 patterns
+ * (mergekernel.cuh:125-130; pass 4224 to reproduce it, 0 = whole chunk, the
+ * default).  Applies to the stage calls and to every fused grade.
+ * pdf_grader_ref_flags, after a grade (or a chunk's cover): PDF_REF_EXCEEDS_U16
+ * when a path count the cover DP computed -- a marked node's fanout sums or
+ * its final c / h -- left the reference's 16-bit fields, where its asserts
+ * fire (coverkernel.cuh:41-43, 210-211, 224-225; the reference also asserts
+ * on unmarked nodes' fanout sums, which this library does not compute);
+ * PDF_REF_EXCEEDS_U32 when a chunk's total passed the reference's uint32
+ * partial sums (coverkernel.cuh:79-101); PDF_REF_SEGMENT_STRIDE when the
+ * reference's merge launches would leave segments unmerged (the stride bug
+ * of mergekernel.cuh:243, *unmerged_segments of them, for the segment length
+ * of the last merge); PDF_REF_MERGE_WINDOW when chunks are longer than 4,224
+ * pairs and no merge window is set.  pdf_ref_unmerged_segments gives that
+ * count for S segments. */
+#define PDF_REF_EXCEEDS_U16 1u
+#define PDF_REF_EXCEEDS_U32 2u
+#define PDF_REF_SEGMENT_STRIDE 4u
+#define PDF_REF_MERGE_WINDOW 8u
+int pdf_grader_set_merge_window(pdf_grader *g, uint32_t pairs);
+int pdf_grader_ref_flags(const pdf_grader *g, uint32_t *flags, uint64_t *unmerged_segments);
+uint64_t pdf_ref_unmerged_segments(uint64_t segments);
+
 /* Tuning knobs (no reference counterpart; the reference fixes its shapes
  * with compile-time constants such as SIM_BLOCK, MARK_BLOCK, PARALLEL_SEGS,
  * simkernel.cuh:29-30, markkernel.cuh:7-8, mergekernel.cuh:49-52).

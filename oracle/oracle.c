@@ -212,6 +212,8 @@ int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8
   if (!c || !h) { free(c); free(h); return -5; }
   uint64_t total = 0;
   for (uint64_t p = 0; p < W; ++p) {
+    int any_new = 0; /* a pair with no new node contributes nothing: a sparse grader skips it */
+    for (uint32_t gi = 0; gi < nl->n && !any_new; ++gi) any_new = newm[(uint64_t)gi * W + p] != 0;
     for (uint32_t gi = nl->n; gi-- > 0;) {
       const uint8_t cache = marks[(uint64_t)gi * W + p];
       uint64_t cc = 0, hh = 0;
@@ -232,6 +234,8 @@ int orc_cover_new_chunk(const orc_netlist *nl, const uint8_t *marks, const uint8
       hh = (cache > 0 && !is_new) ? hh : 0;
       if (nl->type[gi] == INPT) hh = 0; /* coverkernel.cuh:221-223 */
       if (cc > 0xFFFF || hh > 0xFFFF) *flags |= ORC_FLAG_EXCEEDS_U16;
+      if (any_new && cache > 0 && (rc > 0xFFFF || hc > 0xFFFF || cc > 0xFFFF || hh > 0xFFFF))
+        *flags |= ORC_FLAG_EXCEEDS_U16_DP;
       c[gi] = cc;
       h[gi] = hh;
       if (gate_counts && __builtin_add_overflow(gate_counts[gi], cc, &gate_counts[gi]))
@@ -409,4 +413,34 @@ int orc_grade(const orc_netlist *nl, const uint64_t *a_bits, const uint64_t *b_b
   free(codes);
   free(marks);
   return 0;
+}
+
+/* gpuMergeSegments (mergekernel.cuh:229-250): remaining_blocks = ceil(S/20);
+ * each pass launches block_y = min(BLOCK_STEP, remaining_blocks) blocks of
+ * PARALLEL_SEGS = 20 segments at startSegment = count, then count +=
+ * BLOCK_STEP (65535, not 65535 * 20) and remaining_blocks -= BLOCK_STEP (or
+ * becomes 0).  Segment sid is examined iff some pass has count <= sid <
+ * count + 20 * block_y.  Counted here segment by segment. */
+uint64_t orc_ref_unmerged_segments(uint64_t S) {
+  uint64_t remaining = S / 20 + (S % 20 > 0), count = 0, missed = 0;
+  uint64_t passes = 0;
+  uint64_t starts[64], ends[64];
+  do {
+    const uint64_t block_y = remaining > 65535 ? 65535 : remaining;
+    if (passes < 64) {
+      starts[passes] = count;
+      ends[passes] = count + 20 * block_y;
+    }
+    ++passes;
+    count += 65535;
+    if (remaining < 65535) remaining = 0;
+    if (remaining >= 65535) remaining -= 65535;
+  } while (remaining > 0);
+  if (passes > 64) return S; /* beyond any netlist this oracle is run on */
+  for (uint64_t sid = 0; sid < S; ++sid) {
+    int hit = 0;
+    for (uint64_t k = 0; k < passes && !hit; ++k) hit = sid >= starts[k] && sid < ends[k];
+    missed += !hit;
+  }
+  return missed;
 }

@@ -22,6 +22,13 @@ typedef struct orc_options {
 #define ORC_FLAG_EXCEEDS_U16 1u   /* a c/h value left the reference's 16 bits */
 #define ORC_FLAG_EXCEEDS_U32 2u   /* a chunk total left the reference's uint32 */
 #define ORC_FLAG_U64_OVERFLOW 4u  /* exact uint64 accumulation overflowed */
+/* the part of ORC_FLAG_EXCEEDS_U16 a sparse grader can see: a fanout sum or
+ * final c/h past 16 bits at a marked node of a pair where some node is new */
+#define ORC_FLAG_EXCEEDS_U16_DP 8u
+
+/* Segments of S that the reference's merge launch loop never reaches
+ * (gpuMergeSegments, mergekernel.cuh:229-250, count += BLOCK_STEP). */
+uint64_t orc_ref_unmerged_segments(uint64_t S);
 
 const uint8_t *orc_table(int which);
 uint16_t orc_mark_lut(int robust, int idx);

@@ -17,6 +17,7 @@ BSL_PATH = os.path.join(HERE, "libbitsliced.so")
 FLAG_EXCEEDS_U16 = 1
 FLAG_EXCEEDS_U32 = 2
 FLAG_U64_OVERFLOW = 4
+FLAG_EXCEEDS_U16_DP = 8
 REFERENCE_MERGE_WINDOW = 4224  # patterns examined per chunk by mergekernel.cuh:125-130
 
 
@@ -225,3 +226,12 @@ def new_from_firsts(n: int, W: int, start: int, first_t0, first_t1) -> np.ndarra
         sel = (arr != 0xFFFFFFFF) & (arr >= start) & (arr < start + W)
         newm[np.nonzero(sel)[0], arr[sel] - start] = 1
     return newm
+
+
+def ref_unmerged_segments(S: int) -> int:
+    """orc_ref_unmerged_segments: segments the reference's merge launch loop
+    never examines (mergekernel.cuh:229-250)."""
+    f = lib().orc_ref_unmerged_segments
+    f.restype = C.c_uint64
+    f.argtypes = [C.c_uint64]
+    return int(f(C.c_uint64(S)))

@@ -56,6 +56,10 @@ def test_2m_gates_all_pairs_against_bitsliced():
         g.set_tuning(pg.TUNE_LANES, GATES_2M.lanes)
         g.set_tuning(pg.TUNE_WORDS_PER_LANE, GATES_2M.words_per_lane)
         tuned = state(g, g.grade(True))
+        # the reference's merge launches reach 1,310,700 of the 2,004,096
+        # nodes (mergekernel.cuh:243)
+        flags, unmerged = g.ref_flags()
+        assert flags & pg.REF_SEGMENT_STRIDE and unmerged == 693_396
     say("2m gates: 32,768-pair blocks on the GPU")
     with pg.Grader(nl) as g:
         g.set_vectors(bits, P)

@@ -30,6 +30,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -4021,6 +4022,16 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   HIPCHK(hipSetDevice(g->device));
   const uint64_t P = g->num_pairs;
   auto round64 = [](uint64_t x) { return (x + 63) / 64 * 64; };
+  // the caller's configuration, restored if tuning fails
+  const uint64_t chunk0 = g->chunk_pairs;
+  const uint32_t lanes0 = g->tune_lanes, graph0 = g->tune_use_graph, wpl0 = g->tune_words_per_lane;
+  auto restore = [&](int rc) {
+    g->tune_lanes = lanes0;
+    g->tune_use_graph = graph0;
+    g->tune_words_per_lane = wpl0;
+    (void)pdf_grader_set_chunk(g, chunk0);
+    return rc;
+  };
   // Candidates: 1..4 lanes, each with the even split of the pairs and the
   // power-of-two chunks from 32,768 pairs below it (from 65,536 on, 2 words
   // per lane applies), kept when the lanes' chunk buffers fit in 85% of the
@@ -4028,11 +4039,20 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   HIPCHK(hipStreamSynchronize(g->stream));
   free_chunk(g);
   size_t free_b = 0, total_b = 0;
-  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  if (!hip_ok(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return restore(PDF_ERR_HIP);
   const double budget = 0.85 * (double)free_b;
-  auto lane_bytes = [&](uint64_t chunk) {  // what alloc_chunk_buffers takes
+  const double n = g->nl.n;
+  auto lane_bytes = [&](uint64_t chunk) {  // what alloc_chunk_buffers takes for one lane
     const uint64_t words = (chunk + 63) / 64, pitch = (words + 1 + 15) / 16 * 16, ntiles = (words + 63) / 64;
-    return (double)g->nl.n * (8.0 * pitch * (g->seq ? 3 : 4) + 8.0 * ntiles + 4.0 * words);
+    const double per_wave = 16.0 * n;  // the cover's c/h fallback scratch
+    const double waves = std::max(1.0, std::min((double)g->max_cover_waves, std::floor((double)(1ull << 30) / per_wave)));
+    double bytes = n * (8.0 * pitch * (g->seq ? 3 : 4) + 8.0 * ntiles + 4.0 * words) + 4.0 * chunk + 12.0 * words +
+                   waves * per_wave;
+    if (g->tune_walk) {
+      const double tw = g->tune_walk, nv = (double)(((g->seq ? g->tune_walk + 1 : 2 * g->tune_walk) + 2) & ~1u);
+      bytes += n * std::ceil(words / tw) * nv * 8.0;
+    }
+    return bytes;
   };
   struct Cand { uint32_t lanes; uint64_t chunk; };
   std::vector<Cand> cands;
@@ -4049,7 +4069,7 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   }
   if (cands.empty()) {
     set_error("pdf_grader_autotune: no chunk configuration fits in free HBM");
-    return PDF_ERR_STATE;
+    return restore(PDF_ERR_STATE);
   }
   uint64_t ref = 0;
   bool have_ref = false;
@@ -4057,10 +4077,10 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   bt.ms = 1e30f;
   int rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIPCHK(hipEventCreate(&e0));
+  if (!hip_ok(hipEventCreate(&e0), "hipEventCreate")) return restore(PDF_ERR_HIP);
   if (!hip_ok(hipEventCreate(&e1), "hipEventCreate")) {
     (void)hipEventDestroy(e0);
-    return PDF_ERR_HIP;
+    return restore(PDF_ERR_HIP);
   }
   auto run = [&]() -> int {
     for (const Cand &c : cands)
@@ -4070,7 +4090,9 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
           g->tune_lanes = c.lanes;
           g->tune_use_graph = graph;
           g->tune_words_per_lane = wpl;
-          if ((rc = pdf_grader_set_chunk(g, c.chunk))) return rc;
+          rc = pdf_grader_set_chunk(g, c.chunk);
+          if (rc == PDF_ERR_NOMEM) continue;  // the estimate was short: skip this candidate
+          if (rc) return rc;
           std::vector<float> t(reps + 1);
           uint64_t tot = 0;
           for (uint32_t r = 0; r <= reps; ++r) {  // the first run warms up (and captures)
@@ -4091,16 +4113,20 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
           const float med = t[1 + reps / 2];
           if (med < bt.ms) bt = pdf_grader_tuning{c.lanes, c.chunk, graph, wpl, med};
         }
+    if (!have_ref) {
+      set_error("pdf_grader_autotune: no candidate configuration could be allocated");
+      return PDF_ERR_NOMEM;
+    }
     return PDF_OK;
   };
   rc = run();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (rc) return rc;
+  if (rc) return restore(rc);
   g->tune_lanes = bt.lanes;
   g->tune_use_graph = bt.use_graph;
   g->tune_words_per_lane = bt.words_per_lane;
-  if ((rc = pdf_grader_set_chunk(g, bt.chunk_pairs))) return rc;
+  if ((rc = pdf_grader_set_chunk(g, bt.chunk_pairs))) return restore(rc);
   if (best) *best = bt;
   return PDF_OK;
 }

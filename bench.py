@@ -257,18 +257,28 @@ def learned_vs_plain(robust: bool):
         with open(out) as f:
             res = json.load(f)
     s95, g95 = res["summary"]["paths_95"], res["summary"]["segments_95"]
-    per = [c["learned"] for c in res["circuits"]]
+    live = [c for c in res["circuits"] if c["full_paths"] > 0]
+    per = [c["learned"] for c in live] or [c["learned"] for c in res["circuits"]]
+
+    def mean(xs):
+        xs = [x for x in xs if x is not None]
+        return sum(xs) / len(xs) if xs else None
+
     return {
         "mode": "robust" if robust else "non-robust",
-        "time_to_95pct_paths_learned_over_plain_full": s95["learned_over_plain_full"],
-        "time_to_95pct_paths_learned_over_plain_same": s95["learned_over_plain_same_fraction"],
+        # the metric: learned time to 95% of the paths over plain in-order
+        # time to 95%, both graded in the same chunks (circuits with paths)
+        "time_to_95pct_paths_learned_over_plain": s95["learned_over_plain_same_fraction"],
+        "time_to_95pct_paths_learned_over_plain_full_grading": s95["learned_over_plain_full"],
+        "time_to_95pct_paths_ms": {"learned": mean([c["learned"]["t95_paths_ms"] for c in live]),
+                                   "plain": mean([c["plain_curve"]["t95_paths_ms"] for c in live])},
         "pairs_to_95pct_paths": {"learned": s95["pairs_learned_mean"], "plain": s95["pairs_plain_mean"]},
         "pairs_to_95pct_segments": {"learned": g95["pairs_learned_mean"], "plain": g95["pairs_plain_mean"]},
-        "overhead_ms": {k: sum(p[k] for p in per) / len(per)
-                        for k in ("features_score_ms", "rank_ms", "features_ms", "score_ms")},
-        "plain_full_ms": sum(c["plain_ms"] for c in res["circuits"]) / len(res["circuits"]),
-        "heldout_circuits": len(res["circuits"]), "candidates": res["config"]["candidates"],
-        "records": res["records"],
+        "learned_stage_ms": {k: mean([p[k] for p in per])
+                             for k in ("features_score_ms", "rank_ms", "features_ms", "score_ms")},
+        "plain_full_ms": mean([c["plain_ms"] for c in res["circuits"]]),
+        "heldout_circuits": len(res["circuits"]), "circuits_without_paths": s95["circuits_without_any"],
+        "candidates": res["config"]["candidates"], "records": res["records"],
     }
 
 

@@ -138,6 +138,11 @@ __device__ __forceinline__ void pin(uint32_t x) { asm volatile("" : : "v"(x)); }
 // cannot be moved up between loads issued before it.
 __device__ __forceinline__ void hold(uint64_t &x) { asm volatile("" : "+v"(x)); }
 
+// LDS words typed as LDS (address space 3), so every access through them is
+// an LDS instruction with no generic-address check
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 // OR x into a removal accumulator word: a device-scope atomic, or for the
 // level walk (whose accumulator words each belong to one workgroup) a
 // workgroup-scope one.
@@ -513,7 +518,8 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
                                            uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
                                            uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
                                            uint32_t *__restrict__ list_cnt, uint64_t list_stride,
-                                           const uint64_t *fa_src = nullptr, const uint64_t *fb_src = nullptr);
+                                           const uint64_t *fa_src = nullptr, const uint64_t *fb_src = nullptr,
+                                           lds_u32 *walk_cnt = nullptr, uint32_t w0 = 0);
 
 // One (gate, word) item of the reverse pass.
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
@@ -539,14 +545,25 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
 // in the same launch, so the pushes are workgroup-scope atomics and the
 // barrier between levels orders them before the read; the node's frame-1
 // word and frame 2's source word are read from fa_src and fb_src (the walk's
-// tile-major value plane) instead of V / VB.
+// tile-major value plane) instead of V / VB; and the word's list length is
+// counted in LDS (walk_cnt[w - w0], written out by the caller at the end).
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WALK>
 __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
                                            uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
                                            uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
                                            uint32_t *__restrict__ list_cnt, uint64_t list_stride,
-                                           const uint64_t *fa_src, const uint64_t *fb_src) {
+                                           const uint64_t *fa_src, const uint64_t *fb_src, lds_u32 *walk_cnt,
+                                           uint32_t w0) {
   const uint64_t row = (uint64_t)g * pl.pitch + w;
+  if (WALK && DO_REMOVE && !r.po) {
+    // most forward marks die in the removal (no marked fanout pushed into
+    // them): the accumulator alone decides, and a dying mark only needs its
+    // word cleared, with none of the loads below
+    if (!pl.ACC[row]) {
+      pl.M[row] = 0;
+      return;
+    }
+  }
   // second trip, all loads issued before any is used: this node's mark and
   // accumulator; for the merge, its frames as raw words (frame 1 and frame
   // 2's source) and its first-detection pairs so far; for the removal, the
@@ -623,7 +640,9 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
     const uint32_t c1 = t1 ? base + (uint32_t)__builtin_ctzll(t1) : PDF_NONE;
     if (c0 < cur0) atomicMin(&first_t0[g], c0);
     if (c1 < cur1) atomicMin(&first_t1[g], c1);
-    const uint32_t pos = atomicAdd(&list_cnt[w], 1u);
+    const uint32_t pos = WALK ? __hip_atomic_fetch_add(&walk_cnt[w - w0], 1u, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP)
+                              : atomicAdd(&list_cnt[w], 1u);
     list[(uint64_t)w * list_stride + pos] = g;
   }
 }
@@ -720,9 +739,6 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
 // 8 XCDs round-robin, so neighbouring tiles share an XCD and its L2.
 constexpr int kWalkBlock = 1024;
 constexpr uint32_t kWalkLdsBytes = 128 * 1024;
-// the LDS window's words, typed as LDS (address space 3) so every access is
-// an LDS instruction with no generic-address check
-typedef __attribute__((address_space(3))) uint64_t lds_u64;
 // records move as 16-byte words (ds_read_b128 / global_load_dwordx4)
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) u64x2 lds_u64x2;
@@ -1005,25 +1021,47 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateR
                                                           uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
                                                           uint64_t list_stride) {
   using S = WalkShape<SEQ, TW>;
+  __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
+  lds_u32 *cnt = (lds_u32 *)cnt_words;
   uint32_t tile, w0, nw;
   if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
+  if (threadIdx.x < TW) cnt[threadIdx.x] = 0;
   const uint64_t *vt = VT + (uint64_t)tile * net.n * S::NV;
+  auto bits_of = [&](uint32_t g) { return vt[(uint64_t)g * S::NV + S::NF]; };  // mark-present bits of the tile
+  // each lane's first gate of a level: its mark-present bits are loaded one
+  // level ahead, and its netlist record too when a bit is set (neither
+  // changes in this pass)
+  uint64_t bn = 0;
+  GateRec rn{};
+  {
+    const uint32_t g = level_off[levels - 1] + threadIdx.x;
+    if (g < level_off[levels]) bn = bits_of(g);
+    if (bn) rn = recs[g];
+  }
+  __syncthreads();
   for (uint32_t k = levels; k-- > 0;) {
     const uint32_t lo = level_off[k], hi = level_off[k + 1];
+    const uint64_t b0 = bn;
+    const GateRec r0 = rn;
+    const uint32_t gn = k > 0 ? level_off[k - 1] + threadIdx.x : 0;
+    bn = k > 0 && gn < lo ? bits_of(gn) : 0;
     for (uint32_t g = lo + threadIdx.x; g < hi; g += kWalkBlock) {
-      uint64_t nzw = vt[(uint64_t)g * S::NV + S::NF];  // the node's mark-present bits of the tile
+      const bool first = g == lo + threadIdx.x;
+      uint64_t nzw = first ? b0 : bits_of(g);
       if (!nzw) continue;
-      const GateRec r = recs[g];
+      const GateRec r = first ? r0 : recs[g];
       while (nzw) {
         const uint32_t w = w0 + (uint32_t)__builtin_ctzll(nzw);
         nzw &= nzw - 1;
         const uint64_t *fa = vt + (uint64_t)g * S::NV + (w - w0);
         rev_marked<SEQ, true, true, true>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride,
-                                          fa, SEQ ? fa + 1 : fa + TW);
+                                          fa, SEQ ? fa + 1 : fa + TW, cnt, w0);
       }
     }
+    if (bn) rn = recs[gn];
     __syncthreads();  // pushes into lower levels' accumulators are visible
   }
+  if (threadIdx.x < nw) list_cnt[w0 + threadIdx.x] = cnt[threadIdx.x];
 }
 
 // Cross-chunk state to its initial value (one launch per grade).

@@ -222,6 +222,22 @@ __device__ __forceinline__ uint64_t side_ok(const uint64_t (&fa)[kRegFanin], con
   return r;
 }
 
+// side_ok for a gate whose AND/OR class differs between lanes: complementing
+// every frame turns each OR-like predicate into the AND-like one (T0 <-> T1,
+// S0 <-> S1 map LUT_R / LUT_NR's OR/NOR rows onto their AND/NAND rows,
+// markkernel.cuh:36-37), so one branch-free evaluation serves both.
+template <bool ROBUST>
+__device__ __forceinline__ uint64_t side_ok_any(const uint64_t (&fa)[kRegFanin], const uint64_t (&fb)[kRegFanin],
+                                                uint32_t nfi, uint64_t orlike_mask) {
+  uint64_t ca[kRegFanin], cb[kRegFanin];
+#pragma unroll
+  for (int j = 0; j < kRegFanin; ++j) {
+    ca[j] = fa[j] ^ orlike_mask;
+    cb[j] = fb[j] ^ orlike_mask;
+  }
+  return side_ok<ROBUST, true>(ca, cb, nfi);
+}
+
 // Same predicate for gates with more fanins: operands re-read from the planes.
 template <bool SEQ, bool ROBUST, bool ANDLIKE>
 __device__ uint64_t side_ok_mem(const Planes &pl, const uint32_t *fins, uint32_t nfi, uint32_t w) {
@@ -519,7 +535,8 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
                                            uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
                                            uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                            const uint64_t *fa_src = nullptr, const uint64_t *fb_src = nullptr,
-                                           lds_u32 *walk_cnt = nullptr, uint32_t w0 = 0);
+                                           lds_u32 *walk_cnt = nullptr, uint32_t w0 = 0,
+                                           const uint64_t *bits_src = nullptr, uint32_t bits_stride = 0);
 
 // One (gate, word) item of the reverse pass.
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
@@ -545,16 +562,24 @@ __device__ __forceinline__ void rev_item(const DevNet &net, const GateRec *__res
 // in the same launch, so the pushes are workgroup-scope atomics and the
 // barrier between levels orders them before the read; the node's frame-1
 // word and frame 2's source word are read from fa_src and fb_src (the walk's
-// tile-major value plane) instead of V / VB; and the word's list length is
-// counted in LDS (walk_cnt[w - w0], written out by the caller at the end).
+// tile-major value plane) instead of V / VB; the word's list length is
+// counted in LDS (walk_cnt[w - w0], written out by the caller at the end);
+// and with bits_src a fanin f's mark-present bits of the tile are
+// bits_src[f * bits_stride] (the walk's own records, bit u: word w0 + u)
+// instead of its NZ word, which other workgroups' forward walks update with
+// atomics in the same launch.
 template <bool SEQ, bool DO_REMOVE, bool DO_MERGE, bool WALK>
 __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
                                            uint32_t w, uint32_t start, uint32_t *__restrict__ first_t0,
                                            uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
                                            uint32_t *__restrict__ list_cnt, uint64_t list_stride,
                                            const uint64_t *fa_src, const uint64_t *fb_src, lds_u32 *walk_cnt,
-                                           uint32_t w0) {
+                                           uint32_t w0, const uint64_t *bits_src, uint32_t bits_stride) {
   const uint64_t row = (uint64_t)g * pl.pitch + w;
+  auto nz_word = [&](uint32_t f) -> uint64_t {  // bit (w & 63): f has a forward mark at word w
+    if (WALK && bits_src) return bits_src[(uint64_t)f * bits_stride] << (w0 & 63);
+    return pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)];
+  };
   if (WALK && DO_REMOVE && !r.po) {
     // most forward marks die in the removal (no marked fanout pushed into
     // them): the accumulator alone decides, and a dying mark only needs its
@@ -586,7 +611,7 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
     for (int j = 0; j < kRegFanin; ++j) {
       nzf[j] = mf[j] = 0;
       if ((uint32_t)j < r.nfi) {
-        nzf[j] = pl.NZ[(uint64_t)r.fin[j] * pl.ntiles + (w >> 6)];
+        nzf[j] = nz_word(r.fin[j]);
         mf[j] = pl.M[(uint64_t)r.fin[j] * pl.pitch + w];
       }
     }
@@ -623,7 +648,7 @@ __device__ __forceinline__ void rev_marked(const DevNet &net, const GateRec &r, 
         if (mf[j] & m) push_acc<WALK>(&pl.ACC[(uint64_t)r.fin[j] * pl.pitch + w], mf[j] & m);
       for (uint32_t j = kRegFanin; j < r.nfi; ++j) {
         const uint32_t f = net.fin[r.fin_off + j];
-        if (!((pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)] >> (w & 63)) & 1)) continue;
+        if (!((nz_word(f) >> (w & 63)) & 1)) continue;
         const uint64_t x = pl.M[(uint64_t)f * pl.pitch + w] & m;
         if (x) push_acc<WALK>(&pl.ACC[(uint64_t)f * pl.pitch + w], x);
       }
@@ -910,7 +935,7 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
             fa[j] = x[j][u];
             fb[j] = frame2<SEQ, TW>(x[j], u);
           }
-          rr = andlike ? side_ok<ROBUST, true>(fa, fb, nfi) : side_ok<ROBUST, false>(fa, fb, nfi);
+          rr = side_ok_any<ROBUST>(fa, fb, nfi, andlike ? 0ULL : ~0ULL);
         } else {
           // more fanins than registers: each off-path pair re-read from LDS / VT
           rr = 0;
@@ -970,21 +995,14 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
 
 // Forward walk: levels 0..levels-1 of one tile, sim + forward mark.  VT is
 // the tile-major value plane ([tiles][n][NV] words).  The NZ plane must be
-// all zero (a fill kernel runs first); every block also clears its share of
-// the per-chunk counters (clear32) for the merge and cover.
+// all zero (a fill kernel runs first).
 template <bool SEQ, bool ROBUST, int TW>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_fwd(DevNet net, const GateRec *__restrict__ recs, Planes pl,
-                                                          const uint64_t *__restrict__ va,
-                                                          const uint64_t *__restrict__ vb, uint64_t vstride,
-                                                          uint64_t wbase, uint64_t *__restrict__ VT,
-                                                          const uint32_t *__restrict__ level_off, uint32_t levels,
-                                                          uint32_t tiles, uint32_t *__restrict__ clear32,
-                                                          uint32_t clear32_n) {
+__device__ __forceinline__ void walk_fwd_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
+                                              const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
+                                              uint64_t vstride, uint64_t wbase, uint64_t *__restrict__ VT,
+                                              const uint32_t *__restrict__ level_off, uint32_t levels, uint32_t tiles,
+                                              lds_u64 *lds) {
   using S = WalkShape<SEQ, TW>;
-  __shared__ uint64_t lds_words[S::cap * S::NV];
-  lds_u64 *lds = (lds_u64 *)lds_words;
-  for (uint32_t i = blockIdx.x * kWalkBlock + threadIdx.x; i < clear32_n; i += gridDim.x * kWalkBlock)
-    clear32[i] = 0;
   uint32_t tile, w0, nw;
   if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
   uint64_t *vt = VT + (uint64_t)tile * net.n * S::NV;
@@ -1011,23 +1029,25 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_fwd(DevNet net, const GateR
 // the tile, frames from VT).  A node's fanouts are on higher levels, so they
 // have pushed into its accumulator before its level runs; its fanins are on
 // lower levels, processed after the barrier.
-template <bool SEQ, int TW>
-__global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateRec *__restrict__ recs, Planes pl,
-                                                          const uint64_t *__restrict__ VT,
-                                                          const uint32_t *__restrict__ level_off, uint32_t levels,
-                                                          uint32_t tiles, uint32_t start,
-                                                          uint32_t *__restrict__ first_t0,
-                                                          uint32_t *__restrict__ first_t1,
-                                                          uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
-                                                          uint64_t list_stride) {
+// NM: after per-level forward launches instead of the forward walk, so the
+// mark-present bits come from NZ and the frames from V / VB (node-major).
+template <bool SEQ, int TW, bool NM>
+__device__ __forceinline__ void walk_rev_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
+                                              const uint64_t *__restrict__ VT, const uint32_t *__restrict__ level_off,
+                                              uint32_t levels, uint32_t tiles, uint32_t start,
+                                              uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
+                                              uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                              uint64_t list_stride, lds_u32 *cnt) {
   using S = WalkShape<SEQ, TW>;
-  __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
-  lds_u32 *cnt = (lds_u32 *)cnt_words;
   uint32_t tile, w0, nw;
   if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
   if (threadIdx.x < TW) cnt[threadIdx.x] = 0;
-  const uint64_t *vt = VT + (uint64_t)tile * net.n * S::NV;
-  auto bits_of = [&](uint32_t g) { return vt[(uint64_t)g * S::NV + S::NF]; };  // mark-present bits of the tile
+  const uint64_t *vt = NM ? nullptr : VT + (uint64_t)tile * net.n * S::NV;
+  const uint64_t tmask = (1ULL << nw) - 1;  // nw <= 8
+  auto bits_of = [&](uint32_t g) -> uint64_t {  // mark-present bits of the tile, bit u: word w0 + u
+    if (NM) return (pl.NZ[(uint64_t)g * pl.ntiles + (w0 >> 6)] >> (w0 & 63)) & tmask;
+    return vt[(uint64_t)g * S::NV + S::NF];
+  };
   // each lane's first gate of a level: its mark-present bits are loaded one
   // level ahead, and its netlist record too when a bit is set (neither
   // changes in this pass)
@@ -1053,15 +1073,69 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateR
       while (nzw) {
         const uint32_t w = w0 + (uint32_t)__builtin_ctzll(nzw);
         nzw &= nzw - 1;
-        const uint64_t *fa = vt + (uint64_t)g * S::NV + (w - w0);
+        const uint64_t *fa = NM ? pl.V + (uint64_t)g * pl.pitch + w : vt + (uint64_t)g * S::NV + (w - w0);
+        const uint64_t *fb = SEQ ? fa + 1 : NM ? pl.VB + (uint64_t)g * pl.pitch + w : fa + TW;
         rev_marked<SEQ, true, true, true>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride,
-                                          fa, SEQ ? fa + 1 : fa + TW, cnt, w0);
+                                          fa, fb, cnt, w0, NM ? nullptr : vt + S::NF, S::NV);
       }
     }
     if (bn) rn = recs[gn];
     __syncthreads();  // pushes into lower levels' accumulators are visible
   }
   if (threadIdx.x < nw) list_cnt[w0 + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// The reverse walk alone (after per-level forward launches: NM).
+template <bool SEQ, int TW, bool NM>
+__global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                          const uint64_t *__restrict__ VT,
+                                                          const uint32_t *__restrict__ level_off, uint32_t levels,
+                                                          uint32_t tiles, uint32_t start,
+                                                          uint32_t *__restrict__ first_t0,
+                                                          uint32_t *__restrict__ first_t1,
+                                                          uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                          uint64_t list_stride) {
+  __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
+  walk_rev_body<SEQ, TW, NM>(net, recs, pl, VT, level_off, levels, tiles, start, first_t0, first_t1, list, list_cnt,
+                             list_stride, (lds_u32 *)cnt_words);
+}
+
+// Forward and reverse walk of a tile in one launch: a tile's reverse pass
+// needs only its own forward pass, so no workgroup waits for another.
+template <bool SEQ, bool ROBUST, int TW>
+__global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                      const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
+                                                      uint64_t vstride, uint64_t wbase, uint64_t *__restrict__ VT,
+                                                      const uint32_t *__restrict__ level_off, uint32_t levels,
+                                                      uint32_t tiles, uint64_t *__restrict__ actmap,
+                                                      uint32_t *__restrict__ act_count,
+                                                      uint64_t *__restrict__ chunk_total, uint32_t start,
+                                                      uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
+                                                      uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                      uint64_t list_stride) {
+  using S = WalkShape<SEQ, TW>;
+  __shared__ uint64_t lds_words[S::cap * S::NV];
+  __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
+  {
+    // the chunk's event and cover counters, each cleared by the block that
+    // owns it (blocks are not ordered within the launch, so no block may
+    // touch another tile's words): the pair bitmap's words of this tile,
+    // and the pair count and chunk total by the block of tile 0; the list
+    // lengths are written by their tiles at the end
+    uint32_t t, w0, nw;
+    if (walk_tile<TW>(pl, tiles, t, w0, nw)) {
+      if (threadIdx.x < nw) actmap[w0 + threadIdx.x] = 0;
+      if (t == 0 && threadIdx.x == 0) {
+        *act_count = 0;
+        *chunk_total = 0;
+      }
+    }
+  }
+  walk_fwd_body<SEQ, ROBUST, TW>(net, recs, pl, va, vb, vstride, wbase, VT, level_off, levels, tiles,
+                                 (lds_u64 *)lds_words);
+  __syncthreads();  // the forward walk's records are visible to the reverse walk
+  walk_rev_body<SEQ, TW, false>(net, recs, pl, VT, level_off, levels, tiles, start, first_t0, first_t1, list,
+                                list_cnt, list_stride, (lds_u32 *)cnt_words);
 }
 
 // Cross-chunk state to its initial value (one launch per grade).
@@ -2070,6 +2144,7 @@ struct pdf_grader {
   uint32_t tune_use_graph = 1;
   uint32_t tune_walk = 0;  // PDF_TUNE_WALK: 0 level launches, else words per tile of the level walk
   uint32_t merge_window = 0;  // pdf_grader_set_merge_window
+  uint32_t tune_rev_walk = 0;  // PDF_TUNE_REV_WALK: words per tile of a reverse walk after per-level forward launches
   // chunk lanes (PDF_TUNE_LANES): the fused grade runs consecutive chunks on
   // `lanes` streams, each lane with its own chunk buffers; lane 0 is the set
   // of fields above, lanes 1.. are kept here
@@ -2321,19 +2396,50 @@ int launch_reverse(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_lo 
   return PDF_OK;
 }
 
+template <bool SEQ, int TW, bool NM>
+void launch_rev_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tiles) {
+  const uint32_t grid = 8 * ((tiles + 7) / 8);  // walk_tile skips the blocks past the last tile
+  tm_begin(g, kRev);
+  k_walk_rev<SEQ, TW, NM><<<grid, kWalkBlock, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_VT, g->d_level_off,
+                                                              g->nl.L + 1, tiles, (uint32_t)start, g->d_first0,
+                                                              g->d_first1, g->d_list, g->d_list_cnt, g->nl.n);
+  tm_end(g);
+}
+
+// The reverse pass alone as a level walk (tile width tw) over node-major
+// planes, after per-level forward launches.
+int launch_rev_walk(pdf_grader *g, uint64_t start, uint64_t width, uint32_t tw) {
+  const Planes pl = planes(g, width);
+  const uint32_t tiles = (pl.words + tw - 1) / tw;
+  if (g->seq) {
+    switch (tw) {
+      case 1: launch_rev_walk_t<true, 1, true>(g, start, pl, tiles); break;
+      case 2: launch_rev_walk_t<true, 2, true>(g, start, pl, tiles); break;
+      case 4: launch_rev_walk_t<true, 4, true>(g, start, pl, tiles); break;
+      default: launch_rev_walk_t<true, 8, true>(g, start, pl, tiles); break;
+    }
+  } else {
+    switch (tw) {
+      case 1: launch_rev_walk_t<false, 1, true>(g, start, pl, tiles); break;
+      case 2: launch_rev_walk_t<false, 2, true>(g, start, pl, tiles); break;
+      case 4: launch_rev_walk_t<false, 4, true>(g, start, pl, tiles); break;
+      default: launch_rev_walk_t<false, 8, true>(g, start, pl, tiles); break;
+    }
+  }
+  HIPCHK(hipGetLastError());
+  return PDF_OK;
+}
+
 template <bool SEQ, bool R, int TW>
 void launch_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tiles) {
   const DevNet net = g->net();
   const uint32_t grid = 8 * ((tiles + 7) / 8);  // walk_tile skips the blocks past the last tile
   tm_begin(g, kFwd);
-  k_walk_fwd<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride,
-                                                              start / 64, g->d_VT, g->d_level_off, g->nl.L + 1,
-                                                              tiles, g->d_cstate, g->n_cstate);
-  tm_end(g);
-  tm_begin(g, kRev);
-  k_walk_rev<SEQ, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_VT, g->d_level_off, g->nl.L + 1,
-                                                          tiles, (uint32_t)start, g->d_first0, g->d_first1,
-                                                          g->d_list, g->d_list_cnt, g->nl.n);
+  k_walk<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64,
+                                                          g->d_VT, g->d_level_off, g->nl.L + 1, tiles, g->d_actmap,
+                                                          g->d_act_count, g->d_chunk_total, (uint32_t)start,
+                                                          g->d_first0, g->d_first1, g->d_list, g->d_list_cnt,
+                                                          g->nl.n);
   tm_end(g);
 }
 
@@ -2379,7 +2485,8 @@ int launch_passes(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
   g->walk_planes = 0;
   int rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true);
   if (!rc && !g->seq) rc = reset_chunk(g);
-  if (!rc) rc = launch_reverse<true, true>(g, start, width);
+  if (!rc) rc = g->tune_rev_walk ? launch_rev_walk(g, start, width, g->tune_rev_walk)
+                                 : launch_reverse<true, true>(g, start, width);
   return rc;
 }
 
@@ -3085,11 +3192,18 @@ static int queue_grade_lanes(pdf_grader *g, int robust) {
           chunk(c, start, width);
           rc = launch_fwd<true, true>(g, start, width, robust, /*clear_state=*/true, 0xFFFFFFFFu, k, k);
         }
-      for (uint32_t k = g->nl.L + 1; k-- > 0 && !rc;)
+      if (g->tune_rev_walk) {
         for (uint64_t c = c0; c < c1 && !rc; ++c) {
           chunk(c, start, width);
-          rc = launch_reverse<true, true>(g, start, width, k, k);
+          rc = launch_rev_walk(g, start, width, g->tune_rev_walk);
         }
+      } else {
+        for (uint32_t k = g->nl.L + 1; k-- > 0 && !rc;)
+          for (uint64_t c = c0; c < c1 && !rc; ++c) {
+            chunk(c, start, width);
+            rc = launch_reverse<true, true>(g, start, width, k, k);
+          }
+      }
     }
     if (K >= 2)
       for (uint64_t c = c0; c < c1 && !rc; ++c) {
@@ -3331,6 +3445,13 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
       break;
     case PDF_TUNE_USE_GRAPH:
       g->tune_use_graph = value != 0;
+      break;
+    case PDF_TUNE_REV_WALK:
+      if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) {
+        set_error("words per reverse-walk tile must be 0 (per-level launches), 1, 2, 4 or 8");
+        return PDF_ERR_ARG;
+      }
+      g->tune_rev_walk = (uint32_t)value;
       break;
     case PDF_TUNE_COVER_NODES:
       if (value < 1 || value > (int64_t)kCoverCap) { set_error("cover LDS nodes must be 1..512"); return PDF_ERR_ARG; }

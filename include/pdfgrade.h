@@ -304,6 +304,9 @@ uint64_t pdf_ref_unmerged_segments(uint64_t segments);
 #define PDF_TUNE_WALK 8           /* grader: 0 per-level launches (the reference's host level loops); 1/2/4/8 fused
                                    * grades run each chunk's forward and reverse passes as one launch each, a
                                    * workgroup per tile of that many 64-pair words walking every level */
+#define PDF_TUNE_REV_WALK 11      /* grader: 0 per-level reverse launches; 1/2/4/8 after per-level forward launches
+                                   * (PDF_TUNE_WALK 0) the reverse pass of a chunk is one level walk with tiles
+                                   * of that many words */
 #define PDF_TUNE_COVER_NODES 9    /* grader: marked nodes of a pair whose cover DP stays in LDS, 1..512 (512);
                                    * pairs with more keep c/h in global scratch */
 #define PDF_TUNE_COVER_POOL 10    /* grader: fanout ids of those nodes kept in LDS, 0..4096 (4096); the rest are

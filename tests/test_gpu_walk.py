@@ -1,7 +1,8 @@
 """The level walk (PDF_TUNE_WALK: one forward and one reverse launch per chunk,
-a workgroup per tile of 1/2/4/8 words walking every level) against the CPU
-oracle on small circuits and the bit-sliced CPU grader at full size; integer
-work, so every output must be equal."""
+a workgroup per tile of 1/2/4/8 words walking every level) and the reverse
+walk alone after per-level forward launches (PDF_TUNE_REV_WALK) against the
+CPU oracle on small circuits and the bit-sliced CPU grader at full size;
+integer work, so every output must be equal."""
 import sys
 
 import numpy as np
@@ -15,6 +16,8 @@ from helpers import biased_vectors, bitsliced_grade, edge_netlist, oracle_grade,
 pytestmark = pytest.mark.gpu
 
 WALKS = (1, 2, 4, 8)
+# (forward and reverse walk, reverse walk only): every tile width either way
+MODES = [(tw, 0) for tw in WALKS] + [(0, tw) for tw in WALKS]
 
 
 def say(*a):
@@ -22,7 +25,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-def walk_grade(nl, bits, P, chunk, robust, tw, b_bits=None, lanes=1, graph=1, reps=1):
+def walk_grade(nl, bits, P, chunk, robust, tw, b_bits=None, lanes=1, graph=1, reps=1, rev=0):
     with pg.Grader(nl) as g:
         if b_bits is None:
             g.set_vectors(bits, P)
@@ -30,6 +33,7 @@ def walk_grade(nl, bits, P, chunk, robust, tw, b_bits=None, lanes=1, graph=1, re
             g.set_pairs(bits, b_bits, P)
         g.set_chunk(chunk)
         g.set_tuning(pg.TUNE_WALK, tw)
+        g.set_tuning(pg.TUNE_REV_WALK, rev)
         g.set_tuning(pg.TUNE_LANES, lanes)
         g.set_tuning(pg.TUNE_USE_GRAPH, graph)
         out = []
@@ -47,20 +51,20 @@ def assert_same(got, ref, ctx=""):
     assert np.array_equal(got["gate_counts"], ref["gate_counts"]), ctx
 
 
-@pytest.mark.parametrize("tw", WALKS)
-def test_walk_tiny_workload(tw):
+@pytest.mark.parametrize("tw,rev", MODES)
+def test_walk_tiny_workload(tw, rev):
     nl = pg.Netlist.generate(TINY.netlist)
     bits = pg.gen_vectors(TINY.vector_seed, nl.info()["num_pi"], TINY.pairs)
     for robust in (True, False):
         for chunk in (64, 320, 1024):
             say("walk tiny", tw, robust, chunk)
             ref = oracle_grade(nl, bits, TINY.pairs, chunk, robust)
-            for got in walk_grade(nl, bits, TINY.pairs, chunk, robust, tw, reps=2):
-                assert_same(got, ref, (tw, robust, chunk))
+            for got in walk_grade(nl, bits, TINY.pairs, chunk, robust, tw, reps=2, rev=rev):
+                assert_same(got, ref, (tw, rev, robust, chunk))
 
 
-@pytest.mark.parametrize("tw", WALKS)
-def test_walk_random_circuits(tw):
+@pytest.mark.parametrize("tw,rev", MODES)
+def test_walk_random_circuits(tw, rev):
     for seed in range(10):
         cfg = small_config(100 + seed, num_pi=6 + seed % 5, num_gates=40 + 9 * seed, levels=4 + seed % 7,
                            wide=seed % 2 == 0, alpha=2.2 if seed % 3 == 0 else 0.0)
@@ -71,11 +75,11 @@ def test_walk_random_circuits(tw):
             say("walk random", tw, seed, robust)
             chunk = 64 * (1 + seed % 3)
             ref = oracle_grade(nl, bits, P, chunk, robust)
-            assert_same(walk_grade(nl, bits, P, chunk, robust, tw)[0], ref, (tw, seed, robust))
+            assert_same(walk_grade(nl, bits, P, chunk, robust, tw, rev=rev)[0], ref, (tw, rev, seed, robust))
 
 
-@pytest.mark.parametrize("tw", WALKS)
-def test_walk_edge_netlist_and_explicit_pairs(tw):
+@pytest.mark.parametrize("tw,rev", MODES)
+def test_walk_edge_netlist_and_explicit_pairs(tw, rev):
     nl = edge_netlist()
     P = 704  # 11 words: a partial last tile for every tw > 1
     bits = biased_vectors(9, 8, P, p_hold=0.5)
@@ -84,13 +88,14 @@ def test_walk_edge_netlist_and_explicit_pairs(tw):
     for robust in (True, False):
         say("walk edge", tw, robust)
         ref = oracle_grade(nl, bits, P, 448, robust)
-        assert_same(walk_grade(nl, bits, P, 448, robust, tw)[0], ref, ("edge", tw, robust))
+        assert_same(walk_grade(nl, bits, P, 448, robust, tw, rev=rev)[0], ref, ("edge", tw, rev, robust))
         ref = oracle_grade(nl, bits, P, 448, robust, b_bits=b)
-        assert_same(walk_grade(nl, bits, P, 448, robust, tw, b_bits=b)[0], ref, ("explicit", tw, robust))
+        assert_same(walk_grade(nl, bits, P, 448, robust, tw, b_bits=b, rev=rev)[0], ref,
+                    ("explicit", tw, rev, robust))
 
 
-@pytest.mark.parametrize("tw", (2, 4))
-def test_walk_lanes_and_graphs(tw):
+@pytest.mark.parametrize("tw,rev", [(2, 0), (4, 0), (0, 2), (0, 4)])
+def test_walk_lanes_and_graphs(tw, rev):
     """Several chunks on 2 lanes, with and without graph replay, repeated."""
     nl = pg.Netlist.generate(small_config(7, num_pi=16, num_po=24, num_gates=900, levels=14, wide=True,
                                           alpha=2.2))
@@ -100,8 +105,8 @@ def test_walk_lanes_and_graphs(tw):
         ref = oracle_grade(nl, bits, P, 2048, robust)
         for graph in (0, 1):
             say("walk lanes", tw, robust, graph)
-            for got in walk_grade(nl, bits, P, 2048, robust, tw, lanes=2, graph=graph, reps=2):
-                assert_same(got, ref, (tw, robust, graph))
+            for got in walk_grade(nl, bits, P, 2048, robust, tw, lanes=2, graph=graph, reps=2, rev=rev):
+                assert_same(got, ref, (tw, rev, robust, graph))
 
 
 @pytest.mark.parametrize("robust", [True, False])
@@ -112,10 +117,13 @@ def test_walk_50k_full_size_against_bitsliced(robust):
     P = GATES_50K.pairs
     bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
     ref = bitsliced_grade(nl, bits, P, P, robust)
-    for tw in WALKS:
-        say("walk 50k vs bit-sliced", robust, tw)
-        for got in walk_grade(nl, bits, P, P, robust, tw, reps=2):
-            assert_same(got, ref, (robust, tw))
+    for tw, rev in MODES:
+        say("walk 50k vs bit-sliced", robust, tw, rev)
+        for got in walk_grade(nl, bits, P, P, robust, tw, reps=2, rev=rev):
+            assert_same(got, ref, (robust, tw, rev))
+    say("walk 50k vs bit-sliced, 2 lanes x 32,768, reverse walk", robust)
+    for got in walk_grade(nl, bits, P, 32768, robust, 0, lanes=2, reps=2, rev=2):
+        assert_same(got, ref, (robust, "lanes"))
 
 
 def test_walk_50k_hold_biased_against_bitsliced():

@@ -6,21 +6,27 @@ seeded 50,000-gate / 60-level netlist, with the netlist and vectors already
 resident in HBM.  Prints one JSON line.
 
 The second half of the metric, time to 95% of full path coverage with the
-learned pair ranking against plain in-order grading, comes from the held-out
+learned pair ranking against plain in-order grading (both to the same
+fraction of the paths, in the same chunks), comes from the held-out
 experiment of pdfgrade.learned (32 training and 8 held-out 50k-gate circuits,
 1M candidate pairs each), run after the timed steps in both marking modes
 (--learned 0 skips it).
 
---workload batch-256-circuits instead times one batched grade of 256
-circuits (1k-200k gates, 4k-64k pairs each) against grading them one after
-another with per-circuit launches.
+`also` holds the same workload in the other marking mode, the cover-heavy
+50k-gate circuit (many more marked paths: the merge and the cover DP under
+load) and the 256-circuit batch against per-circuit grading (--extra 0 /
+--batch 0 skip them).  --workload cover-heavy-50k-gates or
+batch-256-circuits makes one of them the main line.
 
-The roofline object is for the dominant kernel (the fused sim + forward-mark
-level kernel), timed live with HIP events around each of its launches on the
-grader's stream.  The CPU baseline is the bit-sliced C grader
-(oracle/bitsliced.c, 64 pairs per word, OpenMP over the host's threads) over
-the whole workload on this host, with its one-core rate and the serial
-byte-per-pattern oracle's rate on a bounded sample beside it.
+The roofline object is for the dominant kernel (the level walk with the
+recorded tuning): algorithmic bytes per launch over its average duration
+timed live with HIP events on the grader's stream (frac), over the average of
+the committed rocprofv3 trace (frac_rocprof), the committed FETCH_SIZE +
+WRITE_SIZE traffic over the same time (traffic_frac), and the bytes the
+walk's own design must move (model_frac).  The CPU baseline is the
+bit-sliced C grader (oracle/bitsliced.c, 64 pairs per word, OpenMP over the
+host's threads) over the whole workload on this host, with its one-core rate
+and the serial byte-per-pattern oracle's rate on a bounded sample beside it.
 """
 from __future__ import annotations
 
@@ -49,22 +55,25 @@ def parse():
                     help="single workloads: segment length K (1..8); K >= 2 times pdf_grade_segs")
     ap.add_argument("--graph", type=int, default=None, help="batch workloads: force graph replay on (1) or off (0)")
     ap.add_argument("--learned", type=int, default=1, help="50k workload: run the learned-ranking experiment")
+    ap.add_argument("--extra", type=int, default=1,
+                    help="50k workload: also time the other marking mode, the cover-heavy input and the batch")
+    ap.add_argument("--batch", type=int, default=1, help="with --extra: include the 256-circuit batch line")
     ap.add_argument("--autotune", type=int, default=None,
                     help="run the autotuner before the timed steps (default: no for single workloads, yes for "
                          "batches)")
     return ap.parse_args()
 
 
-def measured_traffic(workload: str, family: str):
-    """HBM bytes per launch of the kernel family, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<workload>.md, its json
-    block), or None."""
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of a kernel (k_walk, k_fwd, k_cover ...), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/traffic_<workload>.md, its json block), or None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload}.md")
     try:
         with open(path) as f:
             text = f.read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
-        return json.loads(block)["bytes_per_launch"].get(f"k_{family}")
+        return json.loads(block)["bytes_per_launch"].get(kernel)
     except (OSError, IndexError, KeyError, ValueError):
         return None
 
@@ -214,10 +223,29 @@ def bench_batch(args, wl):
         # rate in dense-plane terms, not a measured bandwidth
         "dense_plane_equivalent": {"GBps": algo_bytes / dt / 1e9, "over_hbm_peak": algo_bytes / dt / 1e9 / HBM_PEAK_GBS},
         "forward_algorithmic": {"GBps": fwd_bytes / dt / 1e9, "frac": fwd_bytes / dt / 1e9 / HBM_PEAK_GBS},
+        "roofline": batch_roofline(wl.name, fwd_bytes, max(i["levels"] for i in infos) + 1),
         "cpu_baseline": batch_cpu_baseline(nets, vecs, pairs, robust, tot) if args.cpu_sample > 0 else None,
     }
-    print(json.dumps(out))
     b.close()
+    return out
+
+
+def batch_roofline(wl_name, fwd_bytes, launches):
+    """The batch's dominant kernel, k_fwd_batch (one launch per union level:
+    level k of every circuit deep enough): its algorithmic bytes, (5n + I)/8
+    per pair of every circuit, over the average duration in the committed
+    trace of this workload, and the committed counter traffic over it (the
+    batch has no per-kernel event timing)."""
+    per = fwd_bytes / launches
+    rp_ns = trace_avg_ns(wl_name, "k_fwd_batch")
+    traffic = measured_traffic(wl_name, "k_fwd_batch")
+    achieved = per / (rp_ns * 1e-9) / 1e9 if rp_ns else None
+    return {"bound": "hbm", "kernel": "k_fwd_batch", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+            "algorithmic_bytes_per_launch": per, "launches_per_step": launches,
+            "rocprof_avg_us": rp_ns / 1e3 if rp_ns else None,
+            "frac_rocprof": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic_frac": traffic / (rp_ns * 1e-9) / 1e9 / HBM_PEAK_GBS if traffic and rp_ns else None}
 
 
 def batch_cpu_baseline(nets, vecs, pairs, robust, gpu_totals, stride=8):
@@ -282,6 +310,173 @@ def learned_vs_plain(robust: bool):
     }
 
 
+def trace_avg_ns(workload: str, kernel: str):
+    """Average duration of a kernel in the committed rocprofv3 --kernel-trace
+    --stats summary of a workload (profiles/k_<kernel>_<workload>.md, the
+    "avg_ns" map of its json block), or None."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"k_*_{workload}.md"))):
+        try:
+            with open(path) as f:
+                text = f.read()
+            avg = json.loads(text.split("```json", 1)[1].split("```", 1)[0]).get("avg_ns", {})
+        except (OSError, IndexError, ValueError, AttributeError):
+            continue
+        if kernel in avg:
+            return float(avg[kernel])
+    return None
+
+
+def walk_model_bytes(nl, chunk: int, tw: int, seq: bool = True) -> dict:
+    """Bytes one level-walk launch must move in its design (the chunk's
+    records do not fit on chip): every node's tile record written once to
+    the tile-major plane, each fanin record outside the workgroup's LDS window
+    read back once, and the PI words (plus one halo word per tile) read.  The
+    mark words are data-dependent and left out (the counters include them)."""
+    import numpy as np
+
+    a = nl.arrays()
+    n, I = a["n"], a["num_pi"]
+    words = (chunk + 63) // 64
+    tiles = -(-words // tw)
+    nf = tw + 1 if seq else 2 * tw
+    nv = (nf + 2) & ~1
+    cap = (128 * 1024) // (8 * nv)
+    rec = 8 * nv
+    lo = a["level_off"]
+    far = 0
+    types, fin_off, fin = a["type"], a["fin_off"], a["fin"]
+    nfi = np.diff(fin_off)
+    lvl = a["level"]
+    hi = lo[lvl + 1]  # end of each node's level
+    win_lo = np.maximum(hi.astype(np.int64) - cap, 0)
+    # fanin j of gate g is read when it is fanin 0, or any fanin of a
+    # multi-input AND/NAND/OR/NOR
+    multi = (types >= 2) & (types <= 5)  # PDF_AND .. PDF_NOR
+    gate = np.repeat(np.arange(n), nfi)
+    j = np.arange(fin.shape[0]) - np.repeat(fin_off[:-1], nfi)
+    read = (j == 0) | multi[gate]
+    far = int(np.count_nonzero(read & (fin.astype(np.int64) < win_lo[gate])))
+    writes = n * tiles * rec
+    reads = far * tiles * rec + I * (words + (tiles if seq else 0)) * 8
+    return {"bytes": writes + reads, "record_writes": writes, "far_fanin_reads": far * tiles * rec,
+            "pi_reads": I * (words + (tiles if seq else 0)) * 8, "far_fanins": far, "tiles": tiles,
+            "record_bytes": rec, "window_nodes": cap}
+
+
+def roofline_fields(wl_name, fam, kernel, algo_bytes_launch, avg_ms, launches, model=None):
+    """The roofline object of the dominant kernel: algorithmic bytes per
+    launch over its HIP-event average (frac, frac_events), over the average
+    of the committed rocprofv3 trace of this workload (frac_rocprof), the
+    committed counter traffic over the event average (traffic_frac), and the
+    implementation's own byte model when given."""
+    achieved = algo_bytes_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = measured_traffic(wl_name, kernel)
+    rp_ns = trace_avg_ns(wl_name, kernel)
+    out = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "algorithmic_bytes_per_launch": algo_bytes_launch, "avg_launch_us": avg_ms * 1e3,
+           "launches_per_step": launches, "frac_events": achieved / HBM_PEAK_GBS,
+           "rocprof_avg_us": rp_ns / 1e3 if rp_ns else None,
+           "frac_rocprof": algo_bytes_launch / (rp_ns * 1e-9) / 1e9 / HBM_PEAK_GBS if rp_ns else None,
+           "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic and avg_ms > 0 else None}
+    if model:
+        out["model_bytes_per_launch"] = model["bytes"]
+        out["model_frac"] = model["bytes"] / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if avg_ms > 0 else None
+        out["model"] = model
+    return out
+
+
+def setup_grader(pg, nl, bits, wl, robust, autotune):
+    """A grader with the workload's recorded tuning (what pdf_grader_autotune
+    chose on an MI355X), or the autotuner's choice, untimed."""
+    g = pg.Grader(nl, device=0)
+    g.set_vectors(bits, wl.pairs)
+    # chunk first: the lanes' buffers are sized for the current chunk
+    g.set_chunk(wl.chunk)
+    g.set_tuning(pg.TUNE_LANES, wl.lanes)
+    g.set_tuning(pg.TUNE_WORDS_PER_LANE, wl.words_per_lane)
+    g.set_tuning(pg.TUNE_USE_GRAPH, wl.use_graph)
+    g.set_tuning(pg.TUNE_WALK, wl.walk)
+    if autotune:
+        tune = g.autotune(robust, reps=5)
+    else:
+        tune = {"lanes": wl.lanes, "chunk_pairs": wl.chunk, "use_graph": wl.use_graph,
+                "words_per_lane": wl.words_per_lane, "walk": wl.walk, "source": "recorded"}
+    return g, tune
+
+
+def timed_grades(grade, warmup, steps):
+    total = None
+    for _ in range(warmup):
+        total = grade()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = grade()
+        assert total is None or t == total
+        total = t
+    return total, (time.perf_counter() - t0) / steps
+
+
+def dominant(kt):
+    return max(("fwd", "reverse", "cover", "events", "merge"), key=lambda k: kt[f"{k}_ms"])
+
+
+def fam_bytes_per_pair(fam, n, I, K, num_segs, walk):
+    """Algorithmic bytes per pair of a kernel family (the per-stage figures of
+    the measurement plan): sim (2n + I)/8 and forward mark 3n/8 for the fused
+    forward kernel; removal 2n/8 and merge 2n/8 for the fused reverse kernel;
+    the level walk fuses all four; for segments of length K, each of the S
+    segments reads K mark bits and node 0's final-value bit per pair."""
+    if fam == "fwd":
+        return (9 * n + I) / 8.0 if walk else (5 * n + I) / 8.0
+    if fam == "reverse":
+        return 4 * n / 8.0
+    if fam == "merge":
+        return num_segs * (K + 1) / 8.0
+    if fam == "cover":
+        return n / 8.0
+    return 0.0
+
+
+def side_line(pg, nl, bits, wl, robust, steps, warmup, label):
+    """A secondary configuration of the 50k workload (same recorded tuning):
+    throughput, the grade's per-kernel HIP-event times and the paths found."""
+    g, _ = setup_grader(pg, nl, bits, wl, robust, False)
+    total, dt = timed_grades(lambda: g.grade(robust), warmup, steps)
+    t_total, kt = g.grade_timed(robust)
+    assert t_total == total
+    g.close()
+    fam = dominant(kt)
+    out = {"what": label, "mode": "robust" if robust else "non-robust", "value": wl.pairs / dt, "unit": "pairs/s",
+           "ms_per_step": dt * 1e3, "steps": steps, "total_paths": total,
+           "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "merge_ms",
+                                            "wall_ms")},
+           "kernel_launches": {k: kt[k] for k in ("fwd_launches", "reverse_launches", "events_launches",
+                                                  "cover_launches")},
+           "dominant": f"k_{'walk' if fam == 'fwd' and wl.walk else fam}"}
+    return out
+
+
+def cover_fields(wl_name, kt):
+    """k_cover and k_events of a grade: HIP-event average per launch and the
+    committed counter traffic of this workload over it."""
+    out = {}
+    for fam in ("cover", "events"):
+        launches = kt[f"{fam}_launches"]
+        if not launches:
+            continue
+        avg_ms = kt[f"{fam}_ms"] / launches
+        traffic = measured_traffic(wl_name, f"k_{fam}")
+        rp_ns = trace_avg_ns(wl_name, f"k_{fam}")
+        out[f"k_{fam}"] = {"avg_launch_us": avg_ms * 1e3, "launches_per_step": launches, "traffic": traffic,
+                           "traffic_GBps": traffic / (avg_ms * 1e-3) / 1e9 if traffic else None,
+                           "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                           "rocprof_avg_us": rp_ns / 1e3 if rp_ns else None}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -289,67 +484,43 @@ def main():
                                    "not built)"}))
         sys.exit(2)
     import pdfgrade as pg
-    from pdfgrade.configs import BATCH_WORKLOADS, WORKLOADS
+    from pdfgrade.configs import BATCH_WORKLOADS, COVER_HEAVY, WORKLOADS
 
     if args.workload in BATCH_WORKLOADS:
-        return bench_batch(args, BATCH_WORKLOADS[args.workload])
+        return print(json.dumps(bench_batch(args, BATCH_WORKLOADS[args.workload])))
 
     wl = WORKLOADS[args.workload]
     nl = pg.Netlist.generate(wl.netlist)
     info = nl.info()
     n, I = info["n"], info["num_pi"]
     bits = pg.gen_vectors(wl.vector_seed, I, wl.pairs)
+    prof_name = wl.name
     robust = bool(args.robust)
-    g = pg.Grader(nl, device=0)
-    g.set_vectors(bits, wl.pairs)
-    # the workload's recorded tuning (what pdf_grader_autotune chose on an
-    # MI355X); --autotune 1 runs the autotuner again, untimed
-    # (chunk first: the lanes' buffers are sized for the current chunk)
-    g.set_chunk(wl.chunk)
-    g.set_tuning(pg.TUNE_LANES, wl.lanes)
-    g.set_tuning(pg.TUNE_WORDS_PER_LANE, wl.words_per_lane)
-    g.set_tuning(pg.TUNE_USE_GRAPH, wl.use_graph)
-    tune = g.autotune(robust, reps=5) if args.autotune else {"lanes": wl.lanes, "chunk_pairs": wl.chunk,
-                                                              "use_graph": wl.use_graph,
-                                                              "words_per_lane": wl.words_per_lane,
-                                                              "source": "recorded"}
+    g, tune = setup_grader(pg, nl, bits, wl, robust, args.autotune)
     K = args.segs
     grade = (lambda: g.grade(robust)) if K == 1 else (lambda: g.grade_segs(robust, K))
-    total = None
-    for _ in range(args.warmup):
-        total = grade()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t = grade()
-        assert total is None or t == total
-        total = t
-    dt = time.perf_counter() - t0
-    pairs_per_s = wl.pairs * args.steps / dt
-    ms_per_step = dt * 1e3 / args.steps
+    total, dt = timed_grades(grade, args.warmup, args.steps)
+    pairs_per_s = wl.pairs / dt
+    ms_per_step = dt * 1e3
 
     # live per-launch timing of the dominant kernel family: the same chunk
     # shape with the launches serialized (lanes overlap kernels, which would
     # blur one kernel's duration)
     t_total, kt = g.grade_timed(robust) if K == 1 else g.grade_segs(robust, K, timed=True)
     assert t_total == total
-    fam = max(("fwd", "reverse", "cover", "events", "merge"), key=lambda k: kt[f"{k}_ms"])
-    # algorithmic bytes per pair, from the per-stage figures of the
-    # measurement plan: sim (2n + I)/8 and forward mark 3n/8 for the fused
-    # forward kernel; removal 2n/8 and merge 2n/8 for the fused reverse kernel;
-    # for segments of length K, each of the S segments reads K mark bits and
-    # node 0's final-value bit per pair: S (K + 1)/8
-    fwd_bytes = (5 * n + I) / 8.0 * wl.pairs
-    rev_bytes = 4 * n / 8.0 * wl.pairs
+    fam = dominant(kt)
+    walk = bool(wl.walk) and K == 1
+    kernel = "k_walk" if fam == "fwd" and walk else f"k_{fam}"
     num_segs = nl.segments(K).shape[0] if K > 1 else n
-    seg_bytes = num_segs * (K + 1) / 8.0 * wl.pairs
-    fam_bytes = {"fwd": fwd_bytes, "reverse": rev_bytes, "merge": seg_bytes}.get(fam, 0.0)
     launches = max(1, kt[f"{fam}_launches"])
+    algo_launch = fam_bytes_per_pair(fam, n, I, K, num_segs, walk) * wl.pairs / launches
     avg_ms = kt[f"{fam}_ms"] / launches
-    achieved = (fam_bytes / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    model = walk_model_bytes(nl, min(wl.chunk, wl.pairs), wl.walk) if kernel == "k_walk" else None
+    roof = roofline_fields(prof_name, fam, kernel, algo_launch, avg_ms, launches, model)
     # end-to-end algorithmic rate (10n + I bits per pair: sim, mark, remove,
     # merge, cover each reading/writing its dense planes once)
     e2e_bytes = (10 * n + I) / 8.0 * wl.pairs
-    e2e_gbs = e2e_bytes / (ms_per_step * 1e-3) / 1e9
+    e2e_gbs = e2e_bytes / dt / 1e9
 
     cpu = None
     if args.cpu_sample > 0 and K == 1:
@@ -357,15 +528,44 @@ def main():
         cpu = cpu_baseline(nl, bits, wl, min(args.cpu_sample, wl.pairs), robust, gpu)
     elif args.cpu_sample > 0:
         cpu = segs_cpu_baseline(nl, bits, wl, min(args.cpu_sample, wl.pairs), robust, K)
+    g.close()
+
+    headline = wl.name == "single-50k-gates" and K == 1
+    side = None
+    if args.extra and headline:
+        side = {"non_robust": side_line(pg, nl, bits, wl, not robust, args.steps, args.warmup,
+                                        "the same workload in the other marking mode"),
+                "cover_heavy": None, "batch": None}
+        ch = COVER_HEAVY
+        ch_nl = pg.Netlist.generate(ch.netlist)
+        ch_bits = pg.gen_vectors(ch.vector_seed, ch_nl.info()["num_pi"], ch.pairs)
+        line = side_line(pg, ch_nl, ch_bits, ch, robust, args.steps, args.warmup,
+                         f"{ch.name}: 50,000 gates in 16 levels, 70% NOT/BUFF, 4,096 POs; many more marked "
+                         f"paths, so the merge and the cover DP carry the load")
+        ch_kt = dict(line["kernel_ms"], **line["kernel_launches"])
+        ch_n = ch_nl.info()["n"]
+        line["cover_kernels"] = cover_fields(ch.name, ch_kt)
+        # the cover reads the mark plane once: n/8 bytes per pair
+        if ch_kt["cover_launches"]:
+            line["cover_roofline"] = roofline_fields(ch.name, "cover", "k_cover", ch_n / 8.0 * ch.pairs /
+                                                     ch_kt["cover_launches"],
+                                                     ch_kt["cover_ms"] / ch_kt["cover_launches"],
+                                                     ch_kt["cover_launches"])
+        side["cover_heavy"] = line
+        if args.batch:
+            from argparse import Namespace
+            bargs = Namespace(**vars(args))
+            bargs.steps, bargs.warmup = 5, 2
+            side["batch"] = bench_batch(bargs, BATCH_WORKLOADS["batch-256-circuits"])
 
     learned_part = None
-    if args.learned and wl.name == "single-50k-gates" and K == 1:
+    if args.learned and headline:
         learned_part = [learned_vs_plain(robust), learned_vs_plain(not robust)]
 
     out = {
         "metric": ("vector-pairs graded/s at 50k gates; time-to-95%-coverage learned vs plain"
-                   if wl.name == "single-50k-gates" and K == 1 else
-                   f"vector-pairs graded/s ({wl.name}{f', segments of length {K}' if K > 1 else ''})"),
+                   if headline else
+                   f"vector-pairs graded/s ({prof_name}{f', segments of length {K}' if K > 1 else ''})"),
         "value": pairs_per_s,
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -382,17 +582,16 @@ def main():
                    "mode": "robust" if robust else "non-robust", "total_paths": total, "tuning": tune,
                    "segs": K, "segments": num_segs},
         "gate_evals_per_s": pairs_per_s * (n - I),
-        "roofline": {"bound": "hbm", "kernel": f"k_{fam}", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(wl.name, fam),
-                     "avg_launch_us": avg_ms * 1e3, "launches_per_step": launches},
+        "roofline": roof,
         "end_to_end": {"algorithmic_GBps": e2e_gbs, "frac": e2e_gbs / HBM_PEAK_GBS},
         "kernel_ms": {k: kt[k] for k in ("fwd_ms", "reverse_ms", "events_ms", "cover_ms", "fill_ms", "merge_ms",
                                          "wall_ms")},
+        "cover_kernels": cover_fields(prof_name, kt),
         "cpu_baseline": cpu,
+        "also": side,
         "learned_vs_plain": learned_part,
     }
     print(json.dumps(out))
-    g.close()
 
 
 if __name__ == "__main__":

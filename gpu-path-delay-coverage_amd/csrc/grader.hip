@@ -4184,10 +4184,12 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   // the caller's configuration, restored if tuning fails
   const uint64_t chunk0 = g->chunk_pairs;
   const uint32_t lanes0 = g->tune_lanes, graph0 = g->tune_use_graph, wpl0 = g->tune_words_per_lane;
+  const uint32_t walk0 = g->tune_walk;
   auto restore = [&](int rc) {
     g->tune_lanes = lanes0;
     g->tune_use_graph = graph0;
     g->tune_words_per_lane = wpl0;
+    g->tune_walk = walk0;
     (void)pdf_grader_set_chunk(g, chunk0);
     return rc;
   };
@@ -4201,24 +4203,26 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   if (!hip_ok(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo")) return restore(PDF_ERR_HIP);
   const double budget = 0.85 * (double)free_b;
   const double n = g->nl.n;
-  auto lane_bytes = [&](uint64_t chunk) {  // what alloc_chunk_buffers takes for one lane
+  auto lane_bytes = [&](uint64_t chunk, uint32_t walk) {  // what alloc_chunk_buffers takes for one lane
     const uint64_t words = (chunk + 63) / 64, pitch = (words + 1 + 15) / 16 * 16, ntiles = (words + 63) / 64;
     const double per_wave = 16.0 * n;  // the cover's c/h fallback scratch
     const double waves = std::max(1.0, std::min((double)g->max_cover_waves, std::floor((double)(1ull << 30) / per_wave)));
     double bytes = n * (8.0 * pitch * (g->seq ? 3 : 4) + 8.0 * ntiles + 4.0 * words) + 4.0 * chunk + 12.0 * words +
                    waves * per_wave;
-    if (g->tune_walk) {
-      const double tw = g->tune_walk, nv = (double)(((g->seq ? g->tune_walk + 1 : 2 * g->tune_walk) + 2) & ~1u);
+    if (walk) {
+      const double tw = walk, nv = (double)(((g->seq ? walk + 1 : 2 * walk) + 2) & ~1u);
       bytes += n * std::ceil(words / tw) * nv * 8.0;
     }
     return bytes;
   };
-  struct Cand { uint32_t lanes; uint64_t chunk; };
+  struct Cand { uint32_t lanes; uint64_t chunk; uint32_t walk; };
   std::vector<Cand> cands;
   auto add = [&](uint32_t lanes, uint64_t chunk) {
     if ((P + chunk - 1) / chunk < lanes) return;  // fewer chunks than lanes
-    if (lanes * lane_bytes(chunk) > budget) return;
-    cands.push_back({lanes, chunk});
+    // per-level launches, and the level walk with tiles of 2 and 4 words
+    // (1 word is never the faster walk, 8 leaves most CUs idle below 2^19 pairs)
+    for (uint32_t walk : {0u, 2u, 4u})
+      if (lanes * lane_bytes(chunk, walk) <= budget) cands.push_back({lanes, chunk, walk});
   };
   for (uint32_t lanes = 1; lanes <= 4; ++lanes) {
     const uint64_t even = lanes == 1 ? P : round64((P + lanes - 1) / lanes);
@@ -4246,9 +4250,11 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
       for (uint32_t graph = 0; graph < 2; ++graph)
         for (uint32_t wpl = 1; wpl <= 2; ++wpl) {
           if (wpl == 2 && words_per_lane(2, g->seq, true, true, c.chunk) != 2) continue;  // no effect
+          if (wpl == 2 && c.walk) continue;  // the walk's tiles take the place of words per lane
           g->tune_lanes = c.lanes;
           g->tune_use_graph = graph;
           g->tune_words_per_lane = wpl;
+          g->tune_walk = c.walk;
           rc = pdf_grader_set_chunk(g, c.chunk);
           if (rc == PDF_ERR_NOMEM) continue;  // the estimate was short: skip this candidate
           if (rc) return rc;
@@ -4270,7 +4276,7 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
           }
           std::sort(t.begin() + 1, t.end());
           const float med = t[1 + reps / 2];
-          if (med < bt.ms) bt = pdf_grader_tuning{c.lanes, c.chunk, graph, wpl, med};
+          if (med < bt.ms) bt = pdf_grader_tuning{c.lanes, c.chunk, graph, wpl, c.walk, med};
         }
     if (!have_ref) {
       set_error("pdf_grader_autotune: no candidate configuration could be allocated");
@@ -4285,6 +4291,7 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
   g->tune_lanes = bt.lanes;
   g->tune_use_graph = bt.use_graph;
   g->tune_words_per_lane = bt.words_per_lane;
+  g->tune_walk = bt.walk;
   if ((rc = pdf_grader_set_chunk(g, bt.chunk_pairs))) return restore(rc);
   if (best) *best = bt;
   return PDF_OK;

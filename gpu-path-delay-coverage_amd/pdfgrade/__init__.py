@@ -69,7 +69,7 @@ class _Info(C.Structure):
 
 class _GraderTuning(C.Structure):
     _fields_ = [("lanes", C.c_uint32), ("chunk_pairs", C.c_uint64), ("use_graph", C.c_uint32),
-                ("words_per_lane", C.c_uint32), ("ms", C.c_float)]
+                ("words_per_lane", C.c_uint32), ("walk", C.c_uint32), ("ms", C.c_float)]
 
 
 class KernelTimes(C.Structure):
@@ -298,6 +298,26 @@ class Netlist:
         a["n"] = n
         a["L"] = L
         return a
+
+
+def hold_vectors(seed: int, num_pi: int, num_vectors: int, p_hold: float) -> np.ndarray:
+    """Hold-biased vectors: vector 0 iid Bernoulli(1/2), then each PI keeps
+    the previous vector's value with probability p_hold and draws a fresh one
+    otherwise (fewer side inputs switch, so many more paths are sensitized
+    than with iid vectors).  PI-major packed like gen_vectors (numpy
+    Generator `seed`)."""
+    rng = np.random.default_rng(seed)
+    fresh = rng.random((num_vectors, num_pi)) < 0.5
+    held = rng.random((num_vectors, num_pi)) < p_hold
+    held[0] = False
+    src = np.where(held, 0, np.arange(num_vectors)[:, None])
+    np.maximum.accumulate(src, axis=0, out=src)  # the last vector that drew PI i fresh
+    vals = np.take_along_axis(fresh, src, axis=0)
+    words = (num_vectors + 63) // 64
+    padded = np.zeros((words * 64, num_pi), dtype=bool)
+    padded[:num_vectors] = vals
+    packed = np.packbits(padded.T.reshape(num_pi, words, 64), axis=2, bitorder="little")
+    return np.ascontiguousarray(packed.reshape(num_pi, words * 8)).view("<u8").astype(np.uint64)
 
 
 def gen_vectors(seed: int, num_pi: int, num_vectors: int) -> np.ndarray:

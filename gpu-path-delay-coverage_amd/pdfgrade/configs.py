@@ -26,6 +26,7 @@ class Workload:
     lanes: int = 1  # chunks graded concurrently (pdf_grader_autotune's choice, see bench.py)
     words_per_lane: int = 1  # PDF_TUNE_WORDS_PER_LANE (2 needs chunks of a multiple of 65,536 pairs)
     use_graph: int = 1  # PDF_TUNE_USE_GRAPH: replay a captured graph (1) or launch directly (0)
+    walk: int = 0  # PDF_TUNE_WALK: words per level-walk tile (0: one launch per level and pass)
 
 
 TINY = Workload(
@@ -38,10 +39,11 @@ GATES_50K = Workload(
     name="single-50k-gates",
     netlist=GenConfig(seed=2, num_pi=256, num_po=128, num_gates=50_000, levels=60, fanin_p=WIDE_FANIN,
                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64),
-    # graph replay: direct launches on the two lanes' streams are faster on
-    # an idle host (0.721 against 0.753 ms a grade on an MI355X) but take
-    # 0.79-0.94 ms when host threads compete for the CPU
-    vector_seed=102, pairs=65_536, chunk=32_768, lanes=2, use_graph=1)
+    # the level walk, one 65,536-pair chunk in 4-word tiles (256 workgroups,
+    # one per CU): 0.68 ms a grade on an MI355X against 0.75-0.77 ms for
+    # per-level launches on 2 lanes x 32,768 pairs (graph replay), the
+    # fastest configuration without the walk
+    vector_seed=102, pairs=65_536, chunk=65_536, lanes=1, use_graph=1, walk=4)
 
 GATES_2M = Workload(
     name="single-2m-gates",
@@ -92,5 +94,18 @@ class BatchWorkload:
 
 BATCH_256 = BatchWorkload(name="batch-256-circuits", count=256)
 
-WORKLOADS = {w.name: w for w in (TINY, GATES_50K, GATES_2M)}
+COVER_HEAVY = Workload(
+    name="cover-heavy-50k-gates",
+    # 50,000 gates in 16 levels, 70% of them NOT/BUFF (a fan-in-1 gate passes
+    # its input's mark on), 4,096 POs: ~2,800 robust paths over 65,536 iid
+    # pairs against the 50k workload's 4, so the merge and the cover DP
+    # carry the load (the cover takes ~1.2 of ~2.1 ms a grade)
+    netlist=GenConfig(seed=5, num_pi=256, num_po=4096, num_gates=50_000, levels=16,
+                      fanin_p=(0.7, 0.25, 0.05, 0.0), prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64),
+    # pdf_grader_autotune's choice on an MI355X: one chunk, per-level
+    # launches (the level walk is slower here: 3,100 gates a level), direct
+    # launches (2.09 ms a grade)
+    vector_seed=7, pairs=65_536, chunk=65_536, lanes=1, use_graph=0)
+
+WORKLOADS = {w.name: w for w in (TINY, GATES_50K, GATES_2M, COVER_HEAVY)}
 BATCH_WORKLOADS = {BATCH_256.name: BATCH_256}

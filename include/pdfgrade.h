@@ -442,14 +442,16 @@ int pdf_score_pairs(pdf_grader *g, pdf_mlp *m, uint32_t max_level, int robust, u
 /* Autotuner (no reference counterpart): times the grade (reps runs each,
  * median) over chunk lanes 1..4, each with chunk = pairs / lanes and the
  * power-of-two chunks from 32,768 pairs below it (only configurations whose
- * chunk buffers fit in 85% of the free HBM), graph replay on and off and
- * words per lane; checks that every configuration gives the same total and
- * leaves the grader in the fastest one. */
+ * chunk buffers fit in 85% of the free HBM), graph replay on and off, words
+ * per lane, and per-level launches against the level walk with 2- and 4-word
+ * tiles (PDF_TUNE_WALK); checks that every configuration gives the same
+ * total and leaves the grader in the fastest one. */
 typedef struct {
   uint32_t lanes;
   uint64_t chunk_pairs;
   uint32_t use_graph, words_per_lane;
-  float ms; /* median grade time of this configuration */
+  uint32_t walk; /* PDF_TUNE_WALK: 0 = per-level launches */
+  float ms;      /* median grade time of this configuration */
 } pdf_grader_tuning;
 int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf_grader_tuning *best);
 

@@ -346,6 +346,7 @@ def test_grader_autotune_candidates():
         assert 1 <= lanes <= 4
         assert chunk == even or (32768 <= chunk < even and chunk & (chunk - 1) == 0), best
         assert best["use_graph"] in (0, 1) and best["words_per_lane"] in (1, 2) and best["ms"] > 0
+        assert best["walk"] in (0, 2, 4) and not (best["walk"] and best["words_per_lane"] == 2), best
         assert g.num_chunks() == -(-P // chunk)
         for rep in range(2):
             got = dict(total=g.grade(True), gate_counts=g.gate_counts())
@@ -406,8 +407,8 @@ def test_fcount_cli_matches_oracle(tmp_path):
 
 @pytest.mark.parametrize("robust", [True, False])
 def test_50k_gates_full_size_against_bitsliced(robust):
-    """All 65,536 pairs of the 50k-gate workload with its recorded chunk and
-    lane tuning, bit-exact against the bit-sliced CPU grader (an independent
+    """All 65,536 pairs of the 50k-gate workload with its recorded chunk,
+    lane and walk tuning, bit-exact against the bit-sliced CPU grader (an independent
     second CPU implementation, itself checked against the oracle)."""
     say("50k full vs bit-sliced", robust)
     nl = pg.Netlist.generate(GATES_50K.netlist)
@@ -417,6 +418,7 @@ def test_50k_gates_full_size_against_bitsliced(robust):
     with pg.Grader(nl) as g:
         g.set_vectors(bits, P)
         g.set_tuning(pg.TUNE_LANES, GATES_50K.lanes)
+        g.set_tuning(pg.TUNE_WALK, GATES_50K.walk)
         g.set_chunk(GATES_50K.chunk)
         for rep in range(2):
             got = dict(total=g.grade(robust), gate_counts=g.gate_counts())

@@ -238,3 +238,47 @@ def test_library_segments_equal_oracle(seed):
         assert np.array_equal(nl.segments(K), orc.segments(net, K)), K
     with pytest.raises(pg.PdfError):
         nl.segments(9)
+
+
+def test_hold_vectors_layout_and_hold_rate():
+    """pdfgrade.hold_vectors: PI-major packing like gen_vectors (bit v % 64
+    of word v // 64 is vector v), vector 0 drawn fresh, and each PI equal to
+    its previous value at rate p_hold + (1 - p_hold) / 2."""
+    from helpers import pack, unpack
+
+    V, I, p = 20000, 9, 0.8
+    bits = pg.hold_vectors(3, I, V, p)
+    assert bits.shape == (I, (V + 63) // 64) and bits.dtype == np.uint64
+    vals = unpack(bits, V)
+    assert np.array_equal(pack(vals), bits)  # no stray bits past vector V - 1
+    same = (vals[1:] == vals[:-1]).mean()
+    assert abs(same - (p + (1 - p) / 2)) < 0.01
+    assert abs(vals.mean() - 0.5) < 0.02
+    assert np.array_equal(pg.hold_vectors(3, I, V, p), bits)
+    assert not np.array_equal(pg.hold_vectors(4, I, V, p), bits)
+
+
+def test_bench_walk_byte_model_counts_far_fanins():
+    """bench.walk_model_bytes on the 50k-gate netlist: the fanin edges it
+    counts as outside the walk's LDS window equal a direct loop over the
+    gates (fanin 0, or any fanin of AND/NAND/OR/NOR, below the node's level
+    end minus the 2,730-node window), and the byte totals follow."""
+    import bench
+
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    a = nl.arrays()
+    m = bench.walk_model_bytes(nl, 65536, 4)
+    cap = m["window_nodes"]
+    assert cap == 128 * 1024 // 48 and m["record_bytes"] == 48 and m["tiles"] == 256
+    far = 0
+    lo = a["level_off"]
+    for g in range(a["n"]):
+        t = a["type"][g]
+        hi = int(lo[a["level"][g] + 1])
+        for j, f in enumerate(a["fin"][a["fin_off"][g]:a["fin_off"][g + 1]]):
+            if (j == 0 or pg.AND <= t <= pg.NOR) and int(f) < hi - cap:
+                far += 1
+    assert m["far_fanins"] == far > 0
+    assert m["record_writes"] == a["n"] * 256 * 48
+    assert m["far_fanin_reads"] == far * 256 * 48
+    assert m["bytes"] == m["record_writes"] + m["far_fanin_reads"] + m["pi_reads"]

@@ -2691,7 +2691,12 @@ void free_chunk_buffers(pdf_grader *g);
 int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs);
 
 // Frees every lane's chunk buffers (the extra lanes' streams and events too).
+// Every lane's queued work, graph launches included, finishes before any of
+// its graph executables or buffers is released.
 void free_chunk(pdf_grader *g) {
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (auto &l : g->lanes)
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
   drop_graph(g);
   if (!g->lanes.empty()) {
     pdf_grader::Lane l0;
@@ -3553,6 +3558,122 @@ int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t 
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
   g->merged_segs = (uint32_t)segs;
+  return PDF_OK;
+}
+
+// The four stages and the reset queued on a caller's stream: the same
+// launches as the synchronous calls with the grader's stream swapped for the
+// caller's, and nothing else (no allocation, no synchronous copy, no wait, no
+// memset node), so a caller can capture them into its own HIP graph.
+namespace {
+struct OnStream {
+  pdf_grader *g;
+  hipStream_t own;
+  OnStream(pdf_grader *g_, void *stream) : g(g_), own(g_->stream) { g->stream = static_cast<hipStream_t>(stream); }
+  ~OnStream() { g->stream = own; }
+};
+
+int async_begin(pdf_grader *g, void *stream, const char *what) {
+  if (!g || !stream) { set_error(std::string(what) + ": null argument"); return PDF_ERR_ARG; }
+  if (g->debug) {
+    set_error(std::string(what) + ": debug dumps need the synchronous stage calls");
+    return PDF_ERR_STATE;
+  }
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != g->device) HIPCHK(hipSetDevice(g->device));
+  return PDF_OK;
+}
+}  // namespace
+
+int pdf_grader_reset_async(pdf_grader *g, void *stream) {
+  int rc = async_begin(g, stream, "pdf_grader_reset_async");
+  if (rc) return rc;
+  OnStream on(g, stream);
+  rc = queue_reset(g);
+  if (!rc && g->seg_count) rc = queue_seg_reset(g);
+  return rc;
+}
+
+int pdf_sim_async(pdf_grader *g, uint64_t chunk, uint64_t start, void *stream) {
+  int rc = async_begin(g, stream, "pdf_sim_async");
+  if (rc) return rc;
+  uint64_t width;
+  if ((rc = check_chunk(g, chunk, start, &width))) return rc;
+  OnStream on(g, stream);
+  g->walk_planes = 0;
+  if ((rc = launch_fwd<true, false>(g, start, width, 0))) return rc;
+  g->stage_chunk = (int64_t)chunk;
+  g->stage = 1;
+  return PDF_OK;
+}
+
+int pdf_mark_async(pdf_grader *g, uint64_t chunk, uint64_t start, int robust, void *stream) {
+  int rc = async_begin(g, stream, "pdf_mark_async");
+  if (rc) return rc;
+  uint64_t width;
+  if ((rc = check_chunk(g, chunk, start, &width))) return rc;
+  if (g->stage_chunk != (int64_t)chunk || g->stage < 1) {
+    set_error("pdf_mark_async: simulate this chunk first");
+    return PDF_ERR_STATE;
+  }
+  OnStream on(g, stream);
+  if ((rc = launch_fwd<false, true>(g, start, width, robust))) return rc;
+  if ((rc = launch_reverse<true, false>(g, start, width))) return rc;
+  g->stage = 2;
+  g->stage_robust = robust;
+  return PDF_OK;
+}
+
+int pdf_merge_async(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, void *stream) {
+  int rc = async_begin(g, stream, "pdf_merge_async");
+  if (rc) return rc;
+  if (segs < 1 || segs > 8) { set_error("pdf_merge_async: segment length must be 1..8"); return PDF_ERR_ARG; }
+  uint64_t width;
+  if ((rc = check_chunk(g, chunk, start, &width))) return rc;
+  if (g->stage_chunk != (int64_t)chunk || g->stage < 2) {
+    set_error("pdf_merge_async: mark this chunk first");
+    return PDF_ERR_STATE;
+  }
+  if (segs >= 2 && (g->seg_k != (uint32_t)segs || !g->d_NEW)) {
+    set_error("pdf_merge_async: no segment list of this length yet (one pdf_merge or pdf_grade_segs call sets it "
+              "up)");
+    return PDF_ERR_STATE;
+  }
+  OnStream on(g, stream);
+  if ((rc = reset_chunk(g))) return rc;
+  if ((rc = launch_reverse<false, true>(g, start, width))) return rc;
+  if (segs >= 2 && (rc = launch_merge_seg(g, start, width))) return rc;
+  g->stage = 3;
+  g->merged_segs = (uint32_t)segs;
+  return PDF_OK;
+}
+
+int pdf_cover_async(pdf_grader *g, uint64_t chunk, uint64_t start, int segs, void *stream,
+                    uint64_t *chunk_total_dev, uint64_t *per_gate_accum_dev, uint32_t *err_dev) {
+  int rc = async_begin(g, stream, "pdf_cover_async");
+  if (rc) return rc;
+  if (!chunk_total_dev) { set_error("pdf_cover_async: null chunk_total_dev"); return PDF_ERR_ARG; }
+  if (segs < 1 || segs > 8) { set_error("pdf_cover_async: segment length must be 1..8"); return PDF_ERR_ARG; }
+  uint64_t width;
+  if ((rc = check_chunk(g, chunk, start, &width))) return rc;
+  if (g->stage_chunk != (int64_t)chunk || g->stage != 3) {
+    set_error("pdf_cover_async: merge this chunk first");
+    return PDF_ERR_STATE;
+  }
+  if ((uint32_t)segs != g->merged_segs) {
+    set_error("pdf_cover_async: segment length differs from the merge's");
+    return PDF_ERR_STATE;
+  }
+  OnStream on(g, stream);
+  rc = segs >= 2 ? launch_cover_segs(g, start, width) : launch_cover(g, start, width);
+  if (rc) return rc;
+  k_copy_result<<<1, 1, 0, g->stream>>>(g->d_chunk_total, g->d_err, chunk_total_dev, err_dev);
+  HIPCHK(hipGetLastError());
+  if (per_gate_accum_dev)
+    HIPCHK(hipMemcpyAsync(per_gate_accum_dev, g->d_gate, (uint64_t)g->nl.n * sizeof(uint64_t),
+                          hipMemcpyDeviceToDevice, g->stream));
+  g->stage = 4;
   return PDF_OK;
 }
 

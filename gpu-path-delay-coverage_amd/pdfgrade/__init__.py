@@ -118,6 +118,12 @@ SIGNATURES = {
     "pdf_grade_timed": ([_P, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
     "pdf_grade_segs": ([_P, C.c_int, C.c_int, _U64P, C.POINTER(KernelTimes)], C.c_int),
     "pdf_grade_async": ([_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "pdf_grader_reset_async": ([_P, C.c_void_p], C.c_int),
+    "pdf_sim_async": ([_P, C.c_uint64, C.c_uint64, C.c_void_p], C.c_int),
+    "pdf_mark_async": ([_P, C.c_uint64, C.c_uint64, C.c_int, C.c_void_p], C.c_int),
+    "pdf_merge_async": ([_P, C.c_uint64, C.c_uint64, C.c_int, C.c_void_p], C.c_int),
+    "pdf_cover_async": ([_P, C.c_uint64, C.c_uint64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+                        C.c_int),
     "pdf_grade_noalloc": ([_P, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "pdf_grader_set_tuning": ([_P, C.c_int, C.c_int64], C.c_int),
     "pdf_grader_set_merge_window": ([_P, C.c_uint32], C.c_int),
@@ -447,6 +453,26 @@ class Grader:
         cov = C.c_uint64()
         _check(lib().pdf_cover(self._h, chunk, start, segs, C.byref(cov), C.byref(ms)))
         return cov.value, ms.value
+
+    # -- the stages and the reset queued on a caller's HIP stream handle
+    # (e.g. torch.cuda.Stream().cuda_stream); device pointers as ints
+    def reset_async(self, stream: int):
+        _check(lib().pdf_grader_reset_async(self._h, C.c_void_p(stream)))
+
+    def sim_async(self, chunk: int, start: int, stream: int):
+        _check(lib().pdf_sim_async(self._h, chunk, start, C.c_void_p(stream)))
+
+    def mark_async(self, chunk: int, start: int, robust: bool, stream: int):
+        _check(lib().pdf_mark_async(self._h, chunk, start, int(robust), C.c_void_p(stream)))
+
+    def merge_async(self, chunk: int, start: int, segs: int, stream: int):
+        _check(lib().pdf_merge_async(self._h, chunk, start, segs, C.c_void_p(stream)))
+
+    def cover_async(self, chunk: int, start: int, segs: int, stream: int, chunk_total_dev: int,
+                    per_gate_accum_dev: int = 0, err_dev: int = 0):
+        _check(lib().pdf_cover_async(self._h, chunk, start, segs, C.c_void_p(stream), C.c_void_p(chunk_total_dev),
+                                     C.c_void_p(per_gate_accum_dev) if per_gate_accum_dev else None,
+                                     C.c_void_p(err_dev) if err_dev else None))
 
     def grade(self, robust: bool) -> int:
         tot = C.c_uint64()

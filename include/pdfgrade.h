@@ -226,6 +226,29 @@ int pdf_merge(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
 int pdf_cover(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs,
               uint64_t *coverage, float *ms);
 
+/* The four stages and the reset with the stream made explicit: each queues
+ * exactly the launches of its synchronous counterpart above on the caller's
+ * HIP stream (a hipStream_t passed as void *) and returns without waiting,
+ * allocating, copying synchronously or recording a memset, so a chunk's
+ * stages can be captured into the caller's HIP graph.  The same order rules
+ * hold (sim, mark, merge, cover per chunk; one segment length per vector
+ * set, its list set up once by a synchronous pdf_merge or pdf_grade_segs);
+ * debug dumps (pdf_grader_set_debug) need the synchronous calls.
+ * pdf_cover_async writes this chunk's exact total to *chunk_total_dev, the
+ * per-gate counts accumulated since the last reset to per_gate_accum_dev[n]
+ * (if not NULL) and, to *err_dev (if not NULL), non-zero when a uint64 count
+ * overflowed, all device memory, once the stream has run it.  The caller
+ * lets the stream finish before it reads the getters, changes the chunk or
+ * destroys the grader.  (simkernel.cuh:200, markkernel.cuh:295,
+ * mergekernel.cuh:200, coverkernel.cuh:233 with the legacy default stream
+ * replaced by the caller's.) */
+int pdf_grader_reset_async(pdf_grader *g, void *stream);
+int pdf_sim_async(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, void *stream);
+int pdf_mark_async(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int robust, void *stream);
+int pdf_merge_async(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs, void *stream);
+int pdf_cover_async(pdf_grader *g, uint64_t chunk, uint64_t start_pattern, int segs, void *stream,
+                    uint64_t *chunk_total_dev, uint64_t *per_gate_accum_dev, uint32_t *err_dev);
+
 /* Whole chunk loop of main.cu:171-231 (sim, mark, merge, cover per chunk,
  * totals accumulated) using the fused kernels; *total receives the exact
  * total.  Equivalent to calling the four stages per chunk. */
@@ -240,8 +263,9 @@ int pdf_grade(pdf_grader *g, int robust, uint64_t *total);
  * One lane.  The buffers it uses must exist: pdf_grader_set_chunk makes
  * those of segs = 1, one pdf_grade_segs call those of a segs >= 2.  Calls
  * that overlap on one grader race; the getters see the results once the
- * caller's stream has finished (main.cu:171-231 with the stream made
- * explicit). */
+ * caller's stream has finished, and the caller lets it finish before it
+ * changes the chunk or destroys the grader (the grader does not know the
+ * stream) (main.cu:171-231 with the stream made explicit). */
 int pdf_grade_async(pdf_grader *g, int robust, int segs, void *stream, uint64_t *total_dev,
                     uint32_t *err_dev);
 

@@ -330,10 +330,13 @@ def trace_avg_ns(workload: str, kernel: str):
 
 def walk_model_bytes(nl, chunk: int, tw: int, seq: bool = True) -> dict:
     """Bytes one level-walk launch must move in its design (the chunk's
-    records do not fit on chip): every node's tile record written once to
-    the tile-major plane, each fanin record outside the workgroup's LDS window
-    read back once, and the PI words (plus one halo word per tile) read.  The
-    mark words are data-dependent and left out (the counters include them)."""
+    records do not fit on chip): the record of every node that a later
+    level reads from outside the workgroup's LDS window written once per
+    tile (its reach, the end of its last reader's level minus its id,
+    exceeds the window), each such fanin read back once per tile, one
+    mark-present byte per node and tile written and read back, and the PI
+    words (plus one halo word per tile) read.  The mark words are
+    data-dependent and left out (the counters include them)."""
     import numpy as np
 
     a = nl.arrays()
@@ -342,27 +345,31 @@ def walk_model_bytes(nl, chunk: int, tw: int, seq: bool = True) -> dict:
     tiles = -(-words // tw)
     nf = tw + 1 if seq else 2 * tw
     nv = (nf + 2) & ~1
-    cap = (128 * 1024) // (8 * nv)
+    sv = nv + ((tw + 1) & ~1)  # an LDS slot: the record, then the mark words
+    cap = (160 * 1024 - 64) // (8 * sv)
     rec = 8 * nv
     lo = a["level_off"]
-    far = 0
     types, fin_off, fin = a["type"], a["fin_off"], a["fin"]
     nfi = np.diff(fin_off)
     lvl = a["level"]
-    hi = lo[lvl + 1]  # end of each node's level
-    win_lo = np.maximum(hi.astype(np.int64) - cap, 0)
+    hi = lo[lvl + 1].astype(np.int64)  # end of each node's level
     # fanin j of gate g is read when it is fanin 0, or any fanin of a
     # multi-input AND/NAND/OR/NOR
     multi = (types >= 2) & (types <= 5)  # PDF_AND .. PDF_NOR
     gate = np.repeat(np.arange(n), nfi)
     j = np.arange(fin.shape[0]) - np.repeat(fin_off[:-1], nfi)
     read = (j == 0) | multi[gate]
-    far = int(np.count_nonzero(read & (fin.astype(np.int64) < win_lo[gate])))
-    writes = n * tiles * rec
+    dist = hi[gate] - fin.astype(np.int64)
+    far = int(np.count_nonzero(read & (dist > cap)))
+    reach = np.zeros(n, np.int64)
+    np.maximum.at(reach, fin[read].astype(np.int64), dist[read])
+    kept = int(np.count_nonzero(reach > cap))
+    writes = kept * tiles * rec
+    bits = 2 * n * tiles
     reads = far * tiles * rec + I * (words + (tiles if seq else 0)) * 8
-    return {"bytes": writes + reads, "record_writes": writes, "far_fanin_reads": far * tiles * rec,
-            "pi_reads": I * (words + (tiles if seq else 0)) * 8, "far_fanins": far, "tiles": tiles,
-            "record_bytes": rec, "window_nodes": cap}
+    return {"bytes": writes + reads + bits, "record_writes": writes, "far_fanin_reads": far * tiles * rec,
+            "pi_reads": I * (words + (tiles if seq else 0)) * 8, "bit_bytes": bits, "far_fanins": far,
+            "kept_records": kept, "tiles": tiles, "record_bytes": rec, "window_nodes": cap}
 
 
 def roofline_fields(wl_name, fam, kernel, algo_bytes_launch, avg_ms, launches, model=None):

@@ -54,12 +54,18 @@ constexpr int kRegFanin = 4;
 
 // Per-node record read with one 32-byte load (a scalar load when the node is
 // the same for the whole wave): type, PO flag, the first four fanins inline and
-// the CSR offsets of the full fanin and fanout lists.
+// the CSR offsets of the full fanin and fanout lists.  reach: how far past the
+// node's id the last node of the level of its last reader in the level walk's
+// forward pass lies (a reader: a gate that reads it as fanin 0, or as any
+// fanin of an AND/NAND/OR/NOR), at most 65,535; 0 when nothing reads it.
+// The walk reads a fanin from its LDS window unless that distance exceeds
+// the window, so a node whose reach fits the window is never read back.
 struct alignas(32) GateRec {
   uint32_t fin[4];  // first four fanins (0 past nfi)
   uint32_t fin_off, fout_off;
   uint16_t nfi, nfo;
-  uint8_t type, po, pad0, pad1;
+  uint8_t type, po;
+  uint16_t reach;
 };
 static_assert(sizeof(GateRec) == 32, "GateRec is one 32-byte load");
 
@@ -763,7 +769,9 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
 // Workgroup b takes tile (b % 8) * per_xcd + b / 8: blocks are dealt to the
 // 8 XCDs round-robin, so neighbouring tiles share an XCD and its L2.
 constexpr int kWalkBlock = 1024;
-constexpr uint32_t kWalkLdsBytes = 128 * 1024;
+
+// the node window (the rest of the 160 KiB: the tile's list lengths)
+constexpr uint32_t kWalkLdsBytes = 160 * 1024 - 64;
 // records move as 16-byte words (ds_read_b128 / global_load_dwordx4)
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) u64x2 lds_u64x2;
@@ -773,8 +781,22 @@ struct WalkShape {
   static constexpr int NF = SEQ ? TW + 1 : 2 * TW;           // frame words of a node record
   static constexpr int NV = (NF + 2) & ~1;                   // + the tile's mark-present bits (bit u: word u),
                                                              // padded to whole 16-byte words
-  static constexpr uint32_t cap = kWalkLdsBytes / (8 * NV);  // node records in LDS
+  static constexpr int NMK = (TW + 1) & ~1;                  // mark words kept beside a record in LDS
+  static constexpr int SV = NV + NMK;                        // words of an LDS slot: record, then marks
+  static constexpr uint32_t cap = kWalkLdsBytes / (8 * SV);  // node slots in LDS
 };
+
+// Words of a node's record in the level walk (WalkShape::NV): the frame
+// words and the mark-present bits, padded to an even count.
+inline uint64_t walk_record_words(bool seq, uint32_t tw) { return ((seq ? tw + 1 : 2 * tw) + 2) & ~1ULL; }
+
+// Words of one lane's walk planes: the tile-major records ([tiles][n][NV])
+// followed by the tiles' mark-present bits, one byte per node
+// ([tiles][n], bit u: word u of the tile).
+inline uint64_t walk_plane_words(bool seq, uint32_t tw, uint64_t n, uint64_t words) {
+  const uint64_t tiles = (words + tw - 1) / tw;
+  return n * tiles * walk_record_words(seq, tw) + (n * tiles + 7) / 8;
+}
 
 template <int TW>
 __device__ __forceinline__ bool walk_tile(const Planes &pl, uint32_t tiles, uint32_t &tile, uint32_t &w0,
@@ -796,14 +818,15 @@ __device__ __forceinline__ uint64_t frame2(const uint64_t (&rec)[NV], int u) {
 }
 
 // The first N words of node f's NV-word record of the tile: from its LDS
-// slot when f is in the window, else from the tile-major plane.
-template <int N, int NV, uint32_t CAP>
+// slot (SV words apart) when f is in the window, else from the tile-major
+// plane.
+template <int N, int NV, int SV, uint32_t CAP>
 __device__ __forceinline__ void walk_fetch(uint64_t (&dst)[NV], uint32_t f, uint32_t win_lo, const lds_u64 *lds,
                                            const uint64_t *__restrict__ vt) {
-  static_assert(NV % 2 == 0 && N <= NV, "records are whole 16-byte words");
+  static_assert(NV % 2 == 0 && N <= NV && SV % 2 == 0, "records are whole 16-byte words");
   constexpr int Q = (N + 1) / 2;
   if (f >= win_lo) {
-    const lds_u64x2 *s = reinterpret_cast<const lds_u64x2 *>(lds + (f % CAP) * NV);
+    const lds_u64x2 *s = reinterpret_cast<const lds_u64x2 *>(lds + (f % CAP) * SV);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const u64x2 x = s[q];
@@ -821,15 +844,52 @@ __device__ __forceinline__ void walk_fetch(uint64_t (&dst)[NV], uint32_t f, uint
   }
 }
 
+// Node f's TW mark words of the tile: from its LDS slot (after the record)
+// when f is in the window, else from the node-major mark plane (row mrow,
+// from word w0: w0 is a multiple of TW and the pitch a multiple of 16 words
+// at least the chunk's words rounded up, so all TW words are inside the
+// row).  Plane words whose mark-present bit is clear are stale; the LDS
+// copy holds every word.
+template <int TW, int NV, int SV, uint32_t CAP>
+__device__ __forceinline__ void walk_fetch_marks(uint64_t (&dst)[TW], uint32_t f, uint32_t win_lo,
+                                                 const lds_u64 *lds, const uint64_t *__restrict__ mrow) {
+  if (f >= win_lo) {
+    const lds_u64 *s = lds + (f % CAP) * SV + NV;
+    if constexpr (TW == 1) {
+      dst[0] = s[0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < TW / 2; ++q) {
+        const u64x2 y = reinterpret_cast<const lds_u64x2 *>(s)[q];
+        dst[2 * q] = y.x;
+        dst[2 * q + 1] = y.y;
+      }
+    }
+  } else {
+    if constexpr (TW == 1) {
+      dst[0] = mrow[0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < TW / 2; ++q) {
+        const u64x2 y = reinterpret_cast<const u64x2 *>(mrow)[q];
+        dst[2 * q] = y.x;
+        dst[2 * q + 1] = y.y;
+      }
+    }
+  }
+}
+
 // Sim + forward mark of gate g (record r) over words w0 .. w0+nw-1 of the
 // tile; the node's record (frames of every held word) goes to VT and, when
-// `keep`, to its LDS slot.
-template <bool SEQ, bool ROBUST, int TW>
+// `keep`, to its LDS slot.  PRE: read the record at pre_src into `pre`
+// (the lane's gate of the next level) after this gate's loads.
+template <bool SEQ, bool ROBUST, int TW, bool PRE>
 __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &r, const Planes &pl,
                                               const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
                                               uint64_t vstride, uint64_t vbase, uint32_t g, uint32_t w0,
                                               uint32_t nw, uint64_t *__restrict__ vt, lds_u64 *lds,
-                                              uint32_t win_lo, bool keep) {
+                                              uint32_t win_lo, bool keep, const GateRec *pre_src, GateRec &pre,
+                                              uint8_t *__restrict__ bt, bool all_records) {
   using S = WalkShape<SEQ, TW>;
   constexpr int NV = S::NV;
   constexpr int NU = SEQ ? TW + 1 : TW;  // frame-1 words held: the tile's and the halo
@@ -847,10 +907,12 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
 #pragma unroll
       for (int u = 0; u < TW; ++u) b[u] = (uint32_t)u < nw ? srcb[u] : 0;
     }
+    if constexpr (PRE) pre = *pre_src;
 #pragma unroll
     for (int u = 0; u < TW; ++u) {
       if ((uint32_t)u >= nw) continue;
       const uint64_t bu = SEQ ? shift_next(a[u], a[SEQ ? u + 1 : u]) : b[u];
+      b[u] = bu;
       // pairs past the chunk's last one never mark (every other mark needs
       // its fanin 0 marked)
       m[u] = (a[u] ^ bu) & (w0 + u + 1 == pl.words ? pl.last_mask : ~0ULL);
@@ -859,6 +921,13 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
     const uint32_t nfi = r.nfi;
     const uint32_t *more = net.fin + r.fin_off;
     const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+    // (unknown types and XOR/XNOR with >= 2 fanins never mark)
+    const bool passes = ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT ||
+                        (ty >= PDF_AND && ty <= PDF_XNOR && (nfi == 1 || multi));
+    // one round trip: the fanin records (LDS or the tile-major plane), fanin
+    // 0's mark words of the tile (read whatever its mark-present bits say;
+    // words whose bit is clear are stale and masked off below) and the
+    // lane's netlist record for the next level
     uint64_t x[kRegFanin][NV];
 #pragma unroll
     for (int j = 0; j < kRegFanin; ++j) {
@@ -866,14 +935,23 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
       for (int u = 0; u < NV; ++u) x[j][u] = 0;
       if (j == 0 || (multi && (uint32_t)j < nfi)) {
         // fanin 0's record with its mark-present bits, the others' frames
-        if (j == 0) walk_fetch<NV, NV, S::cap>(x[j], r.fin[j], win_lo, lds, vt);
-        else walk_fetch<S::NF, NV, S::cap>(x[j], r.fin[j], win_lo, lds, vt);
+        if (j == 0) walk_fetch<NV, NV, S::SV, S::cap>(x[j], r.fin[j], win_lo, lds, vt);
+        else walk_fetch<S::NF, NV, S::SV, S::cap>(x[j], r.fin[j], win_lo, lds, vt);
       }
     }
+    uint64_t m0[TW];
+#pragma unroll
+    for (int u = 0; u < TW; ++u) m0[u] = 0;
+    if (passes)
+      walk_fetch_marks<TW, NV, S::SV, S::cap>(m0, r.fin[0], win_lo, lds,
+                                              pl.M + (uint64_t)r.fin[0] * pl.pitch + w0);
+    if constexpr (PRE) pre = *pre_src;
 #pragma unroll
     for (int j = 0; j < kRegFanin; ++j)
 #pragma unroll
       for (int u = 0; u < NV; ++u) hold(x[j][u]);
+#pragma unroll
+    for (int u = 0; u < TW; ++u) hold(m0[u]);
     const uint64_t nz0 = x[0][S::NF];  // fanin 0's mark-present bits
     // the chained 2-input fold (simkernel.cuh:131-148) of every held word of
     // every frame
@@ -892,7 +970,7 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
         }
         for (uint32_t j = kRegFanin; j < nfi; ++j) {
           uint64_t y[NV];
-          walk_fetch<S::NF, NV, S::cap>(y, more[j], win_lo, lds, vt);
+          walk_fetch<S::NF, NV, S::SV, S::cap>(y, more[j], win_lo, lds, vt);
 #pragma unroll
           for (int u = 0; u < S::NF; ++u) fold_m(v[u], y[u], om, inv);
         }
@@ -907,17 +985,9 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
 #pragma unroll
     for (int u = 0; u < TW; ++u) b[u] = frame2<SEQ, TW>(v, u);
     // forward mark (markkernel.cuh:228-267): fanin 0's mark words where its
-    // mark-present bit is set and this gate has a transition, one round trip
-    const bool passes = ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT ||
-                        (ty >= PDF_AND && ty <= PDF_XNOR && (nfi == 1 || multi));
-    // (unknown types and XOR/XNOR with >= 2 fanins never mark)
-    uint64_t m0[TW];
+    // mark-present bit is set
 #pragma unroll
-    for (int u = 0; u < TW; ++u) {
-      m0[u] = 0;
-      const uint64_t tr = passes && (uint32_t)u < nw ? a[u] ^ b[u] : 0;
-      if (tr && ((nz0 >> u) & 1)) m0[u] = pl.M[(uint64_t)r.fin[0] * pl.pitch + w0 + u];
-    }
+    for (int u = 0; u < TW; ++u) m0[u] &= (uint64_t)0 - ((nz0 >> u) & 1);
 #pragma unroll
     for (int u = 0; u < TW; ++u) {
       if (!passes || (uint32_t)u >= nw) continue;
@@ -941,13 +1011,13 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
           rr = 0;
           for (uint32_t i = 0; i < nfi; ++i) {
             uint64_t si[NV];
-            walk_fetch<S::NF, NV, S::cap>(si, more[i], win_lo, lds, vt);
+            walk_fetch<S::NF, NV, S::SV, S::cap>(si, more[i], win_lo, lds, vt);
             const uint64_t ai = si[u], bi = frame2<SEQ, TW>(si, u);
             uint64_t acc = ~0ULL;
             for (uint32_t j = 0; j < nfi && acc; ++j) {
               if (j == i) continue;
               uint64_t sj[NV];
-              walk_fetch<S::NF, NV, S::cap>(sj, more[j], win_lo, lds, vt);
+              walk_fetch<S::NF, NV, S::SV, S::cap>(sj, more[j], win_lo, lds, vt);
               const uint64_t aj = sj[u], bj = frame2<SEQ, TW>(sj, u);
               acc &= andlike ? sens<ROBUST, true>(ai, bi, aj, bj) : sens<ROBUST, false>(ai, bi, aj, bj);
             }
@@ -971,41 +1041,59 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
 #pragma unroll
     for (int u = 0; u < TW; ++u) { rec[u] = a[u]; rec[TW + u] = b[u]; }
   }
+  // marks to the node-major mark plane, their T1 pairs (mark & frame 2) to
+  // the value plane's words (the walk keeps values in its own records; the
+  // reverse walk splits its final marks into T0 / T1 by these), both only
+  // where the word holds a mark; the mark-present bits to the tile's byte
   const uint64_t row = (uint64_t)g * pl.pitch + w0;
   uint64_t bits = 0;
 #pragma unroll
   for (int u = 0; u < TW; ++u)
     if (m[u]) {
       pl.M[row + u] = m[u];
+      pl.V[row + u] = m[u] & b[u];
       bits |= 1ULL << u;
     }
   rec[S::NF] = bits;
-  u64x2 *dst = reinterpret_cast<u64x2 *>(vt + (uint64_t)g * NV);
+  bt[g] = (uint8_t)bits;
+  // the record to the tile-major plane only where a later level reads it
+  // back from there (or every record for the sim-plane getters)
+  if (all_records || r.reach > S::cap) {
+    u64x2 *dst = reinterpret_cast<u64x2 *>(vt + (uint64_t)g * NV);
 #pragma unroll
-  for (int q = 0; q < NV / 2; ++q) dst[q] = u64x2{rec[2 * q], rec[2 * q + 1]};
+    for (int q = 0; q < NV / 2; ++q) dst[q] = u64x2{rec[2 * q], rec[2 * q + 1]};
+  }
   if (keep) {
-    lds_u64x2 *slot = reinterpret_cast<lds_u64x2 *>(lds + (g % S::cap) * NV);
+    lds_u64x2 *slot = reinterpret_cast<lds_u64x2 *>(lds + (g % S::cap) * S::SV);
 #pragma unroll
     for (int q = 0; q < NV / 2; ++q) slot[q] = u64x2{rec[2 * q], rec[2 * q + 1]};
+    // every mark word, zeros included (the slot held another node before)
+#pragma unroll
+    for (int q = 0; q < S::NMK / 2; ++q)
+      slot[NV / 2 + q] = u64x2{2 * q < TW ? m[2 * q] : 0, 2 * q + 1 < TW ? m[2 * q + 1] : 0};
   }
   if (bits)
     atomicOr((unsigned long long *)&pl.NZ[(uint64_t)g * pl.ntiles + (w0 >> 6)],
              (unsigned long long)(bits << (w0 & 63)));
 }
 
-// Forward walk: levels 0..levels-1 of one tile, sim + forward mark.  VT is
-// the tile-major value plane ([tiles][n][NV] words).  The NZ plane must be
-// all zero (a fill kernel runs first).
+// Forward walk: levels 0..levels-1 of one tile, sim + forward mark.  recs
+// are the records in the walk order (walk_order: each level's gates sorted
+// by kind, fout_off holding the id).  VT is the tile-major value plane
+// ([tiles][n][NV] words).  The NZ plane must be all zero (a fill kernel
+// runs first).
 template <bool SEQ, bool ROBUST, int TW>
 __device__ __forceinline__ void walk_fwd_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
                                               const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
                                               uint64_t vstride, uint64_t wbase, uint64_t *__restrict__ VT,
+                                              uint8_t *__restrict__ BT, bool all_records,
                                               const uint32_t *__restrict__ level_off, uint32_t levels, uint32_t tiles,
                                               lds_u64 *lds) {
   using S = WalkShape<SEQ, TW>;
   uint32_t tile, w0, nw;
   if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
   uint64_t *vt = VT + (uint64_t)tile * net.n * S::NV;
+  uint8_t *bt = BT + (uint64_t)tile * net.n;
   // each lane's first gate of the next level: its netlist record is loaded
   // one level ahead (records do not change), off the level's critical path
   GateRec rn{};
@@ -1013,15 +1101,119 @@ __device__ __forceinline__ void walk_fwd_body(const DevNet &net, const GateRec *
   for (uint32_t k = 0; k < levels; ++k) {
     const uint32_t lo = level_off[k], hi = level_off[k + 1];
     const uint32_t win_lo = hi > S::cap ? hi - S::cap : 0;
-    const GateRec r0 = rn;
-    if (k + 1 < levels && hi + threadIdx.x < level_off[k + 2]) rn = recs[hi + threadIdx.x];
-    for (uint32_t g = lo + threadIdx.x; g < hi; g += kWalkBlock) {
-      const GateRec r = g == lo + threadIdx.x ? r0 : recs[g];
-      walk_fwd_gate<SEQ, ROBUST, TW>(net, r, pl, va, vb, vstride, wbase + w0, g, w0, nw, vt, lds, win_lo,
-                                     g >= win_lo);
+    {
+      const GateRec r0 = rn;
+      // the lane's gate of the next level, read unconditionally (lanes past
+      // that level's end read a record of this level, never used) into a
+      // register that nothing reads before the next level: no wait of this
+      // level covers it
+      const bool pre = k + 1 < levels && hi + threadIdx.x < level_off[k + 2];
+      const GateRec *pre_src = recs + (pre ? hi + threadIdx.x : lo);
+      // recs is the walk order: lane i of the level takes the i-th record,
+      // whose fout_off is its gate's id
+      uint32_t i = lo + threadIdx.x;
+      if (i < hi) {
+        // the lane's first gate: its record came with the previous level,
+        // and the next level's is issued after this gate's loads
+        const uint32_t g = r0.fout_off;
+        walk_fwd_gate<SEQ, ROBUST, TW, true>(net, r0, pl, va, vb, vstride, wbase + w0, g, w0, nw, vt, lds, win_lo,
+                                             g >= win_lo, pre_src, rn, bt, all_records);
+        for (i += kWalkBlock; i < hi; i += kWalkBlock) {
+          GateRec unused;
+          const GateRec r = recs[i];
+          walk_fwd_gate<SEQ, ROBUST, TW, false>(net, r, pl, va, vb, vstride, wbase + w0, r.fout_off, w0, nw, vt,
+                                                lds, win_lo, r.fout_off >= win_lo, nullptr, unused, bt,
+                                                all_records);
+        }
+      } else {
+        rn = *pre_src;
+      }
     }
     __syncthreads();  // this level's records are visible to the next level's lanes
   }
+}
+
+// Removal and merge of gate g over the marked words of its tile (bit u of
+// nzw: a forward mark at word w0 + u), the walk's form of rev_marked: every
+// load of the gate goes out in one round trip -- its mark, accumulator and
+// T1 words (the forward walk left mark & frame 2 in the value plane), its
+// first-detection pairs and its register fanins' mark-present bytes --
+// whatever the words turn out to need; a mark that dies (no marked fanout
+// pushed into it) then only has its word cleared.  One atomicMin per
+// polarity covers the whole tile: words ascend, so the first word with a T0
+// (T1) mark holds the tile's earliest pair.
+template <bool SEQ, int TW>
+__device__ __forceinline__ void walk_rev_gate(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
+                                              uint32_t w0, uint64_t nzw, const uint8_t *__restrict__ bt,
+                                              uint32_t start, uint32_t *__restrict__ first_t0,
+                                              uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                              uint64_t list_stride, lds_u32 *cnt) {
+  const uint64_t row = (uint64_t)g * pl.pitch + w0;
+  // TW words from w0: inside the row (w0 is a multiple of TW and the pitch
+  // a multiple of 16 words at least the chunk's words rounded up); words
+  // without a mark-present bit are stale and never used
+  auto load_tile = [&](uint64_t (&dst)[TW], const uint64_t *src) {
+    if constexpr (TW == 1) {
+      dst[0] = src[0];
+    } else {
+#pragma unroll
+      for (int q = 0; q < TW / 2; ++q) {
+        const u64x2 y = reinterpret_cast<const u64x2 *>(src)[q];
+        dst[2 * q] = y.x;
+        dst[2 * q + 1] = y.y;
+      }
+    }
+  };
+  uint64_t m[TW], acc[TW], mb[TW];
+  uint32_t fb[kRegFanin];
+  load_tile(m, pl.M + row);
+  load_tile(acc, pl.ACC + row);
+  load_tile(mb, pl.V + row);
+  const uint32_t cur0 = first_t0[g], cur1 = first_t1[g];
+#pragma unroll
+  for (int j = 0; j < kRegFanin; ++j) fb[j] = (uint32_t)j < r.nfi ? bt[r.fin[j]] : 0;
+  __builtin_amdgcn_sched_barrier(0);  // every load above is issued before any use
+  uint32_t c0 = PDF_NONE, c1 = PDF_NONE;
+#pragma unroll
+  for (int u = 0; u < TW; ++u) {
+    if (!((nzw >> u) & 1)) continue;
+    const uint32_t w = w0 + u;
+    uint64_t mu = m[u];
+    const uint64_t au = acc[u];
+    if (!mu) continue;
+    if (au) pl.ACC[row + u] = 0;
+    if (!r.po && (mu & ~au)) {
+      mu &= au;
+      pl.M[row + u] = mu;
+    }
+    if (!mu) continue;
+    // the pushes: this node's final mark into ACC of each fanin with a
+    // forward mark at this word.  Unlike rev_marked the push is not masked
+    // by the fanin's mark word: the fanin keeps (its forward mark) & ACC, so
+    // bits outside its mark change nothing, and it clears every ACC word it
+    // has a mark-present bit for, which are the only words pushed to.
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j)
+      if ((fb[j] >> u) & 1) push_acc<true>(&pl.ACC[(uint64_t)r.fin[j] * pl.pitch + w], mu);
+    for (uint32_t j = kRegFanin; j < r.nfi; ++j) {
+      const uint32_t f = net.fin[r.fin_off + j];
+      if ((bt[f] >> u) & 1) push_acc<true>(&pl.ACC[(uint64_t)f * pl.pitch + w], mu);
+    }
+    // the merge (mergekernel.cuh:141-197): marked implies a transition, and
+    // T1 means frame 2 set, so mu & mb[u] (mb[u] = forward mark & frame 2)
+    // are the T1 pairs of the final mark
+    const uint64_t mm = mu & merge_mask(pl, w);
+    const uint64_t t0 = mm & ~mb[u], t1 = mm & mb[u];
+    const uint32_t base = start + 64u * w;
+    if (t0 && c0 == PDF_NONE) c0 = base + (uint32_t)__builtin_ctzll(t0);
+    if (t1 && c1 == PDF_NONE) c1 = base + (uint32_t)__builtin_ctzll(t1);
+    const uint32_t pos = __hip_atomic_fetch_add(&cnt[u], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    list[(uint64_t)w * list_stride + pos] = g;
+  }
+  // the first-detection pairs only decrease within a grade, so a value read
+  // above is at least the current one: a candidate not below it is skipped
+  if (c0 < cur0) atomicMin(&first_t0[g], c0);
+  if (c1 < cur1) atomicMin(&first_t1[g], c1);
 }
 
 // Reverse walk: levels levels-1..0 of one tile, removal + merge + the
@@ -1033,7 +1225,7 @@ __device__ __forceinline__ void walk_fwd_body(const DevNet &net, const GateRec *
 // mark-present bits come from NZ and the frames from V / VB (node-major).
 template <bool SEQ, int TW, bool NM>
 __device__ __forceinline__ void walk_rev_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
-                                              const uint64_t *__restrict__ VT, const uint32_t *__restrict__ level_off,
+                                              const uint8_t *__restrict__ BT, const uint32_t *__restrict__ level_off,
                                               uint32_t levels, uint32_t tiles, uint32_t start,
                                               uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
                                               uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
@@ -1042,44 +1234,71 @@ __device__ __forceinline__ void walk_rev_body(const DevNet &net, const GateRec *
   uint32_t tile, w0, nw;
   if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
   if (threadIdx.x < TW) cnt[threadIdx.x] = 0;
-  const uint64_t *vt = NM ? nullptr : VT + (uint64_t)tile * net.n * S::NV;
+  const uint8_t *bt = NM ? nullptr : BT + (uint64_t)tile * net.n;
   const uint64_t tmask = (1ULL << nw) - 1;  // nw <= 8
   auto bits_of = [&](uint32_t g) -> uint64_t {  // mark-present bits of the tile, bit u: word w0 + u
     if (NM) return (pl.NZ[(uint64_t)g * pl.ntiles + (w0 >> 6)] >> (w0 & 63)) & tmask;
-    return vt[(uint64_t)g * S::NV + S::NF];
+    return bt[g];
   };
   // each lane's first gate of a level: its mark-present bits are loaded one
   // level ahead, and its netlist record too when a bit is set (neither
   // changes in this pass)
+  // (the walk reads a gate's words in one round trip; NM reads node-major
+  // planes word by word)
   uint64_t bn = 0;
+  bool bn_ok = false;
   GateRec rn{};
   {
     const uint32_t g = level_off[levels - 1] + threadIdx.x;
-    if (g < level_off[levels]) bn = bits_of(g);
-    if (bn) rn = recs[g];
+    if (g < level_off[levels]) {
+      bn = bits_of(g);
+      bn_ok = true;
+      if (NM) {
+        if (bn) rn = recs[g];
+      } else {
+        rn = recs[g];
+      }
+    }
   }
   __syncthreads();
   for (uint32_t k = levels; k-- > 0;) {
     const uint32_t lo = level_off[k], hi = level_off[k + 1];
-    const uint64_t b0 = bn;
+    const uint64_t b0 = bn_ok ? bn : 0;
     const GateRec r0 = rn;
     const uint32_t gn = k > 0 ? level_off[k - 1] + threadIdx.x : 0;
-    bn = k > 0 && gn < lo ? bits_of(gn) : 0;
+    const bool pre = k > 0 && gn < lo;
+    if (!NM) {
+      // the next level's bits and record, read unconditionally from a valid
+      // node (lanes past that level's end read one of this level, never
+      // used) into registers nothing reads before the next level: no wait
+      // of this level covers them
+      const uint32_t gs = pre ? gn : lo;
+      bn = bits_of(gs);
+      rn = recs[gs];
+      bn_ok = pre;
+    } else {
+      bn = pre ? bits_of(gn) : 0;
+      bn_ok = true;
+    }
     for (uint32_t g = lo + threadIdx.x; g < hi; g += kWalkBlock) {
       const bool first = g == lo + threadIdx.x;
       uint64_t nzw = first ? b0 : bits_of(g);
       if (!nzw) continue;
       const GateRec r = first ? r0 : recs[g];
-      while (nzw) {
-        const uint32_t w = w0 + (uint32_t)__builtin_ctzll(nzw);
-        nzw &= nzw - 1;
-        const uint64_t *fa = NM ? pl.V + (uint64_t)g * pl.pitch + w : vt + (uint64_t)g * S::NV + (w - w0);
-        const uint64_t *fb = SEQ ? fa + 1 : NM ? pl.VB + (uint64_t)g * pl.pitch + w : fa + TW;
-        rev_marked<SEQ, true, true, true>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride,
-                                          fa, fb, cnt, w0, NM ? nullptr : vt + S::NF, S::NV);
+      if constexpr (!NM) {
+        walk_rev_gate<SEQ, TW>(net, r, pl, g, w0, nzw, bt, start, first_t0, first_t1, list, list_stride, cnt);
+      } else {
+        while (nzw) {
+          const uint32_t w = w0 + (uint32_t)__builtin_ctzll(nzw);
+          nzw &= nzw - 1;
+          const uint64_t *fa = pl.V + (uint64_t)g * pl.pitch + w;
+          const uint64_t *fb = SEQ ? fa + 1 : pl.VB + (uint64_t)g * pl.pitch + w;
+          rev_marked<SEQ, true, true, true>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt,
+                                            list_stride, fa, fb, cnt, w0, nullptr, S::NV);
+        }
       }
     }
-    if (bn) rn = recs[gn];
+    if (NM && bn) rn = recs[gn];
     __syncthreads();  // pushes into lower levels' accumulators are visible
   }
   if (threadIdx.x < nw) list_cnt[w0 + threadIdx.x] = cnt[threadIdx.x];
@@ -1088,7 +1307,6 @@ __device__ __forceinline__ void walk_rev_body(const DevNet &net, const GateRec *
 // The reverse walk alone (after per-level forward launches: NM).
 template <bool SEQ, int TW, bool NM>
 __global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateRec *__restrict__ recs, Planes pl,
-                                                          const uint64_t *__restrict__ VT,
                                                           const uint32_t *__restrict__ level_off, uint32_t levels,
                                                           uint32_t tiles, uint32_t start,
                                                           uint32_t *__restrict__ first_t0,
@@ -1096,16 +1314,18 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_rev(DevNet net, const GateR
                                                           uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
                                                           uint64_t list_stride) {
   __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
-  walk_rev_body<SEQ, TW, NM>(net, recs, pl, VT, level_off, levels, tiles, start, first_t0, first_t1, list, list_cnt,
-                             list_stride, (lds_u32 *)cnt_words);
+  walk_rev_body<SEQ, TW, NM>(net, recs, pl, nullptr, level_off, levels, tiles, start, first_t0, first_t1, list,
+                             list_cnt, list_stride, (lds_u32 *)cnt_words);
 }
 
 // Forward and reverse walk of a tile in one launch: a tile's reverse pass
 // needs only its own forward pass, so no workgroup waits for another.
 template <bool SEQ, bool ROBUST, int TW>
-__global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+__global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *__restrict__ recs,
+                                                      const GateRec *__restrict__ wrecs, Planes pl,
                                                       const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
                                                       uint64_t vstride, uint64_t wbase, uint64_t *__restrict__ VT,
+                                                      uint8_t *__restrict__ BT, uint32_t all_records,
                                                       const uint32_t *__restrict__ level_off, uint32_t levels,
                                                       uint32_t tiles, uint64_t *__restrict__ actmap,
                                                       uint32_t *__restrict__ act_count,
@@ -1114,7 +1334,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *
                                                       uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
                                                       uint64_t list_stride) {
   using S = WalkShape<SEQ, TW>;
-  __shared__ uint64_t lds_words[S::cap * S::NV];
+  __shared__ uint64_t lds_words[S::cap * S::SV];
   __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
   {
     // the chunk's event and cover counters, each cleared by the block that
@@ -1131,10 +1351,10 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *
       }
     }
   }
-  walk_fwd_body<SEQ, ROBUST, TW>(net, recs, pl, va, vb, vstride, wbase, VT, level_off, levels, tiles,
-                                 (lds_u64 *)lds_words);
-  __syncthreads();  // the forward walk's records are visible to the reverse walk
-  walk_rev_body<SEQ, TW, false>(net, recs, pl, VT, level_off, levels, tiles, start, first_t0, first_t1, list,
+  walk_fwd_body<SEQ, ROBUST, TW>(net, wrecs, pl, va, vb, vstride, wbase, VT, BT, all_records != 0, level_off, levels,
+                                 tiles, (lds_u64 *)lds_words);
+  __syncthreads();  // the forward walk's marks and bits are visible to the reverse walk
+  walk_rev_body<SEQ, TW, false>(net, recs, pl, BT, level_off, levels, tiles, start, first_t0, first_t1, list,
                                 list_cnt, list_stride, (lds_u32 *)cnt_words);
 }
 
@@ -2089,6 +2309,10 @@ struct pdf_grader {
   uint8_t *d_type = nullptr, *d_po = nullptr;
   uint32_t *d_fin_off = nullptr, *d_fin = nullptr, *d_fout_off = nullptr, *d_fout = nullptr, *d_level = nullptr;
   uint32_t *d_level_off = nullptr;
+  // the level walk's forward order: each level's records sorted by how much
+  // work their gate takes (walk_order), fout_off holding the node's id
+  GateRec *d_wrec = nullptr;
+
   // vectors: sequence mode keeps vwords + 1 words per PI row (positions
   // wrapped mod num_pairs, so the word after the last is available)
   uint64_t num_pairs = 0, vwords = 0, vstride = 0;
@@ -2143,6 +2367,7 @@ struct pdf_grader {
   uint32_t tune_rev_tiles = 0;  // PDF_TUNE_REV_TILES (0: by tile count)
   uint32_t tune_use_graph = 1;
   uint32_t tune_walk = 0;  // PDF_TUNE_WALK: 0 level launches, else words per tile of the level walk
+  uint32_t walk_full = 0;  // the last walk wrote every node's record (debug flags were set)
   uint32_t merge_window = 0;  // pdf_grader_set_merge_window
   uint32_t tune_rev_walk = 0;  // PDF_TUNE_REV_WALK: words per tile of a reverse walk after per-level forward launches
   // chunk lanes (PDF_TUNE_LANES): the fused grade runs consecutive chunks on
@@ -2400,7 +2625,7 @@ template <bool SEQ, int TW, bool NM>
 void launch_rev_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tiles) {
   const uint32_t grid = 8 * ((tiles + 7) / 8);  // walk_tile skips the blocks past the last tile
   tm_begin(g, kRev);
-  k_walk_rev<SEQ, TW, NM><<<grid, kWalkBlock, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_VT, g->d_level_off,
+  k_walk_rev<SEQ, TW, NM><<<grid, kWalkBlock, 0, g->stream>>>(g->net(), g->d_rec, pl, g->d_level_off,
                                                               g->nl.L + 1, tiles, (uint32_t)start, g->d_first0,
                                                               g->d_first1, g->d_list, g->d_list_cnt, g->nl.n);
   tm_end(g);
@@ -2434,9 +2659,13 @@ template <bool SEQ, bool R, int TW>
 void launch_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tiles) {
   const DevNet net = g->net();
   const uint32_t grid = 8 * ((tiles + 7) / 8);  // walk_tile skips the blocks past the last tile
+  // the mark-present bytes follow the records of the lane's full chunk
+  const uint64_t tiles_alloc = ((g->chunk_pairs + 63) / 64 + TW - 1) / TW;
+  uint8_t *bt = reinterpret_cast<uint8_t *>(g->d_VT + (uint64_t)g->nl.n * tiles_alloc * walk_record_words(SEQ, TW));
   tm_begin(g, kFwd);
-  k_walk<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride, start / 64,
-                                                          g->d_VT, g->d_level_off, g->nl.L + 1, tiles, g->d_actmap,
+  k_walk<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, g->d_wrec, pl, g->d_va, g->d_vb, g->vstride, start / 64,
+                                                          g->d_VT, bt, g->walk_full, g->d_level_off, g->nl.L + 1,
+                                                          tiles, g->d_actmap,
                                                           g->d_act_count, g->d_chunk_total, (uint32_t)start,
                                                           g->d_first0, g->d_first1, g->d_list, g->d_list_cnt,
                                                           g->nl.n);
@@ -2466,6 +2695,8 @@ int launch_walk(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
   tm_end(g);
   const uint32_t tw = g->tune_walk;
   g->walk_planes = tw;
+  // every record is kept only for the sim-plane getters (debug flags set)
+  g->walk_full = g->debug != 0;
   if (g->seq) {
     if (robust) launch_walk_tw<true, true>(g, start, pl, tw);
     else launch_walk_tw<true, false>(g, start, pl, tw);
@@ -2729,10 +2960,7 @@ int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs) {
   if (!rc) rc = dalloc(&g->d_ACC, n * g->pitch);
   // the removal accumulator is zero between passes (each reader clears it)
   if (!rc && !hip_ok(hipMemset(g->d_ACC, 0, n * g->pitch * 8), "hipMemset")) rc = PDF_ERR_HIP;
-  if (!rc && g->tune_walk) {
-    const uint64_t tw = g->tune_walk, nv = ((g->seq ? tw + 1 : 2 * tw) + 2) & ~1ULL;  // + the mark-present word, even
-    rc = dalloc(&g->d_VT, n * ((words + tw - 1) / tw) * nv);
-  }
+  if (!rc && g->tune_walk) rc = dalloc(&g->d_VT, walk_plane_words(g->seq, g->tune_walk, n, words));
   if (!rc) rc = dalloc(&g->d_list, n * words);
   const uint32_t ct_off = (uint32_t)((3 * words + 2) & ~1ULL);
   g->n_cstate = ct_off + 2;
@@ -2818,6 +3046,40 @@ int copy_rows(pdf_grader *g, uint64_t *dst, const uint64_t *src, uint64_t cols) 
 
 extern "C" {
 
+// The level walk's forward order: within each level, gates sorted by the
+// work their lane takes -- single-input gates that pass their fanin's mark
+// on, then gates that never mark (XOR/XNOR/NOT and unknown types with two or
+// more fanins simulate to S0, simkernel.cuh:89-92), then AND/NAND/OR/NOR by
+// fan-in -- ties by id.  A wave then holds gates of one kind, and the
+// folds and sensitization terms no lane of it needs are branched over.
+// The copies keep every field the walk reads; fout_off (fanouts are not
+// read) holds the node's id.
+std::vector<GateRec> walk_order(const Netlist &nl, const std::vector<GateRec> &rec) {
+  std::vector<GateRec> out(nl.n);
+  std::vector<uint32_t> ids;
+  for (uint32_t k = 0; k <= nl.L; ++k) {
+    const uint32_t lo = nl.level_off[k], hi = nl.level_off[k + 1];
+    ids.resize(hi - lo);
+    for (uint32_t i = lo; i < hi; ++i) ids[i - lo] = i;
+    auto key = [&](uint32_t i) -> uint32_t {
+      const uint8_t ty = nl.type[i];
+      const uint32_t nfi = rec[i].nfi;
+      if (ty == PDF_INPT) return 0;
+      const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+      if (multi && nfi >= 2) return 10 + std::min<uint32_t>(nfi, 5);
+      const bool passes = ty == PDF_BUFF || ty == PDF_DFF || ty == PDF_FROM || ty == PDF_NOT ||
+                          (ty >= PDF_AND && ty <= PDF_XNOR && nfi == 1);
+      return passes ? 1 : 2;
+    };
+    std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+    for (uint32_t i = lo; i < hi; ++i) {
+      out[i] = rec[ids[i - lo]];
+      out[i].fout_off = ids[i - lo];
+    }
+  }
+  return out;
+}
+
 int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   if (!nlh || !out) { set_error("pdf_grader_create: null argument"); return PDF_ERR_ARG; }
   int ndev = 0;
@@ -2859,7 +3121,18 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
     r.type = nl.type[i];
     r.po = nl.po[i];
   }
+  for (uint32_t i = nl.num_pi; i < nl.n; ++i) {
+    const uint8_t ty = nl.type[i];
+    const bool multi = ty >= PDF_AND && ty <= PDF_NOR;
+    const uint32_t end = nl.level_off[nl.level[i] + 1];  // end of the reader's level
+    for (uint32_t e = nl.fin_off[i]; e < nl.fin_off[i + 1]; ++e) {
+      if (e != nl.fin_off[i] && !multi) break;
+      const uint32_t f = nl.fin[e];
+      rec[f].reach = (uint16_t)std::max<uint32_t>(rec[f].reach, std::min<uint32_t>(end - f, 65535u));
+    }
+  }
   up(g->d_rec, rec);
+  up(g->d_wrec, walk_order(nl, rec));
   up(g->d_type, nl.type);
   up(g->d_po, nl.po);
   up(g->d_fin_off, nl.fin_off);
@@ -2893,6 +3166,7 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_fout);
   dfree(g->d_level);
   dfree(g->d_level_off);
+  dfree(g->d_wrec);
   dfree(g->d_va);
   dfree(g->d_vb);
   free_rank(g);
@@ -3884,11 +4158,16 @@ int pdf_get_sim_planes(const pdf_grader *gc, uint64_t *a_plane, uint64_t *b_plan
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
   const uint64_t words = (width + 63) / 64, n = g->nl.n;
   const uint64_t last = width % 64 ? (1ULL << (width % 64)) - 1 : ~0ULL;
+  if (g->walk_planes && !g->walk_full) {
+    set_error("pdf_get_sim_planes: the level walk kept only the records it reads back; set a debug flag "
+              "(pdf_grader_set_debug) before grading to keep every node's values");
+    return PDF_ERR_STATE;
+  }
   if (g->walk_planes) {
     // the level walk's tile-major records: [tiles][n][NV], TW + 1 frame
     // words (frame 1 of the tile's words and the halo) or 2 TW (both
     // frames), then the tile's mark-present bits, padded to an even count
-    const uint64_t tw = g->walk_planes, nv = ((g->seq ? tw + 1 : 2 * tw) + 2) & ~1ULL, tiles = (words + tw - 1) / tw;
+    const uint64_t tw = g->walk_planes, nv = walk_record_words(g->seq, (uint32_t)tw), tiles = (words + tw - 1) / tw;
     std::vector<uint64_t> vt(tiles * n * nv);
     HIPCHK(hipMemcpy(vt.data(), g->d_VT, vt.size() * 8, hipMemcpyDeviceToHost));
     for (uint64_t i = 0; i < n; ++i)
@@ -3942,6 +4221,15 @@ int pdf_grader_set_debug(pdf_grader *g, uint32_t flags) {
   if (flags & ~(uint32_t)(PDF_DEBUG_FORWARD_MARKS | PDF_DEBUG_PI_COVER)) {
     set_error("pdf_grader_set_debug: unknown flag");
     return PDF_ERR_ARG;
+  }
+  if (flags != g->debug) {
+    // captured grades hold the walk's keep-every-record flag as a launch
+    // argument: capture again
+    HIPCHK(hipSetDevice(g->device));
+    if (g->stream) HIPCHK(hipStreamSynchronize(g->stream));
+    for (auto &l : g->lanes)
+      if (l.stream) HIPCHK(hipStreamSynchronize(l.stream));
+    drop_graph(g);
   }
   g->debug = flags;
   return PDF_OK;
@@ -4330,10 +4618,7 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
     const double waves = std::max(1.0, std::min((double)g->max_cover_waves, std::floor((double)(1ull << 30) / per_wave)));
     double bytes = n * (8.0 * pitch * (g->seq ? 3 : 4) + 8.0 * ntiles + 4.0 * words) + 4.0 * chunk + 12.0 * words +
                    waves * per_wave;
-    if (walk) {
-      const double tw = walk, nv = (double)(((g->seq ? walk + 1 : 2 * walk) + 2) & ~1u);
-      bytes += n * std::ceil(words / tw) * nv * 8.0;
-    }
+    if (walk) bytes += 8.0 * (double)walk_plane_words(g->seq, walk, g->nl.n, words);
     return bytes;
   };
   struct Cand { uint32_t lanes; uint64_t chunk; uint32_t walk; };

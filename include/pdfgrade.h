@@ -362,7 +362,10 @@ int pdf_grade_segs(pdf_grader *g, int robust, int segs, uint64_t *total, pdf_ker
 /* Parity getters (the reference's --verbose dumps, simkernel.cuh:250-295;
  * markkernel.cuh:357-395; mergekernel.cuh:25-47; coverkernel.cuh:44-66).
  * Planes are node-major: [n][words] uint64, words = chunk width / 64; bit j
- * of word k is pair start+64k+j of the chunk last simulated / marked. */
+ * of word k is pair start+64k+j of the chunk last simulated / marked.  After
+ * a fused grade in the level walk (PDF_TUNE_WALK) the sim planes are kept
+ * only when a debug flag was set before it (pdf_grader_set_debug); otherwise
+ * pdf_get_sim_planes returns PDF_ERR_STATE. */
 int pdf_get_sim_planes(const pdf_grader *g, uint64_t *a_plane,
                        uint64_t *b_plane);
 int pdf_get_mark_plane(const pdf_grader *g, uint64_t *mark_plane);

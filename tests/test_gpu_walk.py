@@ -138,3 +138,33 @@ def test_walk_50k_hold_biased_against_bitsliced():
         for tw in (2, 4):
             say("walk 50k hold-biased", robust, tw)
             assert_same(walk_grade(nl, bits, P, P, robust, tw)[0], ref, (robust, tw))
+
+
+@pytest.mark.parametrize("tw", [1, 4])
+def test_walk_sim_planes_after_a_grade(tw):
+    """The walk writes back only the records a later level reads outside its
+    LDS window, so after a fused walk grade the sim-plane getter refuses;
+    with a debug flag set before the grade every record is kept and the
+    last chunk's planes equal the oracle's codes.  Results are the same
+    either way."""
+    from oracle import oracle as orc
+
+    nl = pg.Netlist.generate(small_config(21, num_pi=12, num_po=10, num_gates=700, levels=9, wide=True,
+                                          alpha=2.2))
+    P, chunk = 1024, 512
+    bits = biased_vectors(21, nl.info()["num_pi"], P, p_hold=0.4)
+    ref = oracle_grade(nl, bits, P, chunk, True)
+    net = orc.OracleNet(nl.arrays())
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(chunk)
+        g.set_tuning(pg.TUNE_WALK, tw)
+        say("walk planes", tw)
+        assert g.grade(True) == ref["total"]
+        with pytest.raises(pg.PdfError):
+            g.sim_planes(chunk)
+        g.set_debug(pg.DEBUG_FORWARD_MARKS)
+        assert g.grade(True) == ref["total"]
+        a, b = g.sim_planes(chunk)
+        codes = orc.sim_chunk(net, bits, P, P - chunk, chunk)
+        assert np.array_equal(pg.planes_to_codes(a, b, chunk), codes)

@@ -260,17 +260,19 @@ def test_hold_vectors_layout_and_hold_rate():
 
 def test_bench_walk_byte_model_counts_far_fanins():
     """bench.walk_model_bytes on the 50k-gate netlist: the fanin edges it
-    counts as outside the walk's LDS window equal a direct loop over the
-    gates (fanin 0, or any fanin of AND/NAND/OR/NOR, below the node's level
-    end minus the 2,730-node window), and the byte totals follow."""
+    counts as outside the walk's LDS window, and the nodes whose records are
+    written for them, equal a direct loop over the gates (fanin 0, or any
+    fanin of AND/NAND/OR/NOR, below the node's level end minus the
+    2,047-node window of 80-byte slots), and the byte totals follow."""
     import bench
 
     nl = pg.Netlist.generate(GATES_50K.netlist)
     a = nl.arrays()
     m = bench.walk_model_bytes(nl, 65536, 4)
     cap = m["window_nodes"]
-    assert cap == 128 * 1024 // 48 and m["record_bytes"] == 48 and m["tiles"] == 256
+    assert cap == (160 * 1024 - 64) // 80 == 2047 and m["record_bytes"] == 48 and m["tiles"] == 256
     far = 0
+    kept = set()
     lo = a["level_off"]
     for g in range(a["n"]):
         t = a["type"][g]
@@ -278,7 +280,10 @@ def test_bench_walk_byte_model_counts_far_fanins():
         for j, f in enumerate(a["fin"][a["fin_off"][g]:a["fin_off"][g + 1]]):
             if (j == 0 or pg.AND <= t <= pg.NOR) and int(f) < hi - cap:
                 far += 1
+                kept.add(int(f))
     assert m["far_fanins"] == far > 0
-    assert m["record_writes"] == a["n"] * 256 * 48
+    assert m["kept_records"] == len(kept) < a["n"]
+    assert m["record_writes"] == len(kept) * 256 * 48
     assert m["far_fanin_reads"] == far * 256 * 48
-    assert m["bytes"] == m["record_writes"] + m["far_fanin_reads"] + m["pi_reads"]
+    assert m["bit_bytes"] == 2 * a["n"] * 256
+    assert m["bytes"] == m["record_writes"] + m["far_fanin_reads"] + m["pi_reads"] + m["bit_bytes"]

@@ -1216,6 +1216,211 @@ __device__ __forceinline__ void walk_rev_gate(const DevNet &net, const GateRec &
   if (c1 < cur1) atomicMin(&first_t1[g], c1);
 }
 
+// TW words from w0 of a node-major plane row: inside the row (w0 is a
+// multiple of TW and the pitch a multiple of 16 words at least the chunk's
+// words rounded up).
+template <int TW>
+__device__ __forceinline__ void load_tile(uint64_t (&dst)[TW], const uint64_t *src) {
+  if constexpr (TW == 1) {
+    dst[0] = src[0];
+  } else {
+#pragma unroll
+    for (int q = 0; q < TW / 2; ++q) {
+      const u64x2 y = reinterpret_cast<const u64x2 *>(src)[q];
+      dst[2 * q] = y.x;
+      dst[2 * q + 1] = y.y;
+    }
+  }
+}
+
+// What the staged reverse walk reads of one gate a level before the gate's
+// level runs: its mark, T1 and accumulator words of the tile, its
+// first-detection pairs and its register fanins' mark-present bytes.
+// Nothing of it changes before the gate's level except the accumulator
+// words that the level right above pushes, and those go to LDS.
+template <int TW>
+struct StagedGate {
+  uint64_t m[TW], mb[TW], acc[TW];
+  uint32_t cur0, cur1;
+  uint32_t fb[kRegFanin];
+};
+
+template <int TW>
+__device__ __forceinline__ void rev_item_load(StagedGate<TW> &it, const Planes &pl, const GateRec &r, uint32_t g,
+                                              uint32_t w0, const uint8_t *__restrict__ bt,
+                                              const uint32_t *__restrict__ first_t0,
+                                              const uint32_t *__restrict__ first_t1) {
+  const uint64_t row = (uint64_t)g * pl.pitch + w0;
+  load_tile<TW>(it.m, pl.M + row);
+  load_tile<TW>(it.mb, pl.V + row);
+  load_tile<TW>(it.acc, pl.ACC + row);
+  it.cur0 = first_t0[g];
+  it.cur1 = first_t1[g];
+#pragma unroll
+  for (int j = 0; j < kRegFanin; ++j) it.fb[j] = (uint32_t)j < r.nfi ? bt[r.fin[j]] : 0;
+}
+
+// Removal and merge of gate g (level lo_cur .. ) over the marked words of
+// its tile (bit u of nzw), from its staged item and the accumulator words
+// the level above pushed into LDS (accl).  Pushes into fanins of the next
+// level (ids nx_lo .. nx_lo + nx_n) go to that level's LDS accumulator
+// (nx_acc, TW words a node, unmasked: the fanin keeps its forward mark &
+// ACC, so bits outside its mark change nothing, and the buffer is zeroed
+// by its readers); pushes into older fanins go to the ACC plane, and only
+// to words where the fanin has a forward mark (the plane is all zero
+// between passes: each reader clears the words it has a mark-present bit
+// for).  Otherwise as walk_rev_gate.
+template <bool SEQ, int TW>
+__device__ __forceinline__ void walk_rev_apply(const DevNet &net, const GateRec &r, const Planes &pl, uint32_t g,
+                                               uint32_t w0, uint64_t nzw, const StagedGate<TW> &it,
+                                               const uint64_t (&accl)[TW], lds_u64 *nx_acc, uint32_t nx_lo,
+                                               uint32_t nx_n, const uint8_t *__restrict__ bt, uint32_t start,
+                                               uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
+                                               uint32_t *__restrict__ list, uint64_t list_stride, lds_u32 *cnt) {
+  const uint64_t row = (uint64_t)g * pl.pitch + w0;
+  uint32_t c0 = PDF_NONE, c1 = PDF_NONE;
+#pragma unroll
+  for (int u = 0; u < TW; ++u) {
+    if (!((nzw >> u) & 1)) continue;
+    const uint32_t w = w0 + u;
+    uint64_t mu = it.m[u];
+    const uint64_t au = it.acc[u] | accl[u];
+    if (!mu) continue;
+    if (it.acc[u]) pl.ACC[row + u] = 0;
+    if (!r.po && (mu & ~au)) {
+      mu &= au;
+      pl.M[row + u] = mu;
+    }
+    if (!mu) continue;
+#pragma unroll
+    for (int j = 0; j < kRegFanin; ++j) {
+      if ((uint32_t)j >= r.nfi) continue;
+      const uint32_t f = r.fin[j];
+      if (f - nx_lo < nx_n)
+        __hip_atomic_fetch_or(&nx_acc[(f - nx_lo) * TW + u], mu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else if ((it.fb[j] >> u) & 1)
+        push_acc<true>(&pl.ACC[(uint64_t)f * pl.pitch + w], mu);
+    }
+    for (uint32_t j = kRegFanin; j < r.nfi; ++j) {
+      const uint32_t f = net.fin[r.fin_off + j];
+      if (f - nx_lo < nx_n)
+        __hip_atomic_fetch_or(&nx_acc[(f - nx_lo) * TW + u], mu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      else if ((bt[f] >> u) & 1)
+        push_acc<true>(&pl.ACC[(uint64_t)f * pl.pitch + w], mu);
+    }
+    const uint64_t mm = mu & merge_mask(pl, w);
+    const uint64_t t0 = mm & ~it.mb[u], t1 = mm & it.mb[u];
+    const uint32_t base = start + 64u * w;
+    if (t0 && c0 == PDF_NONE) c0 = base + (uint32_t)__builtin_ctzll(t0);
+    if (t1 && c1 == PDF_NONE) c1 = base + (uint32_t)__builtin_ctzll(t1);
+    const uint32_t pos = __hip_atomic_fetch_add(&cnt[u], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    list[(uint64_t)w * list_stride + pos] = g;
+  }
+  if (c0 < it.cur0) atomicMin(&first_t0[g], c0);
+  if (c1 < it.cur1) atomicMin(&first_t1[g], c1);
+}
+
+// The reverse walk staged through registers and LDS (every level of the
+// netlist has at most nx_cap nodes): what a gate reads from HBM is loaded
+// while the level above it runs (StagedGate; its record and bits two levels
+// ahead), and the level above pushes into the gate's accumulator in LDS
+// (two buffers of nx_cap nodes x TW words, by level parity), so no level
+// waits for a load from HBM.  Lane i takes node lo + i of each level.
+template <bool SEQ, int TW>
+__device__ __forceinline__ void walk_rev_staged(const DevNet &net, const GateRec *__restrict__ recs,
+                                                const Planes &pl, const uint8_t *__restrict__ BT,
+                                                const uint32_t *__restrict__ level_off, uint32_t levels,
+                                                uint32_t tiles, uint32_t start, uint32_t *__restrict__ first_t0,
+                                                uint32_t *__restrict__ first_t1, uint32_t *__restrict__ list,
+                                                uint32_t *__restrict__ list_cnt, uint64_t list_stride,
+                                                lds_u64 *accbuf, uint32_t nx_cap, lds_u32 *cnt) {
+  uint32_t tile, w0, nw;
+  if (!walk_tile<TW>(pl, tiles, tile, w0, nw)) return;
+  if (threadIdx.x < TW) cnt[threadIdx.x] = 0;
+  const uint8_t *bt = BT + (uint64_t)tile * net.n;
+  for (uint32_t i = threadIdx.x; i < 2 * nx_cap * TW; i += kWalkBlock) accbuf[i] = 0;
+  const uint32_t tid = threadIdx.x;
+  // A: the lane's gate of the current level (record, bits, item); B: its
+  // gate of the next level (record, bits)
+  GateRec rA{}, rB{};
+  uint32_t bA = 0, bB = 0;
+  bool vB = false;  // the lane has a gate in the next level (bB is its bits)
+  StagedGate<TW> itA{};
+  {
+    const uint32_t lo = level_off[levels - 1], hi = level_off[levels];
+    if (lo + tid < hi) {
+      rA = recs[lo + tid];
+      bA = bt[lo + tid];  // (waited for here, once)
+      if (bA) rev_item_load<TW>(itA, pl, rA, lo + tid, w0, bt, first_t0, first_t1);
+    }
+    if (levels >= 2) {
+      const uint32_t lo1 = level_off[levels - 2];
+      const uint32_t gs = lo1 + tid < lo ? lo1 + tid : lo1;
+      rB = recs[gs];
+      bB = bt[gs];
+      vB = lo1 + tid < lo;
+    }
+  }
+  __syncthreads();  // the buffers are zero
+  for (uint32_t k = levels; k-- > 0;) {
+    const uint32_t lo = level_off[k], hi = level_off[k + 1];
+    const uint32_t nx_lo = k > 0 ? level_off[k - 1] : 0, nx_n = k > 0 ? lo - nx_lo : 0;
+    lds_u64 *own = accbuf + (k & 1) * nx_cap * TW;
+    lds_u64 *nx_acc = accbuf + ((k + 1) & 1) * nx_cap * TW;
+    // the next level's items (its bits came with the previous level) and
+    // the level after's records and bits: issued now, used after the barrier
+    // (validity flags are applied when the values are used, after the
+    // barrier, so no wait of this level covers the loads)
+    StagedGate<TW> itC{};
+    if (k > 0 && vB && bB) rev_item_load<TW>(itC, pl, rB, nx_lo + tid, w0, bt, first_t0, first_t1);
+    GateRec rD{};
+    uint32_t bD = 0;
+    bool vD = false;
+    if (k > 1) {
+      const uint32_t lo2 = level_off[k - 2];
+      const uint32_t gs = lo2 + tid < nx_lo ? lo2 + tid : lo2;
+      rD = recs[gs];
+      bD = bt[gs];
+      vD = lo2 + tid < nx_lo;
+    }
+    uint32_t i = lo + tid;
+    if (i < hi) {
+      uint64_t accl[TW];
+#pragma unroll
+      for (int u = 0; u < TW; ++u) {
+        accl[u] = own[(i - lo) * TW + u];
+        own[(i - lo) * TW + u] = 0;
+      }
+      if (bA)
+        walk_rev_apply<SEQ, TW>(net, rA, pl, i, w0, bA, itA, accl, nx_acc, nx_lo, nx_n, bt, start, first_t0,
+                                first_t1, list, list_stride, cnt);
+      for (i += kWalkBlock; i < hi; i += kWalkBlock) {
+        // levels wider than the block: loaded here, not ahead
+#pragma unroll
+        for (int u = 0; u < TW; ++u) {
+          accl[u] = own[(i - lo) * TW + u];
+          own[(i - lo) * TW + u] = 0;
+        }
+        const uint32_t b = bt[i];
+        if (!b) continue;
+        const GateRec r = recs[i];
+        StagedGate<TW> it;
+        rev_item_load<TW>(it, pl, r, i, w0, bt, first_t0, first_t1);
+        walk_rev_apply<SEQ, TW>(net, r, pl, i, w0, b, it, accl, nx_acc, nx_lo, nx_n, bt, start, first_t0,
+                                first_t1, list, list_stride, cnt);
+      }
+    }
+    __syncthreads();  // pushes into the next levels' accumulators are visible, own buffer zero
+    rA = rB;
+    bA = vB ? bB : 0;
+    itA = itC;
+    rB = rD;
+    bB = bD;
+    vB = vD;
+  }
+  if (threadIdx.x < nw) list_cnt[w0 + threadIdx.x] = cnt[threadIdx.x];
+}
+
 // Reverse walk: levels levels-1..0 of one tile, removal + merge + the
 // per-word lists of marked nodes (rev_marked on every marked (gate, word) of
 // the tile, frames from VT).  A node's fanouts are on higher levels, so they
@@ -1327,7 +1532,8 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *
                                                       uint64_t vstride, uint64_t wbase, uint64_t *__restrict__ VT,
                                                       uint8_t *__restrict__ BT, uint32_t all_records,
                                                       const uint32_t *__restrict__ level_off, uint32_t levels,
-                                                      uint32_t tiles, uint64_t *__restrict__ actmap,
+                                                      uint32_t widest, uint32_t tiles,
+                                                      uint64_t *__restrict__ actmap,
                                                       uint32_t *__restrict__ act_count,
                                                       uint64_t *__restrict__ chunk_total, uint32_t start,
                                                       uint32_t *__restrict__ first_t0, uint32_t *__restrict__ first_t1,
@@ -1354,8 +1560,15 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *
   walk_fwd_body<SEQ, ROBUST, TW>(net, wrecs, pl, va, vb, vstride, wbase, VT, BT, all_records != 0, level_off, levels,
                                  tiles, (lds_u64 *)lds_words);
   __syncthreads();  // the forward walk's marks and bits are visible to the reverse walk
-  walk_rev_body<SEQ, TW, false>(net, recs, pl, BT, level_off, levels, tiles, start, first_t0, first_t1, list,
-                                list_cnt, list_stride, (lds_u32 *)cnt_words);
+  // the reverse walk stages through the forward walk's LDS when two levels'
+  // accumulators fit
+  constexpr uint32_t nx_cap = S::cap * S::SV / (2 * TW);
+  if (widest <= nx_cap)
+    walk_rev_staged<SEQ, TW>(net, recs, pl, BT, level_off, levels, tiles, start, first_t0, first_t1, list, list_cnt,
+                             list_stride, (lds_u64 *)lds_words, nx_cap, (lds_u32 *)cnt_words);
+  else
+    walk_rev_body<SEQ, TW, false>(net, recs, pl, BT, level_off, levels, tiles, start, first_t0, first_t1, list,
+                                  list_cnt, list_stride, (lds_u32 *)cnt_words);
 }
 
 // Cross-chunk state to its initial value (one launch per grade).
@@ -2312,6 +2525,7 @@ struct pdf_grader {
   // the level walk's forward order: each level's records sorted by how much
   // work their gate takes (walk_order), fout_off holding the node's id
   GateRec *d_wrec = nullptr;
+  uint32_t widest = 0;  // nodes of the widest level
 
   // vectors: sequence mode keeps vwords + 1 words per PI row (positions
   // wrapped mod num_pairs, so the word after the last is available)
@@ -2665,7 +2879,7 @@ void launch_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t til
   tm_begin(g, kFwd);
   k_walk<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, g->d_wrec, pl, g->d_va, g->d_vb, g->vstride, start / 64,
                                                           g->d_VT, bt, g->walk_full, g->d_level_off, g->nl.L + 1,
-                                                          tiles, g->d_actmap,
+                                                          g->widest, tiles, g->d_actmap,
                                                           g->d_act_count, g->d_chunk_total, (uint32_t)start,
                                                           g->d_first0, g->d_first1, g->d_list, g->d_list_cnt,
                                                           g->nl.n);
@@ -3042,18 +3256,17 @@ int copy_rows(pdf_grader *g, uint64_t *dst, const uint64_t *src, uint64_t cols) 
   return PDF_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
 // The level walk's forward order: within each level, gates sorted by the
 // work their lane takes -- single-input gates that pass their fanin's mark
 // on, then gates that never mark (XOR/XNOR/NOT and unknown types with two or
 // more fanins simulate to S0, simkernel.cuh:89-92), then AND/NAND/OR/NOR by
 // fan-in -- ties by id.  A wave then holds gates of one kind, and the
 // folds and sensitization terms no lane of it needs are branched over.
-// The copies keep every field the walk reads; fout_off (fanouts are not
-// read) holds the node's id.
+// (Putting the gates that read a fanin from outside the LDS window last,
+// so that waves without such reads compute while the others wait, measured
+// slower: 0.605 against 0.591 ms per 50k grade.)  The copies keep every
+// field the walk reads; fout_off (fanouts are not read) holds the node's
+// id.
 std::vector<GateRec> walk_order(const Netlist &nl, const std::vector<GateRec> &rec) {
   std::vector<GateRec> out(nl.n);
   std::vector<uint32_t> ids;
@@ -3079,6 +3292,10 @@ std::vector<GateRec> walk_order(const Netlist &nl, const std::vector<GateRec> &r
   }
   return out;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   if (!nlh || !out) { set_error("pdf_grader_create: null argument"); return PDF_ERR_ARG; }
@@ -3133,6 +3350,7 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   }
   up(g->d_rec, rec);
   up(g->d_wrec, walk_order(nl, rec));
+  for (uint32_t k = 0; k <= nl.L; ++k) g->widest = std::max(g->widest, nl.level_off[k + 1] - nl.level_off[k]);
   up(g->d_type, nl.type);
   up(g->d_po, nl.po);
   up(g->d_fin_off, nl.fin_off);

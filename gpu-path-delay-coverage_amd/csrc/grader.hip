@@ -825,22 +825,15 @@ __device__ __forceinline__ void walk_fetch(uint64_t (&dst)[NV], uint32_t f, uint
                                            const uint64_t *__restrict__ vt) {
   static_assert(NV % 2 == 0 && N <= NV && SV % 2 == 0, "records are whole 16-byte words");
   constexpr int Q = (N + 1) / 2;
-  if (f >= win_lo) {
-    const lds_u64x2 *s = reinterpret_cast<const lds_u64x2 *>(lds + (f % CAP) * SV);
+  // one generic pointer a lane, no branch: an LDS read and an HBM read in
+  // two branches of a wave would wait for each other's registers
+  const uint64_t *src = f >= win_lo ? (const uint64_t *)(lds + (f % CAP) * SV) : vt + (uint64_t)f * NV;
+  const u64x2 *s = reinterpret_cast<const u64x2 *>(src);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const u64x2 x = s[q];
-      dst[2 * q] = x.x;
-      dst[2 * q + 1] = x.y;
-    }
-  } else {
-    const u64x2 *s = reinterpret_cast<const u64x2 *>(vt + (uint64_t)f * NV);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const u64x2 x = s[q];
-      dst[2 * q] = x.x;
-      dst[2 * q + 1] = x.y;
-    }
+  for (int q = 0; q < Q; ++q) {
+    const u64x2 x = s[q];
+    dst[2 * q] = x.x;
+    dst[2 * q + 1] = x.y;
   }
 }
 
@@ -853,28 +846,15 @@ __device__ __forceinline__ void walk_fetch(uint64_t (&dst)[NV], uint32_t f, uint
 template <int TW, int NV, int SV, uint32_t CAP>
 __device__ __forceinline__ void walk_fetch_marks(uint64_t (&dst)[TW], uint32_t f, uint32_t win_lo,
                                                  const lds_u64 *lds, const uint64_t *__restrict__ mrow) {
-  if (f >= win_lo) {
-    const lds_u64 *s = lds + (f % CAP) * SV + NV;
-    if constexpr (TW == 1) {
-      dst[0] = s[0];
-    } else {
-#pragma unroll
-      for (int q = 0; q < TW / 2; ++q) {
-        const u64x2 y = reinterpret_cast<const lds_u64x2 *>(s)[q];
-        dst[2 * q] = y.x;
-        dst[2 * q + 1] = y.y;
-      }
-    }
+  const uint64_t *src = f >= win_lo ? (const uint64_t *)(lds + (f % CAP) * SV + NV) : mrow;
+  if constexpr (TW == 1) {
+    dst[0] = src[0];
   } else {
-    if constexpr (TW == 1) {
-      dst[0] = mrow[0];
-    } else {
 #pragma unroll
-      for (int q = 0; q < TW / 2; ++q) {
-        const u64x2 y = reinterpret_cast<const u64x2 *>(mrow)[q];
-        dst[2 * q] = y.x;
-        dst[2 * q + 1] = y.y;
-      }
+    for (int q = 0; q < TW / 2; ++q) {
+      const u64x2 y = reinterpret_cast<const u64x2 *>(src)[q];
+      dst[2 * q] = y.x;
+      dst[2 * q + 1] = y.y;
     }
   }
 }

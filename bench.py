@@ -346,7 +346,7 @@ def walk_model_bytes(nl, chunk: int, tw: int, seq: bool = True) -> dict:
     nf = tw + 1 if seq else 2 * tw
     nv = (nf + 2) & ~1
     sv = nv + ((tw + 1) & ~1)  # an LDS slot: the record, then the mark words
-    cap = (160 * 1024 - 64) // (8 * sv)
+    cap = (160 * 1024) // (8 * sv)
     rec = 8 * nv
     lo = a["level_off"]
     types, fin_off, fin = a["type"], a["fin_off"], a["fin"]

@@ -770,8 +770,10 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
 // 8 XCDs round-robin, so neighbouring tiles share an XCD and its L2.
 constexpr int kWalkBlock = 1024;
 
-// the node window (the rest of the 160 KiB: the tile's list lengths)
-constexpr uint32_t kWalkLdsBytes = 160 * 1024 - 64;
+// the node window: all 160 KiB of a CU's LDS (with 4-word tiles of
+// consecutive pairs exactly 2,048 slots, so a node's slot is its id's low
+// bits); the reverse walk reuses it, its list lengths in the last 64 bytes
+constexpr uint32_t kWalkLdsBytes = 160 * 1024;
 // records move as 16-byte words (ds_read_b128 / global_load_dwordx4)
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) u64x2 lds_u64x2;
@@ -1521,7 +1523,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *
                                                       uint64_t list_stride) {
   using S = WalkShape<SEQ, TW>;
   __shared__ uint64_t lds_words[S::cap * S::SV];
-  __shared__ uint32_t cnt_words[TW];  // the tile's per-word list lengths
+  static_assert(S::cap * S::SV >= 8 + 2 * TW, "room for the reverse walk's list lengths");
   {
     // the chunk's event and cover counters, each cleared by the block that
     // owns it (blocks are not ordered within the launch, so no block may
@@ -1541,14 +1543,15 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk(DevNet net, const GateRec *
                                  tiles, (lds_u64 *)lds_words);
   __syncthreads();  // the forward walk's marks and bits are visible to the reverse walk
   // the reverse walk stages through the forward walk's LDS when two levels'
-  // accumulators fit
-  constexpr uint32_t nx_cap = S::cap * S::SV / (2 * TW);
+  // accumulators fit (the tile's per-word list lengths in the last 8 words)
+  constexpr uint32_t nx_cap = (S::cap * S::SV - 8) / (2 * TW);
+  lds_u32 *cnt = (lds_u32 *)((lds_u64 *)lds_words + S::cap * S::SV - 8);
   if (widest <= nx_cap)
     walk_rev_staged<SEQ, TW>(net, recs, pl, BT, level_off, levels, tiles, start, first_t0, first_t1, list, list_cnt,
-                             list_stride, (lds_u64 *)lds_words, nx_cap, (lds_u32 *)cnt_words);
+                             list_stride, (lds_u64 *)lds_words, nx_cap, cnt);
   else
     walk_rev_body<SEQ, TW, false>(net, recs, pl, BT, level_off, levels, tiles, start, first_t0, first_t1, list,
-                                  list_cnt, list_stride, (lds_u32 *)cnt_words);
+                                  list_cnt, list_stride, cnt);
 }
 
 // Cross-chunk state to its initial value (one launch per grade).

@@ -263,14 +263,14 @@ def test_bench_walk_byte_model_counts_far_fanins():
     counts as outside the walk's LDS window, and the nodes whose records are
     written for them, equal a direct loop over the gates (fanin 0, or any
     fanin of AND/NAND/OR/NOR, below the node's level end minus the
-    2,047-node window of 80-byte slots), and the byte totals follow."""
+    2,048-node window of 80-byte slots), and the byte totals follow."""
     import bench
 
     nl = pg.Netlist.generate(GATES_50K.netlist)
     a = nl.arrays()
     m = bench.walk_model_bytes(nl, 65536, 4)
     cap = m["window_nodes"]
-    assert cap == (160 * 1024 - 64) // 80 == 2047 and m["record_bytes"] == 48 and m["tiles"] == 256
+    assert cap == (160 * 1024) // 80 == 2048 and m["record_bytes"] == 48 and m["tiles"] == 256
     far = 0
     kept = set()
     lo = a["level_off"]

@@ -3066,7 +3066,20 @@ void free_rank(pdf_grader *g) {
   g->rk_pairs = g->rk_vsize = g->rk_temp_bytes = 0;
 }
 
+// Waits for a stream's queued work unless the stream is being captured
+// (then nothing of it runs yet).
+void drain(hipStream_t s) {
+  if (!s) return;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusNone) (void)hipStreamSynchronize(s);
+}
+
+// Destroys the captured grades, after every lane's stream has finished what
+// it queued: an executable graph destroyed while a stream still runs it
+// faults the host inside the runtime.
 void drop_graph(pdf_grader *g) {
+  drain(g->stream);
+  for (auto &l : g->lanes) drain(l.stream);
   if (g->graph) (void)hipGraphExecDestroy(g->graph);
   g->graph = nullptr;
   g->graph_robust = -1;

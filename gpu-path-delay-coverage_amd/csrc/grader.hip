@@ -89,6 +89,10 @@ struct Planes {
   uint32_t ntiles;     // NZ words per node
   uint64_t last_mask;  // valid pairs of the chunk's last word
   uint32_t merge_window = 0;  // 0: merge every pair of the chunk; else only its first merge_window pairs
+  // after a level walk the walk's mark-present bytes stand in for NZ: bit
+  // (w & (2^bt_shift - 1)) of bt[(w >> bt_shift) * bt_n + f] for word w of node f
+  const uint8_t *bt = nullptr;
+  uint32_t bt_shift = 0, bt_n = 0;
 };
 
 // Pairs of word w that the merge may take as first detections: all of them,
@@ -168,8 +172,10 @@ __device__ __forceinline__ void frames(const Planes &pl, uint32_t f, uint32_t w,
 
 // Mark word of node f (zero unless written in this chunk).
 __device__ __forceinline__ uint64_t mark_of(const Planes &pl, uint32_t f, uint32_t w) {
-  const uint64_t nz = pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)];
-  return ((nz >> (w & 63)) & 1) ? pl.M[(uint64_t)f * pl.pitch + w] : 0;
+  const uint32_t present =
+      pl.bt ? (uint32_t)(pl.bt[(uint64_t)(w >> pl.bt_shift) * pl.bt_n + f] >> (w & ((1u << pl.bt_shift) - 1)))
+            : (uint32_t)(pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)] >> (w & 63));
+  return (present & 1) ? pl.M[(uint64_t)f * pl.pitch + w] : 0;
 }
 
 // Off-path sensitization predicate S(on, off) of markkernel.cuh:36-41 in
@@ -763,9 +769,9 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
 // level writes the slots of ids >= hi - cap only (one writer per slot).
 // A record's last word holds the tile's mark-present bits of the node (bit u:
 // a mark at word u), so a gate reads fanin 0's with its frames.  Marks also
-// go to the node-major M / NZ planes of the level kernels for the reverse
-// pass and the cover; an NZ word spans 64 / TW tiles, so the forward walk
-// sets its bits with atomicOr on a zeroed plane and never reads it.
+// go to the node-major M plane of the level kernels for the reverse pass
+// and the cover, and the bits to a compact byte plane BT[tile][n] that
+// both read in place of the NZ plane (one writer per byte: nothing to zero).
 // Workgroup b takes tile (b % 8) * per_xcd + b / 8: blocks are dealt to the
 // 8 XCDs round-robin, so neighbouring tiles share an XCD and its L2.
 constexpr int kWalkBlock = 1024;
@@ -1054,16 +1060,12 @@ __device__ __forceinline__ void walk_fwd_gate(const DevNet &net, const GateRec &
     for (int q = 0; q < S::NMK / 2; ++q)
       slot[NV / 2 + q] = u64x2{2 * q < TW ? m[2 * q] : 0, 2 * q + 1 < TW ? m[2 * q + 1] : 0};
   }
-  if (bits)
-    atomicOr((unsigned long long *)&pl.NZ[(uint64_t)g * pl.ntiles + (w0 >> 6)],
-             (unsigned long long)(bits << (w0 & 63)));
 }
 
 // Forward walk: levels 0..levels-1 of one tile, sim + forward mark.  recs
 // are the records in the walk order (walk_order: each level's gates sorted
 // by kind, fout_off holding the id).  VT is the tile-major value plane
-// ([tiles][n][NV] words).  The NZ plane must be all zero (a fill kernel
-// runs first).
+// ([tiles][n][NV] words), BT the mark-present bytes ([tiles][n]).
 template <bool SEQ, bool ROBUST, int TW>
 __device__ __forceinline__ void walk_fwd_body(const DevNet &net, const GateRec *__restrict__ recs, const Planes &pl,
                                               const uint64_t *__restrict__ va, const uint64_t *__restrict__ vb,
@@ -2542,8 +2544,11 @@ struct pdf_grader {
   int64_t dbg_fwd_chunk = -1, dbg_pic_chunk = -1;
   uint64_t *pic_target = nullptr;  // d_dbg_pic while pdf_cover runs with the debug flag
   uint64_t *d_gate = nullptr;
-  uint64_t *d_grand = nullptr;  // exact total over all chunks
+  // exact total over all chunks and the error flags, one allocation
+  // (d_err points at word 1) read back in one copy into h_result (pinned)
+  uint64_t *d_grand = nullptr;
   uint32_t *d_err = nullptr;
+  uint64_t *h_result = nullptr;
   uint32_t *d_list = nullptr, *d_act_list = nullptr;
   // per-chunk counters in one zeroed u32 block: actmap (u64 per word),
   // list_cnt (per word), act_count, chunk total (u64)
@@ -2748,6 +2753,7 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
 template <bool S, bool M>
 int launch_fwd(pdf_grader *g, uint64_t start, uint64_t width, int robust, bool clear_state = false,
                uint32_t last_level = 0xFFFFFFFFu, uint32_t k_lo = 0, uint32_t k_hi = 0xFFFFFFFFu) {
+  g->walk_planes = 0;  // M and NZ now hold this pass's marks
   if (g->seq) launch_fwd_t<true, S, M>(g, start, width, robust, clear_state, last_level, k_lo, k_hi);
   else launch_fwd_t<false, S, M>(g, start, width, robust, false, last_level, k_lo, k_hi);
   HIPCHK(hipGetLastError());
@@ -2852,13 +2858,18 @@ int launch_rev_walk(pdf_grader *g, uint64_t start, uint64_t width, uint32_t tw) 
   return PDF_OK;
 }
 
+// The level walk's mark-present bytes BT[tile][n] in the lane's walk
+// buffer: after the records of every tile of a full chunk.
+uint8_t *walk_bits(const pdf_grader *g, uint32_t tw) {
+  const uint64_t tiles_alloc = ((g->chunk_pairs + 63) / 64 + tw - 1) / tw;
+  return reinterpret_cast<uint8_t *>(g->d_VT + (uint64_t)g->nl.n * tiles_alloc * walk_record_words(g->seq, tw));
+}
+
 template <bool SEQ, bool R, int TW>
 void launch_walk_t(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tiles) {
   const DevNet net = g->net();
   const uint32_t grid = 8 * ((tiles + 7) / 8);  // walk_tile skips the blocks past the last tile
-  // the mark-present bytes follow the records of the lane's full chunk
-  const uint64_t tiles_alloc = ((g->chunk_pairs + 63) / 64 + TW - 1) / TW;
-  uint8_t *bt = reinterpret_cast<uint8_t *>(g->d_VT + (uint64_t)g->nl.n * tiles_alloc * walk_record_words(SEQ, TW));
+  uint8_t *bt = walk_bits(g, TW);
   tm_begin(g, kFwd);
   k_walk<SEQ, R, TW><<<grid, kWalkBlock, 0, g->stream>>>(net, g->d_rec, g->d_wrec, pl, g->d_va, g->d_vb, g->vstride, start / 64,
                                                           g->d_VT, bt, g->walk_full, g->d_level_off, g->nl.L + 1,
@@ -2880,16 +2891,10 @@ void launch_walk_tw(pdf_grader *g, uint64_t start, const Planes &pl, uint32_t tw
   }
 }
 
-// The forward and reverse passes of one chunk as two level walks (words per
-// tile = g->tune_walk), after zeroing the mark-present plane the forward
-// walk ORs into.  The forward walk also zeroes the per-chunk counters.
+// The forward and reverse passes of one chunk in one level-walk launch
+// (words per tile = g->tune_walk), which also zeroes the per-chunk counters.
 int launch_walk(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
   const Planes pl = planes(g, width);
-  const uint64_t nz_words = (uint64_t)g->nl.n * g->ntiles * 2;
-  tm_begin(g, kFill);
-  k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nz_words, 256), 65536), 256, 0, g->stream>>>(
-      reinterpret_cast<uint32_t *>(g->d_NZ), nz_words, 0u);
-  tm_end(g);
   const uint32_t tw = g->tune_walk;
   g->walk_planes = tw;
   // every record is kept only for the sim-plane getters (debug flags set)
@@ -2919,7 +2924,12 @@ int launch_passes(pdf_grader *g, uint64_t start, uint64_t width, int robust) {
 }
 
 int launch_cover(pdf_grader *g, uint64_t start, uint64_t width, uint32_t *pair_total = nullptr) {
-  const Planes pl = planes(g, width);
+  Planes pl = planes(g, width);
+  if (g->walk_planes) {  // the walk's mark-present bytes, not NZ
+    pl.bt = walk_bits(g, g->walk_planes);
+    pl.bt_shift = (uint32_t)__builtin_ctz(g->walk_planes);
+    pl.bt_n = g->nl.n;
+  }
   tm_begin(g, kEvents);
   k_events<<<grid_for(g->nl.n, 256), 256, 0, g->stream>>>(g->nl.n, g->d_first0, g->d_first1, (uint32_t)start,
                                                          (uint32_t)width, g->d_actmap, g->d_act_list, g->d_act_count);
@@ -3358,8 +3368,14 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   if (!rc) rc = dalloc(&g->d_first0, nl.n);
   if (!rc) rc = dalloc(&g->d_first1, nl.n);
   if (!rc) rc = dalloc(&g->d_gate, nl.n);
-  if (!rc) rc = dalloc(&g->d_grand, 1);
-  if (!rc) rc = dalloc(&g->d_err, 1);
+  if (!rc) rc = dalloc(&g->d_grand, 2);
+  if (!rc) g->d_err = reinterpret_cast<uint32_t *>(g->d_grand + 1);
+  if (!rc && hipHostMalloc((void **)&g->h_result, 2 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    g->h_result = nullptr;
+    set_error("hipHostMalloc of the result words failed");
+    rc = PDF_ERR_NOMEM;
+  }
   if (!rc) rc = reset_state(g);
   if (rc) { pdf_grader_destroy(g); return rc; }
   *out = g;
@@ -3389,7 +3405,8 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_first1);
   dfree(g->d_gate);
   dfree(g->d_grand);
-  dfree(g->d_err);
+  g->d_err = nullptr;  // word 1 of d_grand
+  if (g->h_result) (void)hipHostFree(g->h_result);
   for (hipEvent_t e : g->ev_pool) (void)hipEventDestroy(e);
   if (g->lane0_done) (void)hipEventDestroy(g->lane0_done);
   if (g->lane_start) (void)hipEventDestroy(g->lane_start);
@@ -3866,11 +3883,10 @@ static int grade_impl(pdf_grader *g, int robust, uint64_t *total) {
     if ((rc = queue_grade(g, robust))) return rc;
     tm_close(g);
   }
-  uint64_t tot = 0;
-  uint32_t err = 0;
-  HIPCHK(hipMemcpyAsync(&tot, g->d_grand, sizeof(tot), hipMemcpyDeviceToHost, g->stream));
-  HIPCHK(hipMemcpyAsync(&err, g->d_err, sizeof(err), hipMemcpyDeviceToHost, g->stream));
+  HIPCHK(hipMemcpyAsync(g->h_result, g->d_grand, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream));
   HIPCHK(hipStreamSynchronize(g->stream));
+  const uint64_t tot = g->h_result[0];
+  const uint32_t err = (uint32_t)g->h_result[1];  // d_err is the low half of word 1
   g->stage = 4;
   g->stage_chunk = (int64_t)nchunks - 1;
   g->stage_robust = robust;
@@ -4420,9 +4436,18 @@ int pdf_get_mark_plane(const pdf_grader *gc, uint64_t *mark_plane) {
   HIPCHK(hipSetDevice(g->device));
   const uint64_t width = std::min(g->chunk_pairs, g->num_pairs - (uint64_t)g->stage_chunk * g->chunk_pairs);
   const uint64_t words = (width + 63) / 64, n = g->nl.n;
-  std::vector<uint64_t> nz(n * g->ntiles);
   int rc = copy_rows(g, mark_plane, g->d_M, words);
   if (rc) return rc;
+  if (g->walk_planes) {  // the walk's mark-present bytes BT[tile][n], bit k % tw of tile k / tw
+    const uint64_t tw = g->walk_planes, tiles = (words + tw - 1) / tw;
+    std::vector<uint8_t> bt(n * tiles);
+    HIPCHK(hipMemcpy(bt.data(), walk_bits(g, (uint32_t)tw), bt.size(), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i)
+      for (uint64_t k = 0; k < words; ++k)
+        if (!((bt[(k / tw) * n + i] >> (k % tw)) & 1)) mark_plane[i * words + k] = 0;
+    return PDF_OK;
+  }
+  std::vector<uint64_t> nz(n * g->ntiles);
   HIPCHK(hipMemcpy(nz.data(), g->d_NZ, nz.size() * 8, hipMemcpyDeviceToHost));
   for (uint64_t i = 0; i < n; ++i)
     for (uint64_t k = 0; k < words; ++k)

@@ -146,7 +146,8 @@ def test_walk_sim_planes_after_a_grade(tw):
     LDS window, so after a fused walk grade the sim-plane getter refuses;
     with a debug flag set before the grade every record is kept and the
     last chunk's planes equal the oracle's codes.  Results are the same
-    either way."""
+    either way.  The final mark plane (masked by the walk's per-tile
+    mark-present bytes) equals the oracle's marks in both cases."""
     from oracle import oracle as orc
 
     nl = pg.Netlist.generate(small_config(21, num_pi=12, num_po=10, num_gates=700, levels=9, wide=True,
@@ -160,11 +161,14 @@ def test_walk_sim_planes_after_a_grade(tw):
         g.set_chunk(chunk)
         g.set_tuning(pg.TUNE_WALK, tw)
         say("walk planes", tw)
+        codes = orc.sim_chunk(net, bits, P, P - chunk, chunk)
+        marks, _ = orc.mark_chunk(net, codes, True)
         assert g.grade(True) == ref["total"]
         with pytest.raises(pg.PdfError):
             g.sim_planes(chunk)
+        assert np.array_equal(pg.plane_to_bytes(g.mark_plane(chunk), chunk), marks)
         g.set_debug(pg.DEBUG_FORWARD_MARKS)
         assert g.grade(True) == ref["total"]
         a, b = g.sim_planes(chunk)
-        codes = orc.sim_chunk(net, bits, P, P - chunk, chunk)
         assert np.array_equal(pg.planes_to_codes(a, b, chunk), codes)
+        assert np.array_equal(pg.plane_to_bytes(g.mark_plane(chunk), chunk), marks)

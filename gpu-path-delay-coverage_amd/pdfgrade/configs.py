@@ -40,7 +40,7 @@ GATES_50K = Workload(
     netlist=GenConfig(seed=2, num_pi=256, num_po=128, num_gates=50_000, levels=60, fanin_p=WIDE_FANIN,
                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64),
     # the level walk, one 65,536-pair chunk in 4-word tiles (256 workgroups,
-    # one per CU): 0.68 ms a grade on an MI355X against 0.75-0.77 ms for
+    # one per CU): 0.54-0.55 ms a grade on an MI355X against 0.75-0.77 ms for
     # per-level launches on 2 lanes x 32,768 pairs (graph replay), the
     # fastest configuration without the walk
     vector_seed=102, pairs=65_536, chunk=65_536, lanes=1, use_graph=1, walk=4)

@@ -406,11 +406,13 @@ def setup_grader(pg, nl, bits, wl, robust, autotune):
     g.set_tuning(pg.TUNE_WORDS_PER_LANE, wl.words_per_lane)
     g.set_tuning(pg.TUNE_USE_GRAPH, wl.use_graph)
     g.set_tuning(pg.TUNE_WALK, wl.walk)
+    g.set_tuning(pg.TUNE_REV_COMPACT, wl.rev_compact)
     if autotune:
         tune = g.autotune(robust, reps=5)
     else:
         tune = {"lanes": wl.lanes, "chunk_pairs": wl.chunk, "use_graph": wl.use_graph,
-                "words_per_lane": wl.words_per_lane, "walk": wl.walk, "source": "recorded"}
+                "words_per_lane": wl.words_per_lane, "walk": wl.walk, "rev_compact": wl.rev_compact,
+                "source": "recorded"}
     return g, tune
 
 

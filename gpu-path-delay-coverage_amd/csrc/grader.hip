@@ -120,13 +120,33 @@ constexpr uint32_t kErrRefU32 = 4u;  // a chunk total past the reference's uint3
 // its fanin rows are re-read from that XCD's L2.  Blocks from `main_blocks`
 // on (sequence-mode simulation only) compute the extra column `words`, one
 // gate per thread.
+// A (gate, 64-word tile) item of the compacted reverse pass: the gate, the
+// tile and its mark-present bits (bit u: a forward mark at word 64 tile + u).
+struct RevTile {
+  uint32_t gate, tile;
+  uint64_t bits;
+};
+
 struct LevelMap {
   uint32_t T;      // word tiles
   uint32_t twlog;  // log2 tile width in words (<= 6)
   uint32_t words;  // words in the chunk
   uint32_t lo, count;
   uint32_t main_blocks;
+  // the forward pass appends the level's marked (gate, 64-word tile) items
+  // (PDF_TUNE_REV_COMPACT; one gate per wave only) to items_nsub sub-lists,
+  // tile t to sub-list t % items_nsub, each of items_cap entries at
+  // items + sub * items_cap, its count at items_cnt[sub * kRevCntStride]
+  RevTile *items = nullptr;
+  uint32_t *items_cnt = nullptr;
+  uint32_t items_cap = 0, items_nsub = 1;
 };
+
+// Sub-lists per level of the compacted reverse pass, at most: the forward
+// pass's appends spread over one counter per tile (tiles past 64 share),
+// each in a 64-byte line of its own, so they do not queue on one address.
+constexpr uint32_t kRevSubLists = 64;
+constexpr uint32_t kRevCntStride = 16;  // u32 words between counters
 
 __device__ __forceinline__ bool map_item(const LevelMap &mp, uint32_t b, uint32_t &g, uint32_t &w) {
   const uint32_t grp = b / mp.T, tile = b - grp * mp.T;
@@ -511,15 +531,28 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
   if (DO_MARK) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t tw = 1u << mp.twlog;
+    uint64_t tile_bits[NW];  // one gate per wave: the wave's NZ words
 #pragma unroll
     for (int u = 0; u < NW; ++u) {
       const uint32_t wu = w + 64 * u;
       if (live[u] && m[u]) pl.M[(uint64_t)g * pl.pitch + wu] = m[u];
       // one NZ word per (gate, 64-word tile): bits from this gate's lanes
       const uint64_t bal = __ballot(live[u] && m[u] != 0);
+      tile_bits[u] = bal;
       if ((lane & (tw - 1)) == 0 && live[u]) {
         const uint64_t bits = mp.twlog == 6 ? bal : (bal >> lane) & ((1ULL << tw) - 1);
         pl.NZ[(uint64_t)g * pl.ntiles + (wu >> 6)] = bits;
+      }
+    }
+    if (WIDE && mp.items && lane == 0) {
+      // the compacted reverse pass's items (lane 0 holds the wave's first
+      // word, so (w + 64 u) / 64 is tile u's index)
+#pragma unroll
+      for (int u = 0; u < NW; ++u) {
+        if (!tile_bits[u]) continue;
+        const uint32_t t = (w + 64u * u) >> 6, sub = t % mp.items_nsub;
+        const uint32_t pos = atomicAdd(mp.items_cnt + sub * kRevCntStride, 1u);
+        mp.items[(uint64_t)sub * mp.items_cap + pos] = RevTile{g, t, tile_bits[u]};
       }
     }
   }
@@ -743,6 +776,53 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
     const uint32_t w = (tg * RT + k) * 64 + lane;
     if (((nz[k] >> lane) & 1) && w < mp.words)
       rev_marked<SEQ, DO_REMOVE, DO_MERGE>(net, r, pl, g, w, start, first_t0, first_t1, list, list_cnt, list_stride);
+  }
+}
+
+// Compacted reverse pass over one level (PDF_TUNE_REV_COMPACT): the
+// (gate, 64-word tile) items holding a forward mark, appended by the forward
+// pass (RevTile, in no particular order: the items of a level are
+// independent), one item per wave and iteration, striding over the count.
+// Where almost every item is unmarked (2M gates: ~97%) this dispatches waves
+// only for the marked ones, and their mark-present bits come with the item.
+template <bool SEQ, bool DO_REMOVE, bool DO_MERGE>
+__global__ __launch_bounds__(kBlock) void k_reverse_list(DevNet net, const GateRec *__restrict__ recs, Planes pl,
+                                                         const RevTile *__restrict__ items,
+                                                         const uint32_t *__restrict__ items_cnt, uint32_t items_cap,
+                                                         uint32_t start, uint32_t *__restrict__ first_t0,
+                                                         uint32_t *__restrict__ first_t1,
+                                                         uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
+                                                         uint64_t list_stride) {
+  static_assert(kRevSubLists == 64, "one sub-list count per lane");
+  constexpr uint32_t wpb = kBlock / 64;
+  const uint32_t lane = threadIdx.x & 63;
+  // the sub-lists' counts (those past the level's sub-lists stay zero) and
+  // their exclusive prefix sums, lane s: sub-list s
+  const uint32_t c = items_cnt[lane * kRevCntStride];
+  uint32_t incl = c;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += o;
+  }
+  const uint32_t pre = incl - c, cnt = __shfl(incl, 63);
+  for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < cnt; i += gridDim.x * wpb) {
+    // item i lies in the last sub-list whose prefix is <= i (a later one
+    // starts past i; empty ones share their successor's prefix)
+    const uint32_t sub = 63 - (uint32_t)__builtin_clzll(__ballot(pre <= i));
+    const uint32_t off = i - (uint32_t)__shfl(pre, (int)sub);
+    const RevTile it = items[(uint64_t)sub * items_cap + off];
+    const uint32_t g = __builtin_amdgcn_readfirstlane(it.gate);
+    const uint32_t t = __builtin_amdgcn_readfirstlane(it.tile);
+    const GateRec r = recs[g];
+    if (DO_REMOVE) {
+      pin(r.fin[0]);
+      pin((uint32_t)r.nfi);
+      pin((uint32_t)r.po);
+    }
+    // bits are set only for words of the chunk
+    if ((it.bits >> lane) & 1)
+      rev_marked<SEQ, DO_REMOVE, DO_MERGE>(net, r, pl, g, t * 64 + lane, start, first_t0, first_t1, list, list_cnt,
+                                           list_stride);
   }
 }
 
@@ -2567,6 +2647,12 @@ struct pdf_grader {
   // tuning knobs (pdf_grader_set_tuning)
   uint32_t tune_words_per_lane = 1;
   uint32_t tune_rev_tiles = 0;  // PDF_TUNE_REV_TILES (0: by tile count)
+  uint32_t tune_rev_compact = 0;  // PDF_TUNE_REV_COMPACT: reverse levels over the forward pass's item lists
+  // the compacted reverse pass's items of the lane's chunk, level k >= 1 at
+  // (level_off[k] - level_off[1]) * ntiles, and their per-level counts
+  RevTile *d_rtl = nullptr;
+  uint32_t *d_rtl_cnt = nullptr;
+  bool rev_lists = false;  // the last forward pass queued on this lane built the item lists
   uint32_t tune_use_graph = 1;
   uint32_t tune_walk = 0;  // PDF_TUNE_WALK: 0 level launches, else words per tile of the level walk
   uint32_t walk_full = 0;  // the last walk wrote every node's record (debug flags were set)
@@ -2583,6 +2669,9 @@ struct pdf_grader {
     uint32_t *list_cnt = nullptr, *act_count = nullptr;
     uint64_t *cs = nullptr, *hs = nullptr;
     uint64_t *NEW = nullptr;    // new-node plane (segments of length K >= 2)
+    RevTile *rtl = nullptr;
+    uint32_t *rtl_cnt = nullptr;
+    bool rev_lists = false;
     hipEvent_t done = nullptr;  // merges of this lane's latest chunk and all before it
   };
   std::vector<Lane> lanes;      // lanes 1..; empty for one lane
@@ -2699,6 +2788,28 @@ LevelMap level_map(uint32_t words, uint32_t lo, uint32_t count, bool extra_colum
   return mp;
 }
 
+// Blocks of a compacted reverse launch: enough waves for a level's marked
+// items in one or two strides (2M gates: ~10k marked items of ~320k).
+constexpr uint64_t kRevListBlocks = 2048;
+
+// The compacted reverse pass's sub-lists for chunks of `ntiles` 64-word
+// tiles, the entries they hold per gate of a level (a level of `count`
+// gates: sub-lists of count x ceil(ntiles / nsub) entries), and the
+// placement of level k's lists and counters.
+inline uint32_t rev_nsub(uint64_t ntiles) { return (uint32_t)std::min<uint64_t>(ntiles, kRevSubLists); }
+inline uint64_t rev_slots_per_gate(uint64_t ntiles) {
+  const uint64_t ns = rev_nsub(ntiles);
+  return ns * ((ntiles + ns - 1) / ns);
+}
+void rev_lists_of(const pdf_grader *g, uint32_t k, LevelMap &mp) {
+  const uint64_t ns = rev_nsub(g->ntiles);
+  const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
+  mp.items = g->d_rtl + rev_slots_per_gate(g->ntiles) * (lo - g->nl.level_off[1]);
+  mp.items_cnt = g->d_rtl_cnt + (uint64_t)k * kRevSubLists * kRevCntStride;
+  mp.items_cap = (uint32_t)(count * ((g->ntiles + ns - 1) / ns));
+  mp.items_nsub = (uint32_t)ns;
+}
+
 // Words per lane for the fused sequence-mode pass (tuning knob, default 1):
 // two only when the chunk splits into 128-word tiles whose count is a
 // multiple of 8 (each tile keeps its XCD).
@@ -2722,6 +2833,16 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
   const uint32_t nw = words_per_lane(g->tune_words_per_lane, SEQ, S, M, width);
+  // the compacted reverse pass's item lists: built by a marking pass over
+  // every level whose waves hold one gate each (chunks of >= 64 words)
+  g->rev_lists = M && g->d_rtl && k_lo == 0 && k_hi >= g->nl.L && last_level >= g->nl.L && pl.words >= 64 &&
+                 g->nl.L >= 1;
+  if (g->rev_lists) {
+    const uint64_t nc = ((uint64_t)g->nl.L + 1) * kRevSubLists * kRevCntStride;
+    tm_begin(g, kFill);
+    k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nc, 256), 1024), 256, 0, g->stream>>>(g->d_rtl_cnt, nc, 0u);
+    tm_end(g);
+  }
   for (uint32_t k = k_lo; k <= std::min(std::min(g->nl.L, last_level), k_hi); ++k) {
     uint32_t grid;
     const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
@@ -2731,6 +2852,7 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
       mp.main_blocks = mp.T * ((count + (kBlock >> 6) - 1) / (kBlock >> 6));
       grid = mp.main_blocks + grid_for(count, kBlock);
     }
+    if (g->rev_lists && k >= 1) rev_lists_of(g, k, mp);  // level 0's reverse pass stays dense
     const bool last = k == std::min(g->nl.L, last_level) && SEQ && S && clear_state;
     uint32_t *clear = last ? g->d_cstate : nullptr;
     const uint32_t nclear = last ? g->n_cstate : 0;
@@ -2784,6 +2906,18 @@ void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_
     const LevelMap mp =
         level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], false, &grid);
     tm_begin(g, kRev);
+    if (g->rev_lists && k >= 1) {
+      // at most every (gate, tile) item of the level; waves stride over the count
+      const uint32_t lgrid = (uint32_t)std::min<uint64_t>(
+          kRevListBlocks, std::max<uint64_t>(1, ((uint64_t)mp.count * mp.T + kBlock / 64 - 1) / (kBlock / 64)));
+      LevelMap lm = mp;
+      rev_lists_of(g, k, lm);
+      k_reverse_list<SEQ, R, M><<<lgrid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, lm.items, lm.items_cnt,
+                                                                 lm.items_cap, (uint32_t)start, g->d_first0,
+                                                                 g->d_first1, g->d_list, g->d_list_cnt, g->nl.n);
+      tm_end(g);
+      continue;
+    }
     const uint32_t rt = mp.twlog == 6 ? rev_tiles(g->tune_rev_tiles, mp.T, mp.count) : 1;
     const uint32_t rgrid = ((mp.count + kBlock / 64 - 1) / (kBlock / 64)) * ((mp.T + rt - 1) / rt);
     if (rt == 8)
@@ -3116,6 +3250,9 @@ void lane_save(const pdf_grader *g, pdf_grader::Lane &l) {
   l.cs = g->d_cs;
   l.hs = g->d_hs;
   l.NEW = g->d_NEW;
+  l.rtl = g->d_rtl;
+  l.rtl_cnt = g->d_rtl_cnt;
+  l.rev_lists = g->rev_lists;
 }
 
 void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
@@ -3136,6 +3273,9 @@ void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
   g->d_cs = l.cs;
   g->d_hs = l.hs;
   g->d_NEW = l.NEW;
+  g->d_rtl = l.rtl;
+  g->d_rtl_cnt = l.rtl_cnt;
+  g->rev_lists = l.rev_lists;
 }
 
 void free_chunk_buffers(pdf_grader *g);
@@ -3194,6 +3334,13 @@ int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs) {
   if (!rc) rc = dalloc(&g->d_act_list, chunk_pairs);
   if (!rc) rc = dalloc(&g->d_cs, n * g->cover_waves);
   if (!rc) rc = dalloc(&g->d_hs, n * g->cover_waves);
+  if (!rc && g->tune_rev_compact) {
+    // room for every (gate, 64-word tile) item above level 0, and each
+    // level's sub-list counters
+    rc = dalloc(&g->d_rtl, (n - g->nl.level_off[1]) * rev_slots_per_gate(g->ntiles));
+    if (!rc) rc = dalloc(&g->d_rtl_cnt, ((uint64_t)g->nl.L + 1) * kRevSubLists * kRevCntStride);
+  }
+  g->rev_lists = false;
   return rc;
 }
 
@@ -3214,6 +3361,9 @@ void free_chunk_buffers(pdf_grader *g) {
   dfree(g->d_act_list);
   dfree(g->d_cs);
   dfree(g->d_hs);
+  dfree(g->d_rtl);
+  dfree(g->d_rtl_cnt);
+  g->rev_lists = false;
 }
 
 // Queue the reset of the cross-chunk state (no host synchronization).
@@ -3986,6 +4136,12 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
       }
       g->tune_rev_tiles = (uint32_t)value;
       break;
+    case PDF_TUNE_REV_COMPACT:
+      if (value != 0 && value != 1) { set_error("compacted reverse pass must be 0 or 1"); return PDF_ERR_ARG; }
+      if (g->tune_rev_compact == (uint32_t)value) break;
+      g->tune_rev_compact = (uint32_t)value;
+      // the item lists are chunk buffers
+      return g->chunk_pairs ? pdf_grader_set_chunk(g, g->chunk_pairs) : PDF_OK;
     default:
       set_error("unknown tuning key");
       return PDF_ERR_ARG;
@@ -4335,6 +4491,8 @@ int pdf_grade_noalloc(pdf_grader *g, int robust, void *stream, uint64_t *total_d
   g->n_cstate = p.n_cstate;
   g->cover_waves = p.waves;
   g->d_VT = nullptr;  // no tile-major plane in the scratch: per-level launches
+  g->d_rtl = nullptr;  // nor item lists: dense reverse launches
+  g->d_rtl_cnt = nullptr;
   g->grade_segs = 1;
   g->one_lane = true;
   // the removal accumulator must start at zero and the array keeps whatever
@@ -4858,6 +5016,9 @@ extern "C" int pdf_grader_autotune(pdf_grader *g, int robust, uint32_t reps, pdf
     double bytes = n * (8.0 * pitch * (g->seq ? 3 : 4) + 8.0 * ntiles + 4.0 * words) + 4.0 * chunk + 12.0 * words +
                    waves * per_wave;
     if (walk) bytes += 8.0 * (double)walk_plane_words(g->seq, walk, g->nl.n, words);
+    if (g->tune_rev_compact)  // the compacted reverse pass's item lists
+      bytes += (double)sizeof(RevTile) * (n - g->nl.level_off[1]) * rev_slots_per_gate(ntiles) +
+               4.0 * kRevSubLists * kRevCntStride * (g->nl.L + 1);
     return bytes;
   };
   struct Cand { uint32_t lanes; uint64_t chunk; uint32_t walk; };

@@ -422,8 +422,9 @@ class Grader:
         _check(lib().pdf_grader_reset(self._h))
 
     def set_tuning(self, key: int, value: int):
-        """TUNE_WORDS_PER_LANE (1/2), TUNE_USE_GRAPH (0/1), TUNE_LANES (1..4) or
-        TUNE_REV_TILES (0 automatic, 1/2/4/8); results never change."""
+        """TUNE_WORDS_PER_LANE (1/2), TUNE_USE_GRAPH (0/1), TUNE_LANES (1..4),
+        TUNE_REV_TILES (0 automatic, 1/2/4/8), TUNE_REV_COMPACT (0/1), TUNE_WALK
+        and TUNE_REV_WALK (0/1/2/4/8) or the cover caps; results never change."""
         _check(lib().pdf_grader_set_tuning(self._h, key, value))
 
     def autotune(self, robust: bool, reps: int = 5) -> dict:

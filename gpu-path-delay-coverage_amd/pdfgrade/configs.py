@@ -27,6 +27,7 @@ class Workload:
     words_per_lane: int = 1  # PDF_TUNE_WORDS_PER_LANE (2 needs chunks of a multiple of 65,536 pairs)
     use_graph: int = 1  # PDF_TUNE_USE_GRAPH: replay a captured graph (1) or launch directly (0)
     walk: int = 0  # PDF_TUNE_WALK: words per level-walk tile (0: one launch per level and pass)
+    rev_compact: int = 0  # PDF_TUNE_REV_COMPACT: reverse levels over the forward pass's marked-item lists
 
 
 TINY = Workload(
@@ -52,9 +53,11 @@ GATES_2M = Workload(
     # overflow-checked, so no limit is applied here
     netlist=GenConfig(seed=4, num_pi=4096, num_po=2048, num_gates=2_000_000, levels=200, fanin_p=WIDE_FANIN,
                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64, path_limit_log2=0),
-    # 2 lanes x 131,072-pair chunks x 2 words per lane: 204 ms a grade on an
-    # MI355X against 291 ms for 1 lane x 32,768 x 1 (the chunk buffers take 232 GB)
-    vector_seed=104, pairs=1_048_576, chunk=131_072, lanes=2, words_per_lane=2)
+    # 2 lanes x 131,072-pair chunks x 2 words per lane, the compacted reverse
+    # pass: 166-172 ms a grade on an MI355X against 178-186 ms with dense
+    # reverse launches and 291 ms for 1 lane x 32,768 x 1 (the chunk buffers
+    # take 234 GB)
+    vector_seed=104, pairs=1_048_576, chunk=131_072, lanes=2, words_per_lane=2, rev_compact=1)
 
 
 @dataclass(frozen=True)

@@ -323,7 +323,10 @@ uint64_t pdf_ref_unmerged_segments(uint64_t segments);
 #define PDF_TUNE_COVER_WAVES 3    /* batch: waves of the cover launch */
 #define PDF_TUNE_SEGMENT_ORDER 4  /* batch: 0 circuit order, 1 largest segments first */
 #define PDF_TUNE_LANES 5          /* grader: chunks graded concurrently on 1..4 streams */
-#define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items */
+#define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items;
+                                   * grader (per-level launches): 1 the forward pass appends each level's
+                                   * marked (gate, 64-word tile) items to a list and the reverse launches of
+                                   * levels >= 1 run over those lists only */
 #define PDF_TUNE_REV_TILES 7      /* grader: 64-word tiles per reverse-pass wave, 1/2/4/8 (0: by waves per launch) */
 #define PDF_TUNE_WALK 8           /* grader: 0 per-level launches (the reference's host level loops); 1/2/4/8 fused
                                    * grades run each chunk's forward and reverse passes as one launch each, a

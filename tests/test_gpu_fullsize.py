@@ -1,7 +1,8 @@
 """Parity at the workloads' full sizes.
 
 * 2,000,000-gate / 200-level netlist, all 1,048,576 pairs: the recorded
-  tuning (2 lanes x 131,072-pair chunks x 2 words per lane) and 32,768-pair
+  tuning (2 lanes x 131,072-pair chunks x 2 words per lane, the compacted
+  reverse pass) and 32,768-pair
   blocks, each against the bit-sliced CPU grader on every host thread
   (total, first-detection pairs, per-gate counts: u64 accumulation across
   all blocks).
@@ -55,6 +56,7 @@ def test_2m_gates_all_pairs_against_bitsliced():
         g.set_chunk(GATES_2M.chunk)
         g.set_tuning(pg.TUNE_LANES, GATES_2M.lanes)
         g.set_tuning(pg.TUNE_WORDS_PER_LANE, GATES_2M.words_per_lane)
+        g.set_tuning(pg.TUNE_REV_COMPACT, GATES_2M.rev_compact)
         tuned = state(g, g.grade(True))
         # the reference's merge launches reach 1,310,700 of the 2,004,096
         # nodes (mergekernel.cuh:243)

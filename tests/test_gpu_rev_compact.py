@@ -1,0 +1,168 @@
+"""The compacted reverse pass (PDF_TUNE_REV_COMPACT on a grader): the forward
+pass appends each level's marked (gate, 64-word tile) items to a list and the
+reverse launches of levels >= 1 run over those lists only.  Integer work, so
+every output must equal the oracle's (small cases) or the dense reverse
+pass's and the bit-sliced grader's (full sizes)."""
+import sys
+
+import numpy as np
+import pytest
+
+import pdfgrade as pg
+from oracle import oracle as orc
+from pdfgrade.configs import GATES_2M, GATES_50K, TINY
+
+from helpers import biased_vectors, bitsliced_grade, edge_netlist, oracle_grade, pack, small_config, unpack
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def state(g, total, per_chunk=None):
+    f0, f1 = g.first()
+    return dict(total=total, per_chunk=per_chunk, first_t0=f0, first_t1=f1, gate_counts=g.gate_counts())
+
+
+def assert_same(got, ref, ctx=""):
+    assert got["total"] == ref["total"], (ctx, got["total"], ref["total"])
+    assert np.array_equal(got["first_t0"], ref["first_t0"]), ctx
+    assert np.array_equal(got["first_t1"], ref["first_t1"]), ctx
+    assert np.array_equal(got["gate_counts"], ref["gate_counts"]), ctx
+    if got.get("per_chunk") is not None and "chunk_totals" in ref:
+        assert np.array_equal(np.array(got["per_chunk"], np.uint64), ref["chunk_totals"]), ctx
+
+
+def compact_grades(nl, bits, P, chunk, robust, b_bits=None, lanes=1, graph=1, wpl=1, stages=True, reps=1):
+    """Fused grades (reps of them) and, with stages, the staged chunk loop,
+    with the compacted reverse pass on."""
+    out = []
+    with pg.Grader(nl) as g:
+        if b_bits is None:
+            g.set_vectors(bits, P)
+        else:
+            g.set_pairs(bits, b_bits, P)
+        g.set_chunk(chunk)
+        g.set_tuning(pg.TUNE_REV_COMPACT, 1)
+        g.set_tuning(pg.TUNE_LANES, lanes)
+        g.set_tuning(pg.TUNE_USE_GRAPH, graph)
+        g.set_tuning(pg.TUNE_WORDS_PER_LANE, wpl)
+        for _ in range(reps):
+            out.append(("fused", state(g, g.grade(robust))))
+        if stages:
+            total, per_chunk = g.grade_stages(robust)
+            out.append(("stages", state(g, total, per_chunk)))
+    return out
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_compact_ragged_chunks_against_oracle(robust):
+    """1,500 gates over a 65,536-pair chunk (16 tiles) and a ragged
+    40,859-pair last chunk (639 words: 9 full tiles and a partial one), one
+    and two lanes, graph replay and direct launches, one and two words per
+    lane in the forward pass, fused and stage by stage."""
+    cfg = pg.GenConfig(seed=77, num_pi=64, num_po=64, num_gates=1500, levels=20, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 65536 + 40896 - 37
+    bits = biased_vectors(77, nl.info()["num_pi"], P, p_hold=0.6)
+    ref = oracle_grade(nl, bits, P, 65536, robust)
+    assert ref["total"] > 0
+    for lanes, graph, wpl in ((1, 1, 1), (1, 0, 2), (2, 1, 2), (2, 0, 1)):
+        say("compact ragged", robust, lanes, graph, wpl)
+        for kind, got in compact_grades(nl, bits, P, 65536, robust, lanes=lanes, graph=graph, wpl=wpl, reps=2):
+            assert_same(got, ref, (kind, lanes, graph, wpl))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compact_random_circuits(seed):
+    """Random small circuits, both modes, chunks of 64 words and more (the
+    lists), 40 words and 1 word (dense reverse launches: the lists need a
+    whole 64-word tile per wave)."""
+    nl = pg.Netlist.generate(small_config(seed + 300, num_pi=10, num_po=12, num_gates=400, levels=8, wide=True,
+                                          alpha=2.2))
+    P = 9000
+    bits = biased_vectors(seed + 300, nl.info()["num_pi"], P, p_hold=0.5)
+    for robust in (True, False):
+        for chunk in (4096, 8192, 2560, 64):
+            say("compact random", seed, robust, chunk)
+            ref = oracle_grade(nl, bits, P, chunk, robust)
+            for kind, got in compact_grades(nl, bits, P, chunk, robust):
+                assert_same(got, ref, (kind, seed, robust, chunk))
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_compact_edge_netlist_explicit_pairs(robust):
+    """Every gate type, 12-input gates, explicit pairs (two planes)."""
+    nl = edge_netlist()
+    P = 5000
+    va = biased_vectors(11, nl.info()["num_pi"], P, p_hold=0.3)
+    vb = biased_vectors(12, nl.info()["num_pi"], P, p_hold=0.3)
+    ref = oracle_grade(nl, va, P, 4096, robust, b_bits=vb)
+    say("compact edge", robust)
+    for kind, got in compact_grades(nl, va, P, 4096, robust, b_bits=vb):
+        assert_same(got, ref, (kind, robust))
+
+
+def test_compact_tiny_and_mark_planes():
+    """The 200-gate workload at 1,024 pairs in one 16-word chunk (dense) and
+    on 4,096 pairs in one 64-word chunk (lists): totals against the oracle,
+    and the stage calls' mark plane against the oracle's final marks."""
+    nl = pg.Netlist.generate(TINY.netlist)
+    net = orc.OracleNet(nl.arrays())
+    for P in (1024, 4096):
+        bits = pg.gen_vectors(TINY.vector_seed, nl.info()["num_pi"], P)
+        for robust in (True, False):
+            say("compact tiny", P, robust)
+            ref = oracle_grade(nl, bits, P, P, robust)
+            for kind, got in compact_grades(nl, bits, P, P, robust):
+                assert_same(got, ref, (kind, P, robust))
+            with pg.Grader(nl) as g:
+                g.set_vectors(bits, P)
+                g.set_chunk(P)
+                g.set_tuning(pg.TUNE_REV_COMPACT, 1)
+                g.sim(0, 0)
+                g.mark(0, 0, robust)
+                codes = orc.sim_chunk(net, bits, P, 0, P)
+                marks, _ = orc.mark_chunk(net, codes, robust)
+                assert np.array_equal(pg.plane_to_bytes(g.mark_plane(P), P), marks), (P, robust)
+    with pg.Grader(nl) as g, pytest.raises(pg.PdfError):
+        g.set_tuning(pg.TUNE_REV_COMPACT, 2)
+
+
+@pytest.mark.parametrize("robust", [True, False])
+def test_compact_50k_against_bitsliced(robust):
+    """The 50k-gate workload's 65,536 pairs with per-level launches on 2
+    lanes x 32,768 pairs, against the bit-sliced CPU grader."""
+    nl = pg.Netlist.generate(GATES_50K.netlist)
+    P = GATES_50K.pairs
+    bits = pg.gen_vectors(GATES_50K.vector_seed, nl.info()["num_pi"], P)
+    say("compact 50k", robust)
+    ref = bitsliced_grade(nl, bits, P, 32768, robust)
+    for kind, got in compact_grades(nl, bits, P, 32768, robust, lanes=2, stages=False, reps=2):
+        assert_same(got, ref, (kind, robust))
+
+
+def test_compact_2m_against_dense():
+    """The 2,000,000-gate netlist on 262,144 pairs with its recorded tuning
+    (2 lanes x 131,072-pair chunks x 2 words per lane), the compacted reverse
+    pass against the dense one: same total, first detections and counts."""
+    nl = pg.Netlist.generate(GATES_2M.netlist)
+    P = 262144
+    bits = pg.gen_vectors(GATES_2M.vector_seed, nl.info()["num_pi"], P)
+    res = {}
+    for compact in (0, 1):
+        say("compact 2m", compact)
+        with pg.Grader(nl) as g:
+            g.set_vectors(bits, P)
+            g.set_chunk(GATES_2M.chunk)
+            g.set_tuning(pg.TUNE_LANES, GATES_2M.lanes)
+            g.set_tuning(pg.TUNE_WORDS_PER_LANE, GATES_2M.words_per_lane)
+            g.set_tuning(pg.TUNE_REV_COMPACT, compact)
+            res[compact] = [state(g, g.grade(True)) for _ in range(2)]
+    assert res[0][0]["total"] > 0
+    for rep in range(2):
+        assert_same(res[1][rep], res[0][0], rep)

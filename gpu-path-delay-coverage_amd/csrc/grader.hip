@@ -140,6 +140,9 @@ struct LevelMap {
   RevTile *items = nullptr;
   uint32_t *items_cnt = nullptr;
   uint32_t items_cap = 0, items_nsub = 1;
+  // the forward pass's records are the level's in fanin-locality order
+  // (fanin_order), fout_off holding each one's node id
+  uint32_t perm = 0;
 };
 
 // Sub-lists per level of the compacted reverse pass, at most: the forward
@@ -363,8 +366,8 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
     const uint32_t nthreads = (nblk - mp.main_blocks) * kBlock;
     for (uint32_t i = gi; i < clear32_n; i += nthreads) clear32[i] = 0;
     if (gi >= mp.count) return;
-    const uint32_t g = mp.lo + gi;
-    const GateRec r = recs[g];
+    const GateRec r = recs[mp.lo + gi];
+    const uint32_t g = mp.perm ? r.fout_off : mp.lo + gi;
     pin((uint32_t)r.type);
     pin((uint32_t)r.nfi);
     pin(r.fin[0]);
@@ -386,6 +389,7 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
   }
   if (WIDE) g = __builtin_amdgcn_readfirstlane(g);
   const GateRec r = recs[g];
+  if (mp.perm) g = r.fout_off;  // the record's position was its place in the order
   // the whole record in the first round trip (the compiler would otherwise
   // fetch the fanin ids only after testing the type)
   pin((uint32_t)r.type);
@@ -2590,6 +2594,8 @@ struct pdf_grader {
   // the level walk's forward order: each level's records sorted by how much
   // work their gate takes (walk_order), fout_off holding the node's id
   GateRec *d_wrec = nullptr;
+  GateRec *d_frec = nullptr;  // the per-level forward launches' records in fanin_order
+  uint32_t tune_fwd_order = 0;  // PDF_TUNE_FWD_ORDER
   uint32_t widest = 0;  // nodes of the widest level
 
   // vectors: sequence mode keeps vwords + 1 words per PI row (positions
@@ -2652,7 +2658,11 @@ struct pdf_grader {
   // (level_off[k] - level_off[1]) * ntiles, and their per-level counts
   RevTile *d_rtl = nullptr;
   uint32_t *d_rtl_cnt = nullptr;
-  bool rev_lists = false;  // the last forward pass queued on this lane built the item lists
+  // the lists the last forward pass queued filled (its lane's d_rtl, or
+  // null) and its chunk start: a reverse pass of that chunk on that lane
+  // runs over them; every other forward call clears this
+  const RevTile *rev_lists_buf = nullptr;
+  uint64_t rev_lists_start = 0;
   uint32_t tune_use_graph = 1;
   uint32_t tune_walk = 0;  // PDF_TUNE_WALK: 0 level launches, else words per tile of the level walk
   uint32_t walk_full = 0;  // the last walk wrote every node's record (debug flags were set)
@@ -2671,7 +2681,6 @@ struct pdf_grader {
     uint64_t *NEW = nullptr;    // new-node plane (segments of length K >= 2)
     RevTile *rtl = nullptr;
     uint32_t *rtl_cnt = nullptr;
-    bool rev_lists = false;
     hipEvent_t done = nullptr;  // merges of this lane's latest chunk and all before it
   };
   std::vector<Lane> lanes;      // lanes 1..; empty for one lane
@@ -2815,14 +2824,16 @@ void rev_lists_of(const pdf_grader *g, uint32_t k, LevelMap &mp) {
 // multiple of 8 (each tile keeps its XCD).
 uint32_t words_per_lane(uint32_t want, bool seq, bool sim, bool mark, uint64_t width) {
   const uint64_t words = (width + 63) / 64;
-  return (want == 2 && seq && sim && mark && width % 64 == 0 && words % 128 == 0 && (words / 128) % 8 == 0) ? 2 : 1;
+  for (uint32_t nw = want; nw >= 2; nw /= 2)
+    if (seq && sim && mark && width % 64 == 0 && words % (64 * nw) == 0 && (words / (64 * nw)) % 8 == 0) return nw;
+  return 1;
 }
 
 template <bool SEQ, bool S, bool M, bool R, bool WIDE, int NW>
 void fwd_one(pdf_grader *g, uint32_t grid, const DevNet &net, const Planes &pl, uint64_t wbase, const LevelMap &mp,
              uint32_t *clear, uint32_t nclear) {
-  k_fwd<SEQ, S, M, R, WIDE, NW><<<grid, kBlock, 0, g->stream>>>(net, g->d_rec, pl, g->d_va, g->d_vb, g->vstride,
-                                                                 wbase, mp, clear, nclear);
+  k_fwd<SEQ, S, M, R, WIDE, NW><<<grid, kBlock, 0, g->stream>>>(net, mp.perm ? g->d_frec : g->d_rec, pl, g->d_va,
+                                                                 g->d_vb, g->vstride, wbase, mp, clear, nclear);
 }
 
 // clear_state: the last level's extra-column threads zero the per-chunk
@@ -2835,9 +2846,11 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
   const uint32_t nw = words_per_lane(g->tune_words_per_lane, SEQ, S, M, width);
   // the compacted reverse pass's item lists: built by a marking pass over
   // every level whose waves hold one gate each (chunks of >= 64 words)
-  g->rev_lists = M && g->d_rtl && k_lo == 0 && k_hi >= g->nl.L && last_level >= g->nl.L && pl.words >= 64 &&
-                 g->nl.L >= 1;
-  if (g->rev_lists) {
+  const bool lists = M && g->d_rtl && k_lo == 0 && k_hi >= g->nl.L && last_level >= g->nl.L && pl.words >= 64 &&
+                     g->nl.L >= 1;
+  g->rev_lists_buf = lists ? g->d_rtl : nullptr;
+  g->rev_lists_start = start;
+  if (lists) {
     const uint64_t nc = ((uint64_t)g->nl.L + 1) * kRevSubLists * kRevCntStride;
     tm_begin(g, kFill);
     k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nc, 256), 1024), 256, 0, g->stream>>>(g->d_rtl_cnt, nc, 0u);
@@ -2847,12 +2860,13 @@ void launch_fwd_t(pdf_grader *g, uint64_t start, uint64_t width, int robust, boo
     uint32_t grid;
     const uint32_t lo = g->nl.level_off[k], count = g->nl.level_off[k + 1] - lo;
     LevelMap mp = level_map(pl.words, lo, count, SEQ && S, &grid);
-    if (nw == 2) {  // 128-word tiles, one node per wave
-      mp.T = pl.words / 128;
+    if (nw >= 2) {  // 64 nw-word tiles, one node per wave
+      mp.T = pl.words / (64 * nw);
       mp.main_blocks = mp.T * ((count + (kBlock >> 6) - 1) / (kBlock >> 6));
       grid = mp.main_blocks + grid_for(count, kBlock);
     }
-    if (g->rev_lists && k >= 1) rev_lists_of(g, k, mp);  // level 0's reverse pass stays dense
+    if (lists && k >= 1) rev_lists_of(g, k, mp);  // level 0's reverse pass stays dense
+    mp.perm = g->tune_fwd_order && g->d_frec;
     const bool last = k == std::min(g->nl.L, last_level) && SEQ && S && clear_state;
     uint32_t *clear = last ? g->d_cstate : nullptr;
     const uint32_t nclear = last ? g->n_cstate : 0;
@@ -2901,12 +2915,14 @@ void launch_reverse_t(pdf_grader *g, uint64_t start, uint64_t width, uint32_t k_
                       uint32_t k_hi = 0xFFFFFFFFu) {
   const Planes pl = planes(g, width);
   const DevNet net = g->net();
+  // the item lists of this chunk's forward pass on this lane, if it built them
+  const bool lists = g->d_rtl && g->rev_lists_buf == g->d_rtl && g->rev_lists_start == start;
   for (uint32_t k = std::min(g->nl.L, k_hi) + 1; k-- > k_lo;) {
     uint32_t grid;
     const LevelMap mp =
         level_map(pl.words, g->nl.level_off[k], g->nl.level_off[k + 1] - g->nl.level_off[k], false, &grid);
     tm_begin(g, kRev);
-    if (g->rev_lists && k >= 1) {
+    if (lists && k >= 1) {
       // at most every (gate, tile) item of the level; waves stride over the count
       const uint32_t lgrid = (uint32_t)std::min<uint64_t>(
           kRevListBlocks, std::max<uint64_t>(1, ((uint64_t)mp.count * mp.T + kBlock / 64 - 1) / (kBlock / 64)));
@@ -3252,7 +3268,6 @@ void lane_save(const pdf_grader *g, pdf_grader::Lane &l) {
   l.NEW = g->d_NEW;
   l.rtl = g->d_rtl;
   l.rtl_cnt = g->d_rtl_cnt;
-  l.rev_lists = g->rev_lists;
 }
 
 void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
@@ -3275,7 +3290,6 @@ void lane_load(pdf_grader *g, const pdf_grader::Lane &l) {
   g->d_NEW = l.NEW;
   g->d_rtl = l.rtl;
   g->d_rtl_cnt = l.rtl_cnt;
-  g->rev_lists = l.rev_lists;
 }
 
 void free_chunk_buffers(pdf_grader *g);
@@ -3340,7 +3354,7 @@ int alloc_chunk_buffers(pdf_grader *g, uint64_t chunk_pairs) {
     rc = dalloc(&g->d_rtl, (n - g->nl.level_off[1]) * rev_slots_per_gate(g->ntiles));
     if (!rc) rc = dalloc(&g->d_rtl_cnt, ((uint64_t)g->nl.L + 1) * kRevSubLists * kRevCntStride);
   }
-  g->rev_lists = false;
+  g->rev_lists_buf = nullptr;
   return rc;
 }
 
@@ -3363,7 +3377,7 @@ void free_chunk_buffers(pdf_grader *g) {
   dfree(g->d_hs);
   dfree(g->d_rtl);
   dfree(g->d_rtl_cnt);
-  g->rev_lists = false;
+  g->rev_lists_buf = nullptr;
 }
 
 // Queue the reset of the cross-chunk state (no host synchronization).
@@ -3423,6 +3437,51 @@ int copy_rows(pdf_grader *g, uint64_t *dst, const uint64_t *src, uint64_t cols) 
 // slower: 0.605 against 0.591 ms per 50k grade.)  The copies keep every
 // field the walk reads; fout_off (fanouts are not read) holds the node's
 // id.
+// The per-level forward launches' gate order (PDF_TUNE_FWD_ORDER): within
+// each level, a breadth-first walk over "shares a fanin with" from each
+// unvisited gate in id order, so gates reading the same fanin row are
+// dispatched close together and the row is read from HBM once and then from
+// L2 (a level's waves run a few hundred gates at a time).  Records are
+// returned in that order with fout_off holding the node id (the forward pass
+// reads no fanouts).
+std::vector<GateRec> fanin_order(const Netlist &nl, const std::vector<GateRec> &rec) {
+  std::vector<GateRec> out(nl.n);
+  std::vector<std::pair<uint32_t, uint32_t>> readers;  // (fanin, gate) of the level, sorted
+  std::vector<uint8_t> seen;
+  std::vector<uint32_t> queue;
+  for (uint32_t k = 0; k <= nl.L; ++k) {
+    const uint32_t lo = nl.level_off[k], hi = nl.level_off[k + 1];
+    readers.clear();
+    for (uint32_t g = lo; g < hi; ++g)
+      for (uint32_t e = nl.fin_off[g]; e < nl.fin_off[g + 1]; ++e) readers.push_back({nl.fin[e], g});
+    std::sort(readers.begin(), readers.end());
+    seen.assign(hi - lo, 0);
+    queue.clear();
+    for (uint32_t g0 = lo; g0 < hi; ++g0) {
+      if (seen[g0 - lo]) continue;
+      seen[g0 - lo] = 1;
+      size_t head = queue.size();
+      queue.push_back(g0);
+      while (head < queue.size()) {
+        const uint32_t h = queue[head++];
+        for (uint32_t e = nl.fin_off[h]; e < nl.fin_off[h + 1]; ++e) {
+          auto it = std::lower_bound(readers.begin(), readers.end(), std::make_pair(nl.fin[e], 0u));
+          for (; it != readers.end() && it->first == nl.fin[e]; ++it)
+            if (!seen[it->second - lo]) {
+              seen[it->second - lo] = 1;
+              queue.push_back(it->second);
+            }
+        }
+      }
+    }
+    for (uint32_t i = lo; i < hi; ++i) {  // queue holds every gate of the level once
+      out[i] = rec[queue[i - lo]];
+      out[i].fout_off = queue[i - lo];
+    }
+  }
+  return out;
+}
+
 std::vector<GateRec> walk_order(const Netlist &nl, const std::vector<GateRec> &rec) {
   std::vector<GateRec> out(nl.n);
   std::vector<uint32_t> ids;
@@ -3506,6 +3565,7 @@ int pdf_grader_create(int device, const pdf_netlist *nlh, pdf_grader **out) {
   }
   up(g->d_rec, rec);
   up(g->d_wrec, walk_order(nl, rec));
+  up(g->d_frec, fanin_order(nl, rec));
   for (uint32_t k = 0; k <= nl.L; ++k) g->widest = std::max(g->widest, nl.level_off[k + 1] - nl.level_off[k]);
   up(g->d_type, nl.type);
   up(g->d_po, nl.po);
@@ -3547,6 +3607,7 @@ void pdf_grader_destroy(pdf_grader *g) {
   dfree(g->d_level);
   dfree(g->d_level_off);
   dfree(g->d_wrec);
+  dfree(g->d_frec);
   dfree(g->d_va);
   dfree(g->d_vb);
   free_rank(g);
@@ -4135,6 +4196,10 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
         return PDF_ERR_ARG;
       }
       g->tune_rev_tiles = (uint32_t)value;
+      break;
+    case PDF_TUNE_FWD_ORDER:
+      if (value != 0 && value != 1) { set_error("forward gate order must be 0 (ids) or 1 (fanin locality)"); return PDF_ERR_ARG; }
+      g->tune_fwd_order = (uint32_t)value;
       break;
     case PDF_TUNE_REV_COMPACT:
       if (value != 0 && value != 1) { set_error("compacted reverse pass must be 0 or 1"); return PDF_ERR_ARG; }

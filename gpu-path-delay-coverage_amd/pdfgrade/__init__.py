@@ -32,6 +32,7 @@ S0, S1, T0, T1 = 0, 1, 2, 3
 NONE = 0xFFFFFFFF
 TUNE_WORDS_PER_LANE, TUNE_USE_GRAPH, TUNE_COVER_WAVES, TUNE_SEGMENT_ORDER, TUNE_LANES, TUNE_REV_COMPACT = 1, 2, 3, 4, 5, 6
 TUNE_REV_TILES = 7
+TUNE_FWD_ORDER = 12
 TUNE_WALK = 8
 TUNE_COVER_NODES, TUNE_COVER_POOL, TUNE_REV_WALK = 9, 10, 11
 REF_EXCEEDS_U16, REF_EXCEEDS_U32, REF_SEGMENT_STRIDE, REF_MERGE_WINDOW = 1, 2, 4, 8

@@ -338,6 +338,10 @@ uint64_t pdf_ref_unmerged_segments(uint64_t segments);
                                    * pairs with more keep c/h in global scratch */
 #define PDF_TUNE_COVER_POOL 10    /* grader: fanout ids of those nodes kept in LDS, 0..4096 (4096); the rest are
                                    * read from the netlist in the DP (both knobs exist to exercise those paths) */
+#define PDF_TUNE_FWD_ORDER 12     /* grader: the per-level forward launches take each level's gates 0 (default)
+                                   * in id order, 1 grouped by shared fanins (a fanin row is read from HBM once
+                                   * where its readers run together: 17% fewer bytes read at 2M gates, but not
+                                   * faster there) */
 int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value);
 
 /* Diagnostic variant of pdf_grade (no reference counterpart): the same

@@ -166,3 +166,56 @@ def test_compact_2m_against_dense():
     assert res[0][0]["total"] > 0
     for rep in range(2):
         assert_same(res[1][rep], res[0][0], rep)
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_forward_gate_order(order):
+    """The per-level forward launches in id order (0, the default) and in
+    fanin-locality order (1): random circuits with both reverse passes, fused
+    and stage by stage, against the oracle."""
+    for seed in range(3):
+        nl = pg.Netlist.generate(small_config(seed + 500, num_pi=12, num_po=16, num_gates=600, levels=9, wide=True,
+                                              alpha=2.2))
+        P = 8192
+        bits = biased_vectors(seed + 500, nl.info()["num_pi"], P, p_hold=0.5)
+        for robust in (True, False):
+            ref = oracle_grade(nl, bits, P, 4096, robust)
+            for compact in (0, 1):
+                say("forward order", order, seed, robust, compact)
+                with pg.Grader(nl) as g:
+                    g.set_vectors(bits, P)
+                    g.set_chunk(4096)
+                    g.set_tuning(pg.TUNE_FWD_ORDER, order)
+                    g.set_tuning(pg.TUNE_REV_COMPACT, compact)
+                    assert_same(state(g, g.grade(robust)), ref, (order, seed, robust, compact, "fused"))
+                    total, per_chunk = g.grade_stages(robust)
+                    assert_same(state(g, total, per_chunk), ref, (order, seed, robust, compact, "stages"))
+    with pg.Grader(nl) as g, pytest.raises(pg.PdfError):
+        g.set_tuning(pg.TUNE_FWD_ORDER, 2)
+
+
+def test_compact_lists_follow_their_forward_pass():
+    """The lists belong to the forward pass that filled them: an event-timed
+    grade (one lane, lists built and used per chunk) followed by direct
+    launches on two lanes (levels interleaved across the lanes' chunks, so
+    dense reverse launches) and by graph replays, in every order, each equal
+    to the oracle."""
+    cfg = pg.GenConfig(seed=91, num_pi=32, num_po=32, num_gates=900, levels=12, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 4 * 8192 - 640
+    bits = biased_vectors(91, nl.info()["num_pi"], P, p_hold=0.6)
+    for robust in (True, False):
+        ref = oracle_grade(nl, bits, P, 8192, robust)
+        with pg.Grader(nl) as g:
+            g.set_vectors(bits, P)
+            g.set_chunk(8192)
+            g.set_tuning(pg.TUNE_REV_COMPACT, 1)
+            g.set_tuning(pg.TUNE_LANES, 2)
+            for step, graph, timed in ((0, 0, True), (1, 0, False), (2, 1, True), (3, 1, False), (4, 0, True),
+                                       (5, 1, False), (6, 0, False)):
+                say("compact lists order", robust, step)
+                g.set_tuning(pg.TUNE_USE_GRAPH, graph)
+                total = g.grade_timed(robust)[0] if timed else g.grade(robust)
+                assert_same(state(g, total), ref, (robust, step))
+

@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--autotune", type=int, default=None,
                     help="run the autotuner before the timed steps (default: no for single workloads, yes for "
                          "batches)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KNOB=VALUE",
+                    help="single workloads: set a grader tuning knob after the recorded tuning, e.g. fwd_order=1 "
+                         "(knobs: the pdfgrade TUNE_* names in lower case); results never change")
     return ap.parse_args()
 
 
@@ -506,6 +509,11 @@ def main():
     prof_name = wl.name
     robust = bool(args.robust)
     g, tune = setup_grader(pg, nl, bits, wl, robust, args.autotune)
+    for kv in args.tune:
+        knob, value = kv.split("=", 1)
+        g.set_tuning(getattr(pg, "TUNE_" + knob.upper()), int(value))
+        tune[knob] = int(value)
+        tune["source"] = "recorded + --tune"
     K = args.segs
     grade = (lambda: g.grade(robust)) if K == 1 else (lambda: g.grade_segs(robust, K))
     total, dt = timed_grades(grade, args.warmup, args.steps)

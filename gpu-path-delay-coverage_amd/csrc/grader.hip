@@ -170,6 +170,7 @@ __device__ __forceinline__ void pin(uint32_t x) { asm volatile("" : : "v"(x)); }
 // Like pin, and every later use of x reads it after this point, so work on x
 // cannot be moved up between loads issued before it.
 __device__ __forceinline__ void hold(uint64_t &x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void hold(uint32_t &x) { asm volatile("" : "+v"(x)); }
 
 // LDS words typed as LDS (address space 3), so every access through them is
 // an LDS instruction with no generic-address check
@@ -193,12 +194,15 @@ __device__ __forceinline__ void frames(const Planes &pl, uint32_t f, uint32_t w,
   b = SEQ ? shift_next(a, pl.V[at + 1]) : pl.VB[at];
 }
 
+// Whether node f's mark word w was written in this chunk (bit 0).
+__device__ __forceinline__ uint32_t mark_present(const Planes &pl, uint32_t f, uint32_t w) {
+  return pl.bt ? (uint32_t)(pl.bt[(uint64_t)(w >> pl.bt_shift) * pl.bt_n + f] >> (w & ((1u << pl.bt_shift) - 1)))
+               : (uint32_t)(pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)] >> (w & 63));
+}
+
 // Mark word of node f (zero unless written in this chunk).
 __device__ __forceinline__ uint64_t mark_of(const Planes &pl, uint32_t f, uint32_t w) {
-  const uint32_t present =
-      pl.bt ? (uint32_t)(pl.bt[(uint64_t)(w >> pl.bt_shift) * pl.bt_n + f] >> (w & ((1u << pl.bt_shift) - 1)))
-            : (uint32_t)(pl.NZ[(uint64_t)f * pl.ntiles + (w >> 6)] >> (w & 63));
-  return (present & 1) ? pl.M[(uint64_t)f * pl.pitch + w] : 0;
+  return (mark_present(pl, f, w) & 1) ? pl.M[(uint64_t)f * pl.pitch + w] : 0;
 }
 
 // Off-path sensitization predicate S(on, off) of markkernel.cuh:36-41 in
@@ -1691,12 +1695,14 @@ __device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
   return o;
 }
 
-constexpr uint32_t kCoverCap = 512;    // nodes marked at one pair kept in LDS
-constexpr uint32_t kHashSlots = 1024;  // open-addressing node -> LDS index
+// 1,280 nodes (73.5 KB of LDS a wave with the hash and the pool: two waves
+// per CU) hold the ~970 marked nodes of a pair of the cover-heavy input
+constexpr uint32_t kCoverCap = 1280;   // nodes marked at one pair kept in LDS
+constexpr uint32_t kHashSlots = 2048;  // open-addressing node -> LDS index
 constexpr uint32_t kPool = 4096;       // fanout ids of those nodes kept in LDS
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 2654435761u) >> 22; }  // 10 bits
+__device__ __forceinline__ uint32_t hash_slot(uint32_t g) { return (g * 2654435761u) >> 21; }  // 11 bits
 
 // How much of the cover DP stays in LDS: at most `nodes` marked nodes of a
 // pair (else the DP keeps c/h in global scratch) and `pool` fanout ids (the
@@ -1756,17 +1762,39 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
     return first_t0[g] == p || first_t1[g] == p;
   };
   const uint32_t *L = list + (uint64_t)w * list_stride;
-  // 1. the nodes marked at p, in list order
+  // 1. the nodes marked at p, in list order: kScan entries a lane per round,
+  // each round's list ids, then mark-present bits, then mark words loaded
+  // together (three round trips per 64 kScan entries; a word's list holds
+  // thousands of nodes where many paths are marked)
+  constexpr int kScan = 8;
   for (uint32_t t = lane; t < kHashSlots; t += 64) S.h_key[t] = kEmpty;
   uint32_t K = 0;
-  for (uint32_t base = 0; base < len; base += 64) {
-    const uint32_t idx = base + lane;
-    const uint32_t g = idx < len ? L[idx] : 0;
-    const bool marked = idx < len && ((mark_of(pl, g, w) >> j) & 1);
-    const uint64_t bal = __ballot(marked);
-    const uint32_t pos = K + (uint32_t)__builtin_popcountll(bal & below);
-    if (marked && pos < kCoverCap) S.s_node[pos] = g;
-    K += (uint32_t)__builtin_popcountll(bal);
+  for (uint32_t base = 0; base < len; base += 64 * kScan) {
+    uint32_t gs[kScan], pr[kScan];
+    uint64_t mw[kScan];
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) {
+      const uint32_t idx = base + 64 * u + lane;
+      gs[u] = idx < len ? L[idx] : kEmpty;
+    }
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) hold(gs[u]);
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) pr[u] = gs[u] != kEmpty ? mark_present(pl, gs[u], w) : 0;
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) hold(pr[u]);
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) mw[u] = (pr[u] & 1) ? pl.M[(uint64_t)gs[u] * pl.pitch + w] : 0;
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) hold(mw[u]);
+#pragma unroll
+    for (int u = 0; u < kScan; ++u) {
+      const bool marked = (mw[u] >> j) & 1;
+      const uint64_t bal = __ballot(marked);
+      const uint32_t pos = K + (uint32_t)__builtin_popcountll(bal & below);
+      if (marked && pos < kCoverCap) S.s_node[pos] = gs[u];
+      K += (uint32_t)__builtin_popcountll(bal);
+    }
   }
   __syncthreads();
   const bool in_lds = K <= caps.nodes;
@@ -4174,7 +4202,7 @@ int pdf_grader_set_tuning(pdf_grader *g, int key, int64_t value) {
       g->tune_rev_walk = (uint32_t)value;
       break;
     case PDF_TUNE_COVER_NODES:
-      if (value < 1 || value > (int64_t)kCoverCap) { set_error("cover LDS nodes must be 1..512"); return PDF_ERR_ARG; }
+      if (value < 1 || value > (int64_t)kCoverCap) { set_error("cover LDS nodes must be 1..1280"); return PDF_ERR_ARG; }
       g->cover_caps.nodes = (uint32_t)value;
       break;
     case PDF_TUNE_COVER_POOL:

@@ -334,7 +334,7 @@ uint64_t pdf_ref_unmerged_segments(uint64_t segments);
 #define PDF_TUNE_REV_WALK 11      /* grader: 0 per-level reverse launches; 1/2/4/8 after per-level forward launches
                                    * (PDF_TUNE_WALK 0) the reverse pass of a chunk is one level walk with tiles
                                    * of that many words */
-#define PDF_TUNE_COVER_NODES 9    /* grader: marked nodes of a pair whose cover DP stays in LDS, 1..512 (512);
+#define PDF_TUNE_COVER_NODES 9    /* grader: marked nodes of a pair whose cover DP stays in LDS, 1..1280 (1280);
                                    * pairs with more keep c/h in global scratch */
 #define PDF_TUNE_COVER_POOL 10    /* grader: fanout ids of those nodes kept in LDS, 0..4096 (4096); the rest are
                                    * read from the netlist in the DP (both knobs exist to exercise those paths) */

@@ -17,7 +17,7 @@ from helpers import biased_vectors, edge_netlist, oracle_grade, small_config
 
 pytestmark = pytest.mark.gpu
 
-CAPS = [(512, 4096), (4, 4096), (512, 8), (16, 0), (1, 0)]
+CAPS = [(1280, 4096), (4, 4096), (1280, 8), (16, 0), (1, 0)]
 
 
 def say(*a):
@@ -72,7 +72,7 @@ def test_cover_paths_random_circuits(caps):
             assert ref["total"] > 0
             assert_same(grade_with_caps(nl, bits, P, 512, robust, caps), ref, (caps, seed, robust))
     # the global-memory DP ran: some pair had more marked nodes than the cap
-    assert caps[0] == 512 or most > caps[0], (most, caps)
+    assert caps[0] == 1280 or most > caps[0], (most, caps)
 
 
 @pytest.mark.parametrize("caps", CAPS[1:])
@@ -118,7 +118,7 @@ def test_cover_paths_pair_gains(caps):
 def test_cover_knob_limits():
     nl = pg.Netlist.generate(small_config(5))
     with pg.Grader(nl) as g:
-        for key, bad in ((pg.TUNE_COVER_NODES, 0), (pg.TUNE_COVER_NODES, 513), (pg.TUNE_COVER_POOL, -1),
+        for key, bad in ((pg.TUNE_COVER_NODES, 0), (pg.TUNE_COVER_NODES, 1281), (pg.TUNE_COVER_POOL, -1),
                          (pg.TUNE_COVER_POOL, 4097)):
             with pytest.raises(pg.PdfError):
                 g.set_tuning(key, bad)

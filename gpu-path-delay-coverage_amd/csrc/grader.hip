@@ -1801,31 +1801,63 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
   uint64_t tot = 0;
   bool ovf = false, ref16 = false;
   if (in_lds) {
-    // 2. records, hash, and the fanout-id pool offsets
+    // 2. records, hash, and the fanout-id pool offsets: kScan nodes a lane
+    // per round, each round's record tails (fout_off, nfi/nfo, type/po),
+    // levels and first-detection words loaded together
     uint32_t carry = 0;
-    for (uint32_t base = 0; base < K; base += 64) {
-      const uint32_t idx = base + lane;
-      uint32_t nfo = 0;
-      if (idx < K) {
-        const uint32_t g = S.s_node[idx];
-        const GateRec r = net.rec[g];
-        S.s_level[idx] = net.level[g];
-        nfo = r.nfo;
-        S.s_nfo[idx] = r.nfo;
-        S.s_fo_off[idx] = r.fout_off;
-        S.s_flags[idx] = (uint8_t)((r.po == 1 ? 1 : 0) | (r.type == PDF_INPT ? 2 : 0) |
-                                 (is_new(g) ? 4 : 0));
-        uint32_t h = hash_slot(g);
-        while (atomicCAS(&S.h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
-        S.h_val[h] = (uint16_t)idx;
+    for (uint32_t base = 0; base < K; base += 64 * kScan) {
+      uint32_t gs[kScan], lv[kScan], n0[kScan], n1[kScan];
+      uint64_t nwd[kScan];
+      uint4 tail[kScan];  // bytes 16..31 of the record: fin_off, fout_off, nfi | nfo << 16, type | po << 8
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        const uint32_t idx = base + 64 * u + lane;
+        gs[u] = idx < K ? S.s_node[idx] : kEmpty;
+        lv[u] = n0[u] = n1[u] = 0;
+        nwd[u] = 0;
+        tail[u] = uint4{0, 0, 0, 0};
+        if (gs[u] != kEmpty) {
+          tail[u] = reinterpret_cast<const uint4 *>(net.rec + gs[u])[1];
+          lv[u] = net.level[gs[u]];
+          if (SEGNEW) nwd[u] = newp[(uint64_t)gs[u] * pl.pitch + w];
+          else if (!ALL_NEW) {
+            n0[u] = first_t0[gs[u]];
+            n1[u] = first_t1[gs[u]];
+          }
+        }
       }
-      uint32_t incl = nfo;  // inclusive wave scan of the fanout counts
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(incl, off);
-        if (lane >= (uint32_t)off) incl += o;
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        hold(lv[u]);
+        hold(n0[u]);
+        hold(n1[u]);
+        hold(nwd[u]);
       }
-      if (idx < K) S.s_pool_off[idx] = carry + incl - nfo;
-      carry += __shfl(incl, 63);
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        const uint32_t idx = base + 64 * u + lane;
+        uint32_t nfo = 0;
+        if (gs[u] != kEmpty) {
+          const uint32_t g = gs[u];
+          nfo = tail[u].z >> 16;
+          const uint32_t type = tail[u].w & 0xFF, po = (tail[u].w >> 8) & 0xFF;
+          const bool fresh = ALL_NEW || (SEGNEW ? ((nwd[u] >> j) & 1) != 0 : (n0[u] == p || n1[u] == p));
+          S.s_level[idx] = lv[u];
+          S.s_nfo[idx] = (uint16_t)nfo;
+          S.s_fo_off[idx] = tail[u].y;
+          S.s_flags[idx] = (uint8_t)((po == 1 ? 1 : 0) | (type == PDF_INPT ? 2 : 0) | (fresh ? 4 : 0));
+          uint32_t h = hash_slot(g);
+          while (atomicCAS(&S.h_key[h], kEmpty, g) != kEmpty) h = (h + 1) & (kHashSlots - 1);
+          S.h_val[h] = (uint16_t)idx;
+        }
+        uint32_t incl = nfo;  // inclusive wave scan of the fanout counts
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t o = __shfl_up(incl, off);
+          if (lane >= (uint32_t)off) incl += o;
+        }
+        if (gs[u] != kEmpty) S.s_pool_off[idx] = carry + incl - nfo;
+        carry += __shfl(incl, 63);
+      }
     }
     __syncthreads();
     // 3. fanout ids into the pool, one pool slot per lane and eight loads in
@@ -1854,7 +1886,15 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
       }
     }
     __syncthreads();
-    // 4. the DP over runs of equal level, in LDS
+    // 4. the DP over runs of equal level, in LDS.  A run's pooled nodes (a
+    // prefix of it: pool offsets grow along the list) have their fanout ids
+    // in one contiguous pool range; a lane takes one fanout edge of that
+    // range at a time (its node by binary search over the run's offsets),
+    // looks the fanout up in the hash and adds its c / h into the node's
+    // with LDS atomics, so a run costs its edge count over 64 rounds instead
+    // of its largest fanout count.  Nodes past the pool sum their fanouts
+    // from the netlist, one lane each.  Unsigned sums of non-negative terms
+    // wrap iff some partial sum does, so each atomic's old value tells.
     for (uint32_t i = 0; i < K;) {
       const uint32_t idx = i + lane;
       const bool valid = idx < K;
@@ -1862,14 +1902,49 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
       const uint32_t lv0 = __shfl(lv, 0);
       const uint64_t same = __ballot(valid && lv == lv0);
       const uint32_t run = ~same ? (uint32_t)__builtin_ctzll(~same) : 64u;  // contiguous prefix
+      uint32_t off = 0, nfo = 0;
+      uint8_t fl = 0;
       if (lane < run) {
-        const uint8_t fl = S.s_flags[idx];
-        const uint32_t off = S.s_pool_off[idx], nfo = S.s_nfo[idx];
-        const bool pooled = off + nfo <= caps.pool;
+        fl = S.s_flags[idx];
+        off = S.s_pool_off[idx];
+        nfo = S.s_nfo[idx];
+        S.s_c[idx] = 0;       // fanout sums C and H (H from the PO's own path)
+        S.s_h[idx] = fl & 1;
+      }
+      const bool pooled = lane < run && off + nfo <= caps.pool;
+      const uint32_t rp = (uint32_t)__builtin_popcountll(__ballot(pooled));  // pooled prefix of the run
+      const uint32_t e0 = __shfl(off, 0);
+      const uint32_t e1 = rp ? __shfl(off + nfo, (int)rp - 1) : e0;
+      __syncthreads();
+      for (uint32_t e = e0 + lane; e < e1; e += 64) {
+        uint32_t lo = i, hi = i + rp - 1;  // the last pooled node whose range starts at or before e
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi + 1) >> 1;
+          if (S.s_pool_off[mid] <= e) lo = mid;
+          else hi = mid - 1;
+        }
+        const uint32_t fo = S.s_pool[e];
+        uint32_t h = hash_slot(fo);
+        uint32_t key;
+        while ((key = S.h_key[h]) != kEmpty && key != fo) h = (h + 1) & (kHashSlots - 1);
+        if (key == fo) {
+          const uint32_t at = S.h_val[h];
+          const uint64_t vc = S.s_c[at], vh = S.s_h[at];
+          if (vc) {
+            const uint64_t o = atomicAdd((unsigned long long *)&S.s_c[lo], (unsigned long long)vc);
+            ovf |= o + vc < o;
+          }
+          if (vh) {
+            const uint64_t o = atomicAdd((unsigned long long *)&S.s_h[lo], (unsigned long long)vh);
+            ovf |= o + vh < o;
+          }
+        }
+      }
+      if (lane < run && !pooled) {
         const uint32_t fo_off = S.s_fo_off[idx];
         uint64_t C = 0, H = fl & 1;
         for (uint32_t e = 0; e < nfo; ++e) {
-          const uint32_t fo = pooled ? S.s_pool[off + e] : net.fout[fo_off + e];
+          const uint32_t fo = net.fout[fo_off + e];
           uint32_t h = hash_slot(fo);
           uint32_t key;
           while ((key = S.h_key[h]) != kEmpty && key != fo) h = (h + 1) & (kHashSlots - 1);
@@ -1878,6 +1953,12 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
             ovf |= add_ovf(H, S.s_h[S.h_val[h]]);
           }
         }
+        S.s_c[idx] = C;
+        S.s_h[idx] = H;
+      }
+      __syncthreads();
+      if (lane < run) {
+        const uint64_t C = S.s_c[idx], H = S.s_h[idx];
         uint64_t c = C, h = H;
         if (fl & 4) { ovf |= add_ovf(c, H); h = 0; }
         if (fl & 2) h = 0;
@@ -1890,17 +1971,27 @@ __device__ void cover_one(CoverLds &S, const DevNet &net, const Planes &pl, uint
       __syncthreads();
       i += run;
     }
-    // 5. per-gate counts and the PI total, all nodes at once
-    for (uint32_t idx = lane; idx < K; idx += 64) {
-      const uint64_t c = S.s_c[idx];
-      if (!c) continue;
-      if (!ALL_NEW) {
-        const uint64_t old = atomicAdd((unsigned long long *)&gate_counts[S.s_node[idx]], (unsigned long long)c);
-        ovf |= old + c < old;
+    // 5. per-gate counts and the PI total, all nodes at once (kScan a lane
+    // per round, their atomics issued together)
+    for (uint32_t base = 0; base < K; base += 64 * kScan) {
+      uint64_t cv[kScan], old[kScan];
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        const uint32_t idx = base + 64 * u + lane;
+        cv[u] = idx < K ? S.s_c[idx] : 0;
+        old[u] = 0;
+        if (!ALL_NEW && cv[u])
+          old[u] = atomicAdd((unsigned long long *)&gate_counts[S.s_node[idx]], (unsigned long long)cv[u]);
       }
-      if (S.s_flags[idx] & 2) {
-        ovf |= add_ovf(tot, c);
-        if (pi_cover) pi_cover[(uint64_t)S.s_node[idx] * pi_ld + rel] = c;
+#pragma unroll
+      for (int u = 0; u < kScan; ++u) {
+        const uint32_t idx = base + 64 * u + lane;
+        if (!cv[u]) continue;
+        if (!ALL_NEW) ovf |= old[u] + cv[u] < old[u];
+        if (S.s_flags[idx] & 2) {
+          ovf |= add_ovf(tot, cv[u]);
+          if (pi_cover) pi_cover[(uint64_t)S.s_node[idx] * pi_ld + rel] = cv[u];
+        }
       }
     }
   } else {

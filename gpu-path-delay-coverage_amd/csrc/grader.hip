@@ -1665,27 +1665,79 @@ __global__ void k_reset_chunk(uint32_t *state, uint32_t n) {
   if (i < n) state[i] = 0;
 }
 
-// Distinct first-detection patterns of this chunk (relative index).
+// Distinct first-detection patterns of this chunk (relative index): one
+// node's first-detection pairs of the chunk into the pair bitmap, each
+// pair appended to act_list once (by the lane that set its bit).
+// WAVE (every lane of the wave reads the same bitmap and list: k_events):
+// one atomic per distinct pair of the wave and one list reservation per
+// wave; otherwise (a batch wave spans circuits) one of each per node.
+template <bool WAVE>
 __device__ __forceinline__ void events_one(uint32_t g, const uint32_t *__restrict__ first_t0,
                                            const uint32_t *__restrict__ first_t1, uint32_t start, uint32_t width,
                                            uint64_t *__restrict__ actmap, uint32_t *__restrict__ act_list,
                                            uint32_t *__restrict__ act_count) {
   const uint32_t x[2] = {first_t0[g], first_t1[g]};
+  if (!WAVE) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (x[k] == PDF_NONE || x[k] < start || x[k] - start >= width) continue;
+      const uint32_t rel = x[k] - start;
+      const uint64_t bit = 1ULL << (rel & 63);
+      const uint64_t old = atomicOr((unsigned long long *)&actmap[rel >> 6], (unsigned long long)bit);
+      if (!(old & bit)) act_list[atomicAdd(act_count, 1u)] = rel;
+    }
+    return;
+  }
+  const uint32_t lane = threadIdx.x & 63;
+  // one atomic per distinct pair of the wave (most nodes share a few first
+  // pairs, and atomics on one word queue): the lowest lane holding a pair
+  // leads it, the leaders' atomics go out together
+  bool lead[2] = {false, false};
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    if (x[k] == PDF_NONE || x[k] < start || x[k] - start >= width) continue;
+    const bool want = x[k] != PDF_NONE && x[k] >= start && x[k] - start < width;
     const uint32_t rel = x[k] - start;
-    const uint64_t bit = 1ULL << (rel & 63);
-    const uint64_t old = atomicOr((unsigned long long *)&actmap[rel >> 6], (unsigned long long)bit);
-    if (!(old & bit)) act_list[atomicAdd(act_count, 1u)] = rel;
+    uint64_t todo = __ballot(want);
+    while (todo) {
+      const int leader = (int)__builtin_ctzll(todo);
+      const uint32_t v = __shfl(rel, leader);
+      todo &= ~__ballot(want && rel == v);
+      if ((int)lane == leader) lead[k] = true;
+    }
   }
+  // a bit some wave already set (most of them, for the common pairs) needs
+  // no atomic: read the words first (a stale read only costs the atomic)
+  uint64_t old[2] = {0, 0};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (lead[k]) old[k] = __hip_atomic_load(&actmap[(x[k] - start) >> 6], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const uint64_t bit = 1ULL << ((x[k] - start) & 63);
+    if (lead[k] && !(old[k] & bit))
+      old[k] = atomicOr((unsigned long long *)&actmap[(x[k] - start) >> 6], (unsigned long long)bit);
+  }
+  bool fresh[2];  // this lane set the pair's bit: it appends the pair
+#pragma unroll
+  for (int k = 0; k < 2; ++k) fresh[k] = lead[k] && !((old[k] >> ((x[k] - start) & 63)) & 1);
+  // the wave's new pairs appended with one reservation
+  const uint64_t b0 = __ballot(fresh[0]), b1 = __ballot(fresh[1]);
+  const uint32_t n0 = (uint32_t)__builtin_popcountll(b0), cnt = n0 + (uint32_t)__builtin_popcountll(b1);
+  if (!cnt) return;
+  const int first = (int)__builtin_ctzll(b0 | b1);
+  uint32_t base = 0;
+  if ((int)lane == first) base = atomicAdd(act_count, cnt);
+  base = __shfl(base, first);
+  const uint64_t below = (1ULL << lane) - 1;
+  if (fresh[0]) act_list[base + (uint32_t)__builtin_popcountll(b0 & below)] = x[0] - start;
+  if (fresh[1]) act_list[base + n0 + (uint32_t)__builtin_popcountll(b1 & below)] = x[1] - start;
 }
 
 __global__ void k_events(uint32_t n, const uint32_t *__restrict__ first_t0, const uint32_t *__restrict__ first_t1,
                          uint32_t start, uint32_t width, uint64_t *__restrict__ actmap,
                          uint32_t *__restrict__ act_list, uint32_t *__restrict__ act_count) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g < n) events_one(g, first_t0, first_t1, start, width, actmap, act_list, act_count);
+  if (g < n) events_one<true>(g, first_t0, first_t1, start, width, actmap, act_list, act_count);
 }
 
 __device__ __forceinline__ bool add_ovf(uint64_t &acc, uint64_t v) {
@@ -2664,7 +2716,7 @@ __global__ void k_events_batch(const BatchCircuit *__restrict__ circ, const uint
   if (t >= total) return;
   const uint32_t ci = find_circuit(node_base, C, t);
   const BatchCircuit &c = circ[ci];
-  events_one(t - node_base[ci], c.first0, c.first1, 0, c.width, c.actmap, c.act_list, c.act_count);
+  events_one<false>(t - node_base[ci], c.first0, c.first1, 0, c.width, c.actmap, c.act_list, c.act_count);
 }
 
 // Every circuit's first-detection pairs, dealt round-robin to the waves

@@ -102,12 +102,12 @@ COVER_HEAVY = Workload(
     # 50,000 gates in 16 levels, 70% of them NOT/BUFF (a fan-in-1 gate passes
     # its input's mark on), 4,096 POs: ~2,800 robust paths over 65,536 iid
     # pairs against the 50k workload's 4, so the merge and the cover DP
-    # carry the load (the cover takes ~1.2 of ~2.1 ms a grade)
+    # carry the load (the cover takes ~0.4 of ~1.2 ms a grade)
     netlist=GenConfig(seed=5, num_pi=256, num_po=4096, num_gates=50_000, levels=16,
                       fanin_p=(0.7, 0.25, 0.05, 0.0), prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64),
     # pdf_grader_autotune's choice on an MI355X: one chunk, per-level
     # launches (the level walk is slower here: 3,100 gates a level), direct
-    # launches (2.09 ms a grade)
+    # launches (1.22 ms a grade)
     vector_seed=7, pairs=65_536, chunk=65_536, lanes=1, use_graph=0)
 
 WORKLOADS = {w.name: w for w in (TINY, GATES_50K, GATES_2M, COVER_HEAVY)}

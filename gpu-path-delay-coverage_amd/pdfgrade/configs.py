@@ -54,7 +54,7 @@ GATES_2M = Workload(
     netlist=GenConfig(seed=4, num_pi=4096, num_po=2048, num_gates=2_000_000, levels=200, fanin_p=WIDE_FANIN,
                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64, path_limit_log2=0),
     # 2 lanes x 131,072-pair chunks x 2 words per lane, the compacted reverse
-    # pass: 166-172 ms a grade on an MI355X against 178-186 ms with dense
+    # pass: 157-161 ms a grade on an MI355X against 178-186 ms with dense
     # reverse launches and 291 ms for 1 lane x 32,768 x 1 (the chunk buffers
     # take 234 GB)
     vector_seed=104, pairs=1_048_576, chunk=131_072, lanes=2, words_per_lane=2, rev_compact=1)

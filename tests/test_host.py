@@ -2,6 +2,7 @@
 symbol of include/pdfgrade.h, and the netlist/vector host code behaves."""
 import os
 import re
+import sys
 import tempfile
 
 import numpy as np
@@ -287,3 +288,21 @@ def test_bench_walk_byte_model_counts_far_fanins():
     assert m["far_fanin_reads"] == far * 256 * 48
     assert m["bit_bytes"] == 2 * a["n"] * 256
     assert m["bytes"] == m["record_writes"] + m["far_fanin_reads"] + m["pi_reads"] + m["bit_bytes"]
+
+
+def test_bench_tune_option_names_knobs(monkeypatch):
+    """bench.py --tune KNOB=VALUE: each knob name is a pdfgrade TUNE_*
+    constant in lower case, repeatable; the recorded workloads' tunings use
+    values the grader accepts."""
+    import bench
+    from pdfgrade.configs import WORKLOADS
+
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--tune", "fwd_order=1", "--tune", "rev_compact=0"])
+    args = bench.parse()
+    assert args.tune == ["fwd_order=1", "rev_compact=0"]
+    for kv in args.tune:
+        knob, value = kv.split("=", 1)
+        assert isinstance(getattr(pg, "TUNE_" + knob.upper()), int) and int(value) in (0, 1)
+    for wl in WORKLOADS.values():
+        assert wl.walk in (0, 1, 2, 4, 8) and wl.rev_compact in (0, 1) and wl.words_per_lane in (1, 2)
+        assert not (wl.walk and wl.rev_compact), wl.name  # the walk has its own reverse pass

@@ -5738,8 +5738,11 @@ int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value) {
 
 int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best) {
   if (!b || reps == 0) { set_error("pdf_batch_autotune: bad argument"); return PDF_ERR_ARG; }
-  // one knob at a time over its values, the others held at their best so far;
-  // two sweeps, each configuration timed once
+  // every combination of the knobs' values (64 at most: segment order x
+  // cover waves x graph replay x compacted reverse pass x words per lane),
+  // each configuration timed once, so interactions between the knobs (the
+  // forward tile width against the reverse pass's waves, say) are measured
+  // rather than assumed away
   struct Knob {
     int key;
     std::vector<uint32_t> values;
@@ -5784,19 +5787,20 @@ int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning
   };
   float best_ms;
   if ((rc = timed(cur, &best_ms))) return rc;
-  for (int sweep = 0; sweep < 2; ++sweep)
-    for (int i = 0; i < kKnobs; ++i)
-      for (uint32_t v : knobs[i].values) {
-        uint32_t trial[kKnobs];
-        std::copy(cur, cur + kKnobs, trial);
-        trial[i] = v;
-        float ms;
-        if ((rc = timed(trial, &ms))) return rc;
-        if (ms < best_ms) {
-          best_ms = ms;
-          std::copy(trial, trial + kKnobs, cur);
-        }
-      }
+  uint32_t idx[kKnobs] = {};  // odometer over the knobs' value lists
+  for (;;) {
+    uint32_t trial[kKnobs];
+    for (int i = 0; i < kKnobs; ++i) trial[i] = knobs[i].values[idx[i]];
+    float ms;
+    if ((rc = timed(trial, &ms))) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      std::copy(trial, trial + kKnobs, cur);
+    }
+    int i = 0;
+    while (i < kKnobs && ++idx[i] == knobs[i].values.size()) idx[i++] = 0;
+    if (i == kKnobs) break;
+  }
   if ((rc = apply(cur))) return rc;
   if (best) *best = pdf_batch_tuning{cur[0], cur[1], cur[2], cur[3], cur[4], best_ms};
   return PDF_OK;

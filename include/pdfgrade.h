@@ -515,9 +515,10 @@ int pdf_batch_member(pdf_batch *b, uint32_t circuit, pdf_grader **out);
 /* PDF_TUNE_USE_GRAPH, PDF_TUNE_COVER_WAVES, PDF_TUNE_SEGMENT_ORDER, PDF_TUNE_REV_COMPACT and
  * PDF_TUNE_WORDS_PER_LANE (2 takes effect only when every circuit has at least 4,096 pairs). */
 int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value);
-/* Tune the knobs one at a time (each value timed over reps runs, median; two
- * sweeps over the knobs), check that every configuration gives the same
- * totals, keep the fastest and report it in *best. */
+/* Time every combination of the knobs' values (segment order, cover waves,
+ * graph replay, compacted reverse pass, words per lane: 64 at most; each
+ * timed over reps runs, median), check that every configuration gives the
+ * same totals, keep the fastest and report it in *best. */
 int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best);
 
 #ifdef __cplusplus

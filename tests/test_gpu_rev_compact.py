@@ -219,3 +219,20 @@ def test_compact_lists_follow_their_forward_pass():
                 total = g.grade_timed(robust)[0] if timed else g.grade(robust)
                 assert_same(state(g, total), ref, (robust, step))
 
+
+
+def test_compact_more_tiles_than_sublists():
+    """Chunks of 524,288 pairs (128 64-word tiles, two tiles to each of the
+    64 sub-lists) and a ragged last chunk, both modes, against the bit-sliced
+    CPU grader."""
+    cfg = pg.GenConfig(seed=95, num_pi=48, num_po=48, num_gates=800, levels=14, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 524288 + 70000
+    bits = biased_vectors(95, nl.info()["num_pi"], P, p_hold=0.5)
+    for robust in (True, False):
+        say("compact 128 tiles", robust)
+        ref = bitsliced_grade(nl, bits, P, 524288, robust)
+        assert ref["total"] > 0
+        for kind, got in compact_grades(nl, bits, P, 524288, robust, wpl=2, stages=False, reps=2):
+            assert_same(got, ref, (kind, robust))

@@ -66,6 +66,47 @@ def test_noalloc_grade_equals_grade_twice():
                 _same(_state(g), ref, (name, robust))
 
 
+def test_noalloc_grade_with_walk_and_compaction_knobs():
+    """The grader's level-walk and compacted-reverse knobs set: the scratch
+    holds neither the walk's tile-major plane nor the item lists, so the
+    no-alloc grade runs per-level launches with dense reverse launches; it
+    must equal the grader's own grades, and the grader's own grades after it
+    (lists refilled by their forward pass) must too."""
+    cfg = pg.GenConfig(seed=25, num_pi=64, num_po=32, num_gates=5_000, levels=30, fanin_p=(0.1, 0.6, 0.2, 0.1),
+                       prev_level_frac=0.7, fanout_alpha=2.2, fanout_cap=64)
+    nl = pg.Netlist.generate(cfg)
+    P = 65_536
+    bits = biased_vectors(25, nl.info()["num_pi"], P, p_hold=0.6)
+    total = torch.zeros(1, dtype=torch.int64, device="cuda")
+    err = torch.ones(1, dtype=torch.int32, device="cuda")
+    s = torch.cuda.Stream()
+    with pg.Grader(nl) as g:
+        g.set_vectors(bits, P)
+        g.set_chunk(P)
+        for robust in (True, False):
+            g.set_tuning(pg.TUNE_WALK, 0)
+            g.set_tuning(pg.TUNE_REV_COMPACT, 0)
+            ref_total = g.grade(robust)
+            ref = _state(g)
+            assert ref_total > 0
+            for walk, compact in ((0, 1), (4, 0), (4, 1)):
+                say(f"noalloc knobs robust={robust} walk={walk} compact={compact}")
+                g.set_tuning(pg.TUNE_WALK, walk)
+                g.set_tuning(pg.TUNE_REV_COMPACT, compact)
+                assert g.grade(robust) == ref_total
+                _same(_state(g), ref, (robust, walk, compact, "before"))
+                total.fill_(-1)
+                err.fill_(1)
+                torch.cuda.synchronize()
+                g.grade_noalloc(robust, s.cuda_stream, total.data_ptr(), err.data_ptr())
+                s.synchronize()
+                assert int(err.item()) == 0
+                assert int(total.item()) == ref_total, (robust, walk, compact, int(total.item()), ref_total)
+                _same(_state(g), ref, (robust, walk, compact, "noalloc"))
+                assert g.grade(robust) == ref_total
+                _same(_state(g), ref, (robust, walk, compact, "after"))
+
+
 def test_noalloc_grade_explicit_pairs():
     cfg = small_config(23, num_pi=10, num_po=12, num_gates=400, levels=10, wide=True)
     nl = pg.Netlist.generate(cfg)

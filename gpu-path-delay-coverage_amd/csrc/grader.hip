@@ -135,11 +135,14 @@ struct LevelMap {
   uint32_t main_blocks;
   // the forward pass appends the level's marked (gate, 64-word tile) items
   // (PDF_TUNE_REV_COMPACT; one gate per wave only) to items_nsub sub-lists,
-  // tile t to sub-list t % items_nsub, each of items_cap entries at
-  // items + sub * items_cap, its count at items_cnt[sub * kRevCntStride]
+  // tile t to sub-list t % items_nsub (or, with items_by_block, the
+  // launch's block b to sub-list b % 64), each of items_cap entries at
+  // items + sub * items_cap, its count at items_cnt[sub * kRevCntStride];
+  // an item's tile field is t | items_tag (a batch's circuit index)
   RevTile *items = nullptr;
   uint32_t *items_cnt = nullptr;
   uint32_t items_cap = 0, items_nsub = 1;
+  uint32_t items_tag = 0, items_by_block = 0;
   // the forward pass's records are the level's in fanin-locality order
   // (fanin_order), fout_off holding each one's node id
   uint32_t perm = 0;
@@ -558,9 +561,10 @@ __device__ __forceinline__ void fwd_body(const DevNet &net, const GateRec *__res
 #pragma unroll
       for (int u = 0; u < NW; ++u) {
         if (!tile_bits[u]) continue;
-        const uint32_t t = (w + 64u * u) >> 6, sub = t % mp.items_nsub;
+        const uint32_t t = (w + 64u * u) >> 6;
+        const uint32_t sub = mp.items_by_block ? blockIdx.x % kRevSubLists : t % mp.items_nsub;
         const uint32_t pos = atomicAdd(mp.items_cnt + sub * kRevCntStride, 1u);
-        mp.items[(uint64_t)sub * mp.items_cap + pos] = RevTile{g, t, tile_bits[u]};
+        mp.items[(uint64_t)sub * mp.items_cap + pos] = RevTile{g, t | mp.items_tag, tile_bits[u]};
       }
     }
   }
@@ -791,6 +795,30 @@ __global__ __launch_bounds__(kBlock) void k_reverse_tiles(DevNet net, const Gate
   }
 }
 
+// A level's sub-lists of compacted reverse items, read by one wave: lane s
+// loads sub-list s's count (zero past the level's sub-lists); pre is its
+// exclusive prefix sum, cnt the level's item count.
+__device__ __forceinline__ void sublist_prefix(const uint32_t *__restrict__ items_cnt, uint32_t lane, uint32_t &pre,
+                                               uint32_t &cnt) {
+  static_assert(kRevSubLists == 64, "one sub-list count per lane");
+  const uint32_t c = items_cnt[lane * kRevCntStride];
+  uint32_t incl = c;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(incl, off);
+    if (lane >= (uint32_t)off) incl += o;
+  }
+  pre = incl - c;
+  cnt = __shfl(incl, 63);
+}
+
+// Where item i (< cnt) of the level lies: in the last sub-list whose prefix
+// is <= i (a later one starts past i; empty ones share their successor's
+// prefix), at i minus that prefix.
+__device__ __forceinline__ uint64_t sublist_slot(uint32_t pre, uint32_t i, uint32_t items_cap) {
+  const uint32_t sub = 63 - (uint32_t)__builtin_clzll(__ballot(pre <= i));
+  return (uint64_t)sub * items_cap + (i - (uint32_t)__shfl(pre, (int)sub));
+}
+
 // Compacted reverse pass over one level (PDF_TUNE_REV_COMPACT): the
 // (gate, 64-word tile) items holding a forward mark, appended by the forward
 // pass (RevTile, in no particular order: the items of a level are
@@ -805,24 +833,12 @@ __global__ __launch_bounds__(kBlock) void k_reverse_list(DevNet net, const GateR
                                                          uint32_t *__restrict__ first_t1,
                                                          uint32_t *__restrict__ list, uint32_t *__restrict__ list_cnt,
                                                          uint64_t list_stride) {
-  static_assert(kRevSubLists == 64, "one sub-list count per lane");
   constexpr uint32_t wpb = kBlock / 64;
+  uint32_t pre, cnt;
+  sublist_prefix(items_cnt, threadIdx.x & 63, pre, cnt);
   const uint32_t lane = threadIdx.x & 63;
-  // the sub-lists' counts (those past the level's sub-lists stay zero) and
-  // their exclusive prefix sums, lane s: sub-list s
-  const uint32_t c = items_cnt[lane * kRevCntStride];
-  uint32_t incl = c;
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t o = __shfl_up(incl, off);
-    if (lane >= (uint32_t)off) incl += o;
-  }
-  const uint32_t pre = incl - c, cnt = __shfl(incl, 63);
   for (uint32_t i = blockIdx.x * wpb + (threadIdx.x >> 6); i < cnt; i += gridDim.x * wpb) {
-    // item i lies in the last sub-list whose prefix is <= i (a later one
-    // starts past i; empty ones share their successor's prefix)
-    const uint32_t sub = 63 - (uint32_t)__builtin_clzll(__ballot(pre <= i));
-    const uint32_t off = i - (uint32_t)__shfl(pre, (int)sub);
-    const RevTile it = items[(uint64_t)sub * items_cap + off];
+    const RevTile it = items[sublist_slot(pre, i, items_cap)];
     const uint32_t g = __builtin_amdgcn_readfirstlane(it.gate);
     const uint32_t t = __builtin_amdgcn_readfirstlane(it.tile);
     const GateRec r = recs[g];
@@ -2684,6 +2700,36 @@ __global__ __launch_bounds__(256) void k_rev_work_batch(const BatchCircuit *__re
     if (w < c.pl.words)
       rev_item<true, true, true>(c.net, c.recs, c.pl, g, w, 0, c.first0, c.first1, c.list, c.list_cnt,
                                  c.list_stride);
+  }
+}
+
+// The batch's compacted reverse pass from lists its forward launches built
+// (PDF_TUNE_REV_COMPACT = 2): union level k's marked (gate, 64-word tile)
+// items of every circuit, in 64 sub-lists (by forward block), the circuit in
+// the tile field's high bits; a fixed grid of waves strides over them, lane
+// = word of the tile, and the item's mark-present bits come with it.
+constexpr uint32_t kBatchTileBits = 20;  // tiles of a circuit < 2^20, circuits < 2^12
+__global__ __launch_bounds__(256) void k_rev_list_batch(const BatchCircuit *__restrict__ circ,
+                                                        const RevTile *__restrict__ items,
+                                                        const uint32_t *__restrict__ items_cnt, uint32_t items_cap) {
+  const uint32_t lane = threadIdx.x & 63;
+  uint32_t pre, cnt;
+  sublist_prefix(items_cnt, lane, pre, cnt);
+  const uint32_t nwaves = gridDim.x * 4;
+  for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < cnt; i += nwaves) {
+    const RevTile it = items[sublist_slot(pre, i, items_cap)];
+    const uint32_t ci = __builtin_amdgcn_readfirstlane(it.tile >> kBatchTileBits);
+    const uint32_t t = __builtin_amdgcn_readfirstlane(it.tile & ((1u << kBatchTileBits) - 1));
+    const uint32_t g = __builtin_amdgcn_readfirstlane(it.gate);
+    const BatchCircuit &c = circ[ci];
+    const GateRec r = c.recs[g];
+    pin(r.fin[0]);
+    pin((uint32_t)r.nfi);
+    pin((uint32_t)r.po);
+    // bits are set only for words of the circuit's chunk
+    if ((it.bits >> lane) & 1)
+      rev_marked<true, true, true>(c.net, r, c.pl, g, t * 64 + lane, 0, c.first0, c.first1, c.list, c.list_cnt,
+                                   c.list_stride);
   }
 }
 
@@ -5372,8 +5418,16 @@ struct pdf_batch {
   std::vector<uint32_t> scan_off, scan_cnt, scan_grid, work_grid;  // per level
   RevItem *d_work = nullptr;
   uint32_t *d_work_cnt = nullptr;
+  // the lists the forward launches build (rev_compact 2): union level k's
+  // 64 sub-lists of blist_cap[k] items at d_blist + blist_off[k], their
+  // counters at d_blist_cnt + k * kRevSubLists * kRevCntStride
+  RevTile *d_blist = nullptr;
+  uint32_t *d_blist_cnt = nullptr;
+  std::vector<uint64_t> blist_off;
+  std::vector<uint32_t> blist_cap;
   // tuning knobs (pdf_batch_set_tuning / pdf_batch_autotune)
   uint32_t tune_cover_waves = 512, tune_order = 1, tune_use_graph = 1, tune_rev_compact = 1, tune_wpl = 2;
+  // (tune_rev_compact: 2 from pdf_batch_create where every chunk has whole-wave tiles)
   // words per lane of the forward launches: 2 only with whole-wave tiles
   uint32_t fwd_nw() const { return wide && tune_wpl == 2 ? 2 : 1; }
 };
@@ -5429,8 +5483,42 @@ int batch_build_segments(pdf_batch *b) {
       all.insert(all.end(), lv.begin(), lv.end());
     }
   }
+  // lists built by the forward launches: a block appends at most one item
+  // per wave and word of a lane (one gate a wave), to sub-list block % 64
+  dfree(b->d_blist);
+  dfree(b->d_blist_cnt);
+  b->blist_off.assign(b->maxL + 1, 0);
+  b->blist_cap.assign(b->maxL + 1, 0);
+  int rc = PDF_OK;
+  if (b->tune_rev_compact == 2) {
+    const uint64_t per_block = (kBlock / 64) * b->fwd_nw();
+    uint64_t at = 0;
+    for (uint32_t k = 0; k <= b->maxL; ++k) {
+      const uint64_t cap = per_block * ((b->fwd_grid[k] + kRevSubLists - 1) / kRevSubLists);
+      if (cap > 0xFFFFFFFFull) {
+        set_error("pdf_batch: a union level has too many blocks for the forward-built reverse lists");
+        return PDF_ERR_UNSUPPORTED;
+      }
+      b->blist_off[k] = at;
+      b->blist_cap[k] = (uint32_t)cap;
+      at += cap * kRevSubLists;
+    }
+    rc = dalloc(&b->d_blist, std::max<uint64_t>(1, at));
+    if (!rc) rc = dalloc(&b->d_blist_cnt, ((uint64_t)b->maxL + 1) * kRevSubLists * kRevCntStride);
+    if (rc) return rc;
+    for (uint32_t k = 0; k <= b->maxL; ++k)
+      for (uint32_t s = b->fwd_off[k]; s < b->fwd_off[k] + b->fwd_cnt[k]; ++s) {
+        LevelMap &mp = all[s].mp;
+        mp.items = b->d_blist + b->blist_off[k];
+        mp.items_cnt = b->d_blist_cnt + (uint64_t)k * kRevSubLists * kRevCntStride;
+        mp.items_cap = b->blist_cap[k];
+        mp.items_nsub = kRevSubLists;
+        mp.items_by_block = 1;
+        mp.items_tag = all[s].circuit << kBatchTileBits;
+      }
+  }
   dfree(b->d_segs);
-  int rc = dalloc(&b->d_segs, all.size());
+  rc = dalloc(&b->d_segs, all.size());
   if (!rc && !hip_ok(hipMemcpy(b->d_segs, all.data(), all.size() * sizeof(BatchSeg), hipMemcpyHostToDevice),
                      "hipMemcpy"))
     rc = PDF_ERR_HIP;
@@ -5489,6 +5577,10 @@ void batch_queue_t(pdf_batch *b) {
   hipStream_t st = b->stream;
   k_reset_batch<<<grid_for(b->total_nodes, 256), 256, 0, st>>>(b->d_circ, b->d_node_base, b->C, b->total_nodes);
   const uint32_t nw = b->fwd_nw();
+  if (b->tune_rev_compact == 2) {
+    const uint64_t nc = ((uint64_t)b->maxL + 1) * kRevSubLists * kRevCntStride;
+    k_fill_u32<<<(uint32_t)std::min<uint64_t>(grid_for(nc, 256), 1024), 256, 0, st>>>(b->d_blist_cnt, nc, 0u);
+  }
   for (uint32_t k = 0; k <= b->maxL; ++k) {
     const BatchSeg *sg = b->d_segs + b->fwd_off[k];
     if (nw == 2)
@@ -5498,7 +5590,14 @@ void batch_queue_t(pdf_batch *b) {
     else
       k_fwd_batch<R, false, 1><<<b->fwd_grid[k], kBlock, 0, st>>>(b->d_circ, sg, b->fwd_cnt[k]);
   }
-  if (b->tune_rev_compact) {
+  if (b->tune_rev_compact == 2) {
+    for (uint32_t k = b->maxL + 1; k-- > 0;) {
+      if (!b->scan_grid[k]) continue;
+      k_rev_list_batch<<<b->work_grid[k], 256, 0, st>>>(b->d_circ, b->d_blist + b->blist_off[k],
+                                                        b->d_blist_cnt + (uint64_t)k * kRevSubLists * kRevCntStride,
+                                                        b->blist_cap[k]);
+    }
+  } else if (b->tune_rev_compact) {
     k_fill_u32<<<1, 256, 0, st>>>(b->d_work_cnt, b->maxL + 1, 0u);
     for (uint32_t k = b->maxL + 1; k-- > 0;) {
       if (!b->scan_grid[k]) continue;
@@ -5637,6 +5736,9 @@ int pdf_batch_create(int device, uint32_t count, const pdf_netlist *const *nets,
     return PDF_ERR_ARG;
   }
   b->total_nodes = (uint32_t)nodes;
+  // the reverse pass over lists the forward launches build where it applies
+  // (256-circuit batch: 35.7 ms a grade against 37.9 with a scan launch)
+  b->tune_rev_compact = b->wide && count <= (1u << (32 - kBatchTileBits)) ? 2 : 1;
   rc = dalloc(&b->d_circ, count);
   if (!rc) rc = dalloc(&b->d_node_base, count);
   if (!rc) rc = dalloc(&b->d_totals, count);
@@ -5665,6 +5767,8 @@ void pdf_batch_destroy(pdf_batch *b) {
   dfree(b->d_scan);
   dfree(b->d_work);
   dfree(b->d_work_cnt);
+  dfree(b->d_blist);
+  dfree(b->d_blist_cnt);
   dfree(b->d_cs);
   dfree(b->d_hs);
   dfree(b->d_totals);
@@ -5718,11 +5822,18 @@ int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value) {
       b->tune_order = (uint32_t)value;
       batch_drop_graph(b);
       return batch_build_segments(b);
-    case PDF_TUNE_REV_COMPACT:
-      if (value != 0 && value != 1) break;
+    case PDF_TUNE_REV_COMPACT: {
+      if (value < 0 || value > 2) break;
+      if (value == 2 && !(b->wide && b->C <= (1u << (32 - kBatchTileBits)))) {
+        set_error("pdf_batch_set_tuning: forward-built reverse lists need chunks of >= 4,096 pairs and at most " +
+                  std::to_string(1u << (32 - kBatchTileBits)) + " circuits");
+        return PDF_ERR_UNSUPPORTED;
+      }
+      const uint32_t was = b->tune_rev_compact;
       b->tune_rev_compact = (uint32_t)value;
       batch_drop_graph(b);
-      return PDF_OK;
+      return (was == 2) != (value == 2) ? batch_build_segments(b) : PDF_OK;
+    }
     case PDF_TUNE_WORDS_PER_LANE:
       if (value != 1 && value != 2) break;
       b->tune_wpl = (uint32_t)value;
@@ -5738,8 +5849,8 @@ int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value) {
 
 int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best) {
   if (!b || reps == 0) { set_error("pdf_batch_autotune: bad argument"); return PDF_ERR_ARG; }
-  // every combination of the knobs' values (64 at most: segment order x
-  // cover waves x graph replay x compacted reverse pass x words per lane),
+  // every combination of the knobs' values (96 at most: segment order x
+  // cover waves x graph replay x reverse pass x words per lane),
   // each configuration timed once, so interactions between the knobs (the
   // forward tile width against the reverse pass's waves, say) are measured
   // rather than assumed away
@@ -5750,7 +5861,9 @@ int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning
   const Knob knobs[] = {{PDF_TUNE_SEGMENT_ORDER, {0, 1}},
                         {PDF_TUNE_COVER_WAVES, {256, 512, 1024, 2048}},
                         {PDF_TUNE_USE_GRAPH, {0, 1}},
-                        {PDF_TUNE_REV_COMPACT, {0, 1}},
+                        {PDF_TUNE_REV_COMPACT, b->wide && b->C <= (1u << (32 - kBatchTileBits))
+                                                   ? std::vector<uint32_t>{0, 1, 2}
+                                                   : std::vector<uint32_t>{0, 1}},
                         {PDF_TUNE_WORDS_PER_LANE, b->wide ? std::vector<uint32_t>{1, 2} : std::vector<uint32_t>{1}}};
   constexpr int kKnobs = 5;
   uint32_t cur[kKnobs] = {b->tune_order, b->tune_cover_waves, b->tune_use_graph, b->tune_rev_compact,

@@ -323,7 +323,10 @@ uint64_t pdf_ref_unmerged_segments(uint64_t segments);
 #define PDF_TUNE_COVER_WAVES 3    /* batch: waves of the cover launch */
 #define PDF_TUNE_SEGMENT_ORDER 4  /* batch: 0 circuit order, 1 largest segments first */
 #define PDF_TUNE_LANES 5          /* grader: chunks graded concurrently on 1..4 streams */
-#define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items;
+#define PDF_TUNE_REV_COMPACT 6    /* batch: 1 reverse pass over a list of the marked (gate, tile) items
+                                   * that a scan launch builds, 2 over lists the forward launches build
+                                   * (every circuit >= 4,096 pairs, at most 4,096 circuits; the default
+                                   * where it applies, else 1);
                                    * grader (per-level launches): 1 the forward pass appends each level's
                                    * marked (gate, 64-word tile) items to a list and the reverse launches of
                                    * levels >= 1 run over those lists only */
@@ -516,7 +519,7 @@ int pdf_batch_member(pdf_batch *b, uint32_t circuit, pdf_grader **out);
  * PDF_TUNE_WORDS_PER_LANE (2 takes effect only when every circuit has at least 4,096 pairs). */
 int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value);
 /* Time every combination of the knobs' values (segment order, cover waves,
- * graph replay, compacted reverse pass, words per lane: 64 at most; each
+ * graph replay, reverse pass, words per lane: 96 at most; each
  * timed over reps runs, median), check that every configuration gives the
  * same totals, keep the fastest and report it in *best. */
 int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning *best);

@@ -48,6 +48,9 @@ def test_small_batch_matches_oracle(robust):
             np.testing.assert_array_equal(m.gate_counts(), want["gate_counts"])
             assert m.total() == want["total"]
         assert any(t > 0 for t in tot)
+    # lists built by the forward launches need whole-wave tiles (>= 64 words)
+    with pytest.raises(pg.PdfError):
+        b.set_tuning(pg.TUNE_REV_COMPACT, 2)
     b.close()
 
 
@@ -78,7 +81,7 @@ def test_wide_batch_matches_plain_grading_and_tuning():
     for order in (0, 1):
         for waves in (64, 1024):
             for graph in (0, 1):
-                for compact, wpl in ((0, 1), (1, 1), (0, 2), (1, 2)):
+                for compact, wpl in ((0, 1), (1, 1), (2, 1), (0, 2), (1, 2), (2, 2)):
                     b.set_tuning(pg.TUNE_SEGMENT_ORDER, order)
                     b.set_tuning(pg.TUNE_COVER_WAVES, waves)
                     b.set_tuning(pg.TUNE_USE_GRAPH, graph)
@@ -97,7 +100,7 @@ def test_wide_batch_matches_plain_grading_and_tuning():
                         np.testing.assert_array_equal(m.gate_counts(), w[3], err_msg=ctx)
     best = b.autotune(True, reps=2)
     _say("autotune", best)
-    assert best["cover_waves"] in (256, 512, 1024, 2048) and best["rev_compact"] in (0, 1) and best["ms"] > 0
+    assert best["cover_waves"] in (256, 512, 1024, 2048) and best["rev_compact"] in (0, 1, 2) and best["ms"] > 0
     assert best["words_per_lane"] in (1, 2)
     tot, _ = b.grade(True)
     np.testing.assert_array_equal(tot, np.array([w[0] for w in want], np.uint64))
@@ -118,7 +121,7 @@ def test_wide_batch_ragged_pairs_two_words_per_lane(robust):
     assert any(w[0] > 0 for w in want)
     b = pg.Batch(nets, vecs, [p for _, p in specs])
     for wpl in (2, 1):
-        for compact in (1, 0):
+        for compact in (2, 1, 0):
             b.set_tuning(pg.TUNE_WORDS_PER_LANE, wpl)
             b.set_tuning(pg.TUNE_REV_COMPACT, compact)
             tot, _ = b.grade(robust)
@@ -132,6 +135,8 @@ def test_wide_batch_ragged_pairs_two_words_per_lane(robust):
                 np.testing.assert_array_equal(m.gate_counts(), w[3], err_msg=ctx)
     with pytest.raises(pg.PdfError):
         b.set_tuning(pg.TUNE_WORDS_PER_LANE, 3)
+    with pytest.raises(pg.PdfError):
+        b.set_tuning(pg.TUNE_REV_COMPACT, 3)
     b.close()
 
 
@@ -162,6 +167,7 @@ def test_batch_256_circuits_against_bitsliced():
     pairs = [p for _, _, p in members]
     b = pg.Batch(nets, vecs, pairs)
     tot, _ = b.grade(True)
+    checked = []
     for i, (nl, v, p) in enumerate(zip(nets, vecs, pairs)):
         ref = bitsliced_grade(nl, v, p, 16384, True)
         m = b.member(i)
@@ -170,7 +176,20 @@ def test_batch_256_circuits_against_bitsliced():
         np.testing.assert_array_equal(f0, ref["first_t0"], err_msg=str(i))
         np.testing.assert_array_equal(f1, ref["first_t1"], err_msg=str(i))
         np.testing.assert_array_equal(m.gate_counts(), ref["gate_counts"], err_msg=str(i))
+        checked.append((int(tot[i]), f0, f1, m.gate_counts()))
         if i % 32 == 31:
             _say("checked", i + 1)
     assert int(tot.sum()) > 0
+    # the default reverse pass runs over lists the forward launches build;
+    # over the list a scan launch builds: the same bit-exact results
+    b.set_tuning(pg.TUNE_REV_COMPACT, 1)
+    _say("batch 256, scan-built reverse lists")
+    tot, _ = b.grade(True)
+    for i, (t_i, f0, f1, counts) in enumerate(checked):
+        m = b.member(i)
+        g0, g1 = m.first()
+        assert int(tot[i]) == t_i, i
+        np.testing.assert_array_equal(g0, f0, err_msg=str(i))
+        np.testing.assert_array_equal(g1, f1, err_msg=str(i))
+        np.testing.assert_array_equal(m.gate_counts(), counts, err_msg=str(i))
     b.close()

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -5489,23 +5490,31 @@ int batch_build_segments(pdf_batch *b) {
   dfree(b->d_blist_cnt);
   b->blist_off.assign(b->maxL + 1, 0);
   b->blist_cap.assign(b->maxL + 1, 0);
-  int rc = PDF_OK;
+  int rc = PDF_OK, list_rc = PDF_OK;
   if (b->tune_rev_compact == 2) {
     const uint64_t per_block = (kBlock / 64) * b->fwd_nw();
     uint64_t at = 0;
-    for (uint32_t k = 0; k <= b->maxL; ++k) {
+    for (uint32_t k = 0; k <= b->maxL && !list_rc; ++k) {
       const uint64_t cap = per_block * ((b->fwd_grid[k] + kRevSubLists - 1) / kRevSubLists);
       if (cap > 0xFFFFFFFFull) {
         set_error("pdf_batch: a union level has too many blocks for the forward-built reverse lists");
-        return PDF_ERR_UNSUPPORTED;
+        list_rc = PDF_ERR_UNSUPPORTED;
       }
       b->blist_off[k] = at;
-      b->blist_cap[k] = (uint32_t)cap;
+      b->blist_cap[k] = (uint32_t)std::min<uint64_t>(cap, 0xFFFFFFFFull);
       at += cap * kRevSubLists;
     }
-    rc = dalloc(&b->d_blist, std::max<uint64_t>(1, at));
-    if (!rc) rc = dalloc(&b->d_blist_cnt, ((uint64_t)b->maxL + 1) * kRevSubLists * kRevCntStride);
-    if (rc) return rc;
+    if (!list_rc) list_rc = dalloc(&b->d_blist, std::max<uint64_t>(1, at));
+    if (!list_rc) list_rc = dalloc(&b->d_blist_cnt, ((uint64_t)b->maxL + 1) * kRevSubLists * kRevCntStride);
+    if (list_rc) {
+      // no room for the lists: the batch keeps a consistent state with the
+      // scan-built lists (knob value 1), and the caller learns why
+      dfree(b->d_blist);
+      dfree(b->d_blist_cnt);
+      b->tune_rev_compact = 1;
+    }
+  }
+  if (b->tune_rev_compact == 2) {
     for (uint32_t k = 0; k <= b->maxL; ++k)
       for (uint32_t s = b->fwd_off[k]; s < b->fwd_off[k] + b->fwd_cnt[k]; ++s) {
         LevelMap &mp = all[s].mp;
@@ -5560,7 +5569,7 @@ int batch_build_segments(pdf_batch *b) {
       !hip_ok(hipMemcpy(b->d_scan, scan.data(), scan.size() * sizeof(BatchScanSeg), hipMemcpyHostToDevice),
               "hipMemcpy"))
     rc = PDF_ERR_HIP;
-  return rc;
+  return rc ? rc : list_rc;
 }
 
 int batch_alloc_scratch(pdf_batch *b) {
@@ -5882,6 +5891,11 @@ int pdf_batch_autotune(pdf_batch *b, int robust, uint32_t reps, pdf_batch_tuning
     for (const auto &e : seen)
       if (e.first == key) { *med = e.second; return PDF_OK; }
     int r = apply(v);
+    if (r == PDF_ERR_NOMEM) {  // its buffers do not fit: never the choice
+      *med = std::numeric_limits<float>::infinity();
+      seen.emplace_back(key, *med);
+      return PDF_OK;
+    }
     if (r) return r;
     std::vector<float> t(reps + 1);
     for (uint32_t i = 0; i <= reps; ++i)  // the first run warms up (and captures)

@@ -516,7 +516,10 @@ int pdf_batch_grade(pdf_batch *b, int robust, uint64_t *totals, float *ms);
  * pdf_get_total) after pdf_batch_grade; owned by the batch. */
 int pdf_batch_member(pdf_batch *b, uint32_t circuit, pdf_grader **out);
 /* PDF_TUNE_USE_GRAPH, PDF_TUNE_COVER_WAVES, PDF_TUNE_SEGMENT_ORDER, PDF_TUNE_REV_COMPACT and
- * PDF_TUNE_WORDS_PER_LANE (2 takes effect only when every circuit has at least 4,096 pairs). */
+ * PDF_TUNE_WORDS_PER_LANE (2 takes effect only when every circuit has at least 4,096 pairs).
+ * When the lists of PDF_TUNE_REV_COMPACT = 2 do not fit in device memory (after this call or
+ * a call that changes the segments), the batch keeps running with value 1 and the call
+ * returns PDF_ERR_NOMEM. */
 int pdf_batch_set_tuning(pdf_batch *b, int key, int64_t value);
 /* Time every combination of the knobs' values (segment order, cover waves,
  * graph replay, reverse pass, words per lane: 96 at most; each
